@@ -1,0 +1,195 @@
+/*
+ * pyrticle_mi355x.h -- C ABI of libpyrticle_mi355x.so, the MI355X (gfx950)
+ * implementation of pyrticle's particle hot path: localization, shape-function
+ * deposition of rho and j, DG interpolation of E and B, and the relativistic
+ * Lorentz push.
+ *
+ * The reference reaches this path through its Boost.Python module
+ * pyrticle._internal (src/wrapper/).  Each entry point below names the
+ * reference interface it replaces.  Conventions:
+ *
+ *   - every function returns 0 (PYR_OK) or a negative PYR_ERR_* code; the
+ *     message is available from pyr_last_error() (thread-local).  No C++
+ *     exception crosses the ABI.
+ *   - particle arrays are DEVICE pointers in the reference's layout:
+ *     positions n x xdim and momenta n x vdim (float64, C order), charges and
+ *     masses n (float64), containing elements n (uint32).  xdim == vdim == dim.
+ *   - work is ordered on the stream given to pyr_set_stream (default: the
+ *     null stream); pyr_sync() waits for it.  Functions that return a count
+ *     on the host synchronize the stream themselves.
+ *   - pyr_mesh_create copies the mesh from HOST memory into HBM; the handle
+ *     owns it.
+ */
+#ifndef PYRTICLE_MI355X_H
+#define PYRTICLE_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYR_INVALID_ELEMENT 0xFFFFFFFFu
+
+enum {
+  PYR_OK = 0,
+  PYR_ERR_INVALID_ARG = -1,
+  PYR_ERR_HIP = -2,
+  PYR_ERR_NO_BEST_NORMAL = -3, /* particle_state.hpp:176-189 */
+  PYR_ERR_NOT_FOUND = -4,
+  PYR_ERR_INTERNAL = -5
+};
+
+/* precision forms */
+enum {
+  PYR_FP64 = 0,   /* plain FP64 throughout: the drop-in */
+  PYR_FP32C = 1   /* FP32 bulk arithmetic, Kahan-compensated per-node sums,
+                     FP64 positions, face tests and charge normalization */
+};
+
+typedef struct pyr_mesh pyr_mesh;
+
+/* Mesh contract, all HOST pointers (layout: pyrticle_amd/meshdata.py).
+ * Replaces _internal.MeshData + MeshData.fill_from_hedge
+ * (wrap_meshdata.cpp:50-114, meshdata.py:37-133, meshdata.hpp:53-216). */
+typedef struct {
+  int32_t dim, K, Np, nv, nadj;
+  const double *inv_map;        /* K x (d*d + d): A row-major, then b        */
+  const double *jacobian;       /* K                                          */
+  const uint32_t *el_vertices;  /* K x (d+1)                                  */
+  const double *face_normal;    /* K x (d+1) x d                              */
+  const double *face_rhs;       /* K x (d+1)                                  */
+  const double *face_centroid;  /* K x (d+1) x d                              */
+  const double *face_radius;    /* K x (d+1)                                  */
+  const uint32_t *face_neighbor;/* K x (d+1), PYR_INVALID_ELEMENT on boundary  */
+  const uint32_t *face_per_axis;/* K x (d+1), 0xFFFFFFFF if not periodic      */
+  const double *vertices;       /* nv x d                                     */
+  const uint32_t *vadj_starts;  /* nv + 1                                     */
+  const uint32_t *vadj_elements;/* nadj                                       */
+  const double *periodicity;    /* d x 2 (min, max); min == max: none         */
+  const double *nodes;          /* K*Np x d                                   */
+  const int32_t *exponents;     /* Np x d monomial exponents (node tuples)    */
+  const double *vandermonde;    /* Np x Np row-major, V[i][k] = phi_k(node_i) */
+} pyr_mesh_desc;
+
+/* library ------------------------------------------------------------------ */
+int pyr_init(int device);                 /* select the HIP device           */
+const char *pyr_last_error(void);
+int pyr_set_stream(void *hip_stream);     /* hipStream_t; NULL = null stream */
+int pyr_sync(void);
+int pyr_version(void);
+
+/* mesh --------------------------------------------------------------------- */
+int pyr_mesh_create(const pyr_mesh_desc *desc, pyr_mesh **out);
+int pyr_mesh_destroy(pyr_mesh *mesh);
+
+/* meshdata.hpp:208-215 find_containing_element (wrap_meshdata.cpp:110), for n
+ * points: the lowest-numbered element containing each point, or
+ * PYR_INVALID_ELEMENT. */
+int pyr_find_containing_element(pyr_mesh *mesh, int64_t n, const double *x,
+                                uint32_t *elem_out);
+
+/* meshdata.hpp:202-206 is_in_element (wrap_meshdata.cpp:108), for n
+ * (element, point) pairs; inside_out[i] is 0 or 1. */
+int pyr_is_in_element(pyr_mesh *mesh, int64_t n, const uint32_t *elem, const double *x,
+                      uint8_t *inside_out);
+
+/* particles ----------------------------------------------------------------- */
+/* particle_state.hpp:98-125 get_velocities (wrap_pic.cpp:71). */
+int pyr_velocities(int64_t n, int vdim, double vacuum_c, const double *momenta,
+                   const double *masses, double *v_out);
+
+/* particle_state.hpp:130-357 find_new_containing_element, for all n particles
+ * as update_containing_elements does (wrap_pic.cpp:72-75), including the
+ * periodic wrap of boundary_hit, which updates positions in place.  Instead of
+ * calling the boundary-hit listener, particles that must die get dead_out = 1;
+ * pyr_compact_plan turns those flags into the reference's kill order.
+ * counters (device, 4 x uint64: same, neighbor, vertex, global) accumulate
+ * (FindEventCounters, wrap_pic.cpp:90-96); may be NULL. */
+int pyr_locate(pyr_mesh *mesh, int64_t n, double *positions, const double *momenta,
+               uint32_t *containing_elements, uint8_t *dead_out,
+               unsigned long long *counters);
+
+/* The net effect of kill_particle/move_particle (particle_state.hpp:361-407)
+ * applied in update_containing_elements' order to the particles flagged dead:
+ * holes below the new count, in ascending order, receive the surviving tail
+ * particles in descending order.  Writes n_moves (from, to) pairs to device
+ * arrays of capacity n and returns the new count and n_moves on the host. */
+int pyr_compact_plan(int64_t n, const uint8_t *dead, uint32_t *move_from,
+                     uint32_t *move_to, int64_t *n_new_out, int64_t *n_moves_out);
+
+/* Applies moves to one per-particle array of elem_bytes bytes per particle
+ * (NumberShiftableVector.note_move, pyrticle/tools.py:174-175). */
+int pyr_apply_moves(void *buffer, int64_t elem_bytes, int64_t n_moves,
+                    const uint32_t *move_from, const uint32_t *move_to);
+
+/* push_monomial.hpp:219-325 monomial_particle_pusher::forces
+ * (wrap_pusher.cpp:101-119) with make_interpolator (:155-210): interpolates
+ * the six nodal fields (device, K*Np each; NULL = ZeroVector) at the particle
+ * positions and returns F = q e + v x (q b) truncated to vdim.  vis_* (n x 3,
+ * may be NULL) receive pt_e, pt_b, el_force, mag_force as the reference's
+ * visualization listener does (:307-322). */
+int pyr_forces(pyr_mesh *mesh, int64_t n, int vdim, const double *const fields[6],
+               const double *positions, const uint32_t *containing_elements,
+               const double *charges, const double *velocities, double *force_out,
+               double *vis_e, double *vis_b, double *vis_el_force, double *vis_mag_force);
+
+/* Stores six nodal fields (device, K*Np each; NULL = zero) as per-element
+ * monomial coefficients inside the mesh handle, for pyr_lsrk4_steps.  The
+ * static fields of a run are transformed once.  No reference counterpart: the
+ * reference re-solves the interpolation system per call (push_monomial.hpp:
+ * 192-203); mathematically the same interpolant. */
+int pyr_set_static_fields(pyr_mesh *mesh, const double *const fields[6]);
+
+/* Fused hedge LSRK4 (5 stages) of the particle RHS [v, F] in the static fields
+ * of pyr_set_static_fields, nsteps steps of size dt, with relocalization and
+ * kills after every stage exactly as cloud.py:554-603 advance_state and
+ * update_containing_elements order them (kills compact the arrays).  One
+ * kernel pass per step keeps each particle's 5 stages in registers.
+ * *n_inout (host) is the particle count before and after.  ids (device int64,
+ * may be NULL) is moved along with the particles.  precision: PYR_FP64 or
+ * PYR_FP32C (interpolation and force in FP32, positions and momenta FP64). */
+int pyr_lsrk4_steps(pyr_mesh *mesh, int precision, int nsteps, double dt, double vacuum_c,
+                    int64_t *n_inout, double *positions, double *momenta, double *charges,
+                    double *masses, uint32_t *containing_elements, int64_t *ids,
+                    unsigned long long *counters);
+
+/* deposition ---------------------------------------------------------------- */
+/* dep_target.hpp:203-273 deposit_densities / deposit_j / deposit_rho
+ * (wrap_depositor.cpp:45-50, over slice(None)) for InterpolatingDepositor
+ * (shape_function_depositor, dep_shape.hpp:67-150) with the polynomial shape
+ * S(r) = normalizer * (R - r^2/R)^alpha for r <= R (tools.hpp:514-556;
+ * normalizer as tools.cpp:43-58 computes it).  rho (K*Np) and j (K*Np x
+ * vdim) are overwritten; either may be NULL; velocities may be NULL when j is.
+ * Particles whose containing element is invalid contribute nothing.
+ * precision PYR_FP32C evaluates the shape and sums in FP32 with Kahan-
+ * compensated per-node sums; positions relative to the element, q*normalizer
+ * and the outputs stay FP64. */
+int pyr_deposit(pyr_mesh *mesh, double radius, double alpha, double normalizer,
+                int precision, int64_t n, int vdim, const double *positions,
+                const double *charges, const double *velocities,
+                const uint32_t *containing_elements, double *rho_out, double *j_out);
+
+/* Instrumentation (no reference counterpart): when on, pyr_deposit also counts
+ * [node-particle pairs evaluated, pairs inside R, candidate particle visits,
+ * visits passing the per-element test]; pyr_deposit_stats reads the counts of
+ * the last call. */
+int pyr_deposit_set_counting(int on);
+int pyr_deposit_stats(pyr_mesh *mesh, unsigned long long out[4]);
+
+/* Number of (target element, source element) pairs in the gather candidate
+ * lists for radius R (built on first use, cached per radius). */
+int pyr_deposit_candidates(pyr_mesh *mesh, double radius, int64_t *n_pairs_out);
+
+/* Stable sort of particle indices by containing element (the counting/radix
+ * sort the deposition uses): perm_out[j] is the particle at sorted position j
+ * (numpy.argsort(kind="stable") of the element ids); seg_out (K+1, may be NULL)
+ * the start of each element's run.  No reference counterpart. */
+int pyr_sort_by_element(pyr_mesh *mesh, int64_t n, const uint32_t *containing_elements,
+                        uint32_t *perm_out, uint32_t *seg_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

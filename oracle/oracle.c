@@ -1,0 +1,612 @@
+/*
+ * oracle.c -- CPU restatement of pyrticle's particle hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library links, loads or
+ * calls this file; only tests/, the smoke check and the benchmark's CPU-baseline
+ * leg use it, as the checker.  It is single-threaded FP64 C, compiled with
+ * -ffp-contract=off so that every a*b+c rounds twice, as the reference C++
+ * (g++, no FMA contraction on x86-64 by default) does.
+ *
+ * Each function names the reference lines it restates.  Parts that the
+ * reference delegates to third-party code are restated from the published
+ * algorithm:
+ *   - hedge affine_map::operator() = prod(A, x) + b (ublas row-wise dot);
+ *   - LAPACK dgetf2/dgetrs and reference-BLAS dtrsm/dger/dlaswp (netlib);
+ *   - Boost.Math beta(a, b) = Gamma(a) Gamma(b) / Gamma(a + b);
+ *   - hedge LSRK4TimeStepper: Carpenter & Kennedy (1994), NASA TM-109112,
+ *     five-stage fourth-order 2N-storage coefficients.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_INVALID 0xFFFFFFFFu
+
+typedef struct {
+  int dim, K, Np, nv;
+  const double *inv_map;        /* K x (d*d + d): A row-major, then b */
+  const double *jacobian;       /* K */
+  const uint32_t *el_vertices;  /* K x (d+1) */
+  const double *face_normal;    /* K x (d+1) x d */
+  const double *face_rhs;       /* K x (d+1) */
+  const double *face_centroid;  /* K x (d+1) x d */
+  const double *face_radius;    /* K x (d+1) */
+  const uint32_t *face_neighbor;/* K x (d+1) */
+  const uint32_t *face_per_axis;/* K x (d+1) */
+  const double *vertices;       /* nv x d */
+  const uint32_t *vadj_starts;  /* nv + 1 */
+  const uint32_t *vadj_elements;
+  const double *periodicity;    /* d x 2: (min, max); min == max: not periodic */
+  const double *nodes;          /* K*Np x d */
+} orc_mesh;
+
+typedef struct {
+  uint64_t same, neighbor, vertex, global;
+} orc_counters;
+
+static char g_err[512];
+const char *orc_last_error(void) { return g_err; }
+
+/* ---------------------------------------------------------------------- */
+/* meshdata.hpp:38-48 is_in_unit_simplex                                   */
+int orc_is_in_unit_simplex(int d, const double *r, double tol) {
+  for (int i = 0; i < d; ++i)
+    if (r[i] < -1 - tol) return 0;
+  double s = 0;
+  for (int i = 0; i < d; ++i) s += r[i];
+  return s <= (double)(-(d - 2)) + tol;
+}
+
+/* hedge affine_map<double>::operator(): prod(matrix, x) + vector           */
+void orc_inverse_map(const orc_mesh *m, uint32_t en, const double *x, double *r) {
+  const int d = m->dim;
+  const double *a = m->inv_map + (size_t)en * (d * d + d);
+  const double *b = a + d * d;
+  for (int i = 0; i < d; ++i) {
+    double t = 0;
+    for (int j = 0; j < d; ++j) t += a[i * d + j] * x[j];
+    r[i] = t + b[i];
+  }
+}
+
+static int in_element(const orc_mesh *m, uint32_t en, const double *pt) {
+  double r[3];
+  orc_inverse_map(m, en, pt, r);
+  return orc_is_in_unit_simplex(m->dim, r, 1e-10);
+}
+
+/* meshdata.hpp:208-215 find_containing_element (first in index order)      */
+uint32_t orc_find_containing_element(const orc_mesh *m, const double *pt) {
+  for (int en = 0; en < m->K; ++en)
+    if (in_element(m, (uint32_t)en, pt)) return (uint32_t)en;
+  return ORC_INVALID;
+}
+
+/* particle_state.hpp:130-261 find_new_containing_element.
+ * Returns the element, ORC_INVALID, or sets *err when every face normal has
+ * n.p <= 0 (the reference throws "no best normal found--weird").           */
+uint32_t orc_find_new_containing_element(const orc_mesh *m, const double *pt,
+                                         const double *mom, uint32_t prev,
+                                         orc_counters *c, int *err) {
+  const int d = m->dim;
+  if (prev != ORC_INVALID) {
+    if (in_element(m, prev, pt)) {
+      c->same++;
+      return prev;
+    }
+    {
+      int closest = -1;
+      double max_ip = 0;
+      for (int f = 0; f <= d; ++f) {
+        const double *n = m->face_normal + ((size_t)prev * (d + 1) + f) * d;
+        double ip = 0;
+        for (int i = 0; i < d; ++i) ip += n[i] * mom[i];
+        if (ip > max_ip) {
+          closest = f;
+          max_ip = ip;
+        }
+      }
+      if (closest == -1) {
+        snprintf(g_err, sizeof g_err, "no best normal found--weird (momentum=[%g,%g,%g])",
+                 mom[0], d > 1 ? mom[1] : 0.0, d > 2 ? mom[2] : 0.0);
+        *err = 1;
+        return ORC_INVALID;
+      }
+      uint32_t possible = m->face_neighbor[(size_t)prev * (d + 1) + closest];
+      if (possible != ORC_INVALID && in_element(m, possible, pt)) {
+        c->neighbor++;
+        return possible;
+      }
+    }
+    {
+      uint32_t closest_vertex = ORC_INVALID;
+      double min_dist = INFINITY;
+      for (int k = 0; k <= d; ++k) {
+        uint32_t vi = m->el_vertices[(size_t)prev * (d + 1) + k];
+        double s = 0;
+        for (int i = 0; i < d; ++i) {
+          double dv = m->vertices[(size_t)vi * d + i] - pt[i];
+          s += dv * dv;
+        }
+        double dist = sqrt(s);
+        if (dist < min_dist) {
+          closest_vertex = vi;
+          min_dist = dist;
+        }
+      }
+      for (uint32_t k = m->vadj_starts[closest_vertex];
+           k < m->vadj_starts[closest_vertex + 1]; ++k) {
+        uint32_t possible = m->vadj_elements[k];
+        if (in_element(m, possible, pt)) {
+          c->vertex++;
+          return possible;
+        }
+      }
+    }
+  }
+  c->global++;
+  return orc_find_containing_element(m, pt);
+}
+
+/* ---------------------------------------------------------------------- */
+/* particle state with extra per-particle vectors that follow moves
+ * (pyrticle/tools.py:106-178 NumberShiftableVector)                         */
+typedef struct {
+  int xd, vd;
+  int64_t n;
+  double *x, *p, *q, *m;
+  uint32_t *elem;
+  int nextra;
+  double **extra;    /* each extra[k] has width extra_w[k] doubles/particle */
+  const int *extra_w;
+  int64_t *ids;      /* optional identity array, moved along */
+} orc_state;
+
+/* particle_state.hpp:386-407 move_particle                                 */
+static void move_particle(orc_state *s, int64_t from, int64_t to) {
+  s->elem[to] = s->elem[from];
+  for (int k = 0; k < s->nextra; ++k) {
+    int w = s->extra_w[k];
+    for (int i = 0; i < w; ++i) s->extra[k][to * w + i] = s->extra[k][from * w + i];
+  }
+  if (s->ids) s->ids[to] = s->ids[from];
+  for (int i = 0; i < s->xd; ++i) s->x[to * s->xd + i] = s->x[from * s->xd + i];
+  for (int i = 0; i < s->vd; ++i) s->p[to * s->vd + i] = s->p[from * s->vd + i];
+  s->q[to] = s->q[from];
+  s->m[to] = s->m[from];
+}
+
+/* particle_state.hpp:361-377 kill_particle                                 */
+static void kill_particle(orc_state *s, int64_t pn) {
+  --s->n;
+  if (s->n) move_particle(s, s->n, pn);
+}
+
+/* particle_state.hpp:301-357 boundary_hit; returns 1 if the particle was
+ * re-found after a periodic wrap, 0 if it must die.                        */
+static int boundary_hit(const orc_mesh *m, orc_state *s, int64_t pn, orc_counters *c,
+                        int *err) {
+  const int d = m->dim;
+  double pt[3];
+  int trip = 0;
+  for (int i = 0; i < d; ++i) pt[i] = s->x[pn * s->xd + i];
+  for (int axis = 0; axis < d; ++axis) {
+    double mn = m->periodicity[2 * axis], mx = m->periodicity[2 * axis + 1];
+    if (mn != mx) {
+      double xi = pt[axis];
+      if (xi < mn) {
+        pt[axis] += (mx - mn);
+        trip = 1;
+      } else if (xi > mx) {
+        pt[axis] -= (mx - mn);
+        trip = 1;
+      }
+    }
+  }
+  if (trip) {
+    for (int i = 0; i < d; ++i) s->x[pn * s->xd + i] = pt[i];
+    uint32_t ce = orc_find_new_containing_element(m, s->x + pn * s->xd, s->p + pn * s->vd,
+                                                  s->elem[pn], c, err);
+    if (*err) return 0;
+    if (ce != ORC_INVALID) {
+      s->elem[pn] = ce;
+      return 1;
+    }
+  }
+  return 0;
+}
+
+/* particle_state.hpp:266-295 update_containing_elements, with the kill
+ * listener of cloud.py:581-585 (kill_particle on every boundary hit).      */
+static int update_containing_elements(const orc_mesh *m, orc_state *s, orc_counters *c) {
+  int err = 0;
+  for (int64_t pn = 0; pn < s->n;) {
+    uint32_t prev = s->elem[pn];
+    uint32_t ne = orc_find_new_containing_element(m, s->x + pn * s->xd, s->p + pn * s->vd,
+                                                  prev, c, &err);
+    if (err) return -1;
+    if (ne == ORC_INVALID) {
+      int found = boundary_hit(m, s, pn, c, &err);
+      if (err) return -1;
+      if (!found) kill_particle(s, pn);
+      /* a re-found particle keeps pn and is examined again, as the
+         reference's loop does (pn does not advance) */
+    } else {
+      s->elem[pn] = ne;
+      ++pn;
+    }
+  }
+  return 0;
+}
+
+int orc_update_containing_elements(const orc_mesh *m, int xd, int vd, int64_t *n,
+                                   double *x, double *p, double *q, double *mass,
+                                   uint32_t *elem, int nextra, double **extra,
+                                   const int *extra_w, int64_t *ids, orc_counters *c) {
+  orc_state s = {xd, vd, *n, x, p, q, mass, elem, nextra, extra, extra_w, ids};
+  int rc = update_containing_elements(m, &s, c);
+  *n = s.n;
+  return rc;
+}
+
+/* ---------------------------------------------------------------------- */
+/* particle_state.hpp:98-125 get_velocities                                 */
+void orc_velocities(int64_t n, int vd, double c, const double *p, const double *mass,
+                    double *v) {
+  for (int64_t pn = 0; pn < n; ++pn) {
+    const double m = mass[pn];
+    double s = 0;
+    for (int i = 0; i < vd; ++i) s += p[pn * vd + i] * p[pn * vd + i];
+    double pm = sqrt(s);
+    if (pm == 0) {
+      for (int i = 0; i < vd; ++i) v[pn * vd + i] = 0;
+    } else {
+      double vm = c * pm / sqrt(m * m * c * c + pm * pm);
+      double f = vm / pm;
+      for (int i = 0; i < vd; ++i) v[pn * vd + i] = f * p[pn * vd + i];
+    }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* tools.cpp:35-58 sphere_area and the polynomial shape normalizer           */
+double orc_shape_normalizer(double radius, int dims, double alpha) {
+  const double n = dims;
+  double sphere_area = 2 * pow(M_PI, n / 2) / tgamma(n / 2);
+  double a = n / 2, b = alpha + 1;
+  double beta = tgamma(a) * tgamma(b) / tgamma(a + b);
+  return 2 / (sphere_area * pow(radius, n + alpha) * beta);
+}
+
+/* tools.hpp:520-534 polynomial_shape_function::operator()                   */
+static inline double shape_eval(double normalizer, double radius, double alpha,
+                                const double *r, int d) {
+  double r2 = 0;
+  for (int i = 0; i < d; ++i) r2 += r[i] * r[i];
+  if (r2 > radius * radius) return 0;
+  double t = radius - r2 / radius;
+  if (alpha == 2) return normalizer * t * t;
+  return normalizer * pow(t, alpha);
+}
+
+double orc_shape_value(double normalizer, double radius, double alpha, int d,
+                       const double *r) {
+  return shape_eval(normalizer, radius, alpha, r, d);
+}
+
+/* ---------------------------------------------------------------------- */
+/* element_finder.hpp:203-294 face_based_element_finder, feeding
+ * dep_shape.hpp:88-106 element_target and dep_target.hpp:54-186 targets.   */
+typedef struct {
+  const orc_mesh *m;
+  double normalizer, radius, alpha;
+  double charge;
+  const double *vel;   /* vd entries or NULL */
+  int vd;
+  double *rho, *j;
+  int64_t *stamp;
+  int64_t mark;
+  uint64_t visits, elements;
+  int depth, max_depth;
+} dfs_ctx;
+
+static void add_shape_on_element(dfs_ctx *c, const double *center, uint32_t en) {
+  const orc_mesh *m = c->m;
+  const int d = m->dim, np = m->Np;
+  const size_t start = (size_t)en * np;
+  c->elements++;
+  for (int i = 0; i < np; ++i) {
+    double r[3];
+    for (int k = 0; k < d; ++k) r[k] = m->nodes[(start + i) * d + k] - center[k];
+    double el_rho = c->charge * shape_eval(c->normalizer, c->radius, c->alpha, r, d);
+    c->visits++;
+    if (c->rho) c->rho[start + i] += el_rho;
+    if (c->j)
+      for (int a = 0; a < c->vd; ++a) c->j[(start + i) * c->vd + a] += c->vel[a] * el_rho;
+  }
+}
+
+static void recurse(dfs_ctx *c, const double *pos, uint32_t en) {
+  const orc_mesh *m = c->m;
+  const int d = m->dim;
+  if (++c->depth > c->max_depth) c->max_depth = c->depth;
+  add_shape_on_element(c, pos, en);
+  c->stamp[en] = c->mark;
+  for (int f = 0; f <= d; ++f) {
+    const size_t fi = (size_t)en * (d + 1) + f;
+    uint32_t nb = m->face_neighbor[fi];
+    if (nb == ORC_INVALID) continue;
+    if (c->stamp[nb] == c->mark) continue;
+    const double *n = m->face_normal + fi * d;
+    double ip = 0;
+    for (int i = 0; i < d; ++i) ip += pos[i] * n[i];
+    double plane_dist = fabs(ip - m->face_rhs[fi]);
+    if (plane_dist > c->radius) continue;
+    const double *fc = m->face_centroid + fi * d;
+    double s = 0;
+    for (int i = 0; i < d; ++i) {
+      double dv = fc[i] - pos[i];
+      s += dv * dv;
+    }
+    if (sqrt(s) > c->radius + m->face_radius[fi]) continue;
+    uint32_t axis = m->face_per_axis[fi];
+    if (axis == ORC_INVALID) {
+      recurse(c, pos, nb);
+    } else {
+      double pos2[3];
+      for (int i = 0; i < d; ++i) pos2[i] = pos[i];
+      double mn = m->periodicity[2 * axis], mx = m->periodicity[2 * axis + 1];
+      if (pos[axis] - c->radius < mn) {
+        pos2[axis] += (mx - mn);
+        recurse(c, pos2, nb);
+      }
+      if (pos[axis] + c->radius > mx) {
+        pos2[axis] -= (mx - mn);
+        recurse(c, pos2, nb);
+      }
+    }
+  }
+  --c->depth;
+}
+
+/* dep_target.hpp:203-273 deposit_densities / deposit_j / deposit_rho over
+ * the whole particle slice (dep_shape.hpp:128-149).  rho and/or j may be
+ * NULL.  visits_out receives the node-visit and element-visit counts.       */
+int orc_deposit(const orc_mesh *m, double radius, double alpha, int64_t n, int xd, int vd,
+                const double *x, const double *q, const double *v, const uint32_t *elem,
+                double *rho, double *j, uint64_t *visits_out) {
+  const size_t nn = (size_t)m->K * m->Np;
+  if (rho) memset(rho, 0, nn * sizeof(double));
+  if (j) memset(j, 0, nn * vd * sizeof(double));
+  dfs_ctx c;
+  memset(&c, 0, sizeof c);
+  c.m = m;
+  c.radius = radius;
+  c.alpha = alpha;
+  c.normalizer = orc_shape_normalizer(radius, m->dim, alpha);
+  c.vd = vd;
+  c.rho = rho;
+  c.j = j;
+  c.stamp = (int64_t *)malloc(sizeof(int64_t) * (size_t)m->K);
+  if (!c.stamp) return -1;
+  for (int k = 0; k < m->K; ++k) c.stamp[k] = -1;
+  for (int64_t pn = 0; pn < n; ++pn) {
+    double pos[3];
+    for (int i = 0; i < m->dim; ++i) pos[i] = x[pn * xd + i];
+    c.charge = q[pn];
+    c.vel = v ? v + pn * vd : NULL;
+    c.mark = pn;
+    recurse(&c, pos, elem[pn]);
+  }
+  free(c.stamp);
+  if (visits_out) {
+    visits_out[0] = c.visits;
+    visits_out[1] = c.elements;
+    visits_out[2] = (uint64_t)c.max_depth;
+  }
+  return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* LAPACK dgetf2 (netlib): LU with partial pivoting of a column-major n x n
+ * matrix, as wrap_tools.cpp:126-145 (lu -> getrf) uses it.  ipiv 0-based.  */
+int orc_getrf(int n, double *a, int *ipiv) {
+  const double sfmin = 2.2250738585072014e-308; /* dlamch('S') */
+  int info = 0;
+  for (int j = 0; j < n; ++j) {
+    int jp = j;
+    double best = fabs(a[j + j * n]);
+    for (int i = j + 1; i < n; ++i)
+      if (fabs(a[i + j * n]) > best) {
+        best = fabs(a[i + j * n]);
+        jp = i;
+      }
+    ipiv[j] = jp;
+    if (a[jp + j * n] != 0) {
+      if (jp != j)
+        for (int k = 0; k < n; ++k) {
+          double t = a[j + k * n];
+          a[j + k * n] = a[jp + k * n];
+          a[jp + k * n] = t;
+        }
+      if (j < n - 1) {
+        if (fabs(a[j + j * n]) >= sfmin) {
+          double r = 1.0 / a[j + j * n];
+          for (int i = j + 1; i < n; ++i) a[i + j * n] *= r;
+        } else {
+          for (int i = j + 1; i < n; ++i) a[i + j * n] /= a[j + j * n];
+        }
+      }
+    } else if (info == 0) {
+      info = j + 1;
+    }
+    if (j < n - 1) {
+      /* dger: A(j+1:, j+1:) += x * (-y)^T, x = A(j+1:, j), y = A(j, j+1:) */
+      for (int jj = j + 1; jj < n; ++jj) {
+        double yv = a[j + jj * n];
+        if (yv != 0) {
+          double temp = -1.0 * yv;
+          for (int i = j + 1; i < n; ++i) a[i + jj * n] += a[i + j * n] * temp;
+        }
+      }
+    }
+  }
+  return info;
+}
+
+/* LAPACK dgetrs('N') (netlib): dlaswp, then dtrsm lower-unit and
+ * upper-nonunit, column by column on B (n x nrhs, column-major).           */
+void orc_getrs(int n, int nrhs, const double *lu, const int *ipiv, double *b) {
+  for (int c = 0; c < nrhs; ++c) {
+    double *bc = b + (size_t)c * n;
+    for (int i = 0; i < n; ++i)
+      if (ipiv[i] != i) {
+        double t = bc[i];
+        bc[i] = bc[ipiv[i]];
+        bc[ipiv[i]] = t;
+      }
+    for (int k = 0; k < n; ++k)
+      if (bc[k] != 0)
+        for (int i = k + 1; i < n; ++i) bc[i] -= bc[k] * lu[i + k * n];
+    for (int k = n - 1; k >= 0; --k)
+      if (bc[k] != 0) {
+        bc[k] = bc[k] / lu[k + k * n];
+        for (int i = 0; i < k; ++i) bc[i] -= bc[k] * lu[i + k * n];
+      }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* push_monomial.hpp:45-70 monomial_basis_function, :155-210 make_interpolator,
+ * :99-111 interpolator::operator(), :219-325 forces, tools.hpp:253-264 cross.
+ * exps: nb x d exponents; lu/ipiv: LU of V^T (column-major) from orc_getrf.
+ * fields[6] = ex, ey, ez, bx, by, bz nodal vectors; NULL = ZeroVector.       */
+typedef struct {
+  int nb;
+  const int *exps;
+  const double *lu;
+  const int *ipiv;
+} orc_interp;
+
+static void lagrange_weights(const orc_mesh *m, const orc_interp *ip, uint32_t en,
+                             const double *x, double *coef) {
+  double r[3];
+  orc_inverse_map(m, en, x, r);
+  for (int k = 0; k < ip->nb; ++k) {
+    double result = 1;
+    for (int i = 0; i < m->dim; ++i) result *= pow(r[i], (double)ip->exps[k * m->dim + i]);
+    coef[k] = result;
+  }
+  orc_getrs(ip->nb, 1, ip->lu, ip->ipiv, coef);
+}
+
+int orc_forces(const orc_mesh *m, int nb, const int *exps, const double *lu,
+               const int *ipiv, int64_t n, int xd, int vd, const double *x,
+               const uint32_t *elem, const double *q, const double *v,
+               const double *const *fields, double *force, double *vis_e,
+               double *vis_b, double *vis_el, double *vis_mag) {
+  orc_interp ip = {nb, exps, lu, ipiv};
+  double coef[64];
+  if (nb > 64) return -1;
+  for (int64_t pn = 0; pn < n; ++pn) {
+    uint32_t en = elem[pn];
+    lagrange_weights(m, &ip, en, x + pn * xd, coef);
+    double f6[6];
+    for (int c = 0; c < 6; ++c) {
+      if (!fields[c]) {
+        f6[c] = 0;
+        continue;
+      }
+      const double *data = fields[c] + (size_t)en * nb;
+      double t = 0;
+      for (int k = 0; k < nb; ++k) t += coef[k] * data[k];
+      f6[c] = t;
+    }
+    const double charge = q[pn];
+    double el[3], qb[3], vv[3], mag[3];
+    for (int i = 0; i < 3; ++i) {
+      el[i] = charge * f6[i];
+      qb[i] = charge * f6[3 + i];
+      vv[i] = i < vd ? v[pn * vd + i] : 0;
+    }
+    mag[0] = vv[1] * qb[2] - vv[2] * qb[1];
+    mag[1] = vv[2] * qb[0] - vv[0] * qb[2];
+    mag[2] = vv[0] * qb[1] - vv[1] * qb[0];
+    for (int i = 0; i < vd; ++i) force[pn * vd + i] = el[i] + mag[i];
+    if (vis_e)
+      for (int i = 0; i < 3; ++i) {
+        vis_e[pn * 3 + i] = f6[i];
+        vis_b[pn * 3 + i] = f6[3 + i];
+        vis_el[pn * 3 + i] = el[i];
+        vis_mag[pn * 3 + i] = mag[i];
+      }
+  }
+  return 0;
+}
+
+/* Lagrange weights of one point, for identity checks of the interpolator.  */
+void orc_lagrange_weights(const orc_mesh *m, int nb, const int *exps, const double *lu,
+                          const int *ipiv, uint32_t en, const double *x, double *coef) {
+  orc_interp ip = {nb, exps, lu, ipiv};
+  lagrange_weights(m, &ip, en, x, coef);
+}
+
+/* ---------------------------------------------------------------------- */
+/* hedge LSRK4TimeStepper (Carpenter & Kennedy 1994 coefficients), driving
+ * the pyrticle particle RHS (cloud.py:238-311: [v, F, 0]) and advance_state
+ * (cloud.py:554-603: x += dx, p += dp, relocalize and kill, with the RK
+ * residual vectors following every move, tools.py:106-178).                */
+static const double RK4A[5] = {
+    0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+    -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+static const double RK4B[5] = {
+    1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+    1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+    2277821191437.0 / 14882151754819.0};
+
+void orc_lsrk4_coefficients(double *a, double *b) {
+  for (int i = 0; i < 5; ++i) {
+    a[i] = RK4A[i];
+    b[i] = RK4B[i];
+  }
+}
+
+int orc_lsrk4_run(const orc_mesh *m, int nb, const int *exps, const double *lu,
+                  const int *ipiv, const double *const *fields, double c_light, int xd,
+                  int vd, int64_t *n_inout, double *x, double *p, double *q, double *mass,
+                  uint32_t *elem, int64_t *ids, double dt, int nsteps, orc_counters *cnt) {
+  int64_t n = *n_inout;
+  double *rx = (double *)calloc((size_t)(n ? n : 1) * xd, sizeof(double));
+  double *rp = (double *)calloc((size_t)(n ? n : 1) * vd, sizeof(double));
+  double *v = (double *)malloc(sizeof(double) * (size_t)(n ? n : 1) * vd);
+  double *f = (double *)malloc(sizeof(double) * (size_t)(n ? n : 1) * vd);
+  if (!rx || !rp || !v || !f) return -1;
+  double *extra[2] = {rx, rp};
+  int extra_w[2] = {xd, vd};
+  int rc = 0;
+  for (int step = 0; step < nsteps && rc == 0; ++step) {
+    for (int s = 0; s < 5; ++s) {
+      orc_velocities(n, vd, c_light, p, mass, v);
+      rc = orc_forces(m, nb, exps, lu, ipiv, n, xd, vd, x, elem, q, v, fields, f, NULL,
+                      NULL, NULL, NULL);
+      if (rc) break;
+      const double a = RK4A[s], b = RK4B[s];
+      for (int64_t i = 0; i < n * xd; ++i) rx[i] = a * rx[i] + dt * v[i];
+      for (int64_t i = 0; i < n * vd; ++i) rp[i] = a * rp[i] + dt * f[i];
+      for (int64_t i = 0; i < n * xd; ++i) x[i] += b * rx[i];
+      for (int64_t i = 0; i < n * vd; ++i) p[i] += b * rp[i];
+      orc_state st = {xd, vd, n, x, p, q, mass, elem, 2, extra, extra_w, ids};
+      rc = update_containing_elements(m, &st, cnt);
+      n = st.n;
+      if (rc) break;
+    }
+  }
+  free(rx);
+  free(rp);
+  free(v);
+  free(f);
+  *n_inout = n;
+  return rc;
+}

@@ -1,0 +1,860 @@
+// deposit.hip -- shape-function deposition of rho and j onto DG element nodes.
+//
+// The reference scatters: for every particle it walks the face-connected
+// elements its shape touches and adds q S(node - x) to each node
+// (dep_shape.hpp:128-149, element_finder.hpp:203-294, dep_target.hpp:54-186).
+// Here the same sum is gathered per target element, with no float atomics:
+//
+//   1. particles are sorted by containing element with a stable LSD radix
+//      sort (8-bit digits), giving contiguous per-element segments;
+//   2. each target element owns a candidate list of source elements that can
+//      hold a particle whose shape reaches it (built on the host once per
+//      radius from bounding-sphere and face-plane separation tests);
+//   3. a block per target element streams the candidate particles, drops
+//      those whose ball cannot reach its nodes, queues the rest in LDS and
+//      evaluates them one particle per lane against the element's nodes, held
+//      in scalar registers, accumulating rho and j per node in registers.
+//
+// Summation order is fixed by the candidate order and the stable sort, so
+// results are bitwise reproducible run to run.  In a convex conforming mesh the
+// nodes within R of a particle are exactly the nodes of elements its ball
+// meets, so the gather and the reference's face walk sum the same terms.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <thread>
+
+#include "pyr_internal.h"
+
+namespace pyr {
+
+// ---------------------------------------------------------------------------
+// stable LSD radix sort of (key, index) pairs, 8-bit digits
+// ---------------------------------------------------------------------------
+constexpr int RS_BS = 256, RS_ROUNDS = 16, RS_TILE = RS_BS * RS_ROUNDS;
+
+__global__ __launch_bounds__(RS_BS) void k_rs_hist(const uint32_t *__restrict__ keys, int64_t n,
+                                                   int shift, uint32_t *__restrict__ hist,
+                                                   int ntiles, uint32_t kmax) {
+  __shared__ uint32_t h[256];
+  const int tid = threadIdx.x;
+  h[tid] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  for (int k = 0; k < RS_ROUNDS; ++k) {
+    int64_t idx = base + (int64_t)k * RS_BS + tid;
+    if (idx < n) atomicAdd(&h[(min(keys[idx], kmax) >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(size_t)tid * ntiles + blockIdx.x] = h[tid];
+}
+
+__global__ __launch_bounds__(RS_BS) void k_rs_scatter(const uint32_t *__restrict__ kin,
+                                                      const uint32_t *__restrict__ vin,
+                                                      uint32_t *__restrict__ kout,
+                                                      uint32_t *__restrict__ vout, int64_t n,
+                                                      int shift, const uint32_t *__restrict__ offs,
+                                                      int ntiles, uint32_t kmax) {
+  __shared__ uint32_t base_pos[256];
+  __shared__ uint32_t wcnt[RS_BS / 64][256];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  base_pos[tid] = offs[(size_t)tid * ntiles + blockIdx.x];
+  const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  for (int k = 0; k < RS_ROUNDS; ++k) {
+    const int64_t idx = base + (int64_t)k * RS_BS + tid;
+    const bool valid = idx < n;
+    const uint32_t key = valid ? min(kin[idx], kmax) : 0u;
+    const uint32_t d = (key >> shift) & 255u;
+#pragma unroll
+    for (int w = 0; w < RS_BS / 64; ++w) wcnt[w][tid] = 0;
+    __syncthreads();
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+    if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = base_pos[d] + rank;
+      for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
+      kout[pos] = key;
+      vout[pos] = vin ? vin[idx] : (uint32_t)idx;
+    }
+    __syncthreads();
+    uint32_t add = 0;
+#pragma unroll
+    for (int w = 0; w < RS_BS / 64; ++w) add += wcnt[w][tid];
+    base_pos[tid] += add;
+  }
+}
+
+// counts[K] collects particles with an invalid element; they sort last and
+// no segment references them.
+__global__ void k_hist_keys(const uint32_t *__restrict__ keys, int64_t n,
+                            uint32_t *__restrict__ counts, uint32_t kmax) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&counts[min(keys[i], kmax)], 1u);
+}
+
+// staged particle record, in element-sorted order:
+// x0 x1 x2 | a = q*normalizer | b = a*v (3) | pad
+__global__ void k_stage(int D, int64_t n, const uint32_t *__restrict__ perm,
+                        const double *__restrict__ x, const double *__restrict__ q,
+                        const double *__restrict__ v, double normalizer,
+                        double *__restrict__ staged) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t i = perm[j];
+  double rec[8];
+  const double a = q[i] * normalizer;
+  for (int k = 0; k < 3; ++k) {
+    rec[k] = k < D ? x[(size_t)i * D + k] : 0.0;
+    rec[4 + k] = (v && k < D) ? a * v[(size_t)i * D + k] : 0.0;
+  }
+  rec[3] = a;
+  rec[7] = 0.0;
+  double4 *dst = reinterpret_cast<double4 *>(staged + (size_t)j * 8);
+  dst[0] = make_double4(rec[0], rec[1], rec[2], rec[3]);
+  dst[1] = make_double4(rec[4], rec[5], rec[6], rec[7]);
+}
+
+// ---------------------------------------------------------------------------
+// gather kernel
+// ---------------------------------------------------------------------------
+struct GatherArgs {
+  int K, Np;
+  double R, invR, R2;
+  double alpha;
+  double shift_len[3];              // periodic extents (shift code 1: +L, 2: -L)
+  const uint32_t *cand_start;       // K + 1
+  const uint32_t *cand;             // source element ids
+  const uint8_t *cand_shift;        // 2 bits per axis
+  const uint32_t *seg;              // K + 1 segment starts into staged
+  const double *staged;             // n x 8
+  const double *geom;               // K x 8: centroid (3), bound radius^2, ...
+  const double *face_normal;        // mesh arrays
+  const double *face_rhs;
+  const void *node_tab;             // K x Np x 4 (T): n - c, |n - c|^2 / R
+  double *rho;                      // K*Np or null
+  double *j;                        // K*Np x D or null
+  unsigned long long *stats;        // [pairs evaluated, pairs inside, candidates, passed]
+};
+
+template <int D>
+__device__ __forceinline__ void shift_vec(const GatherArgs &g, uint32_t code, double *s) {
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    uint32_t c = (code >> (2 * a)) & 3u;
+    s[a] = c == 1 ? g.shift_len[a] : (c == 2 ? -g.shift_len[a] : 0.0);
+  }
+}
+
+// Kahan-compensated accumulator for the FP32 form
+struct Kahan {
+  float s, c;
+  __device__ __forceinline__ void add(float y) {
+    float yy = y - c;
+    float t = s + yy;
+    c = (t - s) - yy;
+    s = t;
+  }
+};
+
+// XCD-aware block -> element map: consecutive elements (spatial neighbours
+// sharing candidate particles) land on the same XCD and its L2.
+__device__ __forceinline__ int xcd_element(int bid, int nblocks) {
+  const int nx = 8;
+  const int per = (nblocks + nx - 1) / nx;
+  const int xcd = bid % nx, slot = bid / nx;
+  const int e = xcd * per + slot;
+  return e;
+}
+
+template <int D, int NP, int NPW, bool FP32, bool A2, bool COUNT>
+__global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherArgs g) {
+  using T = typename std::conditional<FP32, float, double>::type;
+  constexpr int G = (NP + NPW - 1) / NPW;   // waves per block = node groups
+  constexpr int BS = 64 * G;
+  constexpr int QCAP = 64 + BS;
+  constexpr int NV = D + 1;                 // rho + j components
+  constexpr int REC = 8;                    // record: m (3), c, a, b (3)
+
+  const int e = xcd_element(blockIdx.x, gridDim.x);
+  if (e >= g.K) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int node0 = wave * NPW;
+
+  __shared__ T q_rec[QCAP][REC];
+  __shared__ uint32_t w_start[64], w_pre[65];
+  __shared__ uint8_t w_shift[64];
+  __shared__ int wave_pass[G];
+
+  // target element geometry (uniform)
+  const double cx = g.geom[(size_t)e * 8 + 0], cy = g.geom[(size_t)e * 8 + 1],
+               cz = g.geom[(size_t)e * 8 + 2], bound2 = g.geom[(size_t)e * 8 + 3];
+  const double *fn = g.face_normal + (size_t)e * (D + 1) * D;
+  const double *fr = g.face_rhs + (size_t)e * (D + 1);
+  const T *ntab = static_cast<const T *>(g.node_tab) + (size_t)e * NP * 4;
+
+  T acc[NPW][NV];
+  Kahan kacc[FP32 ? NPW : 1][FP32 ? NV : 1];
+#pragma unroll
+  for (int i = 0; i < NPW; ++i)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      acc[i][v] = T(0);
+      if constexpr (FP32) kacc[i][v] = Kahan{0.f, 0.f};
+    }
+  unsigned long long n_eval = 0, n_in = 0, n_cand = 0, n_pass = 0;
+  int q_count = 0;
+  bool any = false;
+
+  auto process_batch = [&](int count) {
+    // one queued particle per lane, all of this wave's nodes
+    if (lane < count) {
+      T rec[REC];
+#pragma unroll
+      for (int k = 0; k < REC; ++k) rec[k] = q_rec[lane][k];
+#pragma unroll
+      for (int i = 0; i < NPW; ++i) {
+        if (node0 + i < NP) {
+          const T *nd = ntab + (size_t)(node0 + i) * 4;
+          T t = rec[3] - nd[3];
+#pragma unroll
+          for (int k = D - 1; k >= 0; --k) t = fma(nd[k], rec[k], t);
+          T w = t > T(0) ? t : T(0);
+          T s;
+          if constexpr (A2)
+            s = w * w;
+          else
+            s = (T)pow((double)w, g.alpha);
+          if constexpr (COUNT) n_in += t > T(0) ? 1 : 0;
+          acc[i][0] = fma(rec[4], s, acc[i][0]);
+#pragma unroll
+          for (int k = 0; k < D; ++k) acc[i][1 + k] = fma(rec[5 + k], s, acc[i][1 + k]);
+        }
+      }
+      if constexpr (COUNT) n_eval += (unsigned long long)min(NPW, NP - node0);
+    }
+    if constexpr (FP32) {
+      // fold the batch partials into the compensated sums
+#pragma unroll
+      for (int i = 0; i < NPW; ++i)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          kacc[i][v].add(acc[i][v]);
+          acc[i][v] = 0.f;
+        }
+    }
+  };
+
+  const uint32_t c_begin = g.cand_start[e], c_end = g.cand_start[e + 1];
+  for (uint32_t cw = c_begin; cw < c_end; cw += 64) {
+    // candidate window: 64 source elements, their segments and shifts
+    uint32_t len = 0, start = 0;
+    uint8_t sc = 0;
+    if (cw + lane < c_end) {
+      const uint32_t src = g.cand[cw + lane];
+      start = g.seg[src];
+      len = g.seg[src + 1] - start;
+      sc = g.cand_shift[cw + lane];
+    }
+    uint32_t incl = len;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      uint32_t t = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += t;
+    }
+    const uint32_t total = __shfl(incl, 63, 64);
+    if (total == 0) continue;   // uniform across the block
+    __syncthreads();
+    if (wave == 0) {
+      w_start[lane] = start;
+      w_pre[lane] = incl - len;
+      w_shift[lane] = sc;
+      if (lane == 63) w_pre[64] = incl;
+    }
+    __syncthreads();
+    for (uint32_t f0 = 0; f0 < total; f0 += BS) {
+      const uint32_t f = f0 + tid;
+      bool pass = false;
+      T rec[REC];
+      if (f < total) {
+        // candidate k holding flat index f: last k with w_pre[k] <= f
+        int lo = 0, hi = 63;
+        while (lo < hi) {
+          int mid = (lo + hi + 1) >> 1;
+          if (w_pre[mid] <= f) lo = mid; else hi = mid - 1;
+        }
+        const uint32_t pidx = w_start[lo] + (f - w_pre[lo]);
+        double sh[3] = {0, 0, 0};
+        if (w_shift[lo]) shift_vec<D>(g, w_shift[lo], sh);
+        const double4 *src = reinterpret_cast<const double4 *>(g.staged + (size_t)pidx * 8);
+        const double4 r0 = src[0], r1 = src[1];
+        double x[3] = {r0.x + sh[0], r0.y + sh[1], D == 3 ? r0.z + sh[2] : 0.0};
+        double d[3] = {x[0] - cx, x[1] - cy, x[2] - cz};
+        double dd = d[0] * d[0] + d[1] * d[1] + (D == 3 ? d[2] * d[2] : 0.0);
+        pass = dd <= bound2;
+#pragma unroll
+        for (int f2 = 0; f2 <= D; ++f2) {
+          double ip = 0;
+#pragma unroll
+          for (int k = 0; k < D; ++k) ip += fn[f2 * D + k] * x[k];
+          pass = pass && (ip - fr[f2] <= g.R);
+        }
+        if constexpr (COUNT) n_cand += 1;
+        if (pass) {
+          if constexpr (COUNT) n_pass += 1;
+          rec[0] = (T)(2.0 * d[0] * g.invR);
+          rec[1] = (T)(2.0 * d[1] * g.invR);
+          rec[2] = (T)(2.0 * d[2] * g.invR);
+          rec[3] = (T)(g.R - dd * g.invR);
+          rec[4] = (T)r0.w;
+          rec[5] = (T)r1.x;
+          rec[6] = (T)r1.y;
+          rec[7] = (T)r1.z;
+        }
+      }
+      const uint64_t bal = __ballot(pass);
+      const int my_pass = __popcll(bal);
+      if (lane == 0) wave_pass[wave] = my_pass;
+      __syncthreads();
+      int off = q_count;
+      for (int w = 0; w < wave; ++w) off += wave_pass[w];
+      int added = 0;
+      for (int w = 0; w < G; ++w) added += wave_pass[w];
+      if (pass) {
+        const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+        const int slot = off + __popcll(bal & lt);
+#pragma unroll
+        for (int k = 0; k < REC; ++k) q_rec[slot][k] = rec[k];
+      }
+      q_count += added;
+      __syncthreads();
+      while (q_count >= 64) {
+        any = true;
+        process_batch(64);
+        __syncthreads();
+        // move the tail of the queue to the front
+        for (int s = tid; s < q_count - 64; s += BS)
+#pragma unroll
+          for (int k = 0; k < REC; ++k) q_rec[s][k] = q_rec[s + 64][k];
+        q_count -= 64;
+        __syncthreads();
+      }
+    }
+  }
+  if (q_count > 0) {
+    any = true;
+    process_batch(q_count);
+  }
+
+  // reduce across lanes in a fixed tree and write this wave's nodes
+#pragma unroll
+  for (int i = 0; i < NPW; ++i) {
+    const int node = node0 + i;
+    if (node >= NP) continue;
+    double out[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      double val;
+      if constexpr (FP32)
+        val = any ? (double)kacc[i][v].s + (double)kacc[i][v].c : 0.0;
+      else
+        val = any ? (double)acc[i][v] : 0.0;
+      if (any)
+        for (int off = 32; off > 0; off >>= 1) val += __shfl_xor(val, off, 64);
+      out[v] = val;
+    }
+    if (lane == 0) {
+      const size_t gn = (size_t)e * NP + node;
+      if (g.rho) g.rho[gn] = out[0];
+      if (g.j)
+#pragma unroll
+        for (int k = 0; k < D; ++k) g.j[gn * D + k] = out[1 + k];
+    }
+  }
+  if constexpr (COUNT) {
+    unsigned long long vals[4] = {n_eval, n_in, n_cand, n_pass};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      unsigned long long v = vals[k];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lane == 0 && v) atomicAdd(g.stats + k, v);
+    }
+  }
+}
+
+// node table: n - c_e and |n - c_e|^2 / R, per target element
+template <class T>
+__global__ void k_node_table(int D, int K, int np, const double *__restrict__ nodes,
+                             const double *__restrict__ geom, double invR,
+                             T *__restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)K * np) return;
+  const int e = (int)(t / np);
+  double d[3] = {0, 0, 0}, s = 0;
+  for (int k = 0; k < D; ++k) {
+    d[k] = nodes[(size_t)t * D + k] - geom[(size_t)e * 8 + k];
+    s += d[k] * d[k];
+  }
+  out[(size_t)t * 4 + 0] = (T)d[0];
+  out[(size_t)t * 4 + 1] = (T)d[1];
+  out[(size_t)t * 4 + 2] = (T)d[2];
+  out[(size_t)t * 4 + 3] = (T)(s * invR);
+}
+
+}  // namespace pyr
+
+using namespace pyr;
+
+// ---------------------------------------------------------------------------
+// per-mesh deposition cache
+// ---------------------------------------------------------------------------
+struct DepositCache {
+  double radius = -1;
+  int64_t n_cand = 0;
+  DevBuf cand_start, cand, cand_shift, geom, tab64, tab32, tab_R;
+  double tab_radius = -1;
+  // sort / staging scratch
+  DevBuf keys_a, keys_b, vals_a, vals_b, hist, hist_scan, counts, seg, tmp, staged, stats;
+  std::vector<double> h_geom;   // K x 8
+};
+
+void pyr_deposit_cache_free(pyr_mesh *mesh) {
+  if (!mesh || !mesh->dep) return;
+  DepositCache *c = mesh->dep;
+  for (DevBuf *b : {&c->cand_start, &c->cand, &c->cand_shift, &c->geom, &c->tab64, &c->tab32,
+                    &c->tab_R, &c->keys_a, &c->keys_b, &c->vals_a, &c->vals_b, &c->hist,
+                    &c->hist_scan, &c->counts, &c->seg, &c->tmp, &c->staged, &c->stats})
+    b->release();
+  delete c;
+  mesh->dep = nullptr;
+}
+
+// Host mirror of the mesh arrays the candidate builder needs.
+struct HostGeom {
+  int D, K, Np;
+  std::vector<double> verts;      // K x (D+1) x D (element vertex coordinates)
+  std::vector<double> fn, fr;     // face normals, rhs
+  std::vector<uint32_t> nb, pax;  // face neighbours, periodic axes
+  std::vector<double> nodes;      // K*Np x D
+  double per_min[3], per_max[3];
+};
+
+static HostGeom download_geom(pyr_mesh *h) {
+  HostGeom g;
+  const int D = h->dim, K = h->K, F = D + 1;
+  g.D = D;
+  g.K = K;
+  g.Np = h->Np;
+  std::vector<uint32_t> elv((size_t)K * F);
+  std::vector<double> vtx((size_t)h->nv * D);
+  g.fn.resize((size_t)K * F * D);
+  g.fr.resize((size_t)K * F);
+  g.nb.resize((size_t)K * F);
+  g.pax.resize((size_t)K * F);
+  g.nodes.resize((size_t)K * h->Np * D);
+  const DevMesh &m = h->dev;
+  PYR_HIP(hipMemcpy(elv.data(), m.el_vertices, elv.size() * 4, hipMemcpyDeviceToHost));
+  PYR_HIP(hipMemcpy(vtx.data(), m.vertices, vtx.size() * 8, hipMemcpyDeviceToHost));
+  PYR_HIP(hipMemcpy(g.fn.data(), m.face_normal, g.fn.size() * 8, hipMemcpyDeviceToHost));
+  PYR_HIP(hipMemcpy(g.fr.data(), m.face_rhs, g.fr.size() * 8, hipMemcpyDeviceToHost));
+  PYR_HIP(hipMemcpy(g.nb.data(), m.face_neighbor, g.nb.size() * 4, hipMemcpyDeviceToHost));
+  PYR_HIP(hipMemcpy(g.pax.data(), m.face_per_axis, g.pax.size() * 4, hipMemcpyDeviceToHost));
+  PYR_HIP(hipMemcpy(g.nodes.data(), m.nodes, g.nodes.size() * 8, hipMemcpyDeviceToHost));
+  g.verts.resize((size_t)K * F * D);
+  for (int e = 0; e < K; ++e)
+    for (int k = 0; k < F; ++k)
+      for (int i = 0; i < D; ++i)
+        g.verts[((size_t)e * F + k) * D + i] = vtx[(size_t)elv[(size_t)e * F + k] * D + i];
+  for (int i = 0; i < 3; ++i) {
+    g.per_min[i] = m.per_min[i];
+    g.per_max[i] = m.per_max[i];
+  }
+  return g;
+}
+
+// Builds, for every target element, the list of (source element, periodic
+// shift) pairs whose particles may lie within R of one of its nodes.  A BFS
+// over face neighbours expands only accepted elements; the region within R of
+// a convex element is convex, so every element meeting it is reached.
+static void build_candidates(pyr_mesh *h, DepositCache *c, double R) {
+  HostGeom g = download_geom(h);
+  const int D = g.D, K = g.K, F = D + 1, np = g.Np;
+  // per element: vertex centroid, vertex radius, node bound
+  std::vector<double> cen((size_t)K * 3, 0.0), vrad(K, 0.0);
+  c->h_geom.assign((size_t)K * 8, 0.0);
+  for (int e = 0; e < K; ++e) {
+    for (int k = 0; k < F; ++k)
+      for (int i = 0; i < D; ++i) cen[(size_t)e * 3 + i] += g.verts[((size_t)e * F + k) * D + i] / F;
+    double r = 0;
+    for (int k = 0; k < F; ++k) {
+      double s = 0;
+      for (int i = 0; i < D; ++i) {
+        double dv = g.verts[((size_t)e * F + k) * D + i] - cen[(size_t)e * 3 + i];
+        s += dv * dv;
+      }
+      r = std::max(r, std::sqrt(s));
+    }
+    vrad[e] = r;
+    double rn = 0;
+    for (int i = 0; i < np; ++i) {
+      double s = 0;
+      for (int k = 0; k < D; ++k) {
+        double dv = g.nodes[((size_t)e * np + i) * D + k] - cen[(size_t)e * 3 + k];
+        s += dv * dv;
+      }
+      rn = std::max(rn, std::sqrt(s));
+    }
+    for (int i = 0; i < 3; ++i) c->h_geom[(size_t)e * 8 + i] = cen[(size_t)e * 3 + i];
+    const double b = (R + rn) * (1 + 1e-12) + 1e-300;
+    c->h_geom[(size_t)e * 8 + 3] = b * b;
+  }
+  const double slack = 1e-9 * R;
+  auto accept = [&](int e, int s, const double *sh) {
+    // bounding spheres
+    double dd = 0;
+    for (int i = 0; i < D; ++i) {
+      double dv = cen[(size_t)s * 3 + i] + sh[i] - cen[(size_t)e * 3 + i];
+      dd += dv * dv;
+    }
+    if (std::sqrt(dd) > R + vrad[e] + vrad[s] + slack) return false;
+    // face planes of e against the vertices of s, and vice versa
+    for (int f = 0; f < F; ++f) {
+      const double *n = &g.fn[((size_t)e * F + f) * D];
+      double mn = INFINITY;
+      for (int k = 0; k < F; ++k) {
+        double ip = 0;
+        for (int i = 0; i < D; ++i) ip += n[i] * (g.verts[((size_t)s * F + k) * D + i] + sh[i]);
+        mn = std::min(mn, ip);
+      }
+      if (mn - g.fr[(size_t)e * F + f] > R + slack) return false;
+    }
+    for (int f = 0; f < F; ++f) {
+      const double *n = &g.fn[((size_t)s * F + f) * D];
+      double rhs = g.fr[(size_t)s * F + f];
+      for (int i = 0; i < D; ++i) rhs += n[i] * sh[i];
+      double mn = INFINITY;
+      for (int k = 0; k < F; ++k) {
+        double ip = 0;
+        for (int i = 0; i < D; ++i) ip += n[i] * g.verts[((size_t)e * F + k) * D + i];
+        mn = std::min(mn, ip);
+      }
+      if (mn - rhs > R + slack) return false;
+    }
+    return true;
+  };
+  const int nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::vector<uint32_t>> lists(K);
+  std::vector<std::vector<uint8_t>> shifts(K);
+  std::atomic<int> next{0};
+  std::atomic<int> failed{0};
+  auto worker = [&]() {
+    std::vector<int> stamp(K, -1);
+    std::vector<uint8_t> stamp_code(K, 0);
+    std::vector<std::pair<uint32_t, uint8_t>> queue;
+    for (;;) {
+      int e = next.fetch_add(1);
+      if (e >= K) break;
+      queue.clear();
+      queue.emplace_back((uint32_t)e, 0);
+      stamp[e] = e;
+      stamp_code[e] = 0;
+      std::vector<std::pair<uint32_t, uint8_t>> acc;
+      for (size_t qi = 0; qi < queue.size(); ++qi) {
+        const uint32_t s = queue[qi].first;
+        const uint8_t code = queue[qi].second;
+        double sh[3] = {0, 0, 0};
+        for (int a = 0; a < D; ++a) {
+          int cc = (code >> (2 * a)) & 3;
+          sh[a] = cc == 1 ? (g.per_max[a] - g.per_min[a])
+                          : (cc == 2 ? -(g.per_max[a] - g.per_min[a]) : 0.0);
+        }
+        if (!accept(e, (int)s, sh)) continue;
+        acc.emplace_back(s, code);
+        for (int f = 0; f < F; ++f) {
+          const uint32_t nb = g.nb[(size_t)s * F + f];
+          if (nb == INVALID) continue;
+          uint8_t ncode = code;
+          const uint32_t ax = g.pax[(size_t)s * F + f];
+          if (ax != INVALID) {
+            // crossing a periodic face: the neighbour sits one period away
+            const double L = g.per_max[ax] - g.per_min[ax];
+            double fcx = 0;
+            for (int k = 0; k < F; ++k) fcx += g.verts[((size_t)s * F + k) * D + ax] / F;
+            const bool at_max = fcx > 0.5 * (g.per_min[ax] + g.per_max[ax]);
+            int cur = (code >> (2 * ax)) & 3;
+            int val = cur == 1 ? 1 : (cur == 2 ? -1 : 0);
+            val += at_max ? 1 : -1;
+            if (val < -1 || val > 1) {
+              failed = 1;
+              continue;
+            }
+            (void)L;
+            ncode = (uint8_t)((code & ~(3u << (2 * ax))) |
+                              ((val == 1 ? 1u : (val == -1 ? 2u : 0u)) << (2 * ax)));
+          }
+          if (stamp[nb] == e) {
+            if (stamp_code[nb] != ncode) failed = 1;
+            continue;
+          }
+          stamp[nb] = e;
+          stamp_code[nb] = ncode;
+          queue.emplace_back(nb, ncode);
+        }
+      }
+      std::sort(acc.begin(), acc.end());
+      lists[e].reserve(acc.size());
+      shifts[e].reserve(acc.size());
+      for (auto &p : acc) {
+        lists[e].push_back(p.first);
+        shifts[e].push_back(p.second);
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nthreads; ++t) pool.emplace_back(worker);
+  for (auto &t : pool) t.join();
+  if (failed)
+    throw Error(PYR_ERR_INVALID_ARG,
+                "shape radius reaches the same element through two periodic images; the "
+                "period must exceed the shape support");
+  std::vector<uint32_t> starts((size_t)K + 1, 0);
+  for (int e = 0; e < K; ++e) starts[e + 1] = starts[e] + (uint32_t)lists[e].size();
+  std::vector<uint32_t> flat(starts[K]);
+  std::vector<uint8_t> flat_s(starts[K]);
+  for (int e = 0; e < K; ++e) {
+    std::copy(lists[e].begin(), lists[e].end(), flat.begin() + starts[e]);
+    std::copy(shifts[e].begin(), shifts[e].end(), flat_s.begin() + starts[e]);
+  }
+  upload(c->cand_start, starts.data(), starts.size());
+  upload(c->cand, flat.data(), std::max<size_t>(1, flat.size()));
+  upload(c->cand_shift, flat_s.data(), std::max<size_t>(1, flat_s.size()));
+  upload(c->geom, c->h_geom.data(), c->h_geom.size());
+  c->n_cand = starts[K];
+  c->radius = R;
+}
+
+static unsigned blocks_for(int64_t n, int bs = 256) {
+  return (unsigned)std::max<int64_t>(1, (n + bs - 1) / bs);
+}
+
+// sorts elem keys, returns perm in vals_a-or-b and segment starts in c->seg
+static const uint32_t *sort_by_element(pyr_mesh *h, DepositCache *c, int64_t n,
+                                       const uint32_t *elem) {
+  const int K = h->K;
+  hipStream_t s = stream();
+  c->counts.reserve(sizeof(uint32_t) * ((size_t)K + 2));
+  c->seg.reserve(sizeof(uint32_t) * ((size_t)K + 2));
+  const int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
+  const int64_t hist_n = std::max<int64_t>(256 * ntiles, K + 1);
+  c->tmp.reserve(sizeof(uint32_t) * scan_tmp_words(hist_n));
+  PYR_HIP(hipMemsetAsync(c->counts.ptr, 0, sizeof(uint32_t) * ((size_t)K + 2), s));
+  hipLaunchKernelGGL(k_hist_keys, dim3(blocks_for(n)), dim3(256), 0, s, elem, n,
+                     c->counts.as<uint32_t>(), (uint32_t)K);
+  PYR_CHECK_LAUNCH();
+  scan_exclusive_u32(c->counts.as<uint32_t>(), c->seg.as<uint32_t>(), (int64_t)K + 1, nullptr,
+                     c->tmp.as<uint32_t>(), s);
+  int bits = 1;   // keys are clamped to [0, K]
+  while (bits < 32 && ((uint64_t)1 << bits) <= (uint64_t)K) ++bits;
+  const int passes = (bits + 7) / 8;
+  c->keys_a.reserve(sizeof(uint32_t) * n);
+  c->keys_b.reserve(sizeof(uint32_t) * n);
+  c->vals_a.reserve(sizeof(uint32_t) * n);
+  c->vals_b.reserve(sizeof(uint32_t) * n);
+  c->hist.reserve(sizeof(uint32_t) * 256 * ntiles);
+  c->hist_scan.reserve(sizeof(uint32_t) * 256 * ntiles);
+  const uint32_t *kin = elem, *vin = nullptr;
+  uint32_t *kout = c->keys_a.as<uint32_t>(), *vout = c->vals_a.as<uint32_t>();
+  for (int p = 0; p < passes; ++p) {
+    hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)ntiles), dim3(RS_BS), 0, s, kin, n, 8 * p,
+                       c->hist.as<uint32_t>(), (int)ntiles, (uint32_t)K);
+    PYR_CHECK_LAUNCH();
+    scan_exclusive_u32(c->hist.as<uint32_t>(), c->hist_scan.as<uint32_t>(), 256 * ntiles,
+                       nullptr, c->tmp.as<uint32_t>(), s);
+    hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)ntiles), dim3(RS_BS), 0, s, kin, vin, kout,
+                       vout, n, 8 * p, c->hist_scan.as<uint32_t>(), (int)ntiles, (uint32_t)K);
+    PYR_CHECK_LAUNCH();
+    kin = kout;
+    vin = vout;
+    if (kout == c->keys_a.as<uint32_t>()) {
+      kout = c->keys_b.as<uint32_t>();
+      vout = c->vals_b.as<uint32_t>();
+    } else {
+      kout = c->keys_a.as<uint32_t>();
+      vout = c->vals_a.as<uint32_t>();
+    }
+  }
+  return vin;
+}
+
+template <int D, int NP, int NPW>
+static void launch_gather(const GatherArgs &ga, int K, int precision, bool a2, bool count) {
+  constexpr int G = (NP + NPW - 1) / NPW;
+  // the XCD-aware element map needs a multiple of 8 blocks
+  dim3 grid((unsigned)(((int64_t)K + 7) / 8 * 8)), block(64 * G);
+  hipStream_t s = stream();
+#define PYR_G(FP, AA, CC) \
+  hipLaunchKernelGGL((k_gather<D, NP, NPW, FP, AA, CC>), grid, block, 0, s, ga)
+  if (precision == PYR_FP64) {
+    if (a2) {
+      if (count) PYR_G(false, true, true); else PYR_G(false, true, false);
+    } else {
+      if (count) PYR_G(false, false, true); else PYR_G(false, false, false);
+    }
+  } else {
+    if (a2) {
+      if (count) PYR_G(true, true, true); else PYR_G(true, true, false);
+    } else {
+      if (count) PYR_G(true, false, true); else PYR_G(true, false, false);
+    }
+  }
+#undef PYR_G
+  PYR_CHECK_LAUNCH();
+}
+
+static void dispatch_gather(int D, int Np, const GatherArgs &ga, int K, int precision, bool a2,
+                            bool count) {
+  if (D == 2) {
+    switch (Np) {
+      case 3: return launch_gather<2, 3, 3>(ga, K, precision, a2, count);
+      case 6: return launch_gather<2, 6, 6>(ga, K, precision, a2, count);
+      case 10: return launch_gather<2, 10, 10>(ga, K, precision, a2, count);
+      case 15: return launch_gather<2, 15, 15>(ga, K, precision, a2, count);
+    }
+  } else {
+    switch (Np) {
+      case 4: return launch_gather<3, 4, 4>(ga, K, precision, a2, count);
+      case 10: return launch_gather<3, 10, 10>(ga, K, precision, a2, count);
+      case 20: return launch_gather<3, 20, 10>(ga, K, precision, a2, count);
+      case 35: return launch_gather<3, 35, 12>(ga, K, precision, a2, count);
+    }
+  }
+  throw Error(PYR_ERR_INVALID_ARG, "deposition supports P1..P4 (2D) and P1..P4 (3D) elements");
+}
+
+static int g_deposit_count_mode = 0;
+
+extern "C" {
+
+int pyr_deposit(pyr_mesh *mesh, double radius, double alpha, double normalizer, int precision,
+                int64_t n, int vdim, const double *x, const double *q, const double *v,
+                const uint32_t *elem, double *rho, double *j) {
+  return guarded([&] {
+    PYR_REQUIRE(mesh, "null mesh");
+    PYR_REQUIRE(radius > 0 && std::isfinite(radius), "shape radius must be positive");
+    PYR_REQUIRE(alpha > 0, "shape exponent must be positive");
+    PYR_REQUIRE(precision == PYR_FP64 || precision == PYR_FP32C, "unknown precision");
+    PYR_REQUIRE(vdim == mesh->dim, "vdim must equal the mesh dimension");
+    PYR_REQUIRE(n >= 0 && n < (int64_t)0x7FFFFFFF, "invalid particle count");
+    PYR_REQUIRE(!j || v, "j requires velocities");
+    if (!rho && !j) return;
+    if (!mesh->dep) mesh->dep = new DepositCache();
+    DepositCache *c = mesh->dep;
+    const int K = mesh->K, Np = mesh->Np, D = mesh->dim;
+    hipStream_t s = stream();
+    if (c->radius != radius) build_candidates(mesh, c, radius);
+    if (c->tab_radius != radius) {
+      c->tab64.reserve(sizeof(double) * (size_t)K * Np * 4);
+      c->tab32.reserve(sizeof(float) * (size_t)K * Np * 4);
+      hipLaunchKernelGGL(k_node_table<double>, dim3(blocks_for((int64_t)K * Np)), dim3(256), 0,
+                         s, D, K, Np, mesh->dev.nodes, c->geom.as<double>(), 1.0 / radius,
+                         c->tab64.as<double>());
+      hipLaunchKernelGGL(k_node_table<float>, dim3(blocks_for((int64_t)K * Np)), dim3(256), 0,
+                         s, D, K, Np, mesh->dev.nodes, c->geom.as<double>(), 1.0 / radius,
+                         c->tab32.as<float>());
+      PYR_CHECK_LAUNCH();
+      c->tab_radius = radius;
+    }
+    c->staged.reserve(sizeof(double) * 8 * (size_t)std::max<int64_t>(1, n));
+    if (n > 0) {
+      PYR_REQUIRE(x && q && elem, "null pointer");
+      const uint32_t *perm = sort_by_element(mesh, c, n, elem);
+      hipLaunchKernelGGL(k_stage, dim3(blocks_for(n)), dim3(256), 0, s, D, n, perm, x, q,
+                         j ? v : nullptr, normalizer, c->staged.as<double>());
+      PYR_CHECK_LAUNCH();
+    } else {
+      c->seg.reserve(sizeof(uint32_t) * ((size_t)K + 1));
+      PYR_HIP(hipMemsetAsync(c->seg.ptr, 0, sizeof(uint32_t) * ((size_t)K + 1), s));
+    }
+    GatherArgs ga{};
+    ga.K = K;
+    ga.Np = Np;
+    ga.R = radius;
+    ga.invR = 1.0 / radius;
+    ga.R2 = radius * radius;
+    ga.alpha = alpha;
+    for (int a = 0; a < 3; ++a)
+      ga.shift_len[a] = mesh->dev.per_max[a] - mesh->dev.per_min[a];
+    ga.cand_start = c->cand_start.as<uint32_t>();
+    ga.cand = c->cand.as<uint32_t>();
+    ga.cand_shift = c->cand_shift.as<uint8_t>();
+    ga.seg = c->seg.as<uint32_t>();
+    ga.staged = c->staged.as<double>();
+    ga.geom = c->geom.as<double>();
+    ga.face_normal = mesh->dev.face_normal;
+    ga.face_rhs = mesh->dev.face_rhs;
+    ga.node_tab = precision == PYR_FP64 ? (const void *)c->tab64.ptr : (const void *)c->tab32.ptr;
+    ga.rho = rho;
+    ga.j = j;
+    c->stats.reserve(4 * sizeof(unsigned long long));
+    ga.stats = c->stats.as<unsigned long long>();
+    const bool count = g_deposit_count_mode != 0;
+    if (count) PYR_HIP(hipMemsetAsync(ga.stats, 0, 4 * sizeof(unsigned long long), s));
+    dispatch_gather(D, Np, ga, K, precision, alpha == 2.0, count);
+  });
+}
+
+int pyr_deposit_set_counting(int on) {
+  g_deposit_count_mode = on;
+  return PYR_OK;
+}
+
+int pyr_deposit_stats(pyr_mesh *mesh, unsigned long long out[4]) {
+  return guarded([&] {
+    PYR_REQUIRE(mesh && out, "null argument");
+    for (int k = 0; k < 4; ++k) out[k] = 0;
+    if (!mesh->dep || !mesh->dep->stats.ptr) return;
+    PYR_HIP(hipStreamSynchronize(stream()));
+    PYR_HIP(hipMemcpy(out, mesh->dep->stats.ptr, 4 * sizeof(unsigned long long),
+                      hipMemcpyDeviceToHost));
+  });
+}
+
+int pyr_deposit_candidates(pyr_mesh *mesh, double radius, int64_t *n_pairs_out) {
+  return guarded([&] {
+    PYR_REQUIRE(mesh && n_pairs_out && radius > 0, "invalid argument");
+    if (!mesh->dep) mesh->dep = new DepositCache();
+    if (mesh->dep->radius != radius) build_candidates(mesh, mesh->dep, radius);
+    *n_pairs_out = mesh->dep->n_cand;
+  });
+}
+
+int pyr_sort_by_element(pyr_mesh *mesh, int64_t n, const uint32_t *elem, uint32_t *perm_out,
+                        uint32_t *seg_out) {
+  return guarded([&] {
+    PYR_REQUIRE(mesh && n >= 0 && n < (int64_t)0x7FFFFFFF, "invalid argument");
+    if (!mesh->dep) mesh->dep = new DepositCache();
+    DepositCache *c = mesh->dep;
+    if (n == 0) {
+      if (seg_out)
+        PYR_HIP(hipMemsetAsync(seg_out, 0, sizeof(uint32_t) * ((size_t)mesh->K + 1), stream()));
+      return;
+    }
+    PYR_REQUIRE(elem && perm_out, "null pointer");
+    const uint32_t *perm = sort_by_element(mesh, c, n, elem);
+    PYR_HIP(hipMemcpyAsync(perm_out, perm, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice,
+                           stream()));
+    if (seg_out)
+      PYR_HIP(hipMemcpyAsync(seg_out, c->seg.ptr, sizeof(uint32_t) * ((size_t)mesh->K + 1),
+                             hipMemcpyDeviceToDevice, stream()));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,422 @@
+// push.hip -- DG interpolation of E and B, the Lorentz force and the fused
+// five-stage LSRK4 particle push with relocalization after every stage.
+//
+// Interpolation uses per-element monomial coefficients c = V^{-1} f, so that
+// f(x) = sum_k c_k phi_k(r(x)) with phi_k = prod_i r_i^{e_ik}.  That is the
+// interpolant the reference evaluates as sum_j l_j(x) f_j with Lagrange weights
+// l = V^{-T} phi from an LU solve per particle (push_monomial.hpp:155-210,
+// :99-111); here the solve is folded into the coefficients once per field set
+// and each particle-stage costs Np products plus 6*Np FMAs.
+#include <cmath>
+
+#include "pyr_internal.h"
+
+namespace pyr {
+
+// ---------------------------------------------------------------------------
+// monomial basis in node_tuples() order (graded, see discretization.py)
+// ---------------------------------------------------------------------------
+template <int D, int N>
+struct Basis {
+  static constexpr int NP = D == 2 ? (N + 1) * (N + 2) / 2 : (N + 1) * (N + 2) * (N + 3) / 6;
+  int e[NP][3];
+  constexpr Basis() : e{} {
+    int k = 0;
+    for (int total = 0; total <= N; ++total) {
+      if (D == 2) {
+        for (int j = 0; j <= total; ++j) {
+          e[k][0] = total - j;
+          e[k][1] = j;
+          e[k][2] = 0;
+          ++k;
+        }
+      } else {
+        for (int j = 0; j <= total; ++j)
+          for (int l = 0; l <= total - j; ++l) {
+            e[k][0] = total - j - l;
+            e[k][1] = j;
+            e[k][2] = l;
+            ++k;
+          }
+      }
+    }
+  }
+};
+
+template <int D, int N, class T>
+__device__ __forceinline__ void monomials(const T *r, T *phi) {
+  constexpr Basis<D, N> B{};
+  T pw[D][N + 1];
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    pw[i][0] = T(1);
+#pragma unroll
+    for (int e = 1; e <= N; ++e) pw[i][e] = pw[i][e - 1] * r[i];
+  }
+#pragma unroll
+  for (int k = 0; k < B.NP; ++k) {
+    T v = pw[0][B.e[k][0]];
+#pragma unroll
+    for (int i = 1; i < D; ++i) v = v * pw[i][B.e[k][i]];
+    phi[k] = v;
+  }
+}
+
+// six interpolated field values at unit coordinates r in element en
+template <int D, int N, class T>
+__device__ __forceinline__ void interp6(const T *__restrict__ coef, uint32_t en, const T *r,
+                                        T *f6) {
+  constexpr int NP = Basis<D, N>::NP;
+  T phi[NP];
+  monomials<D, N, T>(r, phi);
+  const T *c = coef + (size_t)en * 6 * NP;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    T acc = T(0);
+#pragma unroll
+    for (int k = 0; k < NP; ++k) acc = fma(c[f * NP + k], phi[k], acc);
+    f6[f] = acc;
+  }
+}
+
+// push_monomial.hpp:265-305 with tools.hpp:253-264 cross: F = q e + v x (q b),
+// the velocity padded with zeros to three components (entry_or_zero).
+template <int VD, class T>
+__device__ __forceinline__ void lorentz(T q, const T *f6, const T *v, T *force, T *el,
+                                        T *mag) {
+  T qb[3], vv[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    el[i] = q * f6[i];
+    qb[i] = q * f6[3 + i];
+    vv[i] = i < VD ? v[i] : T(0);
+  }
+  mag[0] = vv[1] * qb[2] - vv[2] * qb[1];
+  mag[1] = vv[2] * qb[0] - vv[0] * qb[2];
+  mag[2] = vv[0] * qb[1] - vv[1] * qb[0];
+#pragma unroll
+  for (int i = 0; i < VD; ++i) force[i] = el[i] + mag[i];
+}
+
+// ---------------------------------------------------------------------------
+// nodal -> modal coefficients: out[e][f][k] = sum_j Minv[k][j] field_f[e*Np+j]
+// ---------------------------------------------------------------------------
+struct Fields6 {
+  const double *f[6];
+};
+
+template <class T>
+__global__ void k_modal(int K, int np, Fields6 fl, const double *__restrict__ minv,
+                        T *__restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)K * np) return;
+  int e = (int)(t / np), k = (int)(t % np);
+  for (int f = 0; f < 6; ++f) {
+    double acc = 0;
+    if (fl.f[f]) {
+      const double *src = fl.f[f] + (size_t)e * np;
+      for (int j = 0; j < np; ++j) acc = fma(minv[k * np + j], src[j], acc);
+    }
+    out[((size_t)e * 6 + f) * np + k] = (T)acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// drop-in forces (push_monomial.hpp:219-325)
+// ---------------------------------------------------------------------------
+template <int D, int N>
+__global__ __launch_bounds__(256) void k_forces(DevMesh m, const double *__restrict__ coef, int64_t n,
+                         const double *__restrict__ x, const uint32_t *__restrict__ elem,
+                         const double *__restrict__ q, const double *__restrict__ v,
+                         double *__restrict__ force, double *vis_e, double *vis_b,
+                         double *vis_el, double *vis_mag) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double xi[D], r[D], f6[6], vi[D], F[D], el[3], mag[3];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    xi[k] = x[i * D + k];
+    vi[k] = v[i * D + k];
+  }
+  const uint32_t en = elem[i];
+  if (en >= (uint32_t)m.K) {   // not located: the reference has no defined result
+#pragma unroll
+    for (int k = 0; k < D; ++k) force[i * D + k] = __builtin_nan("");
+    return;
+  }
+  inverse_map<D>(m, en, xi, r);
+  interp6<D, N, double>(coef, en, r, f6);
+  lorentz<D, double>(q[i], f6, vi, F, el, mag);
+#pragma unroll
+  for (int k = 0; k < D; ++k) force[i * D + k] = F[k];
+  if (vis_e) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      vis_e[i * 3 + k] = f6[k];
+      vis_b[i * 3 + k] = f6[3 + k];
+      vis_el[i * 3 + k] = el[k];
+      vis_mag[i * 3 + k] = mag[k];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// fused LSRK4 step (hedge LSRK4TimeStepper, Carpenter & Kennedy 1994)
+// ---------------------------------------------------------------------------
+__constant__ double c_rk4a[5] = {
+    0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+    -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+__constant__ double c_rk4b[5] = {
+    1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+    1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+    2277821191437.0 / 14882151754819.0};
+
+template <int D, int N, class T>
+__global__ __launch_bounds__(256) void k_lsrk4_step(
+    DevMesh m, const T *__restrict__ coef, int64_t n, double dt, double c,
+    double *__restrict__ x, double *__restrict__ p, const double *__restrict__ q,
+    const double *__restrict__ mass, uint32_t *__restrict__ elem,
+    uint32_t *__restrict__ death, unsigned long long *counters, int *err,
+    unsigned int *n_dead) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  FindCounters cnt;
+  if (i < n) {
+    double xi[D], pi[D], r[D], rx[D], rp[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      xi[k] = x[i * D + k];
+      pi[k] = p[i * D + k];
+      rx[k] = 0;
+      rp[k] = 0;
+    }
+    const double qi = q[i], mi = mass[i];
+    uint32_t e = elem[i];
+    uint32_t died = 0;
+    int e_flag = 0;
+    if (e >= (uint32_t)m.K) died = 1;   // not located: removed at the first stage
+    else inverse_map<D>(m, e, xi, r);
+#pragma unroll 1
+    for (int s = 0; s < 5 && !died; ++s) {
+      double v[D], F[D];
+      velocity<D>(pi, mi, c, v);
+      if constexpr (sizeof(T) == 8) {
+        double f6[6], el[3], mag[3];
+        interp6<D, N, double>(coef, e, r, f6);
+        lorentz<D, double>(qi, f6, v, F, el, mag);
+      } else {
+        float rf[D], f6[6], vf[D], Ff[D], el[3], mag[3];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          rf[k] = (float)r[k];
+          vf[k] = (float)v[k];
+        }
+        interp6<D, N, float>(coef, e, rf, f6);
+        lorentz<D, float>((float)qi, f6, vf, Ff, el, mag);
+#pragma unroll
+        for (int k = 0; k < D; ++k) F[k] = (double)Ff[k];
+      }
+      const double a = c_rk4a[s], b = c_rk4b[s];
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        rx[k] = a * rx[k] + dt * v[k];
+        rp[k] = a * rp[k] + dt * F[k];
+      }
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        xi[k] += b * rx[k];
+        pi[k] += b * rp[k];
+      }
+      if (!relocate<D>(m, xi, pi, e, cnt, e_flag, r)) {
+        died = (uint32_t)s + 1;
+        break;
+      }
+    }
+    if (e_flag) atomicOr(err, 1);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      x[i * D + k] = xi[k];
+      p[i * D + k] = pi[k];
+    }
+    elem[i] = e;
+    death[i] = died;
+    if (died) atomicAdd(n_dead, 1u);
+  }
+  flush_counters(cnt, counters);
+}
+
+__global__ void k_stage_flags(int64_t n, const uint32_t *__restrict__ death, uint32_t s,
+                              uint8_t *__restrict__ flags) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flags[i] = death[i] == s ? 1 : 0;
+}
+
+static inline unsigned blocks_for(int64_t n, int bs = 256) {
+  return (unsigned)((n + bs - 1) / bs > 0 ? (n + bs - 1) / bs : 1);
+}
+
+static int order_of(const pyr_mesh *h) {
+  // the exponents must be the graded node_tuples() order for some N
+  const int D = h->dim;
+  for (int N = 1; N <= 6; ++N) {
+    int np = D == 2 ? (N + 1) * (N + 2) / 2 : (N + 1) * (N + 2) * (N + 3) / 6;
+    if (np != h->Np) continue;
+    std::vector<int32_t> ex;
+    for (int total = 0; total <= N; ++total) {
+      if (D == 2)
+        for (int j = 0; j <= total; ++j) {
+          ex.push_back(total - j);
+          ex.push_back(j);
+        }
+      else
+        for (int j = 0; j <= total; ++j)
+          for (int l = 0; l <= total - j; ++l) {
+            ex.push_back(total - j - l);
+            ex.push_back(j);
+            ex.push_back(l);
+          }
+    }
+    if (ex == h->h_exps) return N;
+  }
+  throw Error(PYR_ERR_INVALID_ARG,
+              "unsupported monomial basis (need graded node tuples, order 1..6 in 2D, "
+              "1..4 in 3D)");
+}
+
+template <class F>
+static void dispatch_order(int D, int N, F &&f) {
+#define PYR_CASE(DD, NN) \
+  if (D == DD && N == NN) return f(std::integral_constant<int, DD>{}, std::integral_constant<int, NN>{});
+  PYR_CASE(2, 1) PYR_CASE(2, 2) PYR_CASE(2, 3) PYR_CASE(2, 4) PYR_CASE(2, 5) PYR_CASE(2, 6)
+  PYR_CASE(3, 1) PYR_CASE(3, 2) PYR_CASE(3, 3) PYR_CASE(3, 4)
+#undef PYR_CASE
+  throw Error(PYR_ERR_INVALID_ARG, "unsupported dimension/order");
+}
+
+static void modal_transform(pyr_mesh *h, const double *const fields[6], double *out64,
+                            float *out32) {
+  Fields6 fl;
+  for (int f = 0; f < 6; ++f) fl.f[f] = fields ? fields[f] : nullptr;
+  const int64_t work = (int64_t)h->K * h->Np;
+  if (out64) {
+    hipLaunchKernelGGL(k_modal<double>, dim3(blocks_for(work)), dim3(256), 0, stream(), h->K,
+                       h->Np, fl, h->minv.as<double>(), out64);
+    PYR_CHECK_LAUNCH();
+  }
+  if (out32) {
+    hipLaunchKernelGGL(k_modal<float>, dim3(blocks_for(work)), dim3(256), 0, stream(), h->K,
+                       h->Np, fl, h->minv.as<double>(), out32);
+    PYR_CHECK_LAUNCH();
+  }
+}
+
+}  // namespace pyr
+
+using namespace pyr;
+
+extern "C" {
+
+int pyr_forces(pyr_mesh *mesh, int64_t n, int vdim, const double *const fields[6],
+               const double *x, const uint32_t *elem, const double *q, const double *v,
+               double *force, double *vis_e, double *vis_b, double *vis_el, double *vis_mag) {
+  return guarded([&] {
+    PYR_REQUIRE(mesh && n >= 0, "invalid argument");
+    PYR_REQUIRE(vdim == mesh->dim, "vdim must equal the mesh dimension");
+    if (n == 0) return;
+    PYR_REQUIRE(x && elem && q && v && force, "null pointer");
+    const bool vis = vis_e != nullptr;
+    PYR_REQUIRE(!vis || (vis_b && vis_el && vis_mag), "all four vis arrays or none");
+    const int N = order_of(mesh);
+    mesh->s_modal.reserve(sizeof(double) * (size_t)mesh->K * 6 * mesh->Np);
+    modal_transform(mesh, fields, mesh->s_modal.as<double>(), nullptr);
+    dispatch_order(mesh->dim, N, [&](auto DD, auto NN) {
+      constexpr int D_ = decltype(DD)::value, N_ = decltype(NN)::value;
+      hipLaunchKernelGGL((k_forces<D_, N_>), dim3(blocks_for(n)), dim3(256), 0, stream(),
+                         mesh->dev, mesh->s_modal.as<double>(), n, x, elem, q, v, force,
+                         vis_e, vis_b, vis_el, vis_mag);
+      PYR_CHECK_LAUNCH();
+    });
+  });
+}
+
+int pyr_set_static_fields(pyr_mesh *mesh, const double *const fields[6]) {
+  return guarded([&] {
+    PYR_REQUIRE(mesh, "null mesh");
+    order_of(mesh);
+    const size_t cnt = (size_t)mesh->K * 6 * mesh->Np;
+    mesh->sf64.reserve(cnt * sizeof(double));
+    mesh->sf32.reserve(cnt * sizeof(float));
+    modal_transform(mesh, fields, mesh->sf64.as<double>(), mesh->sf32.as<float>());
+    mesh->sf_set = true;
+  });
+}
+
+int pyr_lsrk4_steps(pyr_mesh *mesh, int precision, int nsteps, double dt, double vacuum_c,
+                    int64_t *n_inout, double *x, double *p, double *q, double *mass,
+                    uint32_t *elem, int64_t *ids, unsigned long long *counters) {
+  return guarded([&] {
+    PYR_REQUIRE(mesh && n_inout && nsteps >= 0, "invalid argument");
+    PYR_REQUIRE(precision == PYR_FP64 || precision == PYR_FP32C, "unknown precision");
+    PYR_REQUIRE(mesh->sf_set, "pyr_set_static_fields has not been called for this mesh");
+    int64_t n = *n_inout;
+    PYR_REQUIRE(n >= 0 && n < (int64_t)0x7FFFFFFF, "invalid particle count");
+    if (n == 0 || nsteps == 0) return;
+    PYR_REQUIRE(x && p && q && mass && elem, "null pointer");
+    const int D = mesh->dim, N = order_of(mesh);
+    // scratch: death stage (u32), flags (u8), moves (2 x u32), err, n_dead
+    mesh->s_flags.reserve((size_t)n * (sizeof(uint32_t) + 1));
+    mesh->s_holes.reserve((size_t)n * 2 * sizeof(uint32_t));
+    mesh->s_misc.reserve(64);
+    uint32_t *death = mesh->s_flags.as<uint32_t>();
+    uint8_t *flags = reinterpret_cast<uint8_t *>(death + n);
+    uint32_t *mv_from = mesh->s_holes.as<uint32_t>();
+    uint32_t *mv_to = mv_from + n;
+    int *err = mesh->s_misc.as<int>();
+    unsigned int *n_dead = reinterpret_cast<unsigned int *>(err + 1);
+    for (int step = 0; step < nsteps && n > 0; ++step) {
+      PYR_HIP(hipMemsetAsync(err, 0, 2 * sizeof(int), stream()));
+      dispatch_order(D, N, [&](auto DD, auto NN) {
+        constexpr int D_ = decltype(DD)::value, N_ = decltype(NN)::value;
+        if (precision == PYR_FP64)
+          hipLaunchKernelGGL((k_lsrk4_step<D_, N_, double>), dim3(blocks_for(n)), dim3(256),
+                             0, stream(), mesh->dev, mesh->sf64.as<double>(), n, dt, vacuum_c, x,
+                             p, q, mass, elem, death, counters, err, n_dead);
+        else
+          hipLaunchKernelGGL((k_lsrk4_step<D_, N_, float>), dim3(blocks_for(n)), dim3(256),
+                             0, stream(), mesh->dev, mesh->sf32.as<float>(), n, dt, vacuum_c, x,
+                             p, q, mass, elem, death, counters, err, n_dead);
+        PYR_CHECK_LAUNCH();
+      });
+      int h[2] = {0, 0};
+      PYR_HIP(hipMemcpyAsync(h, err, 2 * sizeof(int), hipMemcpyDeviceToHost, stream()));
+      PYR_HIP(hipStreamSynchronize(stream()));
+      if (h[0]) throw Error(PYR_ERR_NO_BEST_NORMAL, "no best normal found--weird");
+      if (h[1] == 0) continue;
+      // the reference compacts after each stage (cloud.py:587-592); replay
+      // the five compactions in stage order on the stage-of-death tags
+      for (uint32_t s = 1; s <= 5 && n > 0; ++s) {
+        hipLaunchKernelGGL(k_stage_flags, dim3(blocks_for(n)), dim3(256), 0, stream(), n,
+                           death, s, flags);
+        PYR_CHECK_LAUNCH();
+        int64_t n_new = 0, n_moves = 0;
+        int rc = pyr_compact_plan(n, flags, mv_from, mv_to, &n_new, &n_moves);
+        if (rc) throw Error(rc, pyr_last_error());
+        if (n_moves) {
+          struct {
+            void *buf;
+            int64_t bytes;
+          } arrays[] = {{x, 8 * D}, {p, 8 * D}, {q, 8}, {mass, 8}, {elem, 4},
+                        {death, 4}, {ids, 8}};
+          for (auto &a : arrays)
+            if (a.buf) {
+              rc = pyr_apply_moves(a.buf, a.bytes, n_moves, mv_from, mv_to);
+              if (rc) throw Error(rc, pyr_last_error());
+            }
+        }
+        n = n_new;
+      }
+    }
+    *n_inout = n;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,334 @@
+// Internal declarations shared by the HIP translation units of
+// libpyrticle_mi355x.so.  Not part of the C ABI (see include/pyrticle_mi355x.h).
+//
+// Everything here is built with -ffp-contract=off: a*b+c rounds twice unless a
+// kernel asks for fma() explicitly.  The localization path relies on that to
+// reproduce the reference's FP64 point-in-simplex decisions bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pyrticle_mi355x.h"
+
+namespace pyr {
+
+constexpr uint32_t INVALID = 0xFFFFFFFFu;
+constexpr double IN_TOL = 1e-10;   // meshdata.hpp:40 default tolerance
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_error(const std::string &msg);
+hipStream_t stream();
+
+#define PYR_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      throw ::pyr::Error(PYR_ERR_HIP, std::string(#call) + ": " +                  \
+                                          hipGetErrorString(e_));                  \
+  } while (0)
+
+#define PYR_CHECK_LAUNCH() PYR_HIP(hipGetLastError())
+
+#define PYR_REQUIRE(cond, msg)                                                     \
+  do {                                                                             \
+    if (!(cond)) throw ::pyr::Error(PYR_ERR_INVALID_ARG, msg);                     \
+  } while (0)
+
+// Wraps an ABI entry point body: exceptions become error codes.
+template <class F>
+int guarded(F &&f) {
+  try {
+    f();
+    return PYR_OK;
+  } catch (const Error &e) {
+    set_error(e.what());
+    return e.code;
+  } catch (const std::exception &e) {
+    set_error(e.what());
+    return PYR_ERR_INTERNAL;
+  }
+}
+
+// Device buffer owned by a handle.  Grows, never shrinks.
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  void reserve(size_t b) {
+    if (b <= bytes) return;
+    if (ptr) PYR_HIP(hipFree(ptr));
+    ptr = nullptr;
+    bytes = 0;
+    PYR_HIP(hipMalloc(&ptr, b ? b : 1));
+    bytes = b;
+  }
+  template <class T>
+  T *as() const { return static_cast<T *>(ptr); }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+};
+
+template <class T>
+T *upload(DevBuf &buf, const T *host, size_t count) {
+  buf.reserve(sizeof(T) * count);
+  if (count) PYR_HIP(hipMemcpy(buf.ptr, host, sizeof(T) * count, hipMemcpyHostToDevice));
+  return buf.as<T>();
+}
+
+// Device view of the mesh contract (pyrticle_amd/meshdata.py layout).
+struct DevMesh {
+  int dim, K, Np, nv;
+  const double *inv_map;        // K x (d*d + d)
+  const uint32_t *el_vertices;  // K x (d+1)
+  const double *face_normal;    // K x (d+1) x d
+  const double *face_rhs;       // K x (d+1)
+  const double *face_centroid;  // K x (d+1) x d
+  const double *face_radius;    // K x (d+1)
+  const uint32_t *face_neighbor;
+  const uint32_t *face_per_axis;
+  const double *vertices;       // nv x d
+  const uint32_t *vadj_starts;  // nv + 1
+  const uint32_t *vadj_elements;
+  const double *nodes;          // K*Np x d
+  double per_min[3], per_max[3];
+  // uniform grid over the (slightly inflated) mesh bounding box for the
+  // global-search fallback; cell lists are sorted by element id.
+  double g_lo[3], g_inv_h[3];
+  int g_n[3];
+  const uint32_t *g_starts;
+  const uint32_t *g_elems;
+};
+
+// ---------------------------------------------------------------------------
+// localization, restating particle_state.hpp:130-261 and meshdata.hpp:38-48.
+// ---------------------------------------------------------------------------
+struct FindCounters {
+  uint32_t same = 0, neighbor = 0, vertex = 0, global = 0;
+};
+
+template <int D>
+__device__ __forceinline__ void inverse_map(const DevMesh &m, uint32_t en, const double *x,
+                                            double *r) {
+  const double *a = m.inv_map + (size_t)en * (D * D + D);
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    double t = 0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) t += a[i * D + j] * x[j];
+    r[i] = t + a[D * D + i];
+  }
+}
+
+template <int D>
+__device__ __forceinline__ bool in_unit_simplex(const double *r) {
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+    if (r[i] < -1 - IN_TOL) return false;
+  double s = 0;
+#pragma unroll
+  for (int i = 0; i < D; ++i) s += r[i];
+  return s <= (double)(-(D - 2)) + IN_TOL;
+}
+
+template <int D>
+__device__ __forceinline__ bool in_element(const DevMesh &m, uint32_t en, const double *x,
+                                           double *r) {
+  inverse_map<D>(m, en, x, r);
+  return in_unit_simplex<D>(r);
+}
+
+// meshdata.hpp:208-215: the lowest element index containing x.  The grid cell
+// of x lists every element whose inflated bounding box meets the cell, in
+// ascending order, so the first hit is the reference's answer.
+template <int D>
+__device__ uint32_t find_global(const DevMesh &m, const double *x, double *r) {
+  int cell = 0;
+#pragma unroll
+  for (int i = D - 1; i >= 0; --i) {
+    double c = (x[i] - m.g_lo[i]) * m.g_inv_h[i];
+    if (!(c >= 0.0) || c >= (double)m.g_n[i]) return INVALID;
+    cell = cell * m.g_n[i] + (int)c;
+  }
+  for (uint32_t k = m.g_starts[cell]; k < m.g_starts[cell + 1]; ++k) {
+    uint32_t en = m.g_elems[k];
+    if (in_element<D>(m, en, x, r)) return en;
+  }
+  return INVALID;
+}
+
+// particle_state.hpp:130-261.  *err is set when the reference would throw
+// "no best normal found--weird".  r receives the unit coordinates of x in the
+// returned element (valid when the result is not INVALID).
+template <int D>
+__device__ uint32_t find_new_containing_element(const DevMesh &m, const double *x,
+                                                const double *p, uint32_t prev,
+                                                FindCounters &cnt, int &err, double *r) {
+  if (prev != INVALID && prev < (uint32_t)m.K) {
+    if (in_element<D>(m, prev, x, r)) {
+      cnt.same++;
+      return prev;
+    }
+    int closest = -1;
+    double max_ip = 0;
+#pragma unroll
+    for (int f = 0; f <= D; ++f) {
+      const double *n = m.face_normal + ((size_t)prev * (D + 1) + f) * D;
+      double ip = 0;
+#pragma unroll
+      for (int i = 0; i < D; ++i) ip += n[i] * p[i];
+      if (ip > max_ip) {
+        closest = f;
+        max_ip = ip;
+      }
+    }
+    if (closest == -1) {
+      err = 1;
+      return INVALID;
+    }
+    uint32_t possible = m.face_neighbor[(size_t)prev * (D + 1) + closest];
+    if (possible != INVALID && in_element<D>(m, possible, x, r)) {
+      cnt.neighbor++;
+      return possible;
+    }
+    uint32_t closest_vertex = INVALID;
+    double min_dist = __builtin_inf();
+#pragma unroll
+    for (int k = 0; k <= D; ++k) {
+      uint32_t vi = m.el_vertices[(size_t)prev * (D + 1) + k];
+      double s = 0;
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        double dv = m.vertices[(size_t)vi * D + i] - x[i];
+        s += dv * dv;
+      }
+      double dist = sqrt(s);
+      if (dist < min_dist) {
+        closest_vertex = vi;
+        min_dist = dist;
+      }
+    }
+    for (uint32_t k = m.vadj_starts[closest_vertex]; k < m.vadj_starts[closest_vertex + 1];
+         ++k) {
+      uint32_t possible2 = m.vadj_elements[k];
+      if (in_element<D>(m, possible2, x, r)) {
+        cnt.vertex++;
+        return possible2;
+      }
+    }
+  }
+  cnt.global++;
+  return find_global<D>(m, x, r);
+}
+
+// update_containing_elements + boundary_hit for one particle
+// (particle_state.hpp:266-357).  Returns false if the particle dies.  x is
+// wrapped in place on a periodic trip, as the reference writes it back (:341).
+template <int D>
+__device__ __forceinline__ bool relocate(const DevMesh &m, double *x, const double *p,
+                                         uint32_t &elem, FindCounters &cnt, int &err,
+                                         double *r) {
+  uint32_t ne = find_new_containing_element<D>(m, x, p, elem, cnt, err, r);
+  if (err) return false;
+  if (ne != INVALID) {
+    elem = ne;
+    return true;
+  }
+  bool trip = false;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    const double mn = m.per_min[a], mx = m.per_max[a];
+    if (mn != mx) {
+      const double xi = x[a];
+      if (xi < mn) {
+        x[a] += (mx - mn);
+        trip = true;
+      } else if (xi > mx) {
+        x[a] -= (mx - mn);
+        trip = true;
+      }
+    }
+  }
+  if (trip) {
+    uint32_t ce = find_new_containing_element<D>(m, x, p, elem, cnt, err, r);
+    if (err) return false;
+    if (ce != INVALID) {
+      // the reference re-examines the particle at the same index, which
+      // counts one more "same" hit (update_containing_elements loop, :274-294)
+      cnt.same++;
+      elem = ce;
+      return true;
+    }
+  }
+  return false;
+}
+
+// particle_state.hpp:98-125 get_velocities, same operation order.
+template <int VD>
+__device__ __forceinline__ void velocity(const double *p, double m, double c, double *v) {
+  double s = 0;
+#pragma unroll
+  for (int i = 0; i < VD; ++i) s += p[i] * p[i];
+  double pm = sqrt(s);
+  if (pm == 0) {
+#pragma unroll
+    for (int i = 0; i < VD; ++i) v[i] = 0;
+  } else {
+    double vm = c * pm / sqrt(m * m * c * c + pm * pm);
+    double f = vm / pm;
+#pragma unroll
+    for (int i = 0; i < VD; ++i) v[i] = f * p[i];
+  }
+}
+
+// Wave-aggregated accumulation of the four find counters.
+__device__ __forceinline__ void flush_counters(const FindCounters &c,
+                                               unsigned long long *out) {
+  if (!out) return;
+  unsigned long long vals[4] = {c.same, c.neighbor, c.vertex, c.global};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    unsigned long long v = vals[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(out + k, v);
+  }
+}
+
+// Exclusive scan of n uint32 values (device pointers, stream-ordered).
+// `total` (device, may be null) receives the sum.  tmp must hold
+// scan_tmp_words(n) words.
+size_t scan_tmp_words(int64_t n);
+void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *total,
+                        uint32_t *tmp, hipStream_t s);
+
+}  // namespace pyr
+
+// Opaque handle behind the ABI's pyr_mesh.
+struct pyr_mesh {
+  pyr::DevMesh dev{};
+  int dim = 0, K = 0, Np = 0, nv = 0;
+  std::vector<pyr::DevBuf> owned;   // mesh arrays
+  // interpolation: modal transform Minv = V^{-1} (Np x Np, row-major), exps
+  pyr::DevBuf minv, minv32, exps;
+  std::vector<int32_t> h_exps;
+  // scratch
+  pyr::DevBuf s_modal, s_flags, s_scan, s_tmp, s_holes, s_misc;
+  // deposition state (see deposit.hip)
+  struct DepositCache *dep = nullptr;
+  pyr::DevBuf counters;             // 4 x u64 scratch for counters
+  // static fields as modal coefficients, K x 6 x Np (pyr_set_static_fields)
+  pyr::DevBuf sf64, sf32;
+  bool sf_set = false;
+  ~pyr_mesh();
+};

@@ -1,0 +1,217 @@
+"""Device-resident mesh handle and the particle-path calls on torch tensors.
+
+Every method forwards to one C-ABI entry point of libpyrticle_mi355x.so on
+torch's current stream.  Particle arrays are torch CUDA tensors in the
+reference's layout (positions/momenta n x d float64, charges/masses n float64,
+containing elements n uint32 stored as int32 bit patterns).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from .meshdata import MeshArrays
+
+INVALID_ELEMENT = _lib.INVALID_ELEMENT
+FP64 = _lib.PYR_FP64
+FP32C = _lib.PYR_FP32C
+
+
+def _host(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def element_tensor(elem_np, device="cuda") -> torch.Tensor:
+    """uint32 element ids -> device int32 tensor with the same bits."""
+    a = np.ascontiguousarray(elem_np, dtype=np.uint32).view(np.int32)
+    return torch.from_numpy(a.copy()).to(device)
+
+
+def element_numpy(elem_t: torch.Tensor) -> np.ndarray:
+    return elem_t.detach().cpu().numpy().view(np.uint32)
+
+
+class DeviceMesh:
+    """Owns one pyr_mesh handle (mesh contract in HBM)."""
+
+    def __init__(self, ma: MeshArrays):
+        self.ma = ma
+        self.dim = ma.dim
+        self.K = ma.K
+        self.Np = ma.Np
+        L = _lib.lib()
+        keep = {}
+
+        def p(name, arr, dt):
+            a = _host(arr, dt)
+            keep[name] = a
+            return a.ctypes.data
+
+        desc = _lib.MeshDesc(
+            dim=ma.dim, K=ma.K, Np=ma.Np, nv=ma.nv, nadj=int(len(ma.vadj_elements)),
+            inv_map=p("inv_map", ma.inv_map, np.float64),
+            jacobian=p("jacobian", ma.jacobian, np.float64),
+            el_vertices=p("el_vertices", ma.el_vertices, np.uint32),
+            face_normal=p("face_normal", ma.face_normal, np.float64),
+            face_rhs=p("face_rhs", ma.face_rhs, np.float64),
+            face_centroid=p("face_centroid", ma.face_centroid, np.float64),
+            face_radius=p("face_radius", ma.face_radius, np.float64),
+            face_neighbor=p("face_neighbor", ma.face_neighbor, np.uint32),
+            face_per_axis=p("face_per_axis", ma.face_per_axis, np.uint32),
+            vertices=p("vertices", ma.vertices, np.float64),
+            vadj_starts=p("vadj_starts", ma.vadj_starts, np.uint32),
+            vadj_elements=p("vadj_elements", ma.vadj_elements if len(ma.vadj_elements)
+                            else np.zeros(1, np.uint32), np.uint32),
+            periodicity=p("periodicity", ma.periodicity, np.float64),
+            nodes=p("nodes", ma.nodes, np.float64),
+            exponents=p("exponents", ma.exponents, np.int32),
+            vandermonde=p("vandermonde", ma.vandermonde, np.float64),
+        )
+        h = C.c_void_p()
+        _lib.check(L.pyr_mesh_create(C.byref(desc), C.byref(h)))
+        self._h = h
+        self._lib = L
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self._lib.pyr_mesh_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- lookup -----------------------------------------------------------
+    def find_containing_element(self, x: torch.Tensor) -> torch.Tensor:
+        _lib.sync_stream()
+        n = x.shape[0]
+        out = torch.empty(n, dtype=torch.int32, device=x.device)
+        _lib.check(self._lib.pyr_find_containing_element(self._h, n, _lib.ptr(x), _lib.ptr(out)))
+        return out
+
+    def is_in_element(self, elem: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+        _lib.sync_stream()
+        n = x.shape[0]
+        out = torch.empty(n, dtype=torch.uint8, device=x.device)
+        _lib.check(self._lib.pyr_is_in_element(self._h, n, _lib.ptr(elem), _lib.ptr(x),
+                                               _lib.ptr(out)))
+        return out
+
+    # -- particles ----------------------------------------------------------
+    def locate(self, x, p, elem, counters=None):
+        """Relocalize in place; returns the dead flags (uint8)."""
+        _lib.sync_stream()
+        n = x.shape[0]
+        dead = torch.empty(n, dtype=torch.uint8, device=x.device)
+        _lib.check(self._lib.pyr_locate(self._h, n, _lib.ptr(x), _lib.ptr(p), _lib.ptr(elem),
+                                        _lib.ptr(dead), _lib.ptr(counters)))
+        return dead
+
+    def forces(self, fields, x, elem, q, v, vis=False):
+        _lib.sync_stream()
+        n, vd = v.shape
+        F = torch.empty((n, vd), dtype=torch.float64, device=x.device)
+        vis_t = [torch.empty((n, 3), dtype=torch.float64, device=x.device)
+                 for _ in range(4)] if vis else [None] * 4
+        fp = (C.c_void_p * 6)(*[None if f is None else _lib.ptr(f) for f in fields])
+        _lib.check(self._lib.pyr_forces(self._h, n, vd, fp, _lib.ptr(x), _lib.ptr(elem),
+                                        _lib.ptr(q), _lib.ptr(v), _lib.ptr(F),
+                                        *[_lib.ptr(t) for t in vis_t]))
+        if vis:
+            return F, dict(zip(("pt_e", "pt_b", "el_force", "mag_force"), vis_t))
+        return F
+
+    def set_static_fields(self, fields):
+        _lib.sync_stream()
+        fp = (C.c_void_p * 6)(*[None if f is None else _lib.ptr(f) for f in fields])
+        _lib.check(self._lib.pyr_set_static_fields(self._h, fp))
+
+    def lsrk4_steps(self, nsteps, dt, c, n, x, p, q, m, elem, ids=None, counters=None,
+                    precision=FP64) -> int:
+        _lib.sync_stream()
+        nn = C.c_int64(n)
+        _lib.check(self._lib.pyr_lsrk4_steps(self._h, precision, nsteps, dt, c, C.byref(nn),
+                                             _lib.ptr(x), _lib.ptr(p), _lib.ptr(q),
+                                             _lib.ptr(m), _lib.ptr(elem), _lib.ptr(ids),
+                                             _lib.ptr(counters)))
+        return int(nn.value)
+
+    def deposit(self, radius, alpha, normalizer, x, q, elem, v=None, want_rho=True,
+                want_j=True, precision=FP64, rho=None, j=None):
+        _lib.sync_stream()
+        n, d = x.shape
+        nn = self.K * self.Np
+        if want_rho and rho is None:
+            rho = torch.empty(nn, dtype=torch.float64, device=x.device)
+        if want_j and v is not None and j is None:
+            j = torch.empty((nn, d), dtype=torch.float64, device=x.device)
+        _lib.check(self._lib.pyr_deposit(
+            self._h, radius, alpha, normalizer, precision, n, d, _lib.ptr(x), _lib.ptr(q),
+            _lib.ptr(v) if v is not None else None, _lib.ptr(elem),
+            _lib.ptr(rho) if want_rho else None,
+            _lib.ptr(j) if (want_j and v is not None) else None))
+        return (rho if want_rho else None), (j if (want_j and v is not None) else None)
+
+    def deposit_stats(self):
+        arr = (C.c_ulonglong * 4)()
+        _lib.check(self._lib.pyr_deposit_stats(self._h, C.byref(arr)))
+        return tuple(int(v) for v in arr)
+
+    def candidate_pairs(self, radius) -> int:
+        out = C.c_int64()
+        _lib.check(self._lib.pyr_deposit_candidates(self._h, radius, C.byref(out)))
+        return int(out.value)
+
+    def sort_by_element(self, elem):
+        _lib.sync_stream()
+        n = elem.shape[0]
+        perm = torch.empty(n, dtype=torch.int32, device=elem.device)
+        seg = torch.empty(self.K + 1, dtype=torch.int32, device=elem.device)
+        _lib.check(self._lib.pyr_sort_by_element(self._h, n, _lib.ptr(elem), _lib.ptr(perm),
+                                                 _lib.ptr(seg)))
+        return perm, seg
+
+
+def set_deposit_counting(on: bool):
+    _lib.check(_lib.lib().pyr_deposit_set_counting(1 if on else 0))
+
+
+def velocities(p: torch.Tensor, m: torch.Tensor, c: float) -> torch.Tensor:
+    _lib.sync_stream()
+    n, vd = p.shape
+    v = torch.empty_like(p)
+    _lib.check(_lib.lib().pyr_velocities(n, vd, c, _lib.ptr(p), _lib.ptr(m), _lib.ptr(v)))
+    return v
+
+
+def compact_plan(dead: torch.Tensor):
+    """(n_new, moves_from, moves_to) for the reference's swap-remove kill order."""
+    _lib.sync_stream()
+    n = dead.shape[0]
+    frm = torch.empty(max(n, 1), dtype=torch.int32, device=dead.device)
+    to = torch.empty(max(n, 1), dtype=torch.int32, device=dead.device)
+    n_new = C.c_int64()
+    n_moves = C.c_int64()
+    _lib.check(_lib.lib().pyr_compact_plan(n, _lib.ptr(dead), _lib.ptr(frm), _lib.ptr(to),
+                                           C.byref(n_new), C.byref(n_moves)))
+    k = int(n_moves.value)
+    return int(n_new.value), frm[:k], to[:k]
+
+
+def apply_moves(buf: torch.Tensor, frm: torch.Tensor, to: torch.Tensor):
+    """buf[to] = buf[frm] along the first axis, in place."""
+    _lib.sync_stream()
+    k = frm.shape[0]
+    if k == 0:
+        return
+    row_bytes = buf.element_size() * (buf.numel() // max(1, buf.shape[0]))
+    _lib.check(_lib.lib().pyr_apply_moves(_lib.ptr(buf), row_bytes, k, _lib.ptr(frm),
+                                          _lib.ptr(to)))

@@ -1,0 +1,249 @@
+"""Parity of every C-ABI entry point with the oracle, on the GPU.
+
+Integer work (element ids, kill order, sort permutation, counters) must match
+bit for bit; floating-point results within the stated tolerances:
+FP64 rho, j, forces and trajectories within 1e-12 relative L2; FP32C rho and j
+within 1e-6 relative L2.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32C_TOL = 1e-6
+
+
+def _dev(a, cuda, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device=cuda, dtype=dtype)
+
+
+@pytest.fixture(scope="module")
+def dmesh2(square, cuda):
+    from pyrticle_amd.device import DeviceMesh
+    return DeviceMesh(square.ma)
+
+
+@pytest.fixture(scope="module")
+def dmesh3(small_cylinder, cuda):
+    from pyrticle_amd.device import DeviceMesh
+    return DeviceMesh(small_cylinder.ma)
+
+
+def _points_2d(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-1.05, 1.05, size=(n, 2))
+
+
+def test_find_containing_element_bit_exact(square, oracle_square, dmesh2, cuda):
+    from pyrticle_amd.device import element_numpy
+    x = _points_2d(4000, 11)
+    # add points exactly on vertices and edge midpoints (ties between elements)
+    verts = square.ma.vertices[:200]
+    mids = 0.5 * (square.ma.vertices[square.ma.el_vertices[:200, 0]]
+                  + square.ma.vertices[square.ma.el_vertices[:200, 1]])
+    x = np.vstack([x, verts, mids])
+    got = element_numpy(dmesh2.find_containing_element(_dev(x, cuda)))
+    want = np.array([O.find_containing_element(oracle_square, xi) for xi in x], dtype=np.uint32)
+    assert np.array_equal(got, want)
+    assert (want == O.INVALID).sum() > 0      # some points are outside the square
+
+
+def _moved_particles(wl, om, scale, seed):
+    rng = np.random.default_rng(seed)
+    n = 3000
+    idx = rng.choice(wl.x.shape[0], size=n, replace=False)
+    x0 = wl.x[idx].copy()
+    el0 = np.array([O.find_containing_element(om, xi) for xi in x0], dtype=np.uint32)
+    x1 = x0 + rng.normal(0, scale, size=x0.shape)
+    p = wl.p[idx].copy()
+    return x1, p, wl.q[idx].copy(), wl.m[idx].copy(), el0
+
+
+@pytest.mark.parametrize("scale", [0.002, 0.02, 0.2])
+def test_locate_and_kill_order_bit_exact(square, oracle_square, dmesh2, cuda, scale):
+    from pyrticle_amd import device as dv
+    x, p, q, m, el = _moved_particles(square, oracle_square, scale, 5)
+    ids = np.arange(x.shape[0], dtype=np.int64)
+    # oracle: sequential update_containing_elements with kills
+    ox, op, oq, om_, oel, oids = x.copy(), p.copy(), q.copy(), m.copy(), el.copy(), ids.copy()
+    n_o, cnt_o = O.update_containing_elements(oracle_square, ox, op, oq, om_, oel, ids=oids)
+    # GPU: locate, then replay the kill order
+    tx, tp, tq, tm = (_dev(a, cuda) for a in (x, p, q, m))
+    tel = dv.element_tensor(el, cuda)
+    tids = _dev(ids, cuda, torch.int64)
+    counters = torch.zeros(4, dtype=torch.int64, device=cuda)
+    dead = dmesh2.locate(tx, tp, tel, counters)
+    n_new, frm, to = dv.compact_plan(dead)
+    for buf in (tx, tp, tq, tm, tel, tids):
+        dv.apply_moves(buf, frm, to)
+    torch.cuda.synchronize()
+    assert n_new == n_o
+    assert np.array_equal(dv.element_numpy(tel[:n_new]), oel[:n_o])
+    assert np.array_equal(tids[:n_new].cpu().numpy(), oids[:n_o])
+    assert np.array_equal(tx[:n_new].cpu().numpy(), ox[:n_o])
+    assert tuple(int(c) for c in counters.cpu()) == cnt_o
+    if scale == 0.2:
+        assert n_new < x.shape[0]          # kills were exercised
+
+
+def test_locate_periodic_wrap(periodic_cylinder, cuda):
+    from pyrticle_amd import device as dv
+    discr, ma = periodic_cylinder
+    om = O.OracleMesh(ma)
+    dm = dv.DeviceMesh(ma)
+    rng = np.random.default_rng(3)
+    n = 2000
+    r = 0.8 * np.sqrt(rng.uniform(size=n))
+    ph = rng.uniform(0, 2 * np.pi, n)
+    x = np.stack([r * np.cos(ph), r * np.sin(ph), rng.uniform(0.01, 1.99, n)], axis=1)
+    el = np.array([O.find_containing_element(om, xi) for xi in x], dtype=np.uint32)
+    p = rng.normal(size=(n, 3))
+    x1 = x.copy()
+    x1[:, 2] += rng.normal(0, 0.5, n)           # many cross z=0 or z=2
+    q = np.ones(n)
+    m = np.ones(n)
+    ox, oel = x1.copy(), el.copy()
+    oids = np.arange(n, dtype=np.int64)
+    n_o, cnt_o = O.update_containing_elements(om, ox, p.copy(), q.copy(), m.copy(), oel,
+                                              ids=oids)
+    tx = _dev(x1, cuda)
+    tel = dv.element_tensor(el, cuda)
+    tids = torch.arange(n, dtype=torch.int64, device=cuda)
+    counters = torch.zeros(4, dtype=torch.int64, device=cuda)
+    dead = dm.locate(tx, _dev(p, cuda), tel, counters)
+    n_new, frm, to = dv.compact_plan(dead)
+    for buf in (tx, tel, tids):
+        dv.apply_moves(buf, frm, to)
+    assert n_new == n_o
+    assert np.array_equal(tids[:n_new].cpu().numpy(), oids[:n_o])
+    assert np.array_equal(dv.element_numpy(tel[:n_new]), oel[:n_o])
+    assert np.array_equal(tx[:n_new].cpu().numpy(), ox[:n_o])
+    assert tuple(int(c) for c in counters.cpu()) == cnt_o
+
+
+def test_velocities(square, cuda):
+    from pyrticle_amd import device as dv
+    p = square.p.copy()
+    p[:5] = 0.0                                  # the |p| == 0 branch
+    v = dv.velocities(_dev(p, cuda), _dev(square.m, cuda), square.c).cpu().numpy()
+    vo = O.velocities(p, square.m, square.c)
+    assert np.max(np.abs(v - vo) / np.maximum(np.abs(vo), 1e-300)) <= 4.5e-16
+    assert np.all(v[:5] == 0)
+
+
+def test_forces_2d(square, oracle_square, dmesh2, cuda):
+    from pyrticle_amd import device as dv
+    n = 4000
+    x, p, q, m = square.x[:n], square.p[:n], square.q[:n], square.m[:n]
+    el = np.array([O.find_containing_element(oracle_square, xi) for xi in x], dtype=np.uint32)
+    v = O.velocities(p, m, square.c)
+    rng = np.random.default_rng(2)
+    nn = square.ma.nnodes
+    fields = [rng.normal(size=nn), rng.normal(size=nn), None, None, None, rng.normal(size=nn)]
+    fo, viso = O.forces(oracle_square, x, el, q, v, fields, vis=True)
+    tf = [None if f is None else _dev(f, cuda) for f in fields]
+    F, vis = dmesh2.forces(tf, _dev(x, cuda), dv.element_tensor(el, cuda), _dev(q, cuda),
+                           _dev(v, cuda), vis=True)
+    assert rel_l2(F.cpu().numpy(), fo) < 1e-13
+    for k in viso:
+        assert rel_l2(vis[k].cpu().numpy(), viso[k]) < 1e-13, k
+
+
+def test_forces_3d(small_cylinder, cuda, dmesh3):
+    from pyrticle_amd import device as dv
+    wl = small_cylinder
+    om = O.OracleMesh(wl.ma)
+    n = 2000
+    x, p, q, m = wl.x[:n], wl.p[:n], wl.q[:n], wl.m[:n]
+    el = dv.element_numpy(dmesh3.find_containing_element(_dev(x, cuda)))
+    v = O.velocities(p, m, wl.c)
+    rng = np.random.default_rng(4)
+    fields = [rng.normal(size=wl.ma.nnodes) for _ in range(6)]
+    fo = O.forces(om, x, el, q, v, fields)
+    F = dmesh3.forces([_dev(f, cuda) for f in fields], _dev(x, cuda),
+                      dv.element_tensor(el, cuda), _dev(q, cuda), _dev(v, cuda))
+    assert rel_l2(F.cpu().numpy(), fo) < 1e-13
+
+
+def test_sort_by_element_is_stable_argsort(square, dmesh2, cuda):
+    from pyrticle_amd import device as dv
+    rng = np.random.default_rng(9)
+    el = rng.integers(0, square.ma.K, size=50_000).astype(np.uint32)
+    perm, seg = dmesh2.sort_by_element(dv.element_tensor(el, cuda))
+    assert np.array_equal(perm.cpu().numpy(), np.argsort(el, kind="stable"))
+    counts = np.bincount(el, minlength=square.ma.K)
+    assert np.array_equal(seg.cpu().numpy(), np.concatenate([[0], np.cumsum(counts)]))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_deposit_2d_matches_oracle(square, oracle_square, dmesh2, cuda, precision):
+    from pyrticle_amd import device as dv
+    wl = square
+    el = np.array([O.find_containing_element(oracle_square, xi) for xi in wl.x],
+                  dtype=np.uint32)
+    v = O.velocities(wl.p, wl.m, wl.c)
+    rho_o, j_o, visits = O.deposit(oracle_square, wl.radius, wl.alpha, wl.x, wl.q, v, el)
+    nrm = O.shape_normalizer(wl.radius, 2, wl.alpha)
+    prec = dv.FP64 if precision == "fp64" else dv.FP32C
+    rho, j = dmesh2.deposit(wl.radius, wl.alpha, nrm, _dev(wl.x, cuda), _dev(wl.q, cuda),
+                            dv.element_tensor(el, cuda), v=_dev(v, cuda), precision=prec)
+    tol = FP64_TOL if precision == "fp64" else FP32C_TOL
+    assert rel_l2(rho.cpu().numpy(), rho_o) < tol
+    assert rel_l2(j.cpu().numpy(), j_o) < tol
+
+
+def test_deposit_3d_matches_oracle(small_cylinder, dmesh3, cuda):
+    from pyrticle_amd import device as dv
+    wl = small_cylinder
+    om = O.OracleMesh(wl.ma)
+    n = 5000
+    x, q = wl.x[:n], wl.q[:n]
+    el = dv.element_numpy(dmesh3.find_containing_element(_dev(x, cuda)))
+    v = O.velocities(wl.p[:n], wl.m[:n], wl.c)
+    rho_o, j_o, _ = O.deposit(om, wl.radius, wl.alpha, x, q, v, el)
+    nrm = O.shape_normalizer(wl.radius, 3, wl.alpha)
+    rho, j = dmesh3.deposit(wl.radius, wl.alpha, nrm, _dev(x, cuda), _dev(q, cuda),
+                            dv.element_tensor(el, cuda), v=_dev(v, cuda))
+    assert rel_l2(rho.cpu().numpy(), rho_o) < FP64_TOL
+    assert rel_l2(j.cpu().numpy(), j_o) < FP64_TOL
+    rho2, j2 = dmesh3.deposit(wl.radius, wl.alpha, nrm, _dev(x, cuda), _dev(q, cuda),
+                              dv.element_tensor(el, cuda), v=_dev(v, cuda))
+    assert torch.equal(rho, rho2) and torch.equal(j, j2)     # deterministic
+
+
+@pytest.mark.parametrize("dt_scale", [1.0, 4.0])
+def test_lsrk4_square_matches_oracle(square, oracle_square, dmesh2, cuda, dt_scale):
+    """Config 1: 10k electrons, 100 LSRK4 steps with relocalization; with a 4x
+    time step particles leave the square and the kill order is exercised."""
+    import dataclasses
+    from pyrticle_amd import device as dv
+    wl = dataclasses.replace(square, dt=square.dt * dt_scale)
+    el = np.array([O.find_containing_element(oracle_square, xi) for xi in wl.x],
+                  dtype=np.uint32)
+    n0 = wl.x.shape[0]
+    ox, op, oq, om_, oel = wl.x.copy(), wl.p.copy(), wl.q.copy(), wl.m.copy(), el.copy()
+    oids = np.arange(n0, dtype=np.int64)
+    n_o, cnt_o = O.lsrk4_run(oracle_square, wl.fields, wl.c, ox, op, oq, om_, oel, wl.dt,
+                             wl.nsteps, ids=oids)
+    tf = [None if f is None else _dev(f, cuda) for f in wl.fields]
+    dmesh2.set_static_fields(tf)
+    tx, tp, tq, tm = (_dev(a, cuda) for a in (wl.x, wl.p, wl.q, wl.m))
+    tel = dv.element_tensor(el, cuda)
+    tids = torch.arange(n0, dtype=torch.int64, device=cuda)
+    counters = torch.zeros(4, dtype=torch.int64, device=cuda)
+    n = dmesh2.lsrk4_steps(wl.nsteps, wl.dt, wl.c, n0, tx, tp, tq, tm, tel, ids=tids,
+                           counters=counters)
+    assert n == n_o
+    if dt_scale > 1:
+        assert n < n0                            # particles left the square
+    assert np.array_equal(tids[:n].cpu().numpy(), oids[:n_o])
+    assert np.array_equal(dv.element_numpy(tel[:n]), oel[:n_o])
+    assert rel_l2(tx[:n].cpu().numpy(), ox[:n_o]) < FP64_TOL
+    assert rel_l2(tp[:n].cpu().numpy(), op[:n_o]) < FP64_TOL
+    assert tuple(int(c) for c in counters.cpu()) == cnt_o
