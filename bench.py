@@ -1,0 +1,276 @@
+#!/usr/bin/env python
+"""Throughput of the particle path on one MI355X: particle-steps/s (push + deposit).
+
+One step = one fused LSRK4 push (5 stages, relocalization and kills after each
+stage) followed by one rho + j shape-function deposition, on the cylinder
+workload: 200,910 P3 tetrahedra (Np=20), 4M electrons, R=0.08, alpha=2, static
+B_z=0.5 T and E_z=1e6 V/m, dt=0.2*h_min/c.  Inputs are resident in HBM before
+the timed region.  Prints one JSON line (see README of the benchmark contract
+in DESIGN.md for the fields).
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+METRIC = ("particle-steps/s (push+deposit), FP32-comp vs FP64 at rel ρ err≤1e-6; "
+          "HBM GB/s")
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def build_state(wl, dm, device, torch):
+    from pyrticle_amd import device as dv
+    x = torch.from_numpy(wl.x).to(device)
+    elem = dm.find_containing_element(x)
+    el_np = dv.element_numpy(elem)
+    assert np.all(el_np != dv.INVALID_ELEMENT), "beam particle outside the mesh"
+    # the beam is loaded element by element (stable order within an element)
+    order = np.argsort(el_np, kind="stable")
+    st = {
+        "x": torch.from_numpy(np.ascontiguousarray(wl.x[order])).to(device),
+        "p": torch.from_numpy(np.ascontiguousarray(wl.p[order])).to(device),
+        "q": torch.from_numpy(np.ascontiguousarray(wl.q[order])).to(device),
+        "m": torch.from_numpy(np.ascontiguousarray(wl.m[order])).to(device),
+        "elem": dv.element_tensor(el_np[order], device),
+        "n": wl.x.shape[0],
+    }
+    return st, order
+
+
+def clone_state(st):
+    out = {k: (v.clone() if hasattr(v, "clone") else v) for k, v in st.items()}
+    return out
+
+
+def run_steps(dm, wl, st, nsteps, precision, normalizer, bufs, torch):
+    """nsteps of push + deposit; returns the sum of alive particles per step."""
+    from pyrticle_amd import device as dv
+    alive = 0
+    for _ in range(nsteps):
+        n = dm.lsrk4_steps(1, wl.dt, wl.c, st["n"], st["x"], st["p"], st["q"], st["m"],
+                           st["elem"], precision=precision)
+        st["n"] = n
+        v = dv.velocities(st["p"][:n], st["m"][:n], wl.c)
+        dm.deposit(wl.radius, wl.alpha, normalizer, st["x"][:n], st["q"][:n],
+                   st["elem"][:n], v=v, precision=precision, rho=bufs["rho"], j=bufs["j"])
+        alive += n
+    return alive
+
+
+def cpu_baseline(wl, st0, sample, steps):
+    """Oracle (single-threaded FP64 C restatement) on a bounded sample.
+
+    The sample starts from the same element ids as the GPU state (the
+    GPU's global search is bit-exact with the oracle's O(K) scan, which would
+    dominate the CPU time here)."""
+    from oracle import oracle as O
+    from pyrticle_amd import device as dv
+    om = O.OracleMesh(wl.ma)
+    stride = max(1, st0["n"] // sample)
+    sl = slice(0, stride * sample, stride)
+    x = np.ascontiguousarray(st0["x"][sl].cpu().numpy())
+    p = np.ascontiguousarray(st0["p"][sl].cpu().numpy())
+    q = np.ascontiguousarray(st0["q"][sl].cpu().numpy())
+    m = np.ascontiguousarray(st0["m"][sl].cpu().numpy())
+    el = np.ascontiguousarray(dv.element_numpy(st0["elem"][sl]))
+    t0 = time.perf_counter()
+    alive = 0
+    visits = None
+    for _ in range(steps):
+        n, _ = O.lsrk4_run(om, wl.fields, wl.c, x, p, q, m, el, wl.dt, 1)
+        x, p, q, m, el = x[:n], p[:n], q[:n], m[:n], el[:n]
+        v = O.velocities(p, m, wl.c)
+        _, _, visits = O.deposit(om, wl.radius, wl.alpha, x, q, v, el)
+        alive += n
+    dt = time.perf_counter() - t0
+    return alive / dt, dt, int(visits[0]) / max(1, n), n
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--particles", type=int, default=4_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=40_000)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32c", action="store_true")
+    args = ap.parse_args(argv)
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py runs on one GPU only: the particle path here is built and measured "
+              "on a single MI355X (multi-GPU sharding is described in DESIGN.md, not run).",
+              file=sys.stderr)
+        return 2
+
+    import torch
+    from pyrticle_amd import device as dv
+    from pyrticle_amd import workloads as W
+    from oracle import oracle as O
+
+    device = torch.device("cuda:0")
+    t_setup = time.perf_counter()
+    log("building cylinder mesh and beam")
+    wl = W.cylinder_3d(n_particles=args.particles)
+    K, Np = wl.ma.K, wl.ma.Np
+    dm = dv.DeviceMesh(wl.ma)
+    dm.set_static_fields([torch.from_numpy(f).to(device) for f in wl.fields])
+    st0, order = build_state(wl, dm, device, torch)
+    normalizer = O.shape_normalizer(wl.radius, 3, wl.alpha)
+    nn = K * Np
+    bufs = {"rho": torch.empty(nn, dtype=torch.float64, device=device),
+            "j": torch.empty((nn, 3), dtype=torch.float64, device=device)}
+    log("setup %.1fs: K=%d Np=%d n=%d dt=%.3e" % (time.perf_counter() - t_setup, K, Np,
+                                                   st0["n"], wl.dt))
+
+    # ---- FP64 (the drop-in form) ------------------------------------------
+    st = clone_state(st0)
+    run_steps(dm, wl, st, args.warmup, dv.FP64, normalizer, bufs, torch)
+    torch.cuda.synchronize()
+    dv.profile_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    alive = run_steps(dm, wl, st, args.steps, dv.FP64, normalizer, bufs, torch)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    gather_ms, gather_n = dv.profile_read("gather")
+    push_ms, push_n = dv.profile_read("push")
+    sort_ms, sort_n = dv.profile_read("sort")
+    dv.profile_enable(False)
+    wall = t1 - t0
+    value = alive / wall
+    log("fp64: %.3f ms/step, %.3e particle-steps/s; gather %.3f ms, push %.3f ms, sort %.3f ms"
+        % (1e3 * wall / args.steps, value, gather_ms / max(1, gather_n),
+           push_ms / max(1, push_n), sort_ms / max(1, sort_n)))
+
+    # work counts of the last state (counting pass, outside the timed region)
+    n_last = st["n"]
+    dv.set_deposit_counting(True)
+    v = dv.velocities(st["p"][:n_last], st["m"][:n_last], wl.c)
+    rho64, j64 = dm.deposit(wl.radius, wl.alpha, normalizer, st["x"][:n_last],
+                            st["q"][:n_last], st["elem"][:n_last], v=v)
+    n_eval, n_in, n_cand, n_pass = dm.deposit_stats()
+    dv.set_deposit_counting(False)
+    occupied = int(torch.unique(st["elem"][:n_last]).numel())
+
+    # ---- FP32C ------------------------------------------------------------
+    fp32c = None
+    if not args.no_fp32c:
+        rho32, j32 = dm.deposit(wl.radius, wl.alpha, normalizer, st["x"][:n_last],
+                                st["q"][:n_last], st["elem"][:n_last], v=v,
+                                precision=dv.FP32C)
+        rho_err = float(torch.linalg.norm(rho32 - rho64) / torch.linalg.norm(rho64))
+        j_err = float(torch.linalg.norm(j32 - j64) / torch.linalg.norm(j64))
+        st32 = clone_state(st0)
+        run_steps(dm, wl, st32, args.warmup, dv.FP32C, normalizer, bufs, torch)
+        torch.cuda.synchronize()
+        dv.profile_enable(True)
+        t0 = time.perf_counter()
+        alive32 = run_steps(dm, wl, st32, args.steps, dv.FP32C, normalizer, bufs, torch)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        g32_ms, g32_n = dv.profile_read("gather")
+        p32_ms, p32_n = dv.profile_read("push")
+        dv.profile_enable(False)
+        n32 = min(st32["n"], st["n"])
+        x_err = float(torch.linalg.norm(st32["x"][:n32] - st["x"][:n32])
+                      / torch.linalg.norm(st["x"][:n32])) if st32["n"] == st["n"] else None
+        p_err = float(torch.linalg.norm(st32["p"][:n32] - st["p"][:n32])
+                      / torch.linalg.norm(st["p"][:n32])) if st32["n"] == st["n"] else None
+        fp32c = {"value": alive32 / (t1 - t0), "ms_per_step": 1e3 * (t1 - t0) / args.steps,
+                 "speedup_vs_fp64": (alive32 / (t1 - t0)) / value,
+                 "rho_rel_l2_vs_fp64": rho_err, "j_rel_l2_vs_fp64": j_err,
+                 "x_rel_l2_vs_fp64": x_err, "p_rel_l2_vs_fp64": p_err,
+                 "gather_ms": g32_ms / max(1, g32_n), "push_ms": p32_ms / max(1, p32_n)}
+        log("fp32c: %.3f ms/step (x%.2f), rho err %.2e" % (fp32c["ms_per_step"],
+                                                           fp32c["speedup_vs_fp64"], rho_err))
+
+    # ---- CPU baseline ------------------------------------------------------
+    cpu = None
+    visits_per_particle = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline: oracle on %d particles x %d steps" % (args.cpu_sample,
+                                                                 args.cpu_steps))
+        rate, secs, visits_per_particle, _ = cpu_baseline(wl, st0, args.cpu_sample,
+                                                          args.cpu_steps)
+        cpu = {"value": rate, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+               "sample": "%d particles (every %dth of the element-sorted beam) x %d steps "
+                         "on the full 200,910-element mesh, %.1f s" % (
+                             args.cpu_sample, max(1, st0["n"] // args.cpu_sample),
+                             args.cpu_steps, secs)}
+
+    # ---- roofline of the dominant kernel (deposition gather, FP64 VALU) ----
+    gather_avg_s = gather_ms / max(1, gather_n) / 1e3
+    # reference work model: 9 flops per node visit of the face walk, 12 more
+    # for each node inside R.  Inside pairs are counted exactly on the GPU;
+    # node visits per particle come from the oracle's face walk on the sample.
+    if visits_per_particle is None:
+        visits_per_particle = float(n_eval) / max(1, n_last)
+    flops = 9.0 * visits_per_particle * n_last + 12.0 * n_in
+    achieved_tflops = flops / gather_avg_s / 1e12 if gather_avg_s > 0 else 0.0
+    push_avg_s = push_ms / max(1, push_n) / 1e3
+    field_bytes = occupied * (6 * Np * 8 + 12 * 8 + 4 * (3 + 1) * 8 + 4 * 4)
+    push_bytes = n_last * (68 + 52) + field_bytes
+    push_gbs = push_bytes / push_avg_s / 1e9 if push_avg_s > 0 else 0.0
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded beam and mesh; static fields)",
+        "config": {"workload": "cylinder_3d: r=1, z in [0,4], %d P3 tets (Np=20), %d "
+                               "electrons (KV-like ellipse 0.3x0.2, sigma_z=0.4, gamma=10, "
+                               "0.1%% spread), R=0.08, alpha=2, Bz=0.5 T, Ez=1e6 V/m, "
+                               "dt=0.2*h_min/c, one step = LSRK4 push + rho/j deposition"
+                               % (K, st0["n"]),
+                   "elements": K, "particles": st0["n"], "precision": "fp64"},
+        "roofline": {"bound": "valu", "kernel": "k_gather (rho+j deposition, FP64)",
+                     "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved_tflops / FP64_PEAK_TFLOPS,
+                     "traffic": None,
+                     "avg_launch_ms": gather_avg_s * 1e3,
+                     "work": {"flops_per_launch": flops, "node_visits_per_particle_ref":
+                              visits_per_particle, "pairs_inside_R": n_in,
+                              "pairs_evaluated": n_eval, "candidate_particle_visits": n_cand,
+                              "prefilter_passed": n_pass, "particles": n_last}},
+        "roofline_push": {"bound": "hbm", "kernel": "k_lsrk4_step (fused 5-stage push, FP64)",
+                          "achieved": push_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": push_gbs / HBM_PEAK_GBS, "traffic": None,
+                          "avg_launch_ms": push_avg_s * 1e3,
+                          "bytes_per_launch": push_bytes, "occupied_elements": occupied},
+        "sort_ms": sort_ms / max(1, sort_n),
+        "fp32c": fp32c,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -79,6 +79,14 @@ int pyr_set_stream(void *hip_stream);     /* hipStream_t; NULL = null stream */
 int pyr_sync(void);
 int pyr_version(void);
 
+/* Region timing with HIP events on the library stream (no reference
+ * counterpart; the reference times with pytools IntervalTimers, pusher.py:41-43,
+ * deposition/__init__.py:50-75).  Regions: 0 the fused push kernel, 1 the
+ * deposition gather kernel, 2 the element sort and staging, 3 compaction.
+ * Enabling clears earlier records; read synchronizes the device. */
+int pyr_profile_enable(int on);
+int pyr_profile_read(int region, double *total_ms, int64_t *launches);
+
 /* mesh --------------------------------------------------------------------- */
 int pyr_mesh_create(const pyr_mesh_desc *desc, pyr_mesh **out);
 int pyr_mesh_destroy(pyr_mesh *mesh);

@@ -43,6 +43,8 @@ _SIGNATURES = {
     "pyr_last_error": (C.c_char_p, []),
     "pyr_set_stream": (C.c_int, [_P]),
     "pyr_sync": (C.c_int, []),
+    "pyr_profile_enable": (C.c_int, [C.c_int]),
+    "pyr_profile_read": (C.c_int, [C.c_int, C.POINTER(_D), C.POINTER(_I64)]),
     "pyr_mesh_create": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(_P)]),
     "pyr_mesh_destroy": (C.c_int, [_P]),
     "pyr_find_containing_element": (C.c_int, [_P, _I64, _P, _P]),

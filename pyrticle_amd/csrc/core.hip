@@ -15,6 +15,36 @@ void set_error(const std::string &msg) { g_last_error = msg; }
 hipStream_t stream() { return g_stream; }
 
 // ---------------------------------------------------------------------------
+// region timing with HIP events
+// ---------------------------------------------------------------------------
+static bool g_profiling = false;
+static std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof[PROF_NREGIONS];
+static hipEvent_t g_prof_open[PROF_NREGIONS];
+
+bool profiling() { return g_profiling; }
+
+void prof_record(int region, bool begin) {
+  hipEvent_t ev;
+  if (hipEventCreate(&ev) != hipSuccess) return;
+  (void)hipEventRecord(ev, g_stream);
+  if (begin) {
+    g_prof_open[region] = ev;
+  } else {
+    g_prof[region].emplace_back(g_prof_open[region], ev);
+  }
+}
+
+static void prof_clear() {
+  for (auto &v : g_prof) {
+    for (auto &pr : v) {
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+    v.clear();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // exclusive scan of uint32 (tiles of 256 threads x 8 items)
 // ---------------------------------------------------------------------------
 constexpr int SCAN_BS = 256, SCAN_IPT = 8, SCAN_TILE = SCAN_BS * SCAN_IPT;
@@ -352,6 +382,29 @@ int pyr_set_stream(void *s) {
 
 int pyr_sync(void) {
   return guarded([&] { PYR_HIP(hipStreamSynchronize(g_stream)); });
+}
+
+int pyr_profile_enable(int on) {
+  return guarded([&] {
+    prof_clear();
+    g_profiling = on != 0;
+  });
+}
+
+int pyr_profile_read(int region, double *total_ms, int64_t *launches) {
+  return guarded([&] {
+    PYR_REQUIRE(region >= 0 && region < PROF_NREGIONS && total_ms && launches,
+                "invalid argument");
+    PYR_HIP(hipDeviceSynchronize());
+    double t = 0;
+    for (auto &pr : g_prof[region]) {
+      float ms = 0;
+      PYR_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+      t += ms;
+    }
+    *total_ms = t;
+    *launches = (int64_t)g_prof[region].size();
+  });
 }
 
 int pyr_mesh_create(const pyr_mesh_desc *d, pyr_mesh **out) {

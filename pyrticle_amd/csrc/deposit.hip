@@ -775,6 +775,7 @@ int pyr_deposit(pyr_mesh *mesh, double radius, double alpha, double normalizer, 
     c->staged.reserve(sizeof(double) * 8 * (size_t)std::max<int64_t>(1, n));
     if (n > 0) {
       PYR_REQUIRE(x && q && elem, "null pointer");
+      ProfScope prof(PROF_SORT);
       const uint32_t *perm = sort_by_element(mesh, c, n, elem);
       hipLaunchKernelGGL(k_stage, dim3(blocks_for(n)), dim3(256), 0, s, D, n, perm, x, q,
                          j ? v : nullptr, normalizer, c->staged.as<double>());
@@ -807,6 +808,7 @@ int pyr_deposit(pyr_mesh *mesh, double radius, double alpha, double normalizer, 
     ga.stats = c->stats.as<unsigned long long>();
     const bool count = g_deposit_count_mode != 0;
     if (count) PYR_HIP(hipMemsetAsync(ga.stats, 0, 4 * sizeof(unsigned long long), s));
+    ProfScope prof(PROF_GATHER);
     dispatch_gather(D, Np, ga, K, precision, alpha == 2.0, count);
   });
 }

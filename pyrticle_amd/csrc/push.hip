@@ -374,18 +374,21 @@ int pyr_lsrk4_steps(pyr_mesh *mesh, int precision, int nsteps, double dt, double
     unsigned int *n_dead = reinterpret_cast<unsigned int *>(err + 1);
     for (int step = 0; step < nsteps && n > 0; ++step) {
       PYR_HIP(hipMemsetAsync(err, 0, 2 * sizeof(int), stream()));
-      dispatch_order(D, N, [&](auto DD, auto NN) {
-        constexpr int D_ = decltype(DD)::value, N_ = decltype(NN)::value;
-        if (precision == PYR_FP64)
-          hipLaunchKernelGGL((k_lsrk4_step<D_, N_, double>), dim3(blocks_for(n)), dim3(256),
-                             0, stream(), mesh->dev, mesh->sf64.as<double>(), n, dt, vacuum_c, x,
-                             p, q, mass, elem, death, counters, err, n_dead);
-        else
-          hipLaunchKernelGGL((k_lsrk4_step<D_, N_, float>), dim3(blocks_for(n)), dim3(256),
-                             0, stream(), mesh->dev, mesh->sf32.as<float>(), n, dt, vacuum_c, x,
-                             p, q, mass, elem, death, counters, err, n_dead);
-        PYR_CHECK_LAUNCH();
-      });
+      {
+        ProfScope prof(PROF_PUSH);
+        dispatch_order(D, N, [&](auto DD, auto NN) {
+          constexpr int D_ = decltype(DD)::value, N_ = decltype(NN)::value;
+          if (precision == PYR_FP64)
+            hipLaunchKernelGGL((k_lsrk4_step<D_, N_, double>), dim3(blocks_for(n)), dim3(256),
+                               0, stream(), mesh->dev, mesh->sf64.as<double>(), n, dt, vacuum_c, x,
+                               p, q, mass, elem, death, counters, err, n_dead);
+          else
+            hipLaunchKernelGGL((k_lsrk4_step<D_, N_, float>), dim3(blocks_for(n)), dim3(256),
+                               0, stream(), mesh->dev, mesh->sf32.as<float>(), n, dt, vacuum_c, x,
+                               p, q, mass, elem, death, counters, err, n_dead);
+          PYR_CHECK_LAUNCH();
+        });
+      }
       int h[2] = {0, 0};
       PYR_HIP(hipMemcpyAsync(h, err, 2 * sizeof(int), hipMemcpyDeviceToHost, stream()));
       PYR_HIP(hipStreamSynchronize(stream()));

@@ -44,6 +44,21 @@ hipStream_t stream();
     if (!(cond)) throw ::pyr::Error(PYR_ERR_INVALID_ARG, msg);                     \
   } while (0)
 
+// Optional HIP-event timing of named kernel regions (pyr_profile_enable).
+// Events are recorded on the library stream around the region's launches.
+bool profiling();
+void prof_record(int region, bool begin);
+enum { PROF_PUSH = 0, PROF_GATHER = 1, PROF_SORT = 2, PROF_COMPACT = 3, PROF_NREGIONS = 4 };
+struct ProfScope {
+  int region;
+  explicit ProfScope(int r) : region(r) {
+    if (profiling()) prof_record(region, true);
+  }
+  ~ProfScope() {
+    if (profiling()) prof_record(region, false);
+  }
+};
+
 // Wraps an ABI entry point body: exceptions become error codes.
 template <class F>
 int guarded(F &&f) {

@@ -215,3 +215,18 @@ def apply_moves(buf: torch.Tensor, frm: torch.Tensor, to: torch.Tensor):
     row_bytes = buf.element_size() * (buf.numel() // max(1, buf.shape[0]))
     _lib.check(_lib.lib().pyr_apply_moves(_lib.ptr(buf), row_bytes, k, _lib.ptr(frm),
                                           _lib.ptr(to)))
+
+
+PROFILE_REGIONS = {"push": 0, "gather": 1, "sort": 2, "compact": 3}
+
+
+def profile_enable(on: bool = True):
+    _lib.check(_lib.lib().pyr_profile_enable(1 if on else 0))
+
+
+def profile_read(region: str):
+    """(total milliseconds, launches) recorded for a region since enabling."""
+    ms = C.c_double()
+    cnt = C.c_int64()
+    _lib.check(_lib.lib().pyr_profile_read(PROFILE_REGIONS[region], C.byref(ms), C.byref(cnt)))
+    return float(ms.value), int(cnt.value)

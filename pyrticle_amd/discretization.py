@@ -278,38 +278,19 @@ class LocalDiscretization:
 # mesh
 # ---------------------------------------------------------------------------
 class Element:
-    """One simplex: vertex ids, affine map and face geometry."""
+    """hedge-shaped view of one simplex of a :class:`Mesh` (built on demand)."""
 
     __slots__ = ("id", "vertex_indices", "map_matrix", "map_vector",
                  "inverse_matrix", "inverse_vector", "face_normals")
 
-    def __init__(self, eid, vertex_indices, points):
-        dim = points.shape[1]
-        self.id = eid
-        self.vertex_indices = tuple(int(v) for v in vertex_indices)
-        verts = points[list(self.vertex_indices)]
-        uv = UNIT_VERTICES[dim]
-        # x = A r + b, unit vertex k -> element vertex k
-        a_mat = np.column_stack([(verts[k] - verts[0]) / 2.0 for k in range(1, dim + 1)])
-        b_vec = verts[0] - a_mat @ uv[0]
-        self.map_matrix = a_mat
-        self.map_vector = b_vec
-        self.inverse_matrix = np.linalg.inv(a_mat)
-        self.inverse_vector = -self.inverse_matrix @ b_vec
-        normals = []
-        centroid = verts.mean(axis=0)
-        for fv in FACE_VERTICES[dim]:
-            fpts = verts[list(fv)]
-            if dim == 2:
-                t = fpts[1] - fpts[0]
-                n = np.array([t[1], -t[0]])
-            else:
-                n = np.cross(fpts[1] - fpts[0], fpts[2] - fpts[0])
-            n = n / np.linalg.norm(n)
-            if np.dot(n, fpts.mean(axis=0) - centroid) < 0:
-                n = -n
-            normals.append(n)
-        self.face_normals = normals
+    def __init__(self, mesh, eid):
+        self.id = int(eid)
+        self.vertex_indices = tuple(int(v) for v in mesh.element_vertices[eid])
+        self.map_matrix = mesh.map_matrix[eid]
+        self.map_vector = mesh.map_vector[eid]
+        self.inverse_matrix = mesh.inverse_matrix[eid]
+        self.inverse_vector = mesh.inverse_vector[eid]
+        self.face_normals = list(mesh.face_normals[eid])
 
     def jacobian(self) -> float:
         return float(np.linalg.det(self.map_matrix))
@@ -319,27 +300,72 @@ class Element:
         return [tuple(vertex_indices[i] for i in fv) for fv in FACE_VERTICES[dim]]
 
 
+class _ElementList:
+    def __init__(self, mesh):
+        self._mesh = mesh
+
+    def __len__(self):
+        return self._mesh.element_vertices.shape[0]
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return [Element(self._mesh, k) for k in range(*i.indices(len(self)))]
+        if i < 0:
+            i += len(self)
+        if not 0 <= i < len(self):
+            raise IndexError(i)
+        return Element(self._mesh, i)
+
+    def __iter__(self):
+        for k in range(len(self)):
+            yield Element(self._mesh, k)
+
+
 class Mesh:
-    """A conforming simplex mesh with optional periodicity.
+    """A conforming simplex mesh with optional periodicity, stored as arrays.
 
     ``periodicity[axis]`` is None or a pair of boundary tags (as in hedge); the
-    periodic extent is the bounding box along that axis.
+    periodic extent is the bounding box along that axis.  Elements are
+    reoriented to positive Jacobian.  Per element: the affine map x = A r + b
+    taking unit vertex k to element vertex k, its inverse, and outward unit face
+    normals in FACE_VERTICES order.
     """
 
     def __init__(self, points, elements, periodicity=None):
-        points = np.asarray(points, dtype=np.float64)
+        points = np.ascontiguousarray(points, dtype=np.float64)
         self.points = points
-        self.dimensions = points.shape[1]
-        dim = self.dimensions
-        els = []
-        for i, ev in enumerate(np.asarray(elements)):
-            ev = list(int(v) for v in ev)
-            verts = points[ev]
-            a = np.column_stack([verts[k] - verts[0] for k in range(1, dim + 1)])
-            if np.linalg.det(a) < 0:
-                ev[1], ev[2] = ev[2], ev[1]
-            els.append(Element(i, ev, points))
-        self.elements = els
+        self.dimensions = dim = points.shape[1]
+        ev = np.array(elements, dtype=np.int64).reshape(-1, dim + 1)
+        verts = points[ev]
+        edges = verts[:, 1:, :] - verts[:, :1, :]
+        det = np.linalg.det(np.transpose(edges, (0, 2, 1)))
+        flip = det < 0
+        ev[flip, 1], ev[flip, 2] = ev[flip, 2].copy(), ev[flip, 1].copy()
+        verts = points[ev]
+        self.element_vertices = ev
+        a = np.transpose(verts[:, 1:, :] - verts[:, :1, :], (0, 2, 1)) / 2.0
+        uv0 = UNIT_VERTICES[dim][0]
+        b = verts[:, 0, :] - np.einsum("kij,j->ki", a, uv0)
+        self.map_matrix = a
+        self.map_vector = b
+        self.inverse_matrix = np.linalg.inv(a)
+        self.inverse_vector = -np.einsum("kij,kj->ki", self.inverse_matrix, b)
+        self.jacobians = np.linalg.det(a)
+        centroid = verts.mean(axis=1)
+        normals = np.empty((ev.shape[0], dim + 1, dim))
+        for f, fv in enumerate(FACE_VERTICES[dim]):
+            fp = verts[:, list(fv), :]
+            if dim == 2:
+                t = fp[:, 1] - fp[:, 0]
+                n = np.stack([t[:, 1], -t[:, 0]], axis=1)
+            else:
+                n = np.cross(fp[:, 1] - fp[:, 0], fp[:, 2] - fp[:, 0])
+            n /= np.linalg.norm(n, axis=1)[:, None]
+            sgn = np.einsum("ki,ki->k", n, fp.mean(axis=1) - centroid)
+            n[sgn < 0] *= -1
+            normals[:, f] = n
+        self.face_normals = normals
+        self.elements = _ElementList(self)
         self.periodicity = list(periodicity) if periodicity is not None else [None] * dim
         self._build_connectivity()
 
@@ -348,70 +374,74 @@ class Mesh:
 
     def _build_connectivity(self):
         dim = self.dimensions
+        K = self.element_vertices.shape[0]
+        F = dim + 1
         lo, hi = self.bounding_box()
-        face_map = {}
-        self.neighbors = np.full((len(self.elements), dim + 1), INVALID_ELEMENT, dtype=np.uint64)
-        self.neighbor_axis = np.full((len(self.elements), dim + 1), INVALID_AXIS, dtype=np.uint64)
-        for el in self.elements:
-            for fi, fv in enumerate(el.face_vertices(el.vertex_indices)):
-                key = frozenset(fv)
-                if key in face_map:
-                    oe, of = face_map.pop(key)
-                    self.neighbors[el.id, fi] = oe
-                    self.neighbors[oe, of] = el.id
-                else:
-                    face_map[key] = (el.id, fi)
-        # periodic partners: match boundary faces whose vertices coincide after
-        # a shift by the period along one axis.
+        fv = np.stack([self.element_vertices[:, list(f)] for f in FACE_VERTICES[dim]], axis=1)
+        keys = np.sort(fv.reshape(K * F, dim), axis=1)
+        order = np.lexsort(keys.T[::-1])
+        sk = keys[order]
+        same = np.all(sk[1:] == sk[:-1], axis=1)
+        nb = np.full(K * F, INVALID_ELEMENT, dtype=np.int64)
+        ax = np.full(K * F, INVALID_AXIS, dtype=np.int64)
+        i0 = order[:-1][same]
+        i1 = order[1:][same]
+        nb[i0] = i1 // F
+        nb[i1] = i0 // F
+        matched = np.zeros(K * F, dtype=bool)
+        matched[i0] = True
+        matched[i1] = True
+        boundary = np.nonzero(~matched)[0]
         self.periodic_opposite_faces = {}
         self.periodic_opposite_vertices = {}
-        boundary = dict(face_map)
+        still_boundary = set(boundary.tolist())
         for axis, per in enumerate(self.periodicity):
             if per is None:
                 continue
             period = hi[axis] - lo[axis]
             tol = 1e-9 * max(1.0, period)
 
-            def key_of(vset, shift):
-                pts = self.points[list(vset)].copy()
+            def key_of(face_flat, shift):
+                pts = self.points[keys[face_flat]].copy()
                 pts[:, axis] += shift
-                return tuple(sorted(tuple(np.round(p / tol).astype(np.int64)) for p in pts))
+                return tuple(sorted(tuple(np.round(q / tol).astype(np.int64)) for q in pts))
 
-            low_faces = {}
-            for fkey, (eid, fi) in boundary.items():
-                pts = self.points[list(fkey)]
+            low = {}
+            for fl in boundary:
+                pts = self.points[keys[fl]]
                 if np.all(np.abs(pts[:, axis] - lo[axis]) < tol):
-                    low_faces[key_of(fkey, 0.0)] = (fkey, eid, fi)
-            for fkey, (eid, fi) in list(boundary.items()):
-                pts = self.points[list(fkey)]
+                    low[key_of(fl, 0.0)] = fl
+            for fl in boundary:
+                pts = self.points[keys[fl]]
                 if np.all(np.abs(pts[:, axis] - hi[axis]) < tol):
-                    k = key_of(fkey, -period)
-                    if k in low_faces:
-                        lkey, leid, lfi = low_faces[k]
-                        self.neighbors[eid, fi] = leid
-                        self.neighbors[leid, lfi] = eid
-                        self.neighbor_axis[eid, fi] = axis
-                        self.neighbor_axis[leid, lfi] = axis
-                        self.periodic_opposite_faces[fkey] = (lkey, axis)
-                        self.periodic_opposite_faces[lkey] = (fkey, axis)
-                        boundary.pop(fkey, None)
-                        boundary.pop(lkey, None)
-            # vertex partners on the two periodic planes
+                    k = key_of(fl, -period)
+                    if k in low:
+                        lf = low[k]
+                        nb[fl] = lf // F
+                        nb[lf] = fl // F
+                        ax[fl] = axis
+                        ax[lf] = axis
+                        a_key, b_key = tuple(keys[fl]), tuple(keys[lf])
+                        self.periodic_opposite_faces[a_key] = (b_key, axis)
+                        self.periodic_opposite_faces[b_key] = (a_key, axis)
+                        still_boundary.discard(int(fl))
+                        still_boundary.discard(int(lf))
             low_v = {}
-            for vi, p in enumerate(self.points):
-                if abs(p[axis] - lo[axis]) < tol:
-                    q = p.copy()
-                    low_v[tuple(np.round(q / tol).astype(np.int64))] = vi
-            for vi, p in enumerate(self.points):
-                if abs(p[axis] - hi[axis]) < tol:
-                    q = p.copy()
-                    q[axis] -= period
-                    k = tuple(np.round(q / tol).astype(np.int64))
-                    if k in low_v:
-                        lv = low_v[k]
-                        self.periodic_opposite_vertices.setdefault(vi, []).append((lv, axis))
-                        self.periodic_opposite_vertices.setdefault(lv, []).append((vi, axis))
-        self.boundary_faces = set(boundary.keys())
+            on_lo = np.nonzero(np.abs(self.points[:, axis] - lo[axis]) < tol)[0]
+            on_hi = np.nonzero(np.abs(self.points[:, axis] - hi[axis]) < tol)[0]
+            for vi in on_lo:
+                low_v[tuple(np.round(self.points[vi] / tol).astype(np.int64))] = int(vi)
+            for vi in on_hi:
+                q = self.points[vi].copy()
+                q[axis] -= period
+                k = tuple(np.round(q / tol).astype(np.int64))
+                if k in low_v:
+                    lv = low_v[k]
+                    self.periodic_opposite_vertices.setdefault(int(vi), []).append((lv, axis))
+                    self.periodic_opposite_vertices.setdefault(lv, []).append((int(vi), axis))
+        self.neighbors = nb.reshape(K, F).astype(np.uint64)
+        self.neighbor_axis = ax.reshape(K, F).astype(np.uint64)
+        self.boundary_faces = {frozenset(keys[fl].tolist()) for fl in still_boundary}
 
 
 @dataclass
@@ -432,14 +462,11 @@ class Discretization:
         self.order = order
         self.dimensions = mesh.dimensions
         ldis = LocalDiscretization(mesh.dimensions, order)
-        self.element_groups = [ElementGroup(ldis, list(mesh.elements))]
+        self.element_groups = [ElementGroup(ldis, mesh.elements)]
         un = ldis.unit_nodes()
         self.np = un.shape[0]
-        nodes = np.empty((len(mesh.elements) * self.np, self.dimensions))
-        for el in mesh.elements:
-            nodes[el.id * self.np:(el.id + 1) * self.np] = (
-                un @ el.map_matrix.T + el.map_vector)
-        self.nodes = nodes
+        nodes = np.einsum("kij,nj->kni", mesh.map_matrix, un) + mesh.map_vector[:, None, :]
+        self.nodes = np.ascontiguousarray(nodes.reshape(-1, self.dimensions))
 
     def __len__(self):
         return self.nodes.shape[0]
@@ -455,7 +482,7 @@ class Discretization:
 
     def integral(self, vec):
         w = self.element_groups[0].local_discretization.integral_weights()
-        jac = np.array([abs(el.jacobian()) for el in self.mesh.elements])
+        jac = np.abs(self.mesh.jacobians)
         v = np.asarray(vec).reshape(len(self.mesh.elements), self.np)
         return float(np.sum(jac[:, None] * w[None, :] * v))
 
@@ -562,9 +589,8 @@ def make_box_mesh(lo, hi, n: int, jitter: float = 0.15, seed: int = 0) -> Mesh:
 
 
 def min_edge_length(mesh: Mesh) -> float:
+    verts = mesh.points[mesh.element_vertices]
     best = np.inf
-    for el in mesh.elements:
-        v = mesh.points[list(el.vertex_indices)]
-        for i, j in combinations(range(len(v)), 2):
-            best = min(best, float(np.linalg.norm(v[i] - v[j])))
+    for i, j in combinations(range(verts.shape[1]), 2):
+        best = min(best, float(np.min(np.linalg.norm(verts[:, i] - verts[:, j], axis=1))))
     return best

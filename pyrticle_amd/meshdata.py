@@ -35,7 +35,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .discretization import INVALID_AXIS, INVALID_ELEMENT, Discretization
+from .discretization import FACE_VERTICES, INVALID_AXIS, INVALID_ELEMENT, Discretization
 
 
 @dataclass
@@ -71,7 +71,9 @@ def build_mesh_arrays(discr: Discretization) -> MeshArrays:
     """Restates ``MeshData.fill_from_hedge`` (pyrticle/meshdata.py:37-133)."""
     mesh = discr.mesh
     d = mesh.dimensions
-    K = len(mesh.elements)
+    ev = mesh.element_vertices
+    K = ev.shape[0]
+    F = d + 1
     lo, hi = mesh.bounding_box()
 
     periodicity = np.zeros((d, 2))
@@ -79,57 +81,54 @@ def build_mesh_arrays(discr: Discretization) -> MeshArrays:
         if per is not None:
             periodicity[axis] = (lo[axis], hi[axis])
 
-    inv_map = np.empty((K, d * d + d))
-    jac = np.empty(K)
-    elv = np.empty((K, d + 1), dtype=np.uint32)
-    fn = np.empty((K, d + 1, d))
-    frhs = np.empty((K, d + 1))
-    fcen = np.empty((K, d + 1, d))
-    frad = np.empty((K, d + 1))
-    fnb = np.asarray(mesh.neighbors, dtype=np.uint32)
-    fax = np.asarray(mesh.neighbor_axis, dtype=np.uint32)
+    inv_map = np.concatenate([mesh.inverse_matrix.reshape(K, d * d), mesh.inverse_vector],
+                             axis=1)
     pts = mesh.points
-    for el in mesh.elements:
-        i = el.id
-        inv_map[i, :d * d] = el.inverse_matrix.ravel()
-        inv_map[i, d * d:] = el.inverse_vector
-        jac[i] = abs(el.jacobian())
-        elv[i] = el.vertex_indices
-        for f, fvi in enumerate(el.face_vertices(el.vertex_indices)):
-            n = el.face_normals[f]
-            fv = pts[list(fvi)]
-            rhs = fv @ n
-            assert np.ptp(rhs) <= 1e-12 * max(1.0, np.abs(rhs).max()), "non-planar face"
-            fn[i, f] = n
-            frhs[i, f] = rhs.mean()
-            fcen[i, f] = fv.mean(axis=0)
-            frad[i, f] = max(np.linalg.norm(p - fcen[i, f]) for p in fv)
+    fn = mesh.face_normals
+    fvi = np.stack([ev[:, list(f)] for f in FACE_VERTICES[d]], axis=1)   # K x F x d
+    fpts = pts[fvi]                                                       # K x F x d x d
+    rhs_each = np.einsum("kfvi,kfi->kfv", fpts, fn)
+    spread = rhs_each.max(axis=2) - rhs_each.min(axis=2)
+    assert np.all(spread <= 1e-12 * np.maximum(1.0, np.abs(rhs_each).max(axis=2))), \
+        "non-planar face"
+    frhs = rhs_each.mean(axis=2)
+    fcen = fpts.mean(axis=2)
+    frad = np.linalg.norm(fpts - fcen[:, :, None, :], axis=3).max(axis=2)
 
-    adj = [set() for _ in range(len(pts))]
-    for el in mesh.elements:
-        for vi in el.vertex_indices:
-            adj[vi].add((el.id, INVALID_AXIS))
-            for other, axis in mesh.periodic_opposite_vertices.get(vi, []):
-                adj[other].add((el.id, axis))
+    # vertex -> element adjacency (meshdata.py:108-130), sorted by (element, axis)
+    v_ids = ev.ravel()
+    e_ids = np.repeat(np.arange(K), F)
+    a_ids = np.full(v_ids.shape, INVALID_AXIS, dtype=np.int64)
+    extra_v, extra_e, extra_a = [], [], []
+    for vi, partners in mesh.periodic_opposite_vertices.items():
+        els = np.nonzero((ev == vi).any(axis=1))[0]
+        for other, axis in partners:
+            extra_v.extend([other] * len(els))
+            extra_e.extend(els.tolist())
+            extra_a.extend([axis] * len(els))
+    v_all = np.concatenate([v_ids, np.asarray(extra_v, dtype=np.int64)])
+    e_all = np.concatenate([e_ids, np.asarray(extra_e, dtype=np.int64)])
+    a_all = np.concatenate([a_ids, np.asarray(extra_a, dtype=np.int64)])
+    triples = np.unique(np.stack([v_all, e_all, a_all], axis=1), axis=0)  # sorted v, e, a
+    counts = np.bincount(triples[:, 0], minlength=len(pts))
     starts = np.zeros(len(pts) + 1, dtype=np.uint32)
-    elems, axes = [], []
-    for vi in range(len(pts)):
-        for eid, axis in sorted(adj[vi]):
-            elems.append(eid)
-            axes.append(axis)
-        starts[vi + 1] = len(elems)
+    starts[1:] = np.cumsum(counts)
 
     ldis = discr.element_groups[0].local_discretization
     exps = np.asarray(ldis.node_tuples(), dtype=np.int32)
     return MeshArrays(
         dim=d, K=K, Np=discr.np, nv=len(pts),
-        inv_map=np.ascontiguousarray(inv_map), jacobian=jac,
-        el_vertices=elv, face_normal=fn, face_rhs=frhs, face_centroid=fcen,
-        face_radius=frad, face_neighbor=np.ascontiguousarray(fnb),
-        face_per_axis=np.ascontiguousarray(fax),
+        inv_map=np.ascontiguousarray(inv_map), jacobian=np.abs(mesh.jacobians),
+        el_vertices=np.ascontiguousarray(ev, dtype=np.uint32),
+        face_normal=np.ascontiguousarray(fn), face_rhs=np.ascontiguousarray(frhs),
+        face_centroid=np.ascontiguousarray(fcen), face_radius=np.ascontiguousarray(frad),
+        face_neighbor=np.ascontiguousarray(mesh.neighbors, dtype=np.uint32),
+        face_per_axis=np.ascontiguousarray(mesh.neighbor_axis, dtype=np.uint32),
         vertices=np.ascontiguousarray(pts, dtype=np.float64),
-        vadj_starts=starts, vadj_elements=np.asarray(elems, dtype=np.uint32),
-        vadj_axes=np.asarray(axes, dtype=np.uint32), periodicity=periodicity,
+        vadj_starts=starts,
+        vadj_elements=np.ascontiguousarray(triples[:, 1], dtype=np.uint32),
+        vadj_axes=np.ascontiguousarray(triples[:, 2], dtype=np.uint32),
+        periodicity=periodicity,
         nodes=np.ascontiguousarray(discr.nodes, dtype=np.float64),
         exponents=exps, vandermonde=ldis.vandermonde(),
     )
