@@ -101,26 +101,25 @@ __global__ void k_hist_keys(const uint32_t *__restrict__ keys, int64_t n,
   if (i < n) atomicAdd(&counts[min(keys[i], kmax)], 1u);
 }
 
-// staged particle record, in element-sorted order:
-// x0 x1 x2 | a = q*normalizer | b = a*v (3) | pad
+// staged particles in element-sorted order, two arrays of double4:
+//   pos[j] = (x0, x1, x2, 0)          read for every candidate visit
+//   wgt[j] = (a, a*v0, a*v1, a*v2)    read only for particles that pass,
+// with a = q * normalizer (the charge normalization stays FP64).
 __global__ void k_stage(int D, int64_t n, const uint32_t *__restrict__ perm,
                         const double *__restrict__ x, const double *__restrict__ q,
                         const double *__restrict__ v, double normalizer,
-                        double *__restrict__ staged) {
+                        double4 *__restrict__ pos, double4 *__restrict__ wgt) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const uint32_t i = perm[j];
-  double rec[8];
+  double xs[3], bs[3];
   const double a = q[i] * normalizer;
   for (int k = 0; k < 3; ++k) {
-    rec[k] = k < D ? x[(size_t)i * D + k] : 0.0;
-    rec[4 + k] = (v && k < D) ? a * v[(size_t)i * D + k] : 0.0;
+    xs[k] = k < D ? x[(size_t)i * D + k] : 0.0;
+    bs[k] = (v && k < D) ? a * v[(size_t)i * D + k] : 0.0;
   }
-  rec[3] = a;
-  rec[7] = 0.0;
-  double4 *dst = reinterpret_cast<double4 *>(staged + (size_t)j * 8);
-  dst[0] = make_double4(rec[0], rec[1], rec[2], rec[3]);
-  dst[1] = make_double4(rec[4], rec[5], rec[6], rec[7]);
+  pos[j] = make_double4(xs[0], xs[1], xs[2], 0.0);
+  wgt[j] = make_double4(a, bs[0], bs[1], bs[2]);
 }
 
 // ---------------------------------------------------------------------------
@@ -134,8 +133,9 @@ struct GatherArgs {
   const uint32_t *cand_start;       // K + 1
   const uint32_t *cand;             // source element ids
   const uint8_t *cand_shift;        // 2 bits per axis
-  const uint32_t *seg;              // K + 1 segment starts into staged
-  const double *staged;             // n x 8
+  const uint32_t *seg;              // K + 1 segment starts into pos/wgt
+  const double4 *pos;               // staged positions
+  const double4 *wgt;               // staged weights
   const double *geom;               // K x 8: centroid (3), bound radius^2, ...
   const double *face_normal;        // mesh arrays
   const double *face_rhs;
@@ -165,14 +165,19 @@ struct Kahan {
   }
 };
 
-// XCD-aware block -> element map: consecutive elements (spatial neighbours
-// sharing candidate particles) land on the same XCD and its L2.
-__device__ __forceinline__ int xcd_element(int bid, int nblocks) {
-  const int nx = 8;
-  const int per = (nblocks + nx - 1) / nx;
-  const int xcd = bid % nx, slot = bid / nx;
-  const int e = xcd * per + slot;
-  return e;
+// Block -> element map.  The hardware deals consecutive blocks round-robin to
+// the 8 XCDs; chunks of XCD_CHUNK consecutive elements (spatial neighbours that
+// share candidate particles) go to one XCD and its L2, and the chunks rotate
+// over the XCDs so that a dense region is spread over all of them.
+constexpr int XCD_CHUNK = 32;
+__device__ __forceinline__ int xcd_element(int bid) {
+  const int xcd = bid & 7, slot = bid >> 3;
+  const int chunk = (slot / XCD_CHUNK) * 8 + xcd;
+  return chunk * XCD_CHUNK + slot % XCD_CHUNK;
+}
+static inline unsigned gather_grid(int K) {
+  const int64_t per = 8 * XCD_CHUNK;
+  return (unsigned)(((int64_t)K + per - 1) / per * per);
 }
 
 template <int D, int NP, int NPW, bool FP32, bool A2, bool COUNT>
@@ -180,25 +185,29 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
   using T = typename std::conditional<FP32, float, double>::type;
   constexpr int G = (NP + NPW - 1) / NPW;   // waves per block = node groups
   constexpr int BS = 64 * G;
-  constexpr int QCAP = 64 + BS;
+  constexpr int QCAP = 64 + BS;             // ring buffer of queued particles
   constexpr int NV = D + 1;                 // rho + j components
   constexpr int REC = 8;                    // record: m (3), c, a, b (3)
 
-  const int e = xcd_element(blockIdx.x, gridDim.x);
+  const int e = xcd_element(blockIdx.x);
   if (e >= g.K) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int node0 = wave * NPW;
+  const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
 
   __shared__ T q_rec[QCAP][REC];
-  __shared__ uint32_t w_start[64], w_pre[65];
-  __shared__ uint8_t w_shift[64];
   __shared__ int wave_pass[G];
 
-  // target element geometry (uniform)
+  // target element geometry (uniform across the block)
   const double cx = g.geom[(size_t)e * 8 + 0], cy = g.geom[(size_t)e * 8 + 1],
                cz = g.geom[(size_t)e * 8 + 2], bound2 = g.geom[(size_t)e * 8 + 3];
-  const double *fn = g.face_normal + (size_t)e * (D + 1) * D;
-  const double *fr = g.face_rhs + (size_t)e * (D + 1);
+  double fnl[D + 1][D], frl[D + 1];
+#pragma unroll
+  for (int f = 0; f <= D; ++f) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) fnl[f][k] = g.face_normal[((size_t)e * (D + 1) + f) * D + k];
+    frl[f] = g.face_rhs[(size_t)e * (D + 1) + f] + g.R;
+  }
   const T *ntab = static_cast<const T *>(g.node_tab) + (size_t)e * NP * 4;
 
   T acc[NPW][NV];
@@ -211,15 +220,17 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
       if constexpr (FP32) kacc[i][v] = Kahan{0.f, 0.f};
     }
   unsigned long long n_eval = 0, n_in = 0, n_cand = 0, n_pass = 0;
-  int q_count = 0;
+  int q_head = 0, q_count = 0;   // identical in every thread of the block
   bool any = false;
 
   auto process_batch = [&](int count) {
     // one queued particle per lane, all of this wave's nodes
     if (lane < count) {
+      int slot = q_head + lane;
+      if (slot >= QCAP) slot -= QCAP;
       T rec[REC];
 #pragma unroll
-      for (int k = 0; k < REC; ++k) rec[k] = q_rec[lane][k];
+      for (int k = 0; k < REC; ++k) rec[k] = q_rec[slot][k];
 #pragma unroll
       for (int i = 0; i < NPW; ++i) {
         if (node0 + i < NP) {
@@ -255,97 +266,85 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
 
   const uint32_t c_begin = g.cand_start[e], c_end = g.cand_start[e + 1];
   for (uint32_t cw = c_begin; cw < c_end; cw += 64) {
-    // candidate window: 64 source elements, their segments and shifts
-    uint32_t len = 0, start = 0;
-    uint8_t sc = 0;
+    // window of 64 candidate source elements: segment starts, lengths, shifts
+    uint32_t len = 0, start = 0, sc = 0;
     if (cw + lane < c_end) {
       const uint32_t src = g.cand[cw + lane];
       start = g.seg[src];
       len = g.seg[src + 1] - start;
       sc = g.cand_shift[cw + lane];
     }
-    uint32_t incl = len;
+    uint64_t nonempty = __ballot(len > 0);
+    while (nonempty) {
+      const int k = __ffsll((unsigned long long)nonempty) - 1;
+      nonempty &= nonempty - 1;
+      const uint32_t s0 = __builtin_amdgcn_readlane(start, k);
+      const uint32_t sl = __builtin_amdgcn_readlane(len, k);
+      const uint32_t code = __builtin_amdgcn_readlane(sc, k);
+      double sh[3] = {0, 0, 0};
+      if (code) shift_vec<D>(g, code, sh);
+      for (uint32_t base = 0; base < sl; base += BS) {
+        const uint32_t idx = base + tid;
+        bool pass = false;
+        T rec[REC];
+        if (idx < sl) {
+          const double4 pp = g.pos[s0 + idx];
+          const double x[3] = {pp.x + sh[0], pp.y + sh[1], D == 3 ? pp.z + sh[2] : 0.0};
+          const double d[3] = {x[0] - cx, x[1] - cy, x[2] - cz};
+          const double dd = d[0] * d[0] + d[1] * d[1] + (D == 3 ? d[2] * d[2] : 0.0);
+          pass = dd <= bound2;
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      uint32_t t = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += t;
-    }
-    const uint32_t total = __shfl(incl, 63, 64);
-    if (total == 0) continue;   // uniform across the block
-    __syncthreads();
-    if (wave == 0) {
-      w_start[lane] = start;
-      w_pre[lane] = incl - len;
-      w_shift[lane] = sc;
-      if (lane == 63) w_pre[64] = incl;
-    }
-    __syncthreads();
-    for (uint32_t f0 = 0; f0 < total; f0 += BS) {
-      const uint32_t f = f0 + tid;
-      bool pass = false;
-      T rec[REC];
-      if (f < total) {
-        // candidate k holding flat index f: last k with w_pre[k] <= f
-        int lo = 0, hi = 63;
-        while (lo < hi) {
-          int mid = (lo + hi + 1) >> 1;
-          if (w_pre[mid] <= f) lo = mid; else hi = mid - 1;
+          for (int f = 0; f <= D; ++f) {
+            double ip = 0;
+#pragma unroll
+            for (int kk = 0; kk < D; ++kk) ip += fnl[f][kk] * x[kk];
+            pass = pass && (ip <= frl[f]);
+          }
+          if constexpr (COUNT) n_cand += 1;
+          if (pass) {
+            if constexpr (COUNT) n_pass += 1;
+            const double4 ww = g.wgt[s0 + idx];
+            rec[0] = (T)(2.0 * d[0] * g.invR);
+            rec[1] = (T)(2.0 * d[1] * g.invR);
+            rec[2] = (T)(2.0 * d[2] * g.invR);
+            rec[3] = (T)(g.R - dd * g.invR);
+            rec[4] = (T)ww.x;
+            rec[5] = (T)ww.y;
+            rec[6] = (T)ww.z;
+            rec[7] = (T)ww.w;
+          }
         }
-        const uint32_t pidx = w_start[lo] + (f - w_pre[lo]);
-        double sh[3] = {0, 0, 0};
-        if (w_shift[lo]) shift_vec<D>(g, w_shift[lo], sh);
-        const double4 *src = reinterpret_cast<const double4 *>(g.staged + (size_t)pidx * 8);
-        const double4 r0 = src[0], r1 = src[1];
-        double x[3] = {r0.x + sh[0], r0.y + sh[1], D == 3 ? r0.z + sh[2] : 0.0};
-        double d[3] = {x[0] - cx, x[1] - cy, x[2] - cz};
-        double dd = d[0] * d[0] + d[1] * d[1] + (D == 3 ? d[2] * d[2] : 0.0);
-        pass = dd <= bound2;
-#pragma unroll
-        for (int f2 = 0; f2 <= D; ++f2) {
-          double ip = 0;
-#pragma unroll
-          for (int k = 0; k < D; ++k) ip += fn[f2 * D + k] * x[k];
-          pass = pass && (ip - fr[f2] <= g.R);
+        const uint64_t bal = __ballot(pass);
+        if (G > 1) {
+          if (lane == 0) wave_pass[wave] = __popcll(bal);
+          __syncthreads();
         }
-        if constexpr (COUNT) n_cand += 1;
+        int off = q_count, added = 0;
+        if (G > 1) {
+#pragma unroll
+          for (int w = 0; w < G; ++w) {
+            if (w < wave) off += wave_pass[w];
+            added += wave_pass[w];
+          }
+        } else {
+          added = __popcll(bal);
+        }
         if (pass) {
-          if constexpr (COUNT) n_pass += 1;
-          rec[0] = (T)(2.0 * d[0] * g.invR);
-          rec[1] = (T)(2.0 * d[1] * g.invR);
-          rec[2] = (T)(2.0 * d[2] * g.invR);
-          rec[3] = (T)(g.R - dd * g.invR);
-          rec[4] = (T)r0.w;
-          rec[5] = (T)r1.x;
-          rec[6] = (T)r1.y;
-          rec[7] = (T)r1.z;
+          int slot = q_head + off + __popcll(bal & lt_mask);
+          if (slot >= QCAP) slot -= QCAP;
+#pragma unroll
+          for (int kk = 0; kk < REC; ++kk) q_rec[slot][kk] = rec[kk];
         }
-      }
-      const uint64_t bal = __ballot(pass);
-      const int my_pass = __popcll(bal);
-      if (lane == 0) wave_pass[wave] = my_pass;
-      __syncthreads();
-      int off = q_count;
-      for (int w = 0; w < wave; ++w) off += wave_pass[w];
-      int added = 0;
-      for (int w = 0; w < G; ++w) added += wave_pass[w];
-      if (pass) {
-        const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-        const int slot = off + __popcll(bal & lt);
-#pragma unroll
-        for (int k = 0; k < REC; ++k) q_rec[slot][k] = rec[k];
-      }
-      q_count += added;
-      __syncthreads();
-      while (q_count >= 64) {
-        any = true;
-        process_batch(64);
+        q_count += added;
         __syncthreads();
-        // move the tail of the queue to the front
-        for (int s = tid; s < q_count - 64; s += BS)
-#pragma unroll
-          for (int k = 0; k < REC; ++k) q_rec[s][k] = q_rec[s + 64][k];
-        q_count -= 64;
-        __syncthreads();
+        while (q_count >= 64) {
+          any = true;
+          process_batch(64);
+          q_head += 64;
+          if (q_head >= QCAP) q_head -= QCAP;
+          q_count -= 64;
+          __syncthreads();   // the freed slots may be refilled next
+        }
       }
     }
   }
@@ -364,11 +363,11 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
     for (int v = 0; v < NV; ++v) {
       double val;
       if constexpr (FP32)
-        val = any ? (double)kacc[i][v].s + (double)kacc[i][v].c : 0.0;
+        val = any ? (double)kacc[i][v].s - (double)kacc[i][v].c : 0.0;
       else
         val = any ? (double)acc[i][v] : 0.0;
       if (any)
-        for (int off = 32; off > 0; off >>= 1) val += __shfl_xor(val, off, 64);
+        for (int o = 32; o > 0; o >>= 1) val += __shfl_xor(val, o, 64);
       out[v] = val;
     }
     if (lane == 0) {
@@ -384,7 +383,7 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       unsigned long long v = vals[k];
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
       if (lane == 0 && v) atomicAdd(g.stats + k, v);
     }
   }
@@ -422,7 +421,7 @@ struct DepositCache {
   DevBuf cand_start, cand, cand_shift, geom, tab64, tab32, tab_R;
   double tab_radius = -1;
   // sort / staging scratch
-  DevBuf keys_a, keys_b, vals_a, vals_b, hist, hist_scan, counts, seg, tmp, staged, stats;
+  DevBuf keys_a, keys_b, vals_a, vals_b, hist, hist_scan, counts, seg, tmp, pos, wgt, stats;
   std::vector<double> h_geom;   // K x 8
 };
 
@@ -431,7 +430,7 @@ void pyr_deposit_cache_free(pyr_mesh *mesh) {
   DepositCache *c = mesh->dep;
   for (DevBuf *b : {&c->cand_start, &c->cand, &c->cand_shift, &c->geom, &c->tab64, &c->tab32,
                     &c->tab_R, &c->keys_a, &c->keys_b, &c->vals_a, &c->vals_b, &c->hist,
-                    &c->hist_scan, &c->counts, &c->seg, &c->tmp, &c->staged, &c->stats})
+                    &c->hist_scan, &c->counts, &c->seg, &c->tmp, &c->pos, &c->wgt, &c->stats})
     b->release();
   delete c;
   mesh->dep = nullptr;
@@ -697,8 +696,7 @@ static const uint32_t *sort_by_element(pyr_mesh *h, DepositCache *c, int64_t n,
 template <int D, int NP, int NPW>
 static void launch_gather(const GatherArgs &ga, int K, int precision, bool a2, bool count) {
   constexpr int G = (NP + NPW - 1) / NPW;
-  // the XCD-aware element map needs a multiple of 8 blocks
-  dim3 grid((unsigned)(((int64_t)K + 7) / 8 * 8)), block(64 * G);
+  dim3 grid(gather_grid(K)), block(64 * G);
   hipStream_t s = stream();
 #define PYR_G(FP, AA, CC) \
   hipLaunchKernelGGL((k_gather<D, NP, NPW, FP, AA, CC>), grid, block, 0, s, ga)
@@ -772,13 +770,15 @@ int pyr_deposit(pyr_mesh *mesh, double radius, double alpha, double normalizer, 
       PYR_CHECK_LAUNCH();
       c->tab_radius = radius;
     }
-    c->staged.reserve(sizeof(double) * 8 * (size_t)std::max<int64_t>(1, n));
+    c->pos.reserve(sizeof(double4) * (size_t)std::max<int64_t>(1, n));
+    c->wgt.reserve(sizeof(double4) * (size_t)std::max<int64_t>(1, n));
     if (n > 0) {
       PYR_REQUIRE(x && q && elem, "null pointer");
       ProfScope prof(PROF_SORT);
       const uint32_t *perm = sort_by_element(mesh, c, n, elem);
       hipLaunchKernelGGL(k_stage, dim3(blocks_for(n)), dim3(256), 0, s, D, n, perm, x, q,
-                         j ? v : nullptr, normalizer, c->staged.as<double>());
+                         j ? v : nullptr, normalizer, c->pos.as<double4>(),
+                         c->wgt.as<double4>());
       PYR_CHECK_LAUNCH();
     } else {
       c->seg.reserve(sizeof(uint32_t) * ((size_t)K + 1));
@@ -797,7 +797,8 @@ int pyr_deposit(pyr_mesh *mesh, double radius, double alpha, double normalizer, 
     ga.cand = c->cand.as<uint32_t>();
     ga.cand_shift = c->cand_shift.as<uint8_t>();
     ga.seg = c->seg.as<uint32_t>();
-    ga.staged = c->staged.as<double>();
+    ga.pos = c->pos.as<double4>();
+    ga.wgt = c->wgt.as<double4>();
     ga.geom = c->geom.as<double>();
     ga.face_normal = mesh->dev.face_normal;
     ga.face_rhs = mesh->dev.face_rhs;

@@ -198,7 +198,8 @@ def test_deposit_2d_matches_oracle(square, oracle_square, dmesh2, cuda, precisio
     assert rel_l2(j.cpu().numpy(), j_o) < tol
 
 
-def test_deposit_3d_matches_oracle(small_cylinder, dmesh3, cuda):
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_deposit_3d_matches_oracle(small_cylinder, dmesh3, cuda, precision):
     from pyrticle_amd import device as dv
     wl = small_cylinder
     om = O.OracleMesh(wl.ma)
@@ -208,13 +209,36 @@ def test_deposit_3d_matches_oracle(small_cylinder, dmesh3, cuda):
     v = O.velocities(wl.p[:n], wl.m[:n], wl.c)
     rho_o, j_o, _ = O.deposit(om, wl.radius, wl.alpha, x, q, v, el)
     nrm = O.shape_normalizer(wl.radius, 3, wl.alpha)
+    prec = dv.FP64 if precision == "fp64" else dv.FP32C
+    tol = FP64_TOL if precision == "fp64" else FP32C_TOL
     rho, j = dmesh3.deposit(wl.radius, wl.alpha, nrm, _dev(x, cuda), _dev(q, cuda),
-                            dv.element_tensor(el, cuda), v=_dev(v, cuda))
-    assert rel_l2(rho.cpu().numpy(), rho_o) < FP64_TOL
-    assert rel_l2(j.cpu().numpy(), j_o) < FP64_TOL
+                            dv.element_tensor(el, cuda), v=_dev(v, cuda), precision=prec)
+    assert rel_l2(rho.cpu().numpy(), rho_o) < tol
+    assert rel_l2(j.cpu().numpy(), j_o) < tol
     rho2, j2 = dmesh3.deposit(wl.radius, wl.alpha, nrm, _dev(x, cuda), _dev(q, cuda),
-                              dv.element_tensor(el, cuda), v=_dev(v, cuda))
+                              dv.element_tensor(el, cuda), v=_dev(v, cuda), precision=prec)
     assert torch.equal(rho, rho2) and torch.equal(j, j2)     # deterministic
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_deposit_dense_beam_matches_oracle(small_cylinder, dmesh3, cuda, precision):
+    """Thousands of particles per element: the gather's LDS queue runs deep."""
+    from pyrticle_amd import device as dv
+    from pyrticle_amd.workloads import cylinder_beam
+    wl = small_cylinder
+    om = O.OracleMesh(wl.ma)
+    x, p, q, m = cylinder_beam(120_000, seed=5, axes=(0.08, 0.06), sigma_z=0.05)
+    el = dv.element_numpy(dmesh3.find_containing_element(_dev(x, cuda)))
+    assert np.bincount(el).max() > 2000
+    v = O.velocities(p, m, wl.c)
+    rho_o, j_o, _ = O.deposit(om, wl.radius, wl.alpha, x, q, v, el)
+    nrm = O.shape_normalizer(wl.radius, 3, wl.alpha)
+    prec = dv.FP64 if precision == "fp64" else dv.FP32C
+    rho, j = dmesh3.deposit(wl.radius, wl.alpha, nrm, _dev(x, cuda), _dev(q, cuda),
+                            dv.element_tensor(el, cuda), v=_dev(v, cuda), precision=prec)
+    tol = FP64_TOL if precision == "fp64" else FP32C_TOL
+    assert rel_l2(rho.cpu().numpy(), rho_o) < tol
+    assert rel_l2(j.cpu().numpy(), j_o) < tol
 
 
 @pytest.mark.parametrize("dt_scale", [1.0, 4.0])
