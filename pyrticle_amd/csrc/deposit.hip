@@ -21,6 +21,7 @@
 // meets, so the gather and the reference's face walk sum the same terms.
 #include <algorithm>
 #include <atomic>
+#include <cstdlib>
 #include <cmath>
 #include <thread>
 
@@ -95,10 +96,22 @@ __global__ __launch_bounds__(RS_BS) void k_rs_scatter(const uint32_t *__restrict
 
 // counts[K] collects particles with an invalid element; they sort last and
 // no segment references them.
+// Runs of equal keys inside a wave add their length with one atomic (element-
+// sorted input has long runs; unsorted input degrades to one atomic per lane).
 __global__ void k_hist_keys(const uint32_t *__restrict__ keys, int64_t n,
                             uint32_t *__restrict__ counts, uint32_t kmax) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) atomicAdd(&counts[min(keys[i], kmax)], 1u);
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool valid = i < n;
+  const uint32_t k = valid ? min(keys[i], kmax) : 0xFFFFFFFFu;
+  const uint32_t left = __shfl_up(k, 1, 64);
+  const bool head = valid && (lane == 0 || left != k);
+  const uint64_t heads = __ballot(head) | (~__ballot(valid));   // invalid lanes end runs
+  if (head) {
+    const uint64_t after = lane == 63 ? 0ull : (heads >> (lane + 1)) << (lane + 1);
+    const int next = after ? __ffsll((unsigned long long)after) - 1 : 64;
+    atomicAdd(&counts[k], (uint32_t)(next - lane));
+  }
 }
 
 // staged particles in element-sorted order, two arrays of double4:
@@ -191,7 +204,8 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
 
   const int e = xcd_element(blockIdx.x);
   if (e >= g.K) return;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // uniform per wave
   const int node0 = wave * NPW;
   const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
 
@@ -206,9 +220,11 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
   for (int f = 0; f <= D; ++f) {
 #pragma unroll
     for (int k = 0; k < D; ++k) fnl[f][k] = g.face_normal[((size_t)e * (D + 1) + f) * D + k];
-    frl[f] = g.face_rhs[(size_t)e * (D + 1) + f] + g.R;
+    frl[f] = g.face_rhs[(size_t)e * (D + 1) + f] + g.R * (1.0 + 1e-12);
   }
-  const T *ntab = static_cast<const T *>(g.node_tab) + (size_t)e * NP * 4;
+  // node data through the constant address space: uniform -> scalar loads
+  using CT = const T __attribute__((address_space(4))) *;
+  const CT ntab = (CT)(static_cast<const T *>(g.node_tab)) + (size_t)e * NP * 4;
 
   T acc[NPW][NV];
   Kahan kacc[FP32 ? NPW : 1][FP32 ? NV : 1];
@@ -234,7 +250,7 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
 #pragma unroll
       for (int i = 0; i < NPW; ++i) {
         if (node0 + i < NP) {
-          const T *nd = ntab + (size_t)(node0 + i) * 4;
+          const CT nd = ntab + (size_t)(node0 + i) * 4;
           T t = rec[3] - nd[3];
 #pragma unroll
           for (int k = D - 1; k >= 0; --k) t = fma(nd[k], rec[k], t);
@@ -291,23 +307,26 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
           const double4 pp = g.pos[s0 + idx];
           const double x[3] = {pp.x + sh[0], pp.y + sh[1], D == 3 ? pp.z + sh[2] : 0.0};
           const double d[3] = {x[0] - cx, x[1] - cy, x[2] - cz};
-          const double dd = d[0] * d[0] + d[1] * d[1] + (D == 3 ? d[2] * d[2] : 0.0);
+          const double dd = fma(d[0], d[0], fma(d[1], d[1], D == 3 ? d[2] * d[2] : 0.0));
           pass = dd <= bound2;
 #pragma unroll
           for (int f = 0; f <= D; ++f) {
-            double ip = 0;
+            // conservative: the ball of radius R misses the element if x lies
+            // beyond one of its face planes by more than R
+            double ip = fnl[f][0] * x[0];
 #pragma unroll
-            for (int kk = 0; kk < D; ++kk) ip += fnl[f][kk] * x[kk];
+            for (int kk = 1; kk < D; ++kk) ip = fma(fnl[f][kk], x[kk], ip);
             pass = pass && (ip <= frl[f]);
           }
           if constexpr (COUNT) n_cand += 1;
           if (pass) {
             if constexpr (COUNT) n_pass += 1;
             const double4 ww = g.wgt[s0 + idx];
-            rec[0] = (T)(2.0 * d[0] * g.invR);
-            rec[1] = (T)(2.0 * d[1] * g.invR);
-            rec[2] = (T)(2.0 * d[2] * g.invR);
-            rec[3] = (T)(g.R - dd * g.invR);
+            const double two_inv_r = 2.0 * g.invR;
+            rec[0] = (T)(d[0] * two_inv_r);
+            rec[1] = (T)(d[1] * two_inv_r);
+            rec[2] = (T)(d[2] * two_inv_r);
+            rec[3] = (T)fma(-dd, g.invR, g.R);
             rec[4] = (T)ww.x;
             rec[5] = (T)ww.y;
             rec[6] = (T)ww.z;
@@ -717,6 +736,13 @@ static void launch_gather(const GatherArgs &ga, int K, int precision, bool a2, b
   PYR_CHECK_LAUNCH();
 }
 
+// Nodes per wave of the 3D gather (launch-shape tuning; the result does not
+// depend on it beyond summation order across waves, which is fixed).
+static int gather_npw(int dflt) {
+  const char *env = std::getenv("PYR_GATHER_NPW");
+  return env ? std::atoi(env) : dflt;
+}
+
 static void dispatch_gather(int D, int Np, const GatherArgs &ga, int K, int precision, bool a2,
                             bool count) {
   if (D == 2) {
@@ -730,8 +756,19 @@ static void dispatch_gather(int D, int Np, const GatherArgs &ga, int K, int prec
     switch (Np) {
       case 4: return launch_gather<3, 4, 4>(ga, K, precision, a2, count);
       case 10: return launch_gather<3, 10, 10>(ga, K, precision, a2, count);
-      case 20: return launch_gather<3, 20, 10>(ga, K, precision, a2, count);
-      case 35: return launch_gather<3, 35, 12>(ga, K, precision, a2, count);
+      case 20:
+        switch (gather_npw(10)) {
+          case 5: return launch_gather<3, 20, 5>(ga, K, precision, a2, count);
+          case 7: return launch_gather<3, 20, 7>(ga, K, precision, a2, count);
+          case 20: return launch_gather<3, 20, 20>(ga, K, precision, a2, count);
+          default: return launch_gather<3, 20, 10>(ga, K, precision, a2, count);
+        }
+      case 35:
+        switch (gather_npw(12)) {
+          case 7: return launch_gather<3, 35, 7>(ga, K, precision, a2, count);
+          case 9: return launch_gather<3, 35, 9>(ga, K, precision, a2, count);
+          default: return launch_gather<3, 35, 12>(ga, K, precision, a2, count);
+        }
     }
   }
   throw Error(PYR_ERR_INVALID_ARG, "deposition supports P1..P4 (2D) and P1..P4 (3D) elements");

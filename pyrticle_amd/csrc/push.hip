@@ -62,14 +62,22 @@ __device__ __forceinline__ void monomials(const T *r, T *phi) {
   }
 }
 
+// Read-only data seen through the constant address space: loads with a
+// wave-uniform address become scalar loads into SGPRs.
+template <class T>
+using const_ptr = const T __attribute__((address_space(4))) *;
+template <class T>
+__device__ __forceinline__ const_ptr<T> as_constant(const T *p) {
+  return (const_ptr<T>)p;
+}
+
 // six interpolated field values at unit coordinates r in element en
-template <int D, int N, class T>
-__device__ __forceinline__ void interp6(const T *__restrict__ coef, uint32_t en, const T *r,
-                                        T *f6) {
+template <int D, int N, class T, class P>
+__device__ __forceinline__ void interp6(P coef, uint32_t en, const T *r, T *f6) {
   constexpr int NP = Basis<D, N>::NP;
   T phi[NP];
   monomials<D, N, T>(r, phi);
-  const T *c = coef + (size_t)en * 6 * NP;
+  const P c = coef + (size_t)en * 6 * NP;
 #pragma unroll
   for (int f = 0; f < 6; ++f) {
     T acc = T(0);
@@ -77,6 +85,24 @@ __device__ __forceinline__ void interp6(const T *__restrict__ coef, uint32_t en,
     for (int k = 0; k < NP; ++k) acc = fma(c[f * NP + k], phi[k], acc);
     f6[f] = acc;
   }
+}
+
+// The element of every active lane, if the wave sits in one element (the
+// common case for element-sorted particles), else INVALID.  A uniform element
+// lets per-element data come through scalar loads instead of per-lane ones.
+__device__ __forceinline__ uint32_t wave_uniform_element(uint32_t e) {
+  const uint32_t e0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)e);
+  return __ballot(e != e0) == 0 ? e0 : INVALID;
+}
+
+template <int D, int N, class T>
+__device__ __forceinline__ void interp6_any(const T *__restrict__ coef, uint32_t e,
+                                            uint32_t e_uni, const T *r, T *f6) {
+  if (e_uni != INVALID)
+    interp6<D, N, T>(as_constant(coef), (uint32_t)__builtin_amdgcn_readfirstlane((int)e_uni),
+                     r, f6);
+  else
+    interp6<D, N, T>(coef, e, r, f6);
 }
 
 // push_monomial.hpp:265-305 with tools.hpp:253-264 cross: F = q e + v x (q b),
@@ -199,9 +225,10 @@ __global__ __launch_bounds__(256) void k_lsrk4_step(
     for (int s = 0; s < 5 && !died; ++s) {
       double v[D], F[D];
       velocity<D>(pi, mi, c, v);
+      const uint32_t e_uni = wave_uniform_element(e);
       if constexpr (sizeof(T) == 8) {
         double f6[6], el[3], mag[3];
-        interp6<D, N, double>(coef, e, r, f6);
+        interp6_any<D, N, double>(coef, e, e_uni, r, f6);
         lorentz<D, double>(qi, f6, v, F, el, mag);
       } else {
         float rf[D], f6[6], vf[D], Ff[D], el[3], mag[3];
@@ -210,7 +237,7 @@ __global__ __launch_bounds__(256) void k_lsrk4_step(
           rf[k] = (float)r[k];
           vf[k] = (float)v[k];
         }
-        interp6<D, N, float>(coef, e, rf, f6);
+        interp6_any<D, N, float>(coef, e, e_uni, rf, f6);
         lorentz<D, float>((float)qi, f6, vf, Ff, el, mag);
 #pragma unroll
         for (int k = 0; k < D; ++k) F[k] = (double)Ff[k];
@@ -226,7 +253,23 @@ __global__ __launch_bounds__(256) void k_lsrk4_step(
         xi[k] += b * rx[k];
         pi[k] += b * rp[k];
       }
-      if (!relocate<D>(m, xi, pi, e, cnt, e_flag, r)) {
+      // fast path: still inside the (wave-uniform) element, same arithmetic
+      // as the first branch of find_new_containing_element
+      bool same = false;
+      if (e_uni != INVALID) {
+        const uint32_t eu = (uint32_t)__builtin_amdgcn_readfirstlane((int)e_uni);
+        const_ptr<double> a = as_constant(m.inv_map) + (size_t)eu * (D * D + D);
+#pragma unroll
+        for (int ii = 0; ii < D; ++ii) {
+          double t = 0;
+#pragma unroll
+          for (int jj = 0; jj < D; ++jj) t += a[ii * D + jj] * xi[jj];
+          r[ii] = t + a[D * D + ii];
+        }
+        same = in_unit_simplex<D>(r);
+        if (same) cnt.same++;
+      }
+      if (!same && !relocate<D>(m, xi, pi, e, cnt, e_flag, r)) {
         died = (uint32_t)s + 1;
         break;
       }
