@@ -1,0 +1,416 @@
+"""The names the reference's Python layer imports from ``pyrticle._internal``
+(src/wrapper/wrap_*.cpp), implemented over libpyrticle_mi355x.so.
+
+Particle arrays are torch CUDA tensors (positions/momenta n x d float64,
+charges/masses float64, containing elements int32 holding uint32 bits); every
+hot-path call goes through the C ABI.  Differences from the Boost.Python
+module, all deliberate:
+
+* ``update_containing_elements`` reports kills as data: it compacts the state
+  in the reference's order itself and notifies the number-shift listener with
+  one batched ``note_move`` (index tensors) and ``note_change_size`` instead of
+  calling a boundary-hit listener per particle (particle_state.hpp:266-357,
+  cloud.py:581-585);
+* ``deposit_*`` accept ``slice(None)`` or a contiguous ``slice(start, stop)``;
+* ``lu`` and the monomial basis objects keep their reference meaning for
+  callers, but the GPU interpolation uses the mesh's Vandermonde matrix
+  directly (same interpolant).
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from . import device as dv
+from .discretization import INVALID_AXIS as _INVALID_AXIS
+from .meshdata import build_mesh_arrays
+
+INVALID_ELEMENT = dv.INVALID_ELEMENT
+
+
+# ---------------------------------------------------------------------------
+# listeners (wrap_tools.cpp:45-120, 220-262; tools.hpp:422-508)
+# ---------------------------------------------------------------------------
+class NumberShiftListener:
+    def note_change_size(self, new_size):
+        pass
+
+    def note_move(self, orig, dest, size):
+        pass
+
+    def note_reset(self, start, size):
+        pass
+
+
+class BoundaryHitListener:
+    def note_boundary_hit(self, pn):
+        pass
+
+
+class VisualizationListener:
+    def store_mesh_vis_vector(self, name, vec):
+        pass
+
+    def store_particle_vis_vector(self, name, vec):
+        pass
+
+
+class WarningListener:
+    _singleton = None
+
+    def __init__(self):
+        WarningListener._singleton = self
+
+    def note_warning(self, message, filename, lineno):
+        pass
+
+    @staticmethod
+    def warn(message, filename="<pyrticle_amd>", lineno=0):
+        if WarningListener._singleton is None:
+            raise RuntimeError("warning with no listener: %s" % message)
+        WarningListener._singleton.note_warning(message, filename, lineno)
+
+
+class ZeroVector:
+    """Stands for an all-zero field component (push_monomial.hpp:128-132)."""
+
+
+class FindEventCounters:
+    """particle_state.hpp:86-92; counters live on the device."""
+
+    def __init__(self):
+        self._t = torch.zeros(4, dtype=torch.int64, device="cuda")
+
+    @property
+    def tensor(self):
+        return self._t
+
+    find_same = property(lambda self: int(self._t[0]))
+    find_by_neighbor = property(lambda self: int(self._t[1]))
+    find_by_vertex = property(lambda self: int(self._t[2]))
+    find_global = property(lambda self: int(self._t[3]))
+
+
+# ---------------------------------------------------------------------------
+# mesh data (wrap_meshdata.cpp:50-114, meshdata.py:37-152)
+# ---------------------------------------------------------------------------
+class PeriodicityAxis:
+    def __init__(self, min=0.0, max=0.0):
+        self.min = min
+        self.max = max
+
+
+class MeshData:
+    INVALID_ELEMENT = INVALID_ELEMENT
+    INVALID_AXIS = _INVALID_AXIS
+
+    def __init__(self, dimensions):
+        self.dimensions = dimensions
+        self.discr = None
+        self.arrays = None
+        self.device = None
+        self.periodicities = []
+
+    @staticmethod
+    def get_INVALID_ELEMENT():
+        return INVALID_ELEMENT
+
+    def fill_from_hedge(self, discr):
+        """meshdata.py:37-133, against the discretization shim."""
+        self.discr = discr
+        self.arrays = build_mesh_arrays(discr)
+        self.periodicities = [PeriodicityAxis(*row) for row in self.arrays.periodicity]
+        self.device = dv.DeviceMesh(self.arrays)
+
+    def node_count(self):
+        return self.arrays.nnodes
+
+    def find_containing_element(self, pt):
+        x = torch.as_tensor(np.asarray(pt, dtype=np.float64).reshape(1, -1), device="cuda")
+        return int(dv.element_numpy(self.device.find_containing_element(x))[0])
+
+    def find_containing_elements(self, x: torch.Tensor) -> torch.Tensor:
+        return self.device.find_containing_element(x)
+
+    def is_in_element(self, en, pt):
+        x = torch.as_tensor(np.asarray(pt, dtype=np.float64).reshape(1, -1), device="cuda")
+        e = dv.element_tensor(np.array([en], dtype=np.uint32), "cuda")
+        return bool(self.device.is_in_element(e, x)[0])
+
+    def min_vertex_distance_for_el(self, en):
+        v = self.arrays.vertices[self.arrays.el_vertices[en]]
+        d = np.linalg.norm(v[:, None, :] - v[None, :, :], axis=2)
+        return float(d[~np.eye(len(v), dtype=bool)].min())
+
+    def _min_vertex_distances(self):
+        v = self.arrays.vertices[self.arrays.el_vertices]
+        d = np.linalg.norm(v[:, :, None, :] - v[:, None, :, :], axis=3)
+        n = v.shape[1]
+        d[:, np.arange(n), np.arange(n)] = np.inf
+        return d.reshape(len(v), -1).min(axis=1)
+
+    def advisable_particle_radius(self):
+        """meshdata.py:144-148."""
+        dists = np.sort(self._min_vertex_distances())
+        return 0.6 * float(dists[int(0.25 * len(dists))])
+
+    def min_vertex_distance(self):
+        return float(self._min_vertex_distances().min())
+
+
+# ---------------------------------------------------------------------------
+# particle state (particle_state.hpp:42-81, wrap_pic.cpp:53-69)
+# ---------------------------------------------------------------------------
+class _ParticleState:
+    xdim = 0
+    vdim = 0
+
+    def __init__(self):
+        self.particle_count = 0
+        dev = "cuda"
+        self.containing_elements = torch.zeros(0, dtype=torch.int32, device=dev)
+        self.positions = torch.zeros((0, self.xdim), dtype=torch.float64, device=dev)
+        self.momenta = torch.zeros((0, self.vdim), dtype=torch.float64, device=dev)
+        self.charges = torch.zeros(0, dtype=torch.float64, device=dev)
+        self.masses = torch.zeros(0, dtype=torch.float64, device=dev)
+
+
+class ParticleState2d2v(_ParticleState):
+    xdim = 2
+    vdim = 2
+
+
+class ParticleState3d3v(_ParticleState):
+    xdim = 3
+    vdim = 3
+
+
+def get_velocities(ps, vacuum_c):
+    """particle_state.hpp:98-125."""
+    n = ps.particle_count
+    return dv.velocities(ps.momenta[:n].contiguous(), ps.masses[:n].contiguous(), vacuum_c)
+
+
+def _state_arrays(ps):
+    return (ps.positions, ps.momenta, ps.charges, ps.masses, ps.containing_elements)
+
+
+def update_containing_elements(mesh_data, ps, nshift_listener=None, counters=None):
+    """particle_state.hpp:266-357 with the kill listener of cloud.py:581-585.
+
+    Relocalizes all particles, wraps periodic ones, and removes the particles
+    that left the mesh in the reference's swap-remove order.  The listener
+    receives the batched moves (from, to) and the new count.  Returns the new
+    particle count.
+    """
+    n = ps.particle_count
+    if n == 0:
+        return 0
+    x = ps.positions[:n]
+    dead = mesh_data.device.locate(x, ps.momenta[:n], ps.containing_elements[:n],
+                                   counters.tensor if counters is not None else None)
+    n_new, frm, to = dv.compact_plan(dead)
+    if frm.numel():
+        for arr in _state_arrays(ps):
+            dv.apply_moves(arr, frm, to)
+        if nshift_listener is not None:
+            nshift_listener.note_move(frm, to, 1)
+    if n_new != n:
+        ps.particle_count = n_new
+        if nshift_listener is not None:
+            nshift_listener.note_change_size(n_new)
+    return n_new
+
+
+def move_particle(ps, frm, to, nshift_listener=None):
+    """particle_state.hpp:386-407."""
+    f = torch.tensor([frm], dtype=torch.int32, device="cuda")
+    t = torch.tensor([to], dtype=torch.int32, device="cuda")
+    for arr in _state_arrays(ps):
+        dv.apply_moves(arr, f, t)
+    if nshift_listener is not None:
+        nshift_listener.note_move(f, t, 1)
+
+
+def kill_particle(ps, pn, nshift_listener=None):
+    """particle_state.hpp:361-377."""
+    ps.particle_count -= 1
+    if ps.particle_count:
+        move_particle(ps, ps.particle_count, pn, nshift_listener)
+    if nshift_listener is not None:
+        nshift_listener.note_change_size(ps.particle_count)
+
+
+def find_new_containing_element(mesh_data, ps, pn, prev, counters=None):
+    """particle_state.hpp:130-261 for one particle (no kill)."""
+    x = ps.positions[pn:pn + 1].clone()
+    e = ps.containing_elements[pn:pn + 1].clone()
+    e[0] = torch.tensor(np.array([prev], dtype=np.uint32).view(np.int32)[0])
+    dead = mesh_data.device.locate(x, ps.momenta[pn:pn + 1].contiguous(), e,
+                                   counters.tensor if counters is not None else None)
+    return INVALID_ELEMENT if int(dead[0]) else int(dv.element_numpy(e)[0])
+
+
+# ---------------------------------------------------------------------------
+# shape function (tools.hpp:514-556, tools.cpp:35-58, wrap_tools.cpp:282-295)
+# ---------------------------------------------------------------------------
+class PolynomialShapeFunction:
+    def __init__(self, radius=1.0, dimensions=1, alpha=2.0):
+        self.radius = float(radius)
+        self.dimensions = int(dimensions)
+        self.exponent = float(alpha)
+        n = float(dimensions)
+        sphere_area = 2 * math.pi ** (n / 2) / math.gamma(n / 2)
+        beta = math.gamma(n / 2) * math.gamma(alpha + 1) / math.gamma(n / 2 + alpha + 1)
+        self.normalizer = 2 / (sphere_area * radius ** (n + alpha) * beta)
+
+    @staticmethod
+    def name():
+        return "polynomial"
+
+    def __call__(self, r):
+        r2 = float(np.dot(r, r))
+        if r2 > self.radius ** 2:
+            return 0.0
+        t = self.radius - r2 / self.radius
+        return self.normalizer * t ** self.exponent
+
+
+def get_shape_function_name():
+    return PolynomialShapeFunction.name()
+
+
+# ---------------------------------------------------------------------------
+# depositor (wrap_depositor.cpp:45-126, dep_target.hpp:203-273)
+# ---------------------------------------------------------------------------
+class _InterpolatingDepositor:
+    class DepositorState:
+        def note_move(self, orig, dest, size):
+            pass
+
+        def note_change_size(self, count):
+            pass
+
+        def clear(self):
+            pass
+
+    def __init__(self, mesh_data):
+        self.mesh_data = mesh_data
+        self.shape_function = None
+        self.precision = dv.FP64
+
+
+class InterpolatingDepositor2d2v(_InterpolatingDepositor):
+    pass
+
+
+class InterpolatingDepositor3d3v(_InterpolatingDepositor):
+    pass
+
+
+def _slice_range(pslice, n):
+    if pslice is None:
+        return 0, n
+    start, stop, step = pslice.indices(n)
+    if step != 1:
+        raise ValueError("only contiguous particle slices are supported")
+    return start, stop
+
+
+def _deposit(dep, ps, node_count, velocities, pslice, want_rho, want_j):
+    sf = dep.shape_function
+    if sf is None:
+        raise RuntimeError("shape function never set")
+    md = dep.mesh_data
+    if node_count != md.node_count():
+        raise ValueError("node count does not match the mesh")
+    a, b = _slice_range(pslice, ps.particle_count)
+    x = ps.positions[a:b]
+    v = velocities[a:b].contiguous() if velocities is not None else None
+    rho, j = md.device.deposit(sf.radius, sf.exponent, sf.normalizer, x, ps.charges[a:b],
+                               ps.containing_elements[a:b], v=v, want_rho=want_rho,
+                               want_j=want_j, precision=dep.precision)
+    return rho, j
+
+
+def deposit_rho(dep, ds, ps, node_count, pslice=slice(None)):
+    return _deposit(dep, ps, node_count, None, pslice, True, False)[0]
+
+
+def deposit_j(dep, ds, ps, node_count, velocities, pslice=slice(None)):
+    """j as node_count x vdim (node-major, as the reference returns it)."""
+    return _deposit(dep, ps, node_count, velocities, pslice, False, True)[1]
+
+
+def deposit_densities(dep, ds, ps, node_count, velocities, pslice=slice(None)):
+    return _deposit(dep, ps, node_count, velocities, pslice, True, True)
+
+
+# ---------------------------------------------------------------------------
+# pusher (wrap_pusher.cpp:85-171, push_monomial.hpp:45-325)
+# ---------------------------------------------------------------------------
+class MonomialBasisFunction:
+    def __init__(self, *exponents):
+        self.exponents = tuple(int(e) for e in exponents)
+
+    def __call__(self, v):
+        return float(np.prod([vi ** e for vi, e in zip(v, self.exponents)]))
+
+
+class LocalMonomialDiscretization:
+    def __init__(self):
+        self.basis = []
+        self.lu_vandermonde_t = None
+        self.lu_piv_vandermonde_t = None
+
+
+def lu(a):
+    """wrap_tools.cpp:126-145 (getrf): (LU, pivots), column-major LAPACK form."""
+    from scipy.linalg import lu_factor
+    a = np.asarray(a, dtype=np.float64)
+    lu_, piv = lu_factor(a, check_finite=True)
+    if np.any(np.diag(lu_) == 0):
+        raise RuntimeError("singular matrix in getrf")
+    return np.asfortranarray(lu_), piv.astype(np.int32) + 1
+
+
+class _MonomialPusher:
+    def __init__(self, mesh_data):
+        self.mesh_data = mesh_data
+        self.local_discretizations = []
+        self.ldis_indices = []
+
+    def forces(self, ex, ey, ez, bx, by, bz, ps, velocities, vis_listener=None):
+        if len(self.local_discretizations) > 1:
+            raise RuntimeError("more than one local discretization is currently not "
+                               "supported")
+        n = ps.particle_count
+        fields = [None if (f is None or isinstance(f, ZeroVector)) else f
+                  for f in (ex, ey, ez, bx, by, bz)]
+        res = self.mesh_data.device.forces(
+            fields, ps.positions[:n], ps.containing_elements[:n], ps.charges[:n],
+            velocities[:n].contiguous(), vis=vis_listener is not None)
+        if vis_listener is not None:
+            forces, vis = res
+            for name in ("pt_e", "pt_b", "el_force", "mag_force"):
+                vis_listener.store_particle_vis_vector(name, vis[name])
+            return forces
+        return res
+
+
+class MonomialPusher2d2v(_MonomialPusher):
+    pass
+
+
+class MonomialPusher3d3v(_MonomialPusher):
+    pass
+
+
+_SUFFIXES = {"2d2v": 2, "3d3v": 3}
+
+
+def class_for(prefix, suffix):
+    return globals()[prefix + suffix]

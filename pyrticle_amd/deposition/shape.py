@@ -1,0 +1,40 @@
+"""Shape-function depositor (pyrticle/deposition/shape.py:31-53)."""
+
+from __future__ import annotations
+
+from .. import _internal
+from .. import device as dv
+from . import Depositor
+
+
+class ShapeFunctionDepositor(Depositor):
+    """Backed by InterpolatingDepositor{2d2v,3d3v} on the GPU.
+
+    ``precision`` selects the plain FP64 form (the drop-in, default) or the
+    FP32-compensated form ("fp32c")."""
+
+    name = "Shape"
+
+    def __init__(self, precision="fp64"):
+        Depositor.__init__(self)
+        self.precision = dv.FP64 if precision == "fp64" else dv.FP32C
+
+    def initialize(self, method):
+        Depositor.initialize(self, method)
+        backend_class = _internal.class_for("InterpolatingDepositor",
+                                            method.get_dimensionality_suffix())
+        self.backend = backend_class(method.mesh_data)
+        self.backend.precision = self.precision
+
+    def make_state(self, state):
+        return self.backend.DepositorState()
+
+    def set_shape_function(self, state, sf):
+        Depositor.set_shape_function(self, state, sf)
+        self.backend.shape_function = sf
+
+    def note_move(self, state, orig, dest, size):
+        pass
+
+    def note_change_size(self, state, count):
+        pass
