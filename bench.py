@@ -5,8 +5,8 @@ One step = one fused LSRK4 push (5 stages, relocalization and kills after each
 stage) followed by one rho + j shape-function deposition, on the cylinder
 workload: 200,910 P3 tetrahedra (Np=20), 4M electrons, R=0.08, alpha=2, static
 B_z=0.5 T and E_z=1e6 V/m, dt=0.2*h_min/c.  Inputs are resident in HBM before
-the timed region.  Prints one JSON line (see README of the benchmark contract
-in DESIGN.md for the fields).
+the timed region.  Prints one JSON line: metric, value (particle-steps/s),
+ms_per_step, roofline of the dominant kernel and a CPU baseline.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3]
 """
@@ -109,15 +109,15 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--particles", type=int, default=4_000_000)
-    ap.add_argument("--cpu-sample", type=int, default=40_000)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-sample", type=int, default=200_000)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32c", action="store_true")
     args = ap.parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py runs on one GPU only: the particle path here is built and measured "
-              "on a single MI355X (multi-GPU sharding is described in DESIGN.md, not run).",
+              "on a single MI355X; multi-GPU sharding is not part of this build.",
               file=sys.stderr)
         return 2
 

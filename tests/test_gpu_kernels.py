@@ -243,7 +243,7 @@ def test_deposit_dense_beam_matches_oracle(small_cylinder, dmesh3, cuda, precisi
 
 @pytest.mark.parametrize("dt_scale", [1.0, 4.0])
 def test_lsrk4_square_matches_oracle(square, oracle_square, dmesh2, cuda, dt_scale):
-    """Config 1: 10k electrons, 100 LSRK4 steps with relocalization; with a 4x
+    """Square workload, 10k electrons, 100 LSRK4 steps with relocalization; with a 4x
     time step particles leave the square and the kill order is exercised."""
     import dataclasses
     from pyrticle_amd import device as dv
