@@ -82,7 +82,8 @@ int pyr_version(void);
 /* Region timing with HIP events on the library stream (no reference
  * counterpart; the reference times with pytools IntervalTimers, pusher.py:41-43,
  * deposition/__init__.py:50-75).  Regions: 0 the fused push kernel, 1 the
- * deposition gather kernel, 2 the element sort and staging, 3 compaction.
+ * deposition gather kernel, 2 the element sort and staging, 3 compaction,
+ * 4 the push adjoint, 5 the deposition adjoint.
  * Enabling clears earlier records; read synchronizes the device. */
 int pyr_profile_enable(int on);
 int pyr_profile_read(int region, double *total_ms, int64_t *launches);
@@ -162,6 +163,37 @@ int pyr_lsrk4_steps(pyr_mesh *mesh, int precision, int nsteps, double dt, double
                     double *masses, uint32_t *containing_elements, int64_t *ids,
                     unsigned long long *counters);
 
+/* adjoint of the push ------------------------------------------------------ */
+/* No reference counterpart (pyrticle has no adjoint); these differentiate the
+ * pyr_lsrk4_steps dynamics for torch.autograd (pyrticle_amd/adjoint.py).
+ *
+ * pyr_lsrk4_forward: nsteps LSRK4 steps as pyr_lsrk4_steps computes them, but
+ * without compaction: a particle that leaves the mesh keeps its slot with
+ * death[i] = 5*step + stage (1-based; 0 = alive) and containing_elements[i] =
+ * INVALID.  If ckpt_xp/ckpt_elem are given, the state at the start of every
+ * ckpt_every-th step is stored there: ckpt_xp[(seg*n + i)*2*dim + k] holds x
+ * (k < dim) then p, for seg = 0 .. ceil(nsteps/ckpt_every)-1. */
+int pyr_lsrk4_forward(pyr_mesh *mesh, int precision, int nsteps, int ckpt_every, double dt,
+                      double vacuum_c, int64_t n, double *positions, double *momenta,
+                      const double *charges, const double *masses,
+                      uint32_t *containing_elements, uint32_t *death, double *ckpt_xp,
+                      uint32_t *ckpt_elem, unsigned long long *counters);
+
+/* Reverse pass of pyr_lsrk4_forward: on entry x_bar/p_bar (n x dim, device)
+ * hold dL/dx and dL/dp of the final state, on return those of the initial
+ * state.  Segments are recomputed from the checkpoints (library scratch of
+ * ckpt_every*n*(16*dim+4) + 5*n*(16*dim+4) bytes).  Particles with death != 0
+ * get zero gradients.  precision PYR_FP32C evaluates fields and field
+ * gradients in FP32 and accumulates the adjoint in FP64. */
+int pyr_lsrk4_adjoint(pyr_mesh *mesh, int precision, int nsteps, int ckpt_every, double dt,
+                      double vacuum_c, int64_t n, const double *charges, const double *masses,
+                      const uint32_t *death, const double *ckpt_xp, const uint32_t *ckpt_elem,
+                      double *x_bar, double *p_bar);
+
+/* Adjoint of pyr_velocities: p_bar = (dv/dp)^T v_bar (overwritten). */
+int pyr_velocities_backward(int64_t n, int vdim, double vacuum_c, const double *momenta,
+                            const double *masses, const double *v_bar, double *p_bar);
+
 /* deposition ---------------------------------------------------------------- */
 /* dep_target.hpp:203-273 deposit_densities / deposit_j / deposit_rho
  * (wrap_depositor.cpp:45-50, over slice(None)) for InterpolatingDepositor
@@ -177,6 +209,17 @@ int pyr_deposit(pyr_mesh *mesh, double radius, double alpha, double normalizer,
                 int precision, int64_t n, int vdim, const double *positions,
                 const double *charges, const double *velocities,
                 const uint32_t *containing_elements, double *rho_out, double *j_out);
+
+/* Adjoint of pyr_deposit (no reference counterpart): given rho_bar = dL/drho
+ * (K*Np, may be NULL) and j_bar = dL/dj (K*Np x vdim, may be NULL), writes
+ * x_bar = dL/dx (n x vdim) and, with j_bar, v_bar = dL/dv (n x vdim); both are
+ * overwritten, and zero for particles with an invalid element.  A gather per
+ * particle over the elements its shape reaches; no atomics.  alpha >= 1. */
+int pyr_deposit_backward(pyr_mesh *mesh, double radius, double alpha, double normalizer,
+                         int precision, int64_t n, int vdim, const double *positions,
+                         const double *charges, const double *velocities,
+                         const uint32_t *containing_elements, const double *rho_bar,
+                         const double *j_bar, double *x_bar, double *v_bar);
 
 /* Instrumentation (no reference counterpart): when on, pyr_deposit also counts
  * [node-particle pairs evaluated, pairs inside R, candidate particle visits,

@@ -63,6 +63,13 @@ _SIGNATURES = {
     "pyr_deposit_stats": (C.c_int, [_P, C.POINTER(C.c_ulonglong * 4)]),
     "pyr_deposit_candidates": (C.c_int, [_P, _D, C.POINTER(_I64)]),
     "pyr_sort_by_element": (C.c_int, [_P, _I64, _P, _P, _P]),
+    "pyr_lsrk4_forward": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _D, _D, _I64, _P, _P, _P,
+                                    _P, _P, _P, _P, _P, _P]),
+    "pyr_lsrk4_adjoint": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _D, _D, _I64, _P, _P, _P,
+                                    _P, _P, _P, _P]),
+    "pyr_velocities_backward": (C.c_int, [_I64, C.c_int, _D, _P, _P, _P, _P]),
+    "pyr_deposit_backward": (C.c_int, [_P, _D, _D, _D, C.c_int, _I64, C.c_int, _P, _P, _P, _P,
+                                       _P, _P, _P, _P]),
 }
 
 _lock = threading.Lock()

@@ -352,6 +352,7 @@ pyr_mesh::~pyr_mesh() {
   s_tmp.release();
   s_holes.release();
   s_misc.release();
+  s_adj.release();
   counters.release();
   sf64.release();
   sf32.release();

@@ -427,6 +427,144 @@ __global__ void k_node_table(int D, int K, int np, const double *__restrict__ no
   out[(size_t)t * 4 + 3] = (T)(s * invR);
 }
 
+// ---------------------------------------------------------------------------
+// adjoint of the deposition: per-particle gather of grad S . (rho_bar, j_bar)
+// ---------------------------------------------------------------------------
+// For particle p with a = q N and the nodes i within R:
+//   rho_i = sum_p a S_ip,  j_ia = sum_p a v_pa S_ip,  S = w^alpha, w = R - |n_i - x_p|^2 / R
+//   x_bar_p = (2/R) sum_i alpha w^(alpha-1) c_ip (n_i - x_p),  c_ip = a rho_bar_i + sum_a a v_pa j_bar_ia
+//   v_bar_pa = a sum_i j_bar_ia S_ip
+// Outputs are per particle, so a wave takes 64 particles of one source element
+// and walks the target elements that element can reach (the transpose of the
+// gather's candidate lists); target node data and adjoint densities are wave-
+// uniform and come through scalar loads.  No atomics.
+struct AdjArgs {
+  int K;
+  double R, invR, alpha;
+  double shift_len[3];
+  const uint32_t *tr_start;   // K + 1, per source element
+  const uint32_t *tr;         // target elements
+  const uint8_t *tr_shift;    // shift code applied to the source particle
+  const uint32_t *seg;        // K + 1 segment starts into pos/wgt
+  const uint32_t *perm;       // sorted position -> particle index
+  const double4 *pos, *wgt;
+  const double *geom;
+  const void *node_tab;
+  const double *rho_bar;      // K*Np or null
+  const double *j_bar;        // K*Np x D or null
+  double *x_bar, *v_bar;      // n x D (v_bar only with j_bar)
+};
+
+template <int D, int NP, bool FP32, bool A2>
+__global__ __launch_bounds__(64) void k_deposit_adj(AdjArgs g) {
+  using T = typename std::conditional<FP32, float, double>::type;
+  using CT = const T __attribute__((address_space(4))) *;
+  using CD = const double __attribute__((address_space(4))) *;
+  const int s = xcd_element(blockIdx.x);
+  if (s >= g.K) return;
+  const uint32_t p0 = g.seg[s], p1 = g.seg[s + 1];
+  if (p0 >= p1) return;
+  const int lane = threadIdx.x;
+  const uint32_t t_begin = g.tr_start[s], t_end = g.tr_start[s + 1];
+  const double two_inv_r = 2.0 * g.invR;
+  const CT ntab_all = (CT)(static_cast<const T *>(g.node_tab));
+  const bool has_rho = g.rho_bar != nullptr, has_j = g.j_bar != nullptr;
+  for (uint32_t base = p0; base < p1; base += 64) {
+    const uint32_t jj = base + lane;
+    const bool valid = jj < p1;
+    double4 pp = make_double4(0, 0, 0, 0), ww = make_double4(0, 0, 0, 0);
+    if (valid) {
+      pp = g.pos[jj];
+      ww = g.wgt[jj];
+    }
+    const T wa = (T)ww.x;
+    const T wv[3] = {(T)ww.y, (T)ww.z, (T)ww.w};
+    double gx[D], gv[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      gx[k] = 0;
+      gv[k] = 0;
+    }
+    for (uint32_t ti = t_begin; ti < t_end; ++ti) {
+      const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.tr[ti]);
+      const uint32_t code = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.tr_shift[ti]);
+      double sh[3] = {0, 0, 0};
+      if (code) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          const uint32_t cc = (code >> (2 * a)) & 3u;
+          sh[a] = cc == 1 ? g.shift_len[a] : (cc == 2 ? -g.shift_len[a] : 0.0);
+        }
+      }
+      const CD gt = (CD)g.geom + (size_t)t * 8;
+      const double d[3] = {pp.x + sh[0] - gt[0], pp.y + sh[1] - gt[1],
+                           D == 3 ? pp.z + sh[2] - gt[2] : 0.0};
+      const double dd = fma(d[0], d[0], fma(d[1], d[1], D == 3 ? d[2] * d[2] : 0.0));
+      const bool pass = valid && dd <= gt[3];
+      if (__ballot(pass) == 0) continue;
+      const T m[3] = {(T)(d[0] * two_inv_r), (T)(d[1] * two_inv_r), (T)(d[2] * two_inv_r)};
+      const T c0 = (T)fma(-dd, g.invR, g.R);
+      const CT nt = ntab_all + (size_t)t * NP * 4;
+      const CD rb = (CD)g.rho_bar + (size_t)t * NP;
+      const CD jb = (CD)g.j_bar + (size_t)t * NP * D;
+      T sD = T(0), sDn[D], sS[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        sDn[k] = T(0);
+        sS[k] = T(0);
+      }
+#pragma unroll
+      for (int i = 0; i < NP; ++i) {
+        T tt = c0 - nt[i * 4 + 3];
+#pragma unroll
+        for (int k = D - 1; k >= 0; --k) tt = fma(nt[i * 4 + k], m[k], tt);
+        const T w = (pass && tt > T(0)) ? tt : T(0);
+        T cp = has_rho ? wa * (T)rb[i] : T(0);
+        T jbi[D];
+        if (has_j) {
+#pragma unroll
+          for (int a = 0; a < D; ++a) {
+            jbi[a] = (T)jb[i * D + a];
+            cp = fma(wv[a], jbi[a], cp);
+          }
+        }
+        T dS, S;
+        if constexpr (A2) {
+          dS = w;   // dS/dw = 2w: the 2 is applied once at the end
+          S = w * w;
+        } else {
+          dS = w > T(0) ? (T)(g.alpha * pow((double)w, g.alpha - 1.0)) : T(0);
+          S = w > T(0) ? (T)pow((double)w, g.alpha) : T(0);
+        }
+        const T Dv = dS * cp;
+        sD += Dv;
+#pragma unroll
+        for (int k = 0; k < D; ++k) sDn[k] = fma(Dv, nt[i * 4 + k], sDn[k]);
+        if (has_j) {
+#pragma unroll
+          for (int a = 0; a < D; ++a) sS[a] = fma(jbi[a], S, sS[a]);
+        }
+      }
+      // fold the target's partial sums in FP64: sum_i D_i (n_i - d)
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        gx[k] += (double)sDn[k] - d[k] * (double)sD;
+        gv[k] += (double)sS[k];
+      }
+    }
+    if (valid) {
+      const uint32_t idx = g.perm[jj];
+      const double fx = two_inv_r * (A2 ? 2.0 : 1.0);
+#pragma unroll
+      for (int k = 0; k < D; ++k) g.x_bar[(size_t)idx * D + k] = fx * gx[k];
+      if (g.v_bar) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) g.v_bar[(size_t)idx * D + k] = ww.x * gv[k];
+      }
+    }
+  }
+}
+
 }  // namespace pyr
 
 using namespace pyr;
@@ -439,6 +577,9 @@ struct DepositCache {
   int64_t n_cand = 0;
   DevBuf cand_start, cand, cand_shift, geom, tab64, tab32, tab_R;
   double tab_radius = -1;
+  // transposed lists (per source element) for the adjoint
+  DevBuf tr_start, tr, tr_shift;
+  double tr_radius = -1;
   // sort / staging scratch
   DevBuf keys_a, keys_b, vals_a, vals_b, hist, hist_scan, counts, seg, tmp, pos, wgt, stats;
   std::vector<double> h_geom;   // K x 8
@@ -448,7 +589,8 @@ void pyr_deposit_cache_free(pyr_mesh *mesh) {
   if (!mesh || !mesh->dep) return;
   DepositCache *c = mesh->dep;
   for (DevBuf *b : {&c->cand_start, &c->cand, &c->cand_shift, &c->geom, &c->tab64, &c->tab32,
-                    &c->tab_R, &c->keys_a, &c->keys_b, &c->vals_a, &c->vals_b, &c->hist,
+                    &c->tab_R, &c->tr_start, &c->tr, &c->tr_shift, &c->keys_a, &c->keys_b,
+                    &c->vals_a, &c->vals_b, &c->hist,
                     &c->hist_scan, &c->counts, &c->seg, &c->tmp, &c->pos, &c->wgt, &c->stats})
     b->release();
   delete c;
@@ -774,6 +916,91 @@ static void dispatch_gather(int D, int Np, const GatherArgs &ga, int K, int prec
   throw Error(PYR_ERR_INVALID_ARG, "deposition supports P1..P4 (2D) and P1..P4 (3D) elements");
 }
 
+// candidate lists and node tables for radius R (cached per radius)
+static void prepare_tables(pyr_mesh *mesh, DepositCache *c, double radius) {
+  const int K = mesh->K, Np = mesh->Np, D = mesh->dim;
+  hipStream_t s = stream();
+  if (c->radius != radius) build_candidates(mesh, c, radius);
+  if (c->tab_radius != radius) {
+    c->tab64.reserve(sizeof(double) * (size_t)K * Np * 4);
+    c->tab32.reserve(sizeof(float) * (size_t)K * Np * 4);
+    hipLaunchKernelGGL(k_node_table<double>, dim3(blocks_for((int64_t)K * Np)), dim3(256), 0, s,
+                       D, K, Np, mesh->dev.nodes, c->geom.as<double>(), 1.0 / radius,
+                       c->tab64.as<double>());
+    hipLaunchKernelGGL(k_node_table<float>, dim3(blocks_for((int64_t)K * Np)), dim3(256), 0, s,
+                       D, K, Np, mesh->dev.nodes, c->geom.as<double>(), 1.0 / radius,
+                       c->tab32.as<float>());
+    PYR_CHECK_LAUNCH();
+    c->tab_radius = radius;
+  }
+}
+
+// Transpose of the candidate lists: for each source element, the target
+// elements whose list holds it, in ascending target order, with the same
+// shift code (the shift moves the source particle next to the target).
+static void build_transpose(pyr_mesh *mesh, DepositCache *c) {
+  const int K = mesh->K;
+  const size_t nc = (size_t)c->n_cand;
+  std::vector<uint32_t> start((size_t)K + 1), cand(std::max<size_t>(1, nc));
+  std::vector<uint8_t> code(std::max<size_t>(1, nc));
+  PYR_HIP(hipMemcpy(start.data(), c->cand_start.ptr, start.size() * 4, hipMemcpyDeviceToHost));
+  if (nc) {
+    PYR_HIP(hipMemcpy(cand.data(), c->cand.ptr, nc * 4, hipMemcpyDeviceToHost));
+    PYR_HIP(hipMemcpy(code.data(), c->cand_shift.ptr, nc, hipMemcpyDeviceToHost));
+  }
+  std::vector<uint32_t> tstart((size_t)K + 1, 0);
+  for (size_t k = 0; k < nc; ++k) tstart[cand[k] + 1]++;
+  for (int e = 0; e < K; ++e) tstart[e + 1] += tstart[e];
+  std::vector<uint32_t> fill(tstart.begin(), tstart.end() - 1);
+  std::vector<uint32_t> tr(std::max<size_t>(1, nc));
+  std::vector<uint8_t> tcode(std::max<size_t>(1, nc));
+  for (int e = 0; e < K; ++e)
+    for (uint32_t k = start[e]; k < start[e + 1]; ++k) {
+      const uint32_t src = cand[k];
+      const uint32_t at = fill[src]++;
+      tr[at] = (uint32_t)e;
+      tcode[at] = code[k];
+    }
+  upload(c->tr_start, tstart.data(), tstart.size());
+  upload(c->tr, tr.data(), tr.size());
+  upload(c->tr_shift, tcode.data(), tcode.size());
+  c->tr_radius = c->radius;
+}
+
+template <int D, int NP>
+static void launch_deposit_adj(const AdjArgs &ga, int K, int precision, bool a2) {
+  dim3 grid(gather_grid(K)), block(64);
+  hipStream_t s = stream();
+  if (precision == PYR_FP64) {
+    if (a2) hipLaunchKernelGGL((k_deposit_adj<D, NP, false, true>), grid, block, 0, s, ga);
+    else hipLaunchKernelGGL((k_deposit_adj<D, NP, false, false>), grid, block, 0, s, ga);
+  } else {
+    if (a2) hipLaunchKernelGGL((k_deposit_adj<D, NP, true, true>), grid, block, 0, s, ga);
+    else hipLaunchKernelGGL((k_deposit_adj<D, NP, true, false>), grid, block, 0, s, ga);
+  }
+  PYR_CHECK_LAUNCH();
+}
+
+static void dispatch_deposit_adj(int D, int Np, const AdjArgs &ga, int K, int precision,
+                                 bool a2) {
+  if (D == 2) {
+    switch (Np) {
+      case 3: return launch_deposit_adj<2, 3>(ga, K, precision, a2);
+      case 6: return launch_deposit_adj<2, 6>(ga, K, precision, a2);
+      case 10: return launch_deposit_adj<2, 10>(ga, K, precision, a2);
+      case 15: return launch_deposit_adj<2, 15>(ga, K, precision, a2);
+    }
+  } else {
+    switch (Np) {
+      case 4: return launch_deposit_adj<3, 4>(ga, K, precision, a2);
+      case 10: return launch_deposit_adj<3, 10>(ga, K, precision, a2);
+      case 20: return launch_deposit_adj<3, 20>(ga, K, precision, a2);
+      case 35: return launch_deposit_adj<3, 35>(ga, K, precision, a2);
+    }
+  }
+  throw Error(PYR_ERR_INVALID_ARG, "deposition supports P1..P4 (2D) and P1..P4 (3D) elements");
+}
+
 static int g_deposit_count_mode = 0;
 
 extern "C" {
@@ -794,19 +1021,7 @@ int pyr_deposit(pyr_mesh *mesh, double radius, double alpha, double normalizer, 
     DepositCache *c = mesh->dep;
     const int K = mesh->K, Np = mesh->Np, D = mesh->dim;
     hipStream_t s = stream();
-    if (c->radius != radius) build_candidates(mesh, c, radius);
-    if (c->tab_radius != radius) {
-      c->tab64.reserve(sizeof(double) * (size_t)K * Np * 4);
-      c->tab32.reserve(sizeof(float) * (size_t)K * Np * 4);
-      hipLaunchKernelGGL(k_node_table<double>, dim3(blocks_for((int64_t)K * Np)), dim3(256), 0,
-                         s, D, K, Np, mesh->dev.nodes, c->geom.as<double>(), 1.0 / radius,
-                         c->tab64.as<double>());
-      hipLaunchKernelGGL(k_node_table<float>, dim3(blocks_for((int64_t)K * Np)), dim3(256), 0,
-                         s, D, K, Np, mesh->dev.nodes, c->geom.as<double>(), 1.0 / radius,
-                         c->tab32.as<float>());
-      PYR_CHECK_LAUNCH();
-      c->tab_radius = radius;
-    }
+    prepare_tables(mesh, c, radius);
     c->pos.reserve(sizeof(double4) * (size_t)std::max<int64_t>(1, n));
     c->wgt.reserve(sizeof(double4) * (size_t)std::max<int64_t>(1, n));
     if (n > 0) {
@@ -848,6 +1063,64 @@ int pyr_deposit(pyr_mesh *mesh, double radius, double alpha, double normalizer, 
     if (count) PYR_HIP(hipMemsetAsync(ga.stats, 0, 4 * sizeof(unsigned long long), s));
     ProfScope prof(PROF_GATHER);
     dispatch_gather(D, Np, ga, K, precision, alpha == 2.0, count);
+  });
+}
+
+int pyr_deposit_backward(pyr_mesh *mesh, double radius, double alpha, double normalizer,
+                         int precision, int64_t n, int vdim, const double *x, const double *q,
+                         const double *v, const uint32_t *elem, const double *rho_bar,
+                         const double *j_bar, double *x_bar, double *v_bar) {
+  return guarded([&] {
+    PYR_REQUIRE(mesh, "null mesh");
+    PYR_REQUIRE(radius > 0 && std::isfinite(radius), "shape radius must be positive");
+    PYR_REQUIRE(alpha >= 1, "the adjoint needs a shape exponent >= 1");
+    PYR_REQUIRE(precision == PYR_FP64 || precision == PYR_FP32C, "unknown precision");
+    PYR_REQUIRE(vdim == mesh->dim, "vdim must equal the mesh dimension");
+    PYR_REQUIRE(n >= 0 && n < (int64_t)0x7FFFFFFF, "invalid particle count");
+    PYR_REQUIRE(!j_bar || (v && v_bar), "j_bar requires velocities and v_bar");
+    if (n == 0) return;
+    PYR_REQUIRE(x && q && elem && x_bar, "null pointer");
+    if (!mesh->dep) mesh->dep = new DepositCache();
+    DepositCache *c = mesh->dep;
+    const int K = mesh->K, Np = mesh->Np, D = mesh->dim;
+    hipStream_t s = stream();
+    PYR_HIP(hipMemsetAsync(x_bar, 0, sizeof(double) * (size_t)n * D, s));
+    if (v_bar) PYR_HIP(hipMemsetAsync(v_bar, 0, sizeof(double) * (size_t)n * D, s));
+    if (!rho_bar && !j_bar) return;
+    prepare_tables(mesh, c, radius);
+    if (c->tr_radius != radius) build_transpose(mesh, c);
+    c->pos.reserve(sizeof(double4) * (size_t)n);
+    c->wgt.reserve(sizeof(double4) * (size_t)n);
+    const uint32_t *perm;
+    {
+      ProfScope prof(PROF_SORT);
+      perm = sort_by_element(mesh, c, n, elem);
+      hipLaunchKernelGGL(k_stage, dim3(blocks_for(n)), dim3(256), 0, s, D, n, perm, x, q,
+                         j_bar ? v : nullptr, normalizer, c->pos.as<double4>(),
+                         c->wgt.as<double4>());
+      PYR_CHECK_LAUNCH();
+    }
+    AdjArgs ga{};
+    ga.K = K;
+    ga.R = radius;
+    ga.invR = 1.0 / radius;
+    ga.alpha = alpha;
+    for (int a = 0; a < 3; ++a) ga.shift_len[a] = mesh->dev.per_max[a] - mesh->dev.per_min[a];
+    ga.tr_start = c->tr_start.as<uint32_t>();
+    ga.tr = c->tr.as<uint32_t>();
+    ga.tr_shift = c->tr_shift.as<uint8_t>();
+    ga.seg = c->seg.as<uint32_t>();
+    ga.perm = perm;
+    ga.pos = c->pos.as<double4>();
+    ga.wgt = c->wgt.as<double4>();
+    ga.geom = c->geom.as<double>();
+    ga.node_tab = precision == PYR_FP64 ? (const void *)c->tab64.ptr : (const void *)c->tab32.ptr;
+    ga.rho_bar = rho_bar;
+    ga.j_bar = j_bar;
+    ga.x_bar = x_bar;
+    ga.v_bar = j_bar ? v_bar : nullptr;
+    ProfScope prof(PROF_DEPOSIT_ADJ);
+    dispatch_deposit_adj(D, Np, ga, K, precision, alpha == 2.0);
   });
 }
 

@@ -197,6 +197,66 @@ __constant__ double c_rk4b[5] = {
     1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
     2277821191437.0 / 14882151754819.0};
 
+// One stage of the fused step for one particle: k = [v(p), F(x, p)],
+// res = a_s res + dt k, y += b_s res, then relocalization.  Returns false if
+// the particle left the mesh.  Every forward variant (per-step, checkpointed,
+// and the adjoint's recompute) runs this one function, so they agree bitwise.
+template <int D, int N, class T>
+__device__ __forceinline__ bool lsrk4_stage(const DevMesh &m, const T *__restrict__ coef, int s,
+                                            double dt, double c, double qi, double mi,
+                                            double *xi, double *pi, double *rx, double *rp,
+                                            double *r, uint32_t &e, FindCounters &cnt,
+                                            int &e_flag) {
+  double v[D], F[D];
+  velocity<D>(pi, mi, c, v);
+  const uint32_t e_uni = wave_uniform_element(e);
+  if constexpr (sizeof(T) == 8) {
+    double f6[6], el[3], mag[3];
+    interp6_any<D, N, double>(coef, e, e_uni, r, f6);
+    lorentz<D, double>(qi, f6, v, F, el, mag);
+  } else {
+    float rf[D], f6[6], vf[D], Ff[D], el[3], mag[3];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      rf[k] = (float)r[k];
+      vf[k] = (float)v[k];
+    }
+    interp6_any<D, N, float>(coef, e, e_uni, rf, f6);
+    lorentz<D, float>((float)qi, f6, vf, Ff, el, mag);
+#pragma unroll
+    for (int k = 0; k < D; ++k) F[k] = (double)Ff[k];
+  }
+  const double a = c_rk4a[s], b = c_rk4b[s];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    rx[k] = a * rx[k] + dt * v[k];
+    rp[k] = a * rp[k] + dt * F[k];
+  }
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    xi[k] += b * rx[k];
+    pi[k] += b * rp[k];
+  }
+  // fast path: still inside the (wave-uniform) element, same arithmetic
+  // as the first branch of find_new_containing_element
+  if (e_uni != INVALID) {
+    const uint32_t eu = (uint32_t)__builtin_amdgcn_readfirstlane((int)e_uni);
+    const_ptr<double> am = as_constant(m.inv_map) + (size_t)eu * (D * D + D);
+#pragma unroll
+    for (int ii = 0; ii < D; ++ii) {
+      double t = 0;
+#pragma unroll
+      for (int jj = 0; jj < D; ++jj) t += am[ii * D + jj] * xi[jj];
+      r[ii] = t + am[D * D + ii];
+    }
+    if (in_unit_simplex<D>(r)) {
+      cnt.same++;
+      return true;
+    }
+  }
+  return relocate<D>(m, xi, pi, e, cnt, e_flag, r);
+}
+
 template <int D, int N, class T>
 __global__ __launch_bounds__(256) void k_lsrk4_step(
     DevMesh m, const T *__restrict__ coef, int64_t n, double dt, double c,
@@ -222,58 +282,9 @@ __global__ __launch_bounds__(256) void k_lsrk4_step(
     if (e >= (uint32_t)m.K) died = 1;   // not located: removed at the first stage
     else inverse_map<D>(m, e, xi, r);
 #pragma unroll 1
-    for (int s = 0; s < 5 && !died; ++s) {
-      double v[D], F[D];
-      velocity<D>(pi, mi, c, v);
-      const uint32_t e_uni = wave_uniform_element(e);
-      if constexpr (sizeof(T) == 8) {
-        double f6[6], el[3], mag[3];
-        interp6_any<D, N, double>(coef, e, e_uni, r, f6);
-        lorentz<D, double>(qi, f6, v, F, el, mag);
-      } else {
-        float rf[D], f6[6], vf[D], Ff[D], el[3], mag[3];
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-          rf[k] = (float)r[k];
-          vf[k] = (float)v[k];
-        }
-        interp6_any<D, N, float>(coef, e, e_uni, rf, f6);
-        lorentz<D, float>((float)qi, f6, vf, Ff, el, mag);
-#pragma unroll
-        for (int k = 0; k < D; ++k) F[k] = (double)Ff[k];
-      }
-      const double a = c_rk4a[s], b = c_rk4b[s];
-#pragma unroll
-      for (int k = 0; k < D; ++k) {
-        rx[k] = a * rx[k] + dt * v[k];
-        rp[k] = a * rp[k] + dt * F[k];
-      }
-#pragma unroll
-      for (int k = 0; k < D; ++k) {
-        xi[k] += b * rx[k];
-        pi[k] += b * rp[k];
-      }
-      // fast path: still inside the (wave-uniform) element, same arithmetic
-      // as the first branch of find_new_containing_element
-      bool same = false;
-      if (e_uni != INVALID) {
-        const uint32_t eu = (uint32_t)__builtin_amdgcn_readfirstlane((int)e_uni);
-        const_ptr<double> a = as_constant(m.inv_map) + (size_t)eu * (D * D + D);
-#pragma unroll
-        for (int ii = 0; ii < D; ++ii) {
-          double t = 0;
-#pragma unroll
-          for (int jj = 0; jj < D; ++jj) t += a[ii * D + jj] * xi[jj];
-          r[ii] = t + a[D * D + ii];
-        }
-        same = in_unit_simplex<D>(r);
-        if (same) cnt.same++;
-      }
-      if (!same && !relocate<D>(m, xi, pi, e, cnt, e_flag, r)) {
+    for (int s = 0; s < 5 && !died; ++s)
+      if (!lsrk4_stage<D, N, T>(m, coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, cnt, e_flag))
         died = (uint32_t)s + 1;
-        break;
-      }
-    }
     if (e_flag) atomicOr(err, 1);
 #pragma unroll
     for (int k = 0; k < D; ++k) {
@@ -285,6 +296,326 @@ __global__ __launch_bounds__(256) void k_lsrk4_step(
     if (died) atomicAdd(n_dead, 1u);
   }
   flush_counters(cnt, counters);
+}
+
+// ---------------------------------------------------------------------------
+// adjoint of the push
+// ---------------------------------------------------------------------------
+// Forward without compaction, nsteps in one launch (particles are independent
+// in static fields).  A particle that leaves the mesh keeps its slot: death[i]
+// = 5*step + stage (1-based) and elem[i] = INVALID.  Every ckpt_every steps the
+// step-start state (x, p, element) is written to the checkpoint arrays,
+// segment-major: ckpt_xp[(seg*n + i)*2D + (0..D-1: x, D..2D-1: p)].
+template <int D, int N, class T>
+__global__ __launch_bounds__(256) void k_lsrk4_ckpt(
+    DevMesh m, const T *__restrict__ coef, int64_t n, int nsteps, int ckpt_every, double dt,
+    double c, double *__restrict__ x, double *__restrict__ p, const double *__restrict__ q,
+    const double *__restrict__ mass, uint32_t *__restrict__ elem, uint32_t *__restrict__ death,
+    double *__restrict__ ckpt_xp, uint32_t *__restrict__ ckpt_e, unsigned long long *counters,
+    int *err) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  FindCounters cnt;
+  if (i < n) {
+    double xi[D], pi[D], r[D], rx[D], rp[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      xi[k] = x[i * D + k];
+      pi[k] = p[i * D + k];
+    }
+    const double qi = q[i], mi = mass[i];
+    uint32_t e = elem[i];
+    uint32_t died = 0;
+    int e_flag = 0;
+    if (e >= (uint32_t)m.K) died = 1;
+    else inverse_map<D>(m, e, xi, r);
+#pragma unroll 1
+    for (int step = 0; step < nsteps && !died; ++step) {
+      if (ckpt_xp && step % ckpt_every == 0) {
+        const size_t seg = (size_t)(step / ckpt_every);
+        double *dst = ckpt_xp + (seg * (size_t)n + (size_t)i) * 2 * D;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          dst[k] = xi[k];
+          dst[D + k] = pi[k];
+        }
+        ckpt_e[seg * (size_t)n + (size_t)i] = e;
+      }
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        rx[k] = 0;
+        rp[k] = 0;
+      }
+#pragma unroll 1
+      for (int s = 0; s < 5; ++s)
+        if (!lsrk4_stage<D, N, T>(m, coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, cnt,
+                                  e_flag)) {
+          died = (uint32_t)(5 * step + s + 1);
+          break;
+        }
+    }
+    if (e_flag) atomicOr(err, 1);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      x[i * D + k] = xi[k];
+      p[i * D + k] = pi[k];
+    }
+    elem[i] = died ? INVALID : e;
+    death[i] = died;
+  }
+  flush_counters(cnt, counters);
+}
+
+// Adds J_f(y)^T kb to yb for the particle right-hand side f(x, p) = [v(p),
+// q E(x) + v(p) x q B(x)] at the stage input y = (x, p) in element e:
+//   vb = kb_x + (q B) x kb_p,   Eb = q kb_p,   Bb = q (kb_p x v),
+//   xb += A^T grad_r (Eb . E + Bb . B),        A = d r / d x (inverse map),
+//   pb += (c/s) (vb - p (p . vb) / s^2),       s = sqrt(m^2 c^2 + |p|^2).
+// With T = float the field values and gradients are evaluated in FP32 on a
+// power-of-two rescaled weight vector; the accumulation stays FP64.
+template <int D, int N, class T>
+__device__ __forceinline__ void rhs_adjoint(const DevMesh &m, const T *__restrict__ coef,
+                                            double c, double qi, double mi, const double *xi,
+                                            const double *pi, uint32_t e, const double *kbx,
+                                            const double *kbp, double *xb, double *pb) {
+  constexpr Basis<D, N> B{};
+  constexpr int NP = Basis<D, N>::NP;
+  double r[D], v[D];
+  inverse_map<D>(m, e, xi, r);
+  velocity<D>(pi, mi, c, v);
+  double v3[3], k3[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    v3[k] = k < D ? v[k] : 0.0;
+    k3[k] = k < D ? kbp[k] : 0.0;
+  }
+  double w6[6];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) w6[k] = qi * k3[k];
+  w6[3] = qi * (k3[1] * v3[2] - k3[2] * v3[1]);
+  w6[4] = qi * (k3[2] * v3[0] - k3[0] * v3[2]);
+  w6[5] = qi * (k3[0] * v3[1] - k3[1] * v3[0]);
+  double wmax = 0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) wmax = fmax(wmax, fabs(w6[k]));
+  int wexp = 0;
+  if (wmax > 0) frexp(wmax, &wexp);
+  T ws[6], rt[D];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) ws[k] = (T)ldexp(w6[k], -wexp);
+#pragma unroll
+  for (int k = 0; k < D; ++k) rt[k] = (T)r[k];
+  T pw[D][N + 1];
+#pragma unroll
+  for (int ii = 0; ii < D; ++ii) {
+    pw[ii][0] = T(1);
+#pragma unroll
+    for (int k = 1; k <= N; ++k) pw[ii][k] = pw[ii][k - 1] * rt[ii];
+  }
+  const T *cf = coef + (size_t)e * 6 * NP;
+  T bf[3] = {T(0), T(0), T(0)}, g[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) g[k] = T(0);
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    T phi = pw[0][B.e[k][0]];
+#pragma unroll
+    for (int ii = 1; ii < D; ++ii) phi = phi * pw[ii][B.e[k][ii]];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) bf[f] = fma(cf[(3 + f) * NP + k], phi, bf[f]);
+    T wk = T(0);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) wk = fma(ws[f], cf[f * NP + k], wk);
+#pragma unroll
+    for (int jj = 0; jj < D; ++jj) {
+      const int ej = B.e[k][jj];
+      if (ej > 0) {
+        T d = T(ej) * pw[jj][ej - 1];
+#pragma unroll
+        for (int ii = 0; ii < D; ++ii)
+          if (ii != jj) d = d * pw[ii][B.e[k][ii]];
+        g[jj] = fma(wk, d, g[jj]);
+      }
+    }
+  }
+  // x: A^T grad_r, A row-major in the inverse map
+  const double *am = m.inv_map + (size_t)e * (D * D + D);
+#pragma unroll
+  for (int jj = 0; jj < D; ++jj) {
+    double acc = 0;
+#pragma unroll
+    for (int ii = 0; ii < D; ++ii) acc += (double)g[ii] * am[ii * D + jj];
+    xb[jj] += ldexp(acc, wexp);
+  }
+  // v: kb_x + (q B) x kb_p, truncated to D components
+  double qb[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) qb[k] = qi * (double)bf[k];
+  double vb3[3];
+  vb3[0] = qb[1] * k3[2] - qb[2] * k3[1];
+  vb3[1] = qb[2] * k3[0] - qb[0] * k3[2];
+  vb3[2] = qb[0] * k3[1] - qb[1] * k3[0];
+  double vb[D], pv = 0, pp = 0;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    vb[k] = kbx[k] + vb3[k];
+    pv += pi[k] * vb[k];
+    pp += pi[k] * pi[k];
+  }
+  const double s2 = mi * mi * c * c + pp, s = sqrt(s2), f = c / s;
+#pragma unroll
+  for (int k = 0; k < D; ++k) pb[k] += f * (vb[k] - pi[k] * (pv / s2));
+}
+
+// Reverse pass over all steps for every live particle, one thread per
+// particle.  For each checkpoint segment, last first: recompute the segment's
+// step-start states into ybuf, then for each step, last first, recompute its
+// five stage inputs into sbuf and run the stages backwards:
+//   rb += b_s yb;  kb = dt rb;  rb *= a_s;  yb += J_f(y_{s-1})^T kb.
+// Particles that died get zero gradients (they are not in the final state).
+template <int D, int N, class T>
+__global__ __launch_bounds__(256) void k_lsrk4_adjoint(
+    DevMesh m, const T *__restrict__ coef, int64_t n, int nsteps, int ckpt_every, double dt,
+    double c, const double *__restrict__ q, const double *__restrict__ mass,
+    const uint32_t *__restrict__ death, const double *__restrict__ ckpt_xp,
+    const uint32_t *__restrict__ ckpt_e, double *__restrict__ ybuf, uint32_t *__restrict__ ebuf,
+    double *__restrict__ sbuf, uint32_t *__restrict__ sebuf, double *__restrict__ x_bar,
+    double *__restrict__ p_bar, int *err) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (death[i]) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      x_bar[i * D + k] = 0;
+      p_bar[i * D + k] = 0;
+    }
+    return;
+  }
+  double ax[D], ap[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    ax[k] = x_bar[i * D + k];
+    ap[k] = p_bar[i * D + k];
+  }
+  const double qi = q[i], mi = mass[i];
+  FindCounters cnt;
+  int e_flag = 0, lost = 0;
+  const size_t nn = (size_t)n, ii = (size_t)i;
+  const int nseg = (nsteps + ckpt_every - 1) / ckpt_every;
+#pragma unroll 1
+  for (int seg = nseg - 1; seg >= 0; --seg) {
+    const int steps = min(ckpt_every, nsteps - seg * ckpt_every);
+    double xi[D], pi[D], r[D], rx[D], rp[D];
+    const double *src = ckpt_xp + ((size_t)seg * nn + ii) * 2 * D;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      xi[k] = src[k];
+      pi[k] = src[D + k];
+    }
+    uint32_t e = ckpt_e[(size_t)seg * nn + ii];
+    inverse_map<D>(m, e, xi, r);
+#pragma unroll 1
+    for (int k = 0; k < steps; ++k) {
+      double *dst = ybuf + ((size_t)k * nn + ii) * 2 * D;
+#pragma unroll
+      for (int kk = 0; kk < D; ++kk) {
+        dst[kk] = xi[kk];
+        dst[D + kk] = pi[kk];
+      }
+      ebuf[(size_t)k * nn + ii] = e;
+      if (k + 1 == steps) break;
+#pragma unroll
+      for (int kk = 0; kk < D; ++kk) {
+        rx[kk] = 0;
+        rp[kk] = 0;
+      }
+#pragma unroll 1
+      for (int s = 0; s < 5; ++s)
+        if (!lsrk4_stage<D, N, T>(m, coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, cnt, e_flag))
+          lost = 1;
+    }
+#pragma unroll 1
+    for (int k = steps - 1; k >= 0; --k) {
+      const double *y = ybuf + ((size_t)k * nn + ii) * 2 * D;
+#pragma unroll
+      for (int kk = 0; kk < D; ++kk) {
+        xi[kk] = y[kk];
+        pi[kk] = y[D + kk];
+        rx[kk] = 0;
+        rp[kk] = 0;
+      }
+      e = ebuf[(size_t)k * nn + ii];
+      inverse_map<D>(m, e, xi, r);
+#pragma unroll 1
+      for (int s = 0; s < 5; ++s) {
+        double *st = sbuf + ((size_t)s * nn + ii) * 2 * D;
+#pragma unroll
+        for (int kk = 0; kk < D; ++kk) {
+          st[kk] = xi[kk];
+          st[D + kk] = pi[kk];
+        }
+        sebuf[(size_t)s * nn + ii] = e;
+        if (s == 4) break;   // the last stage's output is not needed
+        if (!lsrk4_stage<D, N, T>(m, coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, cnt, e_flag))
+          lost = 1;
+      }
+      double rbx[D], rbp[D];
+#pragma unroll
+      for (int kk = 0; kk < D; ++kk) {
+        rbx[kk] = 0;
+        rbp[kk] = 0;
+      }
+#pragma unroll 1
+      for (int s = 4; s >= 0; --s) {
+        const double a = c_rk4a[s], b = c_rk4b[s];
+        double kbx[D], kbp[D];
+#pragma unroll
+        for (int kk = 0; kk < D; ++kk) {
+          rbx[kk] += b * ax[kk];
+          rbp[kk] += b * ap[kk];
+          kbx[kk] = dt * rbx[kk];
+          kbp[kk] = dt * rbp[kk];
+          rbx[kk] *= a;
+          rbp[kk] *= a;
+        }
+        const double *st = sbuf + ((size_t)s * nn + ii) * 2 * D;
+        double xs[D], ps[D];
+#pragma unroll
+        for (int kk = 0; kk < D; ++kk) {
+          xs[kk] = st[kk];
+          ps[kk] = st[D + kk];
+        }
+        rhs_adjoint<D, N, T>(m, coef, c, qi, mi, xs, ps, sebuf[(size_t)s * nn + ii], kbx, kbp,
+                             ax, ap);
+      }
+    }
+  }
+  if (e_flag || lost) atomicOr(err, 2);
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    x_bar[i * D + k] = ax[k];
+    p_bar[i * D + k] = ap[k];
+  }
+}
+
+// p_bar = (dv/dp)^T v_bar for v = c p / sqrt(m^2 c^2 + |p|^2)
+template <int VD>
+__global__ void k_velocities_adjoint(int64_t n, double c, const double *__restrict__ p,
+                                     const double *__restrict__ mass,
+                                     const double *__restrict__ v_bar, double *__restrict__ p_bar) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double pi[VD], vb[VD], pv = 0, pp = 0;
+#pragma unroll
+  for (int k = 0; k < VD; ++k) {
+    pi[k] = p[i * VD + k];
+    vb[k] = v_bar[i * VD + k];
+    pv += pi[k] * vb[k];
+    pp += pi[k] * pi[k];
+  }
+  const double mi = mass[i];
+  const double s2 = mi * mi * c * c + pp, s = sqrt(s2), f = c / s;
+#pragma unroll
+  for (int k = 0; k < VD; ++k) p_bar[i * VD + k] = f * (vb[k] - pi[k] * (pv / s2));
 }
 
 __global__ void k_stage_flags(int64_t n, const uint32_t *__restrict__ death, uint32_t s,
@@ -462,6 +793,119 @@ int pyr_lsrk4_steps(pyr_mesh *mesh, int precision, int nsteps, double dt, double
       }
     }
     *n_inout = n;
+  });
+}
+
+int pyr_lsrk4_forward(pyr_mesh *mesh, int precision, int nsteps, int ckpt_every, double dt,
+                      double vacuum_c, int64_t n, double *x, double *p, const double *q,
+                      const double *mass, uint32_t *elem, uint32_t *death, double *ckpt_xp,
+                      uint32_t *ckpt_elem, unsigned long long *counters) {
+  return guarded([&] {
+    PYR_REQUIRE(mesh && nsteps >= 0 && ckpt_every > 0, "invalid argument");
+    PYR_REQUIRE(precision == PYR_FP64 || precision == PYR_FP32C, "unknown precision");
+    PYR_REQUIRE(mesh->sf_set, "pyr_set_static_fields has not been called for this mesh");
+    PYR_REQUIRE(n >= 0 && n < (int64_t)0x7FFFFFFF, "invalid particle count");
+    PYR_REQUIRE((ckpt_xp == nullptr) == (ckpt_elem == nullptr),
+                "checkpoint positions/momenta and elements go together");
+    if (n == 0) return;
+    PYR_REQUIRE(x && p && q && mass && elem && death, "null pointer");
+    const int D = mesh->dim, N = order_of(mesh);
+    mesh->s_misc.reserve(64);
+    int *err = mesh->s_misc.as<int>();
+    PYR_HIP(hipMemsetAsync(err, 0, sizeof(int), stream()));
+    {
+      ProfScope prof(PROF_PUSH);
+      dispatch_order(D, N, [&](auto DD, auto NN) {
+        constexpr int D_ = decltype(DD)::value, N_ = decltype(NN)::value;
+        if (precision == PYR_FP64)
+          hipLaunchKernelGGL((k_lsrk4_ckpt<D_, N_, double>), dim3(blocks_for(n)), dim3(256), 0,
+                             stream(), mesh->dev, mesh->sf64.as<double>(), n, nsteps, ckpt_every,
+                             dt, vacuum_c, x, p, q, mass, elem, death, ckpt_xp, ckpt_elem,
+                             counters, err);
+        else
+          hipLaunchKernelGGL((k_lsrk4_ckpt<D_, N_, float>), dim3(blocks_for(n)), dim3(256), 0,
+                             stream(), mesh->dev, mesh->sf32.as<float>(), n, nsteps, ckpt_every,
+                             dt, vacuum_c, x, p, q, mass, elem, death, ckpt_xp, ckpt_elem,
+                             counters, err);
+        PYR_CHECK_LAUNCH();
+      });
+    }
+    int h = 0;
+    PYR_HIP(hipMemcpyAsync(&h, err, sizeof(int), hipMemcpyDeviceToHost, stream()));
+    PYR_HIP(hipStreamSynchronize(stream()));
+    if (h) throw Error(PYR_ERR_NO_BEST_NORMAL, "no best normal found--weird");
+  });
+}
+
+int pyr_lsrk4_adjoint(pyr_mesh *mesh, int precision, int nsteps, int ckpt_every, double dt,
+                      double vacuum_c, int64_t n, const double *q, const double *mass,
+                      const uint32_t *death, const double *ckpt_xp, const uint32_t *ckpt_elem,
+                      double *x_bar, double *p_bar) {
+  return guarded([&] {
+    PYR_REQUIRE(mesh && nsteps >= 0 && ckpt_every > 0, "invalid argument");
+    PYR_REQUIRE(precision == PYR_FP64 || precision == PYR_FP32C, "unknown precision");
+    PYR_REQUIRE(mesh->sf_set, "pyr_set_static_fields has not been called for this mesh");
+    PYR_REQUIRE(n >= 0 && n < (int64_t)0x7FFFFFFF, "invalid particle count");
+    if (n == 0 || nsteps == 0) return;
+    PYR_REQUIRE(q && mass && death && ckpt_xp && ckpt_elem && x_bar && p_bar, "null pointer");
+    const int D = mesh->dim, N = order_of(mesh);
+    // ybuf: ckpt_every step-start states; sbuf: five stage inputs
+    const size_t rec = (size_t)2 * D * sizeof(double) + sizeof(uint32_t);
+    const size_t ybytes = (size_t)ckpt_every * (size_t)n * 2 * D * sizeof(double);
+    const size_t ebytes = (size_t)ckpt_every * (size_t)n * sizeof(uint32_t);
+    const size_t sbytes = (size_t)5 * (size_t)n * 2 * D * sizeof(double);
+    const size_t sebytes = (size_t)5 * (size_t)n * sizeof(uint32_t);
+    (void)rec;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    mesh->s_adj.reserve(al(ybytes) + al(ebytes) + al(sbytes) + al(sebytes));
+    char *base = mesh->s_adj.as<char>();
+    double *ybuf = reinterpret_cast<double *>(base);
+    uint32_t *ebuf = reinterpret_cast<uint32_t *>(base + al(ybytes));
+    double *sbuf = reinterpret_cast<double *>(base + al(ybytes) + al(ebytes));
+    uint32_t *sebuf = reinterpret_cast<uint32_t *>(base + al(ybytes) + al(ebytes) + al(sbytes));
+    mesh->s_misc.reserve(64);
+    int *err = mesh->s_misc.as<int>();
+    PYR_HIP(hipMemsetAsync(err, 0, sizeof(int), stream()));
+    {
+      ProfScope prof(PROF_ADJOINT);
+      dispatch_order(D, N, [&](auto DD, auto NN) {
+        constexpr int D_ = decltype(DD)::value, N_ = decltype(NN)::value;
+        if (precision == PYR_FP64)
+          hipLaunchKernelGGL((k_lsrk4_adjoint<D_, N_, double>), dim3(blocks_for(n)), dim3(256),
+                             0, stream(), mesh->dev, mesh->sf64.as<double>(), n, nsteps,
+                             ckpt_every, dt, vacuum_c, q, mass, death, ckpt_xp, ckpt_elem, ybuf,
+                             ebuf, sbuf, sebuf, x_bar, p_bar, err);
+        else
+          hipLaunchKernelGGL((k_lsrk4_adjoint<D_, N_, float>), dim3(blocks_for(n)), dim3(256),
+                             0, stream(), mesh->dev, mesh->sf32.as<float>(), n, nsteps,
+                             ckpt_every, dt, vacuum_c, q, mass, death, ckpt_xp, ckpt_elem, ybuf,
+                             ebuf, sbuf, sebuf, x_bar, p_bar, err);
+        PYR_CHECK_LAUNCH();
+      });
+    }
+    int h = 0;
+    PYR_HIP(hipMemcpyAsync(&h, err, sizeof(int), hipMemcpyDeviceToHost, stream()));
+    PYR_HIP(hipStreamSynchronize(stream()));
+    if (h)
+      throw Error(PYR_ERR_INTERNAL,
+                  "adjoint recompute diverged from the forward pass (checkpoints do not match "
+                  "the static fields or the particle data)");
+  });
+}
+
+int pyr_velocities_backward(int64_t n, int vdim, double c, const double *p, const double *m,
+                            const double *v_bar, double *p_bar) {
+  return guarded([&] {
+    PYR_REQUIRE(n >= 0 && (vdim == 2 || vdim == 3), "invalid argument");
+    if (n == 0) return;
+    PYR_REQUIRE(p && m && v_bar && p_bar, "null pointer");
+    if (vdim == 2)
+      hipLaunchKernelGGL(k_velocities_adjoint<2>, dim3(blocks_for(n)), dim3(256), 0, stream(), n,
+                         c, p, m, v_bar, p_bar);
+    else
+      hipLaunchKernelGGL(k_velocities_adjoint<3>, dim3(blocks_for(n)), dim3(256), 0, stream(), n,
+                         c, p, m, v_bar, p_bar);
+    PYR_CHECK_LAUNCH();
   });
 }
 

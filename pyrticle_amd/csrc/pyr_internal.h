@@ -48,7 +48,15 @@ hipStream_t stream();
 // Events are recorded on the library stream around the region's launches.
 bool profiling();
 void prof_record(int region, bool begin);
-enum { PROF_PUSH = 0, PROF_GATHER = 1, PROF_SORT = 2, PROF_COMPACT = 3, PROF_NREGIONS = 4 };
+enum {
+  PROF_PUSH = 0,
+  PROF_GATHER = 1,
+  PROF_SORT = 2,
+  PROF_COMPACT = 3,
+  PROF_ADJOINT = 4,
+  PROF_DEPOSIT_ADJ = 5,
+  PROF_NREGIONS = 6
+};
 struct ProfScope {
   int region;
   explicit ProfScope(int r) : region(r) {
@@ -339,6 +347,7 @@ struct pyr_mesh {
   std::vector<int32_t> h_exps;
   // scratch
   pyr::DevBuf s_modal, s_flags, s_scan, s_tmp, s_holes, s_misc;
+  pyr::DevBuf s_adj;                // adjoint recompute buffers (push.hip)
   // deposition state (see deposit.hip)
   struct DepositCache *dep = nullptr;
   pyr::DevBuf counters;             // 4 x u64 scratch for counters

@@ -160,6 +160,41 @@ class DeviceMesh:
             _lib.ptr(j) if (want_j and v is not None) else None))
         return (rho if want_rho else None), (j if (want_j and v is not None) else None)
 
+    def deposit_backward(self, radius, alpha, normalizer, x, q, elem, rho_bar=None, j_bar=None,
+                         v=None, precision=FP64):
+        """(x_bar, v_bar) from the adjoint densities; v_bar is None without j_bar."""
+        _lib.sync_stream()
+        n, d = x.shape
+        x_bar = torch.empty((n, d), dtype=torch.float64, device=x.device)
+        v_bar = torch.empty((n, d), dtype=torch.float64, device=x.device) \
+            if j_bar is not None else None
+        _lib.check(self._lib.pyr_deposit_backward(
+            self._h, radius, alpha, normalizer, precision, n, d, _lib.ptr(x), _lib.ptr(q),
+            _lib.ptr(v), _lib.ptr(elem), _lib.ptr(rho_bar), _lib.ptr(j_bar), _lib.ptr(x_bar),
+            _lib.ptr(v_bar)))
+        return x_bar, v_bar
+
+    def lsrk4_forward(self, nsteps, dt, c, x, p, q, m, elem, death, ckpt_every=1,
+                      ckpt_xp=None, ckpt_elem=None, counters=None, precision=FP64):
+        """nsteps LSRK4 steps in place without compaction (dead particles keep
+        their slots, death[i] = 5*step + stage); optional checkpoints."""
+        _lib.sync_stream()
+        n = x.shape[0]
+        _lib.check(self._lib.pyr_lsrk4_forward(
+            self._h, precision, nsteps, ckpt_every, dt, c, n, _lib.ptr(x), _lib.ptr(p),
+            _lib.ptr(q), _lib.ptr(m), _lib.ptr(elem), _lib.ptr(death), _lib.ptr(ckpt_xp),
+            _lib.ptr(ckpt_elem), _lib.ptr(counters)))
+
+    def lsrk4_adjoint(self, nsteps, ckpt_every, dt, c, q, m, death, ckpt_xp, ckpt_elem, x_bar,
+                      p_bar, precision=FP64):
+        """In place: final-state gradients -> initial-state gradients."""
+        _lib.sync_stream()
+        n = x_bar.shape[0]
+        _lib.check(self._lib.pyr_lsrk4_adjoint(
+            self._h, precision, nsteps, ckpt_every, dt, c, n, _lib.ptr(q), _lib.ptr(m),
+            _lib.ptr(death), _lib.ptr(ckpt_xp), _lib.ptr(ckpt_elem), _lib.ptr(x_bar),
+            _lib.ptr(p_bar)))
+
     def deposit_stats(self):
         arr = (C.c_ulonglong * 4)()
         _lib.check(self._lib.pyr_deposit_stats(self._h, C.byref(arr)))
@@ -192,6 +227,17 @@ def velocities(p: torch.Tensor, m: torch.Tensor, c: float) -> torch.Tensor:
     return v
 
 
+def velocities_backward(p: torch.Tensor, m: torch.Tensor, c: float,
+                        v_bar: torch.Tensor) -> torch.Tensor:
+    """p_bar = (dv/dp)^T v_bar."""
+    _lib.sync_stream()
+    n, vd = p.shape
+    p_bar = torch.empty_like(p)
+    _lib.check(_lib.lib().pyr_velocities_backward(n, vd, c, _lib.ptr(p), _lib.ptr(m),
+                                                  _lib.ptr(v_bar), _lib.ptr(p_bar)))
+    return p_bar
+
+
 def compact_plan(dead: torch.Tensor):
     """(n_new, moves_from, moves_to) for the reference's swap-remove kill order."""
     _lib.sync_stream()
@@ -217,7 +263,8 @@ def apply_moves(buf: torch.Tensor, frm: torch.Tensor, to: torch.Tensor):
                                           _lib.ptr(to)))
 
 
-PROFILE_REGIONS = {"push": 0, "gather": 1, "sort": 2, "compact": 3}
+PROFILE_REGIONS = {"push": 0, "gather": 1, "sort": 2, "compact": 3, "adjoint": 4,
+                   "deposit_adjoint": 5}
 
 
 def profile_enable(on: bool = True):
