@@ -103,6 +103,120 @@ def cpu_baseline(wl, st0, sample, steps):
     return alive / dt, dt, int(visits[0]) / max(1, n), n
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel from the committed counter profile
+    (FETCH_SIZE and WRITE_SIZE of rocprofv3 --pmc runs, each scaled by what the
+    counter reads for a 512 MiB copy), or None if no profile is present."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "traffic_cylinder3d_4M_fp64.json")
+    try:
+        with open(path) as f:
+            return float(json.load(f)["kernels"][kernel]["hbm_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def adjoint_leg(wl, dm, device, torch, normalizer, n, nsteps, ckpt, fd_n):
+    """Gradient of the density-matching loss w.r.t. the 6 beam parameters and
+    all initial phase-space coordinates: forward = nsteps push steps + final
+    deposition, backward = deposition adjoint + push adjoint with checkpoints
+    every ckpt steps.  FP32C and FP64 at full size; central finite differences
+    (FP64, relative step 1e-4) on a fd_n-particle beam that stays in the mesh."""
+    from pyrticle_amd import adjoint as A
+    from pyrticle_amd import device as dv
+    from pyrticle_amd.units import SIUnitsWithNaturalConstants as U
+    fields = [torch.from_numpy(f).to(device) for f in wl.fields]
+
+    def problem(nn, z_cut):
+        q = torch.full((nn,), -U.EL_CHARGE, dtype=torch.float64, device=device)
+        m = torch.full((nn,), U.EL_MASS, dtype=torch.float64, device=device)
+        noise = A.beam_noise(nn, 1, z_cut=z_cut, device=device)
+        tnoise = A.beam_noise(nn, 2, z_cut=z_cut, device=device)
+
+        def cfg_for(x0, prec):
+            return A.PushDepositConfig(
+                dmesh=dm, fields=fields, q=q, m=m,
+                elem0=dm.find_containing_element(x0.detach().contiguous()), nsteps=nsteps,
+                dt=wl.dt, c=wl.c, radius=wl.radius, alpha=wl.alpha, normalizer=normalizer,
+                precision=prec, ckpt_every=ckpt)
+
+        theta0 = A.default_beam_theta(device)
+        theta_t = theta0.clone()
+        theta_t[0] *= 1.1
+        theta_t[1] *= 1.1
+        with torch.no_grad():
+            xt, pt = A.beam_state(theta_t, tnoise)
+            rho_t, _ = A.push_and_deposit(xt.contiguous(), pt.contiguous(),
+                                          cfg_for(xt, dv.FP64))
+        norm_t = (rho_t * rho_t).sum()
+
+        def loss(theta, prec, keep=None):
+            x0, p0 = A.beam_state(theta, noise)
+            x0 = x0.contiguous()
+            p0 = p0.contiguous()
+            if keep is not None:
+                x0.retain_grad()
+                p0.retain_grad()
+                keep["x0"], keep["p0"] = x0, p0
+            cfg = cfg_for(x0, prec)
+            rho, _ = A.push_and_deposit(x0, p0, cfg)
+            if keep is not None:
+                keep["alive"] = cfg.stats["alive"]
+            return ((rho - rho_t) ** 2).sum() / norm_t
+
+        return theta0, loss
+
+    res = {"workload": "%d particles, %d push+deposit steps, checkpoints every %d steps; "
+                       "loss ||rho_T - rho*||^2/||rho*||^2 against a seed-2 beam with 1.1x "
+                       "semi-axes" % (n, nsteps, ckpt)}
+    theta0, loss = problem(n, 4.5)
+    grads = {}
+    for name, prec in (("fp32c", dv.FP32C), ("fp64", dv.FP64)):
+        for rep in range(2):   # the first pass warms up allocations and tables
+            th = theta0.clone().requires_grad_(True)
+            keep = {}
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            L = loss(th, prec, keep)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            L.backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+        grads[name] = (th.grad.detach().clone(), keep["x0"].grad.detach().clone(),
+                       keep["p0"].grad.detach().clone())
+        res[name] = {"loss": float(L.detach()), "alive": keep["alive"],
+                     "forward_ms": 1e3 * (t1 - t0), "backward_ms": 1e3 * (t2 - t1),
+                     "grad_theta": [float(v) for v in grads[name][0].cpu()]}
+        log("adjoint %s: forward %.1f ms, backward %.1f ms" % (name, 1e3 * (t1 - t0),
+                                                              1e3 * (t2 - t1)))
+
+    def rel(a, b):
+        return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+
+    res["fp32c_vs_fp64_rel_l2"] = {"theta": rel(grads["fp32c"][0], grads["fp64"][0]),
+                                   "x0": rel(grads["fp32c"][1], grads["fp64"][1]),
+                                   "p0": rel(grads["fp32c"][2], grads["fp64"][2])}
+    # finite differences on a beam subset that stays inside the mesh
+    theta0, loss = problem(fd_n, 1.5)
+    th = theta0.clone().requires_grad_(True)
+    loss(th, dv.FP64).backward()
+    g = th.grad.detach().cpu().numpy()
+    fd = np.zeros(6)
+    with torch.no_grad():
+        for k in range(6):
+            h = 1e-4 * float(theta0[k])
+            tp, tm = theta0.clone(), theta0.clone()
+            tp[k] += h
+            tm[k] -= h
+            fd[k] = (float(loss(tp, dv.FP64)) - float(loss(tm, dv.FP64))) / (2 * h)
+    res["finite_differences"] = {
+        "particles": fd_n, "h_relative": 1e-4, "grad_adjoint": g.tolist(), "grad_fd": fd.tolist(),
+        "max_rel_err": float(np.max(np.abs(fd - g)) / np.max(np.abs(g)))}
+    log("adjoint fd check: max rel err %.2e" % res["finite_differences"]["max_rel_err"])
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +227,11 @@ def main(argv=None):
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32c", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run two 512 MiB device-to-device copies (the reference "
+                         "kernel for scaling FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
     args = ap.parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -122,9 +241,17 @@ def main(argv=None):
         return 2
 
     import torch
+    if args.copy_probe:
+        a = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(2):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 8}), flush=True)
+        return 0
     from pyrticle_amd import device as dv
     from pyrticle_amd import workloads as W
-    from oracle import oracle as O
+    from pyrticle_amd._internal import PolynomialShapeFunction
 
     device = torch.device("cuda:0")
     t_setup = time.perf_counter()
@@ -134,7 +261,7 @@ def main(argv=None):
     dm = dv.DeviceMesh(wl.ma)
     dm.set_static_fields([torch.from_numpy(f).to(device) for f in wl.fields])
     st0, order = build_state(wl, dm, device, torch)
-    normalizer = O.shape_normalizer(wl.radius, 3, wl.alpha)
+    normalizer = PolynomialShapeFunction(wl.radius, 3, wl.alpha).normalizer
     nn = K * Np
     bufs = {"rho": torch.empty(nn, dtype=torch.float64, device=device),
             "j": torch.empty((nn, 3), dtype=torch.float64, device=device)}
@@ -203,6 +330,13 @@ def main(argv=None):
         log("fp32c: %.3f ms/step (x%.2f), rho err %.2e" % (fp32c["ms_per_step"],
                                                            fp32c["speedup_vs_fp64"], rho_err))
 
+    # ---- adjoint (gradients of a density-matching loss) -------------------
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_leg(wl, dm, device, torch, normalizer, args.particles,
+                              args.adjoint_steps, 20, 10_000)
+        dm.set_static_fields([torch.from_numpy(f).to(device) for f in wl.fields])
+
     # ---- CPU baseline ------------------------------------------------------
     cpu = None
     visits_per_particle = None
@@ -253,7 +387,7 @@ def main(argv=None):
         "roofline": {"bound": "valu", "kernel": "k_gather (rho+j deposition, FP64)",
                      "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / FP64_PEAK_TFLOPS,
-                     "traffic": None,
+                     "traffic": measured_traffic("k_gather"),
                      "avg_launch_ms": gather_avg_s * 1e3,
                      "work": {"flops_per_launch": flops, "node_visits_per_particle_ref":
                               visits_per_particle, "pairs_inside_R": n_in,
@@ -261,11 +395,13 @@ def main(argv=None):
                               "prefilter_passed": n_pass, "particles": n_last}},
         "roofline_push": {"bound": "hbm", "kernel": "k_lsrk4_step (fused 5-stage push, FP64)",
                           "achieved": push_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": push_gbs / HBM_PEAK_GBS, "traffic": None,
+                          "frac": push_gbs / HBM_PEAK_GBS,
+                          "traffic": measured_traffic("k_lsrk4_step"),
                           "avg_launch_ms": push_avg_s * 1e3,
                           "bytes_per_launch": push_bytes, "occupied_elements": occupied},
         "sort_ms": sort_ms / max(1, sort_n),
         "fp32c": fp32c,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
