@@ -149,13 +149,13 @@ struct GatherArgs {
   const uint32_t *seg;              // K + 1 segment starts into pos/wgt
   const double4 *pos;               // staged positions
   const double4 *wgt;               // staged weights
-  const double *geom;               // K x 8: centroid (3), bound radius^2, ...
+  const double *geom;               // K x 8: centroid (3), (R + node radius)^2, vertex radius
   const double *face_normal;        // mesh arrays
   const double *face_rhs;
   const void *node_tab;             // K x Np x 4 (T): n - c, |n - c|^2 / R
   double *rho;                      // K*Np or null
   double *j;                        // K*Np x D or null
-  unsigned long long *stats;        // [pairs evaluated, pairs inside, candidates, passed]
+  unsigned long long *stats;        // [pairs evaluated, pairs inside R, candidates, passed]
 };
 
 template <int D>
@@ -193,6 +193,13 @@ static inline unsigned gather_grid(int K) {
   return (unsigned)(((int64_t)K + per - 1) / per * per);
 }
 
+// Gather per target element.  A block holds G waves, each owning NPW of the
+// element's nodes.  The block streams the candidate particles, drops those
+// whose ball cannot reach the element (bounding sphere of the nodes inflated
+// by R, face planes moved out by R), queues the rest in an LDS ring buffer and
+// evaluates full batches of 64 queued particles one particle per lane against
+// each wave's nodes (node data in scalar registers), accumulating rho and j
+// per node in registers.
 template <int D, int NP, int NPW, bool FP32, bool A2, bool COUNT>
 __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherArgs g) {
   using T = typename std::conditional<FP32, float, double>::type;
@@ -238,6 +245,18 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
   unsigned long long n_eval = 0, n_in = 0, n_cand = 0, n_pass = 0;
   int q_head = 0, q_count = 0;   // identical in every thread of the block
   bool any = false;
+  constexpr int FOLD = 8;        // FP32: batches per compensated fold
+  int n_fold = 0;
+
+  auto fold = [&]() {
+#pragma unroll
+    for (int i = 0; i < NPW; ++i)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        kacc[i][v].add(acc[i][v]);
+        acc[i][v] = T(0);
+      }
+  };
 
   auto process_batch = [&](int count) {
     // one queued particle per lane, all of this wave's nodes
@@ -249,7 +268,9 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
       for (int k = 0; k < REC; ++k) rec[k] = q_rec[slot][k];
 #pragma unroll
       for (int i = 0; i < NPW; ++i) {
-        if (node0 + i < NP) {
+        // compile-time true when the waves tile the nodes exactly, so that
+        // the node loads are not split by branches
+        if (NP % NPW == 0 || node0 + i < NP) {
           const CT nd = ntab + (size_t)(node0 + i) * 4;
           T t = rec[3] - nd[3];
 #pragma unroll
@@ -269,14 +290,12 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
       if constexpr (COUNT) n_eval += (unsigned long long)min(NPW, NP - node0);
     }
     if constexpr (FP32) {
-      // fold the batch partials into the compensated sums
-#pragma unroll
-      for (int i = 0; i < NPW; ++i)
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          kacc[i][v].add(acc[i][v]);
-          acc[i][v] = 0.f;
-        }
+      // plain FP32 partials over at most FOLD particles per lane, then a
+      // compensated fold
+      if (++n_fold == FOLD) {
+        n_fold = 0;
+        fold();
+      }
     }
   };
 
@@ -308,16 +327,17 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
           const double x[3] = {pp.x + sh[0], pp.y + sh[1], D == 3 ? pp.z + sh[2] : 0.0};
           const double d[3] = {x[0] - cx, x[1] - cy, x[2] - cz};
           const double dd = fma(d[0], d[0], fma(d[1], d[1], D == 3 ? d[2] * d[2] : 0.0));
-          pass = dd <= bound2;
+          // conservative: the ball of radius R misses the element if x lies
+          // beyond one of its face planes by more than R
+          double viol = -INFINITY;
 #pragma unroll
           for (int f = 0; f <= D; ++f) {
-            // conservative: the ball of radius R misses the element if x lies
-            // beyond one of its face planes by more than R
             double ip = fnl[f][0] * x[0];
 #pragma unroll
             for (int kk = 1; kk < D; ++kk) ip = fma(fnl[f][kk], x[kk], ip);
-            pass = pass && (ip <= frl[f]);
+            viol = fmax(viol, ip - frl[f]);
           }
+          pass = (dd <= bound2) & (viol <= 0.0);
           if constexpr (COUNT) n_cand += 1;
           if (pass) {
             if constexpr (COUNT) n_pass += 1;
@@ -371,6 +391,7 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
     any = true;
     process_batch(q_count);
   }
+  if constexpr (FP32) fold();
 
   // reduce across lanes in a fixed tree and write this wave's nodes
 #pragma unroll
@@ -406,6 +427,37 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
       if (lane == 0 && v) atomicAdd(g.stats + k, v);
     }
   }
+}
+
+
+// sort keys for the deposition: element id, then a 12-bit Morton code of the
+// position in a 16^D grid over the element's vertex-bounding cube, so that
+// consecutive particles of an element are spatial neighbours.  Invalid
+// elements get the largest key.
+constexpr int MORTON_BITS = 12;
+template <int D>
+__global__ void k_sort_keys(int64_t n, int K, const uint32_t *__restrict__ elem,
+                            const double *__restrict__ x, const double *__restrict__ geom,
+                            uint32_t *__restrict__ keys) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t e = elem[i];
+  if (e >= (uint32_t)K) {
+    keys[i] = (uint32_t)K << MORTON_BITS;
+    return;
+  }
+  constexpr int BPA = MORTON_BITS / D;   // bits per axis
+  const double vr = geom[(size_t)e * 8 + 4];
+  const double scale = vr > 0 ? (double)(1 << BPA) / (2.0 * vr) : 0.0;
+  uint32_t code = 0;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    const double u = (x[i * D + a] - geom[(size_t)e * 8 + a] + vr) * scale;
+    const int qv = (int)fmin(fmax(u, 0.0), (double)((1 << BPA) - 1));
+#pragma unroll
+    for (int b = 0; b < BPA; ++b) code |= (uint32_t)((qv >> b) & 1) << (b * D + a);
+  }
+  keys[i] = (e << MORTON_BITS) | code;
 }
 
 // node table: n - c_e and |n - c_e|^2 / R, per target element
@@ -582,6 +634,7 @@ struct DepositCache {
   double tr_radius = -1;
   // sort / staging scratch
   DevBuf keys_a, keys_b, vals_a, vals_b, hist, hist_scan, counts, seg, tmp, pos, wgt, stats;
+  DevBuf skeys;   // Morton sort keys
   std::vector<double> h_geom;   // K x 8
 };
 
@@ -590,7 +643,7 @@ void pyr_deposit_cache_free(pyr_mesh *mesh) {
   DepositCache *c = mesh->dep;
   for (DevBuf *b : {&c->cand_start, &c->cand, &c->cand_shift, &c->geom, &c->tab64, &c->tab32,
                     &c->tab_R, &c->tr_start, &c->tr, &c->tr_shift, &c->keys_a, &c->keys_b,
-                    &c->vals_a, &c->vals_b, &c->hist,
+                    &c->vals_a, &c->vals_b, &c->skeys, &c->hist,
                     &c->hist_scan, &c->counts, &c->seg, &c->tmp, &c->pos, &c->wgt, &c->stats})
     b->release();
   delete c;
@@ -675,6 +728,7 @@ static void build_candidates(pyr_mesh *h, DepositCache *c, double R) {
     for (int i = 0; i < 3; ++i) c->h_geom[(size_t)e * 8 + i] = cen[(size_t)e * 3 + i];
     const double b = (R + rn) * (1 + 1e-12) + 1e-300;
     c->h_geom[(size_t)e * 8 + 3] = b * b;
+    c->h_geom[(size_t)e * 8 + 4] = vrad[e];
   }
   const double slack = 1e-9 * R;
   auto accept = [&](int e, int s, const double *sh) {
@@ -805,9 +859,12 @@ static unsigned blocks_for(int64_t n, int bs = 256) {
   return (unsigned)std::max<int64_t>(1, (n + bs - 1) / bs);
 }
 
-// sorts elem keys, returns perm in vals_a-or-b and segment starts in c->seg
+// Stable radix sort of particle indices by keys (element ids, or element ids
+// followed by a Morton code when key_shift > 0); returns the permutation (in
+// vals_a or vals_b) and leaves per-element segment starts in c->seg.
 static const uint32_t *sort_by_element(pyr_mesh *h, DepositCache *c, int64_t n,
-                                       const uint32_t *elem) {
+                                       const uint32_t *elem, const uint32_t *keys = nullptr,
+                                       int key_shift = 0) {
   const int K = h->K;
   hipStream_t s = stream();
   c->counts.reserve(sizeof(uint32_t) * ((size_t)K + 2));
@@ -821,8 +878,13 @@ static const uint32_t *sort_by_element(pyr_mesh *h, DepositCache *c, int64_t n,
   PYR_CHECK_LAUNCH();
   scan_exclusive_u32(c->counts.as<uint32_t>(), c->seg.as<uint32_t>(), (int64_t)K + 1, nullptr,
                      c->tmp.as<uint32_t>(), s);
-  int bits = 1;   // keys are clamped to [0, K]
-  while (bits < 32 && ((uint64_t)1 << bits) <= (uint64_t)K) ++bits;
+  if (!keys) {
+    keys = elem;
+    key_shift = 0;
+  }
+  const uint32_t kmax = (uint32_t)K << key_shift;   // keys are clamped to [0, kmax]
+  int bits = 1;
+  while (bits < 32 && ((uint64_t)1 << bits) <= (uint64_t)kmax) ++bits;
   const int passes = (bits + 7) / 8;
   c->keys_a.reserve(sizeof(uint32_t) * n);
   c->keys_b.reserve(sizeof(uint32_t) * n);
@@ -830,16 +892,16 @@ static const uint32_t *sort_by_element(pyr_mesh *h, DepositCache *c, int64_t n,
   c->vals_b.reserve(sizeof(uint32_t) * n);
   c->hist.reserve(sizeof(uint32_t) * 256 * ntiles);
   c->hist_scan.reserve(sizeof(uint32_t) * 256 * ntiles);
-  const uint32_t *kin = elem, *vin = nullptr;
+  const uint32_t *kin = keys, *vin = nullptr;
   uint32_t *kout = c->keys_a.as<uint32_t>(), *vout = c->vals_a.as<uint32_t>();
   for (int p = 0; p < passes; ++p) {
     hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)ntiles), dim3(RS_BS), 0, s, kin, n, 8 * p,
-                       c->hist.as<uint32_t>(), (int)ntiles, (uint32_t)K);
+                       c->hist.as<uint32_t>(), (int)ntiles, kmax);
     PYR_CHECK_LAUNCH();
     scan_exclusive_u32(c->hist.as<uint32_t>(), c->hist_scan.as<uint32_t>(), 256 * ntiles,
                        nullptr, c->tmp.as<uint32_t>(), s);
     hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)ntiles), dim3(RS_BS), 0, s, kin, vin, kout,
-                       vout, n, 8 * p, c->hist_scan.as<uint32_t>(), (int)ntiles, (uint32_t)K);
+                       vout, n, 8 * p, c->hist_scan.as<uint32_t>(), (int)ntiles, kmax);
     PYR_CHECK_LAUNCH();
     kin = kout;
     vin = vout;
@@ -852,6 +914,38 @@ static const uint32_t *sort_by_element(pyr_mesh *h, DepositCache *c, int64_t n,
     }
   }
   return vin;
+}
+
+// Sorts the particles by (element, Morton code inside the element) and stages
+// positions and weights in that order; returns the permutation.
+static const uint32_t *sort_and_stage(pyr_mesh *mesh, DepositCache *c, int64_t n,
+                                      const double *x, const double *q, const double *v,
+                                      const uint32_t *elem, double normalizer) {
+  const int K = mesh->K, D = mesh->dim;
+  hipStream_t s = stream();
+  const uint32_t *keys = nullptr;
+  int shift = 0;
+  static const bool morton = [] {
+    const char *env = std::getenv("PYR_DEPOSIT_MORTON");
+    return !env || std::atoi(env) != 0;
+  }();
+  if (morton && (uint64_t)K < ((uint64_t)1 << (32 - MORTON_BITS))) {
+    c->skeys.reserve(sizeof(uint32_t) * (size_t)n);
+    if (D == 2)
+      hipLaunchKernelGGL(k_sort_keys<2>, dim3(blocks_for(n)), dim3(256), 0, s, n, K, elem, x,
+                         c->geom.as<double>(), c->skeys.as<uint32_t>());
+    else
+      hipLaunchKernelGGL(k_sort_keys<3>, dim3(blocks_for(n)), dim3(256), 0, s, n, K, elem, x,
+                         c->geom.as<double>(), c->skeys.as<uint32_t>());
+    PYR_CHECK_LAUNCH();
+    keys = c->skeys.as<uint32_t>();
+    shift = MORTON_BITS;
+  }
+  const uint32_t *perm = sort_by_element(mesh, c, n, elem, keys, shift);
+  hipLaunchKernelGGL(k_stage, dim3(blocks_for(n)), dim3(256), 0, s, D, n, perm, x, q, v,
+                     normalizer, c->pos.as<double4>(), c->wgt.as<double4>());
+  PYR_CHECK_LAUNCH();
+  return perm;
 }
 
 template <int D, int NP, int NPW>
@@ -899,7 +993,7 @@ static void dispatch_gather(int D, int Np, const GatherArgs &ga, int K, int prec
       case 4: return launch_gather<3, 4, 4>(ga, K, precision, a2, count);
       case 10: return launch_gather<3, 10, 10>(ga, K, precision, a2, count);
       case 20:
-        switch (gather_npw(10)) {
+        switch (gather_npw(precision == PYR_FP64 ? 10 : 5)) {
           case 5: return launch_gather<3, 20, 5>(ga, K, precision, a2, count);
           case 7: return launch_gather<3, 20, 7>(ga, K, precision, a2, count);
           case 20: return launch_gather<3, 20, 20>(ga, K, precision, a2, count);
@@ -1024,17 +1118,15 @@ int pyr_deposit(pyr_mesh *mesh, double radius, double alpha, double normalizer, 
     prepare_tables(mesh, c, radius);
     c->pos.reserve(sizeof(double4) * (size_t)std::max<int64_t>(1, n));
     c->wgt.reserve(sizeof(double4) * (size_t)std::max<int64_t>(1, n));
-    if (n > 0) {
-      PYR_REQUIRE(x && q && elem, "null pointer");
+    if (n > 0) PYR_REQUIRE(x && q && elem, "null pointer");
+    {
       ProfScope prof(PROF_SORT);
-      const uint32_t *perm = sort_by_element(mesh, c, n, elem);
-      hipLaunchKernelGGL(k_stage, dim3(blocks_for(n)), dim3(256), 0, s, D, n, perm, x, q,
-                         j ? v : nullptr, normalizer, c->pos.as<double4>(),
-                         c->wgt.as<double4>());
-      PYR_CHECK_LAUNCH();
-    } else {
-      c->seg.reserve(sizeof(uint32_t) * ((size_t)K + 1));
-      PYR_HIP(hipMemsetAsync(c->seg.ptr, 0, sizeof(uint32_t) * ((size_t)K + 1), s));
+      if (n > 0) {
+        sort_and_stage(mesh, c, n, x, q, j ? v : nullptr, elem, normalizer);
+      } else {
+        c->seg.reserve(sizeof(uint32_t) * ((size_t)K + 1));
+        PYR_HIP(hipMemsetAsync(c->seg.ptr, 0, sizeof(uint32_t) * ((size_t)K + 1), s));
+      }
     }
     GatherArgs ga{};
     ga.K = K;
@@ -1094,11 +1186,7 @@ int pyr_deposit_backward(pyr_mesh *mesh, double radius, double alpha, double nor
     const uint32_t *perm;
     {
       ProfScope prof(PROF_SORT);
-      perm = sort_by_element(mesh, c, n, elem);
-      hipLaunchKernelGGL(k_stage, dim3(blocks_for(n)), dim3(256), 0, s, D, n, perm, x, q,
-                         j_bar ? v : nullptr, normalizer, c->pos.as<double4>(),
-                         c->wgt.as<double4>());
-      PYR_CHECK_LAUNCH();
+      perm = sort_and_stage(mesh, c, n, x, q, j_bar ? v : nullptr, elem, normalizer);
     }
     AdjArgs ga{};
     ga.K = K;
