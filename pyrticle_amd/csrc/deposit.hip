@@ -25,7 +25,7 @@
 #include <cmath>
 #include <thread>
 
-#include "pyr_internal.h"
+#include "deposit_kernels.h"
 
 namespace pyr {
 
@@ -135,299 +135,41 @@ __global__ void k_stage(int D, int64_t n, const uint32_t *__restrict__ perm,
   wgt[j] = make_double4(a, bs[0], bs[1], bs[2]);
 }
 
-// ---------------------------------------------------------------------------
-// gather kernel
-// ---------------------------------------------------------------------------
-struct GatherArgs {
-  int K, Np;
-  double R, invR, R2;
-  double alpha;
-  double shift_len[3];              // periodic extents (shift code 1: +L, 2: -L)
-  const uint32_t *cand_start;       // K + 1
-  const uint32_t *cand;             // source element ids
-  const uint8_t *cand_shift;        // 2 bits per axis
-  const uint32_t *seg;              // K + 1 segment starts into pos/wgt
-  const double4 *pos;               // staged positions
-  const double4 *wgt;               // staged weights
-  const double *geom;               // K x 8: centroid (3), (R + node radius)^2, vertex radius
-  const double *face_normal;        // mesh arrays
-  const double *face_rhs;
-  const void *node_tab;             // K x Np x 4 (T): n - c, |n - c|^2 / R
-  double *rho;                      // K*Np or null
-  double *j;                        // K*Np x D or null
-  unsigned long long *stats;        // [pairs evaluated, pairs inside R, candidates, passed]
-};
 
-template <int D>
-__device__ __forceinline__ void shift_vec(const GatherArgs &g, uint32_t code, double *s) {
-#pragma unroll
-  for (int a = 0; a < D; ++a) {
-    uint32_t c = (code >> (2 * a)) & 3u;
-    s[a] = c == 1 ? g.shift_len[a] : (c == 2 ? -g.shift_len[a] : 0.0);
+// FP32 staging of the FP32-compensated gather, same order as k_stage:
+//   pos32[j]             = (x_j - c_e, 0)  offset from the vertex centroid of
+//                          the particle's own element e, formed in FP64 and
+//                          rounded once;
+//   wgtT[4 (j/4) + c][j % 4] = w_j[c],  w_j = (a, a v0, a v1, a v2),
+//                          a = q * scale formed in FP64 (scale: the shape
+//                          normalizer times P^alpha, P = 2^ceil(log2 R), since
+//                          the gather evaluates S / P^alpha): the weights of
+//                          every aligned quad of particles, transposed.
+// Runs over n rounded up to a multiple of 4; the padding entries are zero.
+// Particles without a valid element sort last and are not referenced.
+__global__ void k_stage32(int D, int K, int64_t n, const uint32_t *__restrict__ perm,
+                          const uint32_t *__restrict__ elem, const double *__restrict__ x,
+                          const double *__restrict__ q, const double *__restrict__ v,
+                          double scale, const double *__restrict__ geom,
+                          float4 *__restrict__ pos, float *__restrict__ wgt_t) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ((n + 3) & ~(int64_t)3)) return;
+  float xs[3] = {0.f, 0.f, 0.f}, w[4] = {0.f, 0.f, 0.f, 0.f};
+  if (j < n) {
+    const uint32_t i = perm[j];
+    const uint32_t e = elem[i];
+    const double a = q[i] * scale;
+    w[0] = (float)a;
+    for (int k = 0; k < D; ++k) {
+      if (e < (uint32_t)K) xs[k] = (float)(x[(size_t)i * D + k] - geom[(size_t)e * 8 + k]);
+      if (v) w[1 + k] = (float)(a * v[(size_t)i * D + k]);
+    }
   }
+  pos[j] = make_float4(xs[0], xs[1], xs[2], 0.f);
+  const int64_t quad = j & ~(int64_t)3;
+  for (int c = 0; c < 4; ++c) wgt_t[(quad + c) * 4 + (j & 3)] = w[c];
 }
 
-// Kahan-compensated accumulator for the FP32 form
-struct Kahan {
-  float s, c;
-  __device__ __forceinline__ void add(float y) {
-    float yy = y - c;
-    float t = s + yy;
-    c = (t - s) - yy;
-    s = t;
-  }
-};
-
-// Block -> element map.  The hardware deals consecutive blocks round-robin to
-// the 8 XCDs; chunks of XCD_CHUNK consecutive elements (spatial neighbours that
-// share candidate particles) go to one XCD and its L2, and the chunks rotate
-// over the XCDs so that a dense region is spread over all of them.
-constexpr int XCD_CHUNK = 32;
-__device__ __forceinline__ int xcd_element(int bid) {
-  const int xcd = bid & 7, slot = bid >> 3;
-  const int chunk = (slot / XCD_CHUNK) * 8 + xcd;
-  return chunk * XCD_CHUNK + slot % XCD_CHUNK;
-}
-static inline unsigned gather_grid(int K) {
-  const int64_t per = 8 * XCD_CHUNK;
-  return (unsigned)(((int64_t)K + per - 1) / per * per);
-}
-
-// Gather per target element.  A block holds G waves, each owning NPW of the
-// element's nodes.  The block streams the candidate particles, drops those
-// whose ball cannot reach the element (bounding sphere of the nodes inflated
-// by R, face planes moved out by R), queues the rest in an LDS ring buffer and
-// evaluates full batches of 64 queued particles one particle per lane against
-// each wave's nodes (node data in scalar registers), accumulating rho and j
-// per node in registers.
-template <int D, int NP, int NPW, bool FP32, bool A2, bool COUNT>
-__global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherArgs g) {
-  using T = typename std::conditional<FP32, float, double>::type;
-  constexpr int G = (NP + NPW - 1) / NPW;   // waves per block = node groups
-  constexpr int BS = 64 * G;
-  constexpr int QCAP = 64 + BS;             // ring buffer of queued particles
-  constexpr int NV = D + 1;                 // rho + j components
-  constexpr int REC = 8;                    // record: m (3), c, a, b (3)
-
-  const int e = xcd_element(blockIdx.x);
-  if (e >= g.K) return;
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // uniform per wave
-  const int node0 = wave * NPW;
-  const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
-
-  __shared__ T q_rec[QCAP][REC];
-  __shared__ int wave_pass[G];
-
-  // target element geometry (uniform across the block)
-  const double cx = g.geom[(size_t)e * 8 + 0], cy = g.geom[(size_t)e * 8 + 1],
-               cz = g.geom[(size_t)e * 8 + 2], bound2 = g.geom[(size_t)e * 8 + 3];
-  double fnl[D + 1][D], frl[D + 1];
-#pragma unroll
-  for (int f = 0; f <= D; ++f) {
-#pragma unroll
-    for (int k = 0; k < D; ++k) fnl[f][k] = g.face_normal[((size_t)e * (D + 1) + f) * D + k];
-    frl[f] = g.face_rhs[(size_t)e * (D + 1) + f] + g.R * (1.0 + 1e-12);
-  }
-  // node data through the constant address space: uniform -> scalar loads
-  using CT = const T __attribute__((address_space(4))) *;
-  const CT ntab = (CT)(static_cast<const T *>(g.node_tab)) + (size_t)e * NP * 4;
-
-  T acc[NPW][NV];
-  Kahan kacc[FP32 ? NPW : 1][FP32 ? NV : 1];
-#pragma unroll
-  for (int i = 0; i < NPW; ++i)
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      acc[i][v] = T(0);
-      if constexpr (FP32) kacc[i][v] = Kahan{0.f, 0.f};
-    }
-  unsigned long long n_eval = 0, n_in = 0, n_cand = 0, n_pass = 0;
-  int q_head = 0, q_count = 0;   // identical in every thread of the block
-  bool any = false;
-  constexpr int FOLD = 8;        // FP32: batches per compensated fold
-  int n_fold = 0;
-
-  auto fold = [&]() {
-#pragma unroll
-    for (int i = 0; i < NPW; ++i)
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        kacc[i][v].add(acc[i][v]);
-        acc[i][v] = T(0);
-      }
-  };
-
-  auto process_batch = [&](int count) {
-    // one queued particle per lane, all of this wave's nodes
-    if (lane < count) {
-      int slot = q_head + lane;
-      if (slot >= QCAP) slot -= QCAP;
-      T rec[REC];
-#pragma unroll
-      for (int k = 0; k < REC; ++k) rec[k] = q_rec[slot][k];
-#pragma unroll
-      for (int i = 0; i < NPW; ++i) {
-        // compile-time true when the waves tile the nodes exactly, so that
-        // the node loads are not split by branches
-        if (NP % NPW == 0 || node0 + i < NP) {
-          const CT nd = ntab + (size_t)(node0 + i) * 4;
-          T t = rec[3] - nd[3];
-#pragma unroll
-          for (int k = D - 1; k >= 0; --k) t = fma(nd[k], rec[k], t);
-          T w = t > T(0) ? t : T(0);
-          T s;
-          if constexpr (A2)
-            s = w * w;
-          else
-            s = (T)pow((double)w, g.alpha);
-          if constexpr (COUNT) n_in += t > T(0) ? 1 : 0;
-          acc[i][0] = fma(rec[4], s, acc[i][0]);
-#pragma unroll
-          for (int k = 0; k < D; ++k) acc[i][1 + k] = fma(rec[5 + k], s, acc[i][1 + k]);
-        }
-      }
-      if constexpr (COUNT) n_eval += (unsigned long long)min(NPW, NP - node0);
-    }
-    if constexpr (FP32) {
-      // plain FP32 partials over at most FOLD particles per lane, then a
-      // compensated fold
-      if (++n_fold == FOLD) {
-        n_fold = 0;
-        fold();
-      }
-    }
-  };
-
-  const uint32_t c_begin = g.cand_start[e], c_end = g.cand_start[e + 1];
-  for (uint32_t cw = c_begin; cw < c_end; cw += 64) {
-    // window of 64 candidate source elements: segment starts, lengths, shifts
-    uint32_t len = 0, start = 0, sc = 0;
-    if (cw + lane < c_end) {
-      const uint32_t src = g.cand[cw + lane];
-      start = g.seg[src];
-      len = g.seg[src + 1] - start;
-      sc = g.cand_shift[cw + lane];
-    }
-    uint64_t nonempty = __ballot(len > 0);
-    while (nonempty) {
-      const int k = __ffsll((unsigned long long)nonempty) - 1;
-      nonempty &= nonempty - 1;
-      const uint32_t s0 = __builtin_amdgcn_readlane(start, k);
-      const uint32_t sl = __builtin_amdgcn_readlane(len, k);
-      const uint32_t code = __builtin_amdgcn_readlane(sc, k);
-      double sh[3] = {0, 0, 0};
-      if (code) shift_vec<D>(g, code, sh);
-      for (uint32_t base = 0; base < sl; base += BS) {
-        const uint32_t idx = base + tid;
-        bool pass = false;
-        T rec[REC];
-        if (idx < sl) {
-          const double4 pp = g.pos[s0 + idx];
-          const double x[3] = {pp.x + sh[0], pp.y + sh[1], D == 3 ? pp.z + sh[2] : 0.0};
-          const double d[3] = {x[0] - cx, x[1] - cy, x[2] - cz};
-          const double dd = fma(d[0], d[0], fma(d[1], d[1], D == 3 ? d[2] * d[2] : 0.0));
-          // conservative: the ball of radius R misses the element if x lies
-          // beyond one of its face planes by more than R
-          double viol = -INFINITY;
-#pragma unroll
-          for (int f = 0; f <= D; ++f) {
-            double ip = fnl[f][0] * x[0];
-#pragma unroll
-            for (int kk = 1; kk < D; ++kk) ip = fma(fnl[f][kk], x[kk], ip);
-            viol = fmax(viol, ip - frl[f]);
-          }
-          pass = (dd <= bound2) & (viol <= 0.0);
-          if constexpr (COUNT) n_cand += 1;
-          if (pass) {
-            if constexpr (COUNT) n_pass += 1;
-            const double4 ww = g.wgt[s0 + idx];
-            const double two_inv_r = 2.0 * g.invR;
-            rec[0] = (T)(d[0] * two_inv_r);
-            rec[1] = (T)(d[1] * two_inv_r);
-            rec[2] = (T)(d[2] * two_inv_r);
-            rec[3] = (T)fma(-dd, g.invR, g.R);
-            rec[4] = (T)ww.x;
-            rec[5] = (T)ww.y;
-            rec[6] = (T)ww.z;
-            rec[7] = (T)ww.w;
-          }
-        }
-        const uint64_t bal = __ballot(pass);
-        if (G > 1) {
-          if (lane == 0) wave_pass[wave] = __popcll(bal);
-          __syncthreads();
-        }
-        int off = q_count, added = 0;
-        if (G > 1) {
-#pragma unroll
-          for (int w = 0; w < G; ++w) {
-            if (w < wave) off += wave_pass[w];
-            added += wave_pass[w];
-          }
-        } else {
-          added = __popcll(bal);
-        }
-        if (pass) {
-          int slot = q_head + off + __popcll(bal & lt_mask);
-          if (slot >= QCAP) slot -= QCAP;
-#pragma unroll
-          for (int kk = 0; kk < REC; ++kk) q_rec[slot][kk] = rec[kk];
-        }
-        q_count += added;
-        __syncthreads();
-        while (q_count >= 64) {
-          any = true;
-          process_batch(64);
-          q_head += 64;
-          if (q_head >= QCAP) q_head -= QCAP;
-          q_count -= 64;
-          __syncthreads();   // the freed slots may be refilled next
-        }
-      }
-    }
-  }
-  if (q_count > 0) {
-    any = true;
-    process_batch(q_count);
-  }
-  if constexpr (FP32) fold();
-
-  // reduce across lanes in a fixed tree and write this wave's nodes
-#pragma unroll
-  for (int i = 0; i < NPW; ++i) {
-    const int node = node0 + i;
-    if (node >= NP) continue;
-    double out[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      double val;
-      if constexpr (FP32)
-        val = any ? (double)kacc[i][v].s - (double)kacc[i][v].c : 0.0;
-      else
-        val = any ? (double)acc[i][v] : 0.0;
-      if (any)
-        for (int o = 32; o > 0; o >>= 1) val += __shfl_xor(val, o, 64);
-      out[v] = val;
-    }
-    if (lane == 0) {
-      const size_t gn = (size_t)e * NP + node;
-      if (g.rho) g.rho[gn] = out[0];
-      if (g.j)
-#pragma unroll
-        for (int k = 0; k < D; ++k) g.j[gn * D + k] = out[1 + k];
-    }
-  }
-  if constexpr (COUNT) {
-    unsigned long long vals[4] = {n_eval, n_in, n_cand, n_pass};
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      unsigned long long v = vals[k];
-      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-      if (lane == 0 && v) atomicAdd(g.stats + k, v);
-    }
-  }
-}
 
 
 // sort keys for the deposition: element id, then a 12-bit Morton code of the
@@ -479,144 +221,6 @@ __global__ void k_node_table(int D, int K, int np, const double *__restrict__ no
   out[(size_t)t * 4 + 3] = (T)(s * invR);
 }
 
-// ---------------------------------------------------------------------------
-// adjoint of the deposition: per-particle gather of grad S . (rho_bar, j_bar)
-// ---------------------------------------------------------------------------
-// For particle p with a = q N and the nodes i within R:
-//   rho_i = sum_p a S_ip,  j_ia = sum_p a v_pa S_ip,  S = w^alpha, w = R - |n_i - x_p|^2 / R
-//   x_bar_p = (2/R) sum_i alpha w^(alpha-1) c_ip (n_i - x_p),  c_ip = a rho_bar_i + sum_a a v_pa j_bar_ia
-//   v_bar_pa = a sum_i j_bar_ia S_ip
-// Outputs are per particle, so a wave takes 64 particles of one source element
-// and walks the target elements that element can reach (the transpose of the
-// gather's candidate lists); target node data and adjoint densities are wave-
-// uniform and come through scalar loads.  No atomics.
-struct AdjArgs {
-  int K;
-  double R, invR, alpha;
-  double shift_len[3];
-  const uint32_t *tr_start;   // K + 1, per source element
-  const uint32_t *tr;         // target elements
-  const uint8_t *tr_shift;    // shift code applied to the source particle
-  const uint32_t *seg;        // K + 1 segment starts into pos/wgt
-  const uint32_t *perm;       // sorted position -> particle index
-  const double4 *pos, *wgt;
-  const double *geom;
-  const void *node_tab;
-  const double *rho_bar;      // K*Np or null
-  const double *j_bar;        // K*Np x D or null
-  double *x_bar, *v_bar;      // n x D (v_bar only with j_bar)
-};
-
-template <int D, int NP, bool FP32, bool A2>
-__global__ __launch_bounds__(64) void k_deposit_adj(AdjArgs g) {
-  using T = typename std::conditional<FP32, float, double>::type;
-  using CT = const T __attribute__((address_space(4))) *;
-  using CD = const double __attribute__((address_space(4))) *;
-  const int s = xcd_element(blockIdx.x);
-  if (s >= g.K) return;
-  const uint32_t p0 = g.seg[s], p1 = g.seg[s + 1];
-  if (p0 >= p1) return;
-  const int lane = threadIdx.x;
-  const uint32_t t_begin = g.tr_start[s], t_end = g.tr_start[s + 1];
-  const double two_inv_r = 2.0 * g.invR;
-  const CT ntab_all = (CT)(static_cast<const T *>(g.node_tab));
-  const bool has_rho = g.rho_bar != nullptr, has_j = g.j_bar != nullptr;
-  for (uint32_t base = p0; base < p1; base += 64) {
-    const uint32_t jj = base + lane;
-    const bool valid = jj < p1;
-    double4 pp = make_double4(0, 0, 0, 0), ww = make_double4(0, 0, 0, 0);
-    if (valid) {
-      pp = g.pos[jj];
-      ww = g.wgt[jj];
-    }
-    const T wa = (T)ww.x;
-    const T wv[3] = {(T)ww.y, (T)ww.z, (T)ww.w};
-    double gx[D], gv[D];
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      gx[k] = 0;
-      gv[k] = 0;
-    }
-    for (uint32_t ti = t_begin; ti < t_end; ++ti) {
-      const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.tr[ti]);
-      const uint32_t code = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.tr_shift[ti]);
-      double sh[3] = {0, 0, 0};
-      if (code) {
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-          const uint32_t cc = (code >> (2 * a)) & 3u;
-          sh[a] = cc == 1 ? g.shift_len[a] : (cc == 2 ? -g.shift_len[a] : 0.0);
-        }
-      }
-      const CD gt = (CD)g.geom + (size_t)t * 8;
-      const double d[3] = {pp.x + sh[0] - gt[0], pp.y + sh[1] - gt[1],
-                           D == 3 ? pp.z + sh[2] - gt[2] : 0.0};
-      const double dd = fma(d[0], d[0], fma(d[1], d[1], D == 3 ? d[2] * d[2] : 0.0));
-      const bool pass = valid && dd <= gt[3];
-      if (__ballot(pass) == 0) continue;
-      const T m[3] = {(T)(d[0] * two_inv_r), (T)(d[1] * two_inv_r), (T)(d[2] * two_inv_r)};
-      const T c0 = (T)fma(-dd, g.invR, g.R);
-      const CT nt = ntab_all + (size_t)t * NP * 4;
-      const CD rb = (CD)g.rho_bar + (size_t)t * NP;
-      const CD jb = (CD)g.j_bar + (size_t)t * NP * D;
-      T sD = T(0), sDn[D], sS[D];
-#pragma unroll
-      for (int k = 0; k < D; ++k) {
-        sDn[k] = T(0);
-        sS[k] = T(0);
-      }
-#pragma unroll
-      for (int i = 0; i < NP; ++i) {
-        T tt = c0 - nt[i * 4 + 3];
-#pragma unroll
-        for (int k = D - 1; k >= 0; --k) tt = fma(nt[i * 4 + k], m[k], tt);
-        const T w = (pass && tt > T(0)) ? tt : T(0);
-        T cp = has_rho ? wa * (T)rb[i] : T(0);
-        T jbi[D];
-        if (has_j) {
-#pragma unroll
-          for (int a = 0; a < D; ++a) {
-            jbi[a] = (T)jb[i * D + a];
-            cp = fma(wv[a], jbi[a], cp);
-          }
-        }
-        T dS, S;
-        if constexpr (A2) {
-          dS = w;   // dS/dw = 2w: the 2 is applied once at the end
-          S = w * w;
-        } else {
-          dS = w > T(0) ? (T)(g.alpha * pow((double)w, g.alpha - 1.0)) : T(0);
-          S = w > T(0) ? (T)pow((double)w, g.alpha) : T(0);
-        }
-        const T Dv = dS * cp;
-        sD += Dv;
-#pragma unroll
-        for (int k = 0; k < D; ++k) sDn[k] = fma(Dv, nt[i * 4 + k], sDn[k]);
-        if (has_j) {
-#pragma unroll
-          for (int a = 0; a < D; ++a) sS[a] = fma(jbi[a], S, sS[a]);
-        }
-      }
-      // fold the target's partial sums in FP64: sum_i D_i (n_i - d)
-#pragma unroll
-      for (int k = 0; k < D; ++k) {
-        gx[k] += (double)sDn[k] - d[k] * (double)sD;
-        gv[k] += (double)sS[k];
-      }
-    }
-    if (valid) {
-      const uint32_t idx = g.perm[jj];
-      const double fx = two_inv_r * (A2 ? 2.0 : 1.0);
-#pragma unroll
-      for (int k = 0; k < D; ++k) g.x_bar[(size_t)idx * D + k] = fx * gx[k];
-      if (g.v_bar) {
-#pragma unroll
-        for (int k = 0; k < D; ++k) g.v_bar[(size_t)idx * D + k] = ww.x * gv[k];
-      }
-    }
-  }
-}
-
 }  // namespace pyr
 
 using namespace pyr;
@@ -634,6 +238,8 @@ struct DepositCache {
   double tr_radius = -1;
   // sort / staging scratch
   DevBuf keys_a, keys_b, vals_a, vals_b, hist, hist_scan, counts, seg, tmp, pos, wgt, stats;
+  DevBuf pos32, wgt32;   // FP32 staging
+  DevBuf order;          // target elements in Morton order of their centroids
   DevBuf skeys;   // Morton sort keys
   std::vector<double> h_geom;   // K x 8
 };
@@ -644,7 +250,8 @@ void pyr_deposit_cache_free(pyr_mesh *mesh) {
   for (DevBuf *b : {&c->cand_start, &c->cand, &c->cand_shift, &c->geom, &c->tab64, &c->tab32,
                     &c->tab_R, &c->tr_start, &c->tr, &c->tr_shift, &c->keys_a, &c->keys_b,
                     &c->vals_a, &c->vals_b, &c->skeys, &c->hist,
-                    &c->hist_scan, &c->counts, &c->seg, &c->tmp, &c->pos, &c->wgt, &c->stats})
+                    &c->hist_scan, &c->counts, &c->seg, &c->tmp, &c->pos, &c->wgt, &c->stats,
+                    &c->pos32, &c->wgt32, &c->order})
     b->release();
   delete c;
   mesh->dep = nullptr;
@@ -852,6 +459,31 @@ static void build_candidates(pyr_mesh *h, DepositCache *c, double R) {
   upload(c->cand_shift, flat_s.data(), std::max<size_t>(1, flat_s.size()));
   upload(c->geom, c->h_geom.data(), c->h_geom.size());
   c->n_cand = starts[K];
+  // processing order of the targets: Morton order of the centroids (10 bits
+  // per axis over the mesh's bounding box), so that the blocks resident on an
+  // XCD at one time are spatial neighbours sharing candidate particles in L2
+  {
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int e = 0; e < K; ++e)
+      for (int i = 0; i < D; ++i) {
+        lo[i] = std::min(lo[i], cen[(size_t)e * 3 + i]);
+        hi[i] = std::max(hi[i], cen[(size_t)e * 3 + i]);
+      }
+    std::vector<std::pair<uint64_t, uint32_t>> keyed((size_t)K);
+    for (int e = 0; e < K; ++e) {
+      uint64_t code = 0;
+      for (int i = 0; i < D; ++i) {
+        const double span = hi[i] > lo[i] ? hi[i] - lo[i] : 1.0;
+        const uint64_t qv = (uint64_t)std::min(1023.0, std::max(0.0, (cen[(size_t)e * 3 + i] - lo[i]) / span * 1024.0));
+        for (int b = 0; b < 10; ++b) code |= ((qv >> b) & 1u) << (b * D + i);
+      }
+      keyed[e] = {code, (uint32_t)e};
+    }
+    std::sort(keyed.begin(), keyed.end());
+    std::vector<uint32_t> ord((size_t)K);
+    for (int e = 0; e < K; ++e) ord[e] = keyed[e].second;
+    upload(c->order, ord.data(), ord.size());
+  }
   c->radius = R;
 }
 
@@ -920,7 +552,9 @@ static const uint32_t *sort_by_element(pyr_mesh *h, DepositCache *c, int64_t n,
 // positions and weights in that order; returns the permutation.
 static const uint32_t *sort_and_stage(pyr_mesh *mesh, DepositCache *c, int64_t n,
                                       const double *x, const double *q, const double *v,
-                                      const uint32_t *elem, double normalizer) {
+                                      const uint32_t *elem, double normalizer,
+                                      bool stage64 = true, bool stage32 = false,
+                                      double radius = 1.0, double alpha = 1.0) {
   const int K = mesh->K, D = mesh->dim;
   hipStream_t s = stream();
   const uint32_t *keys = nullptr;
@@ -942,72 +576,29 @@ static const uint32_t *sort_and_stage(pyr_mesh *mesh, DepositCache *c, int64_t n
     shift = MORTON_BITS;
   }
   const uint32_t *perm = sort_by_element(mesh, c, n, elem, keys, shift);
-  hipLaunchKernelGGL(k_stage, dim3(blocks_for(n)), dim3(256), 0, s, D, n, perm, x, q, v,
-                     normalizer, c->pos.as<double4>(), c->wgt.as<double4>());
-  PYR_CHECK_LAUNCH();
+  if (stage64) {
+    c->pos.reserve(sizeof(double4) * (size_t)n);
+    c->wgt.reserve(sizeof(double4) * (size_t)n);
+    hipLaunchKernelGGL(k_stage, dim3(blocks_for(n)), dim3(256), 0, s, D, n, perm, x, q, v,
+                       normalizer, c->pos.as<double4>(), c->wgt.as<double4>());
+    PYR_CHECK_LAUNCH();
+  }
+  if (stage32) {
+    const int64_t n4 = (n + 3) & ~(int64_t)3;
+    c->pos32.reserve(sizeof(float4) * (size_t)n4);
+    c->wgt32.reserve(sizeof(float4) * (size_t)n4);
+    // the matrix-core gather evaluates S / P^alpha with P = 2^ceil(log2 R):
+    // P^alpha goes into the weights
+    int p_exp = 0;
+    const double p_mant = std::frexp(radius, &p_exp);
+    const double pscale = std::ldexp(1.0, p_mant == 0.5 ? p_exp - 1 : p_exp);
+    const double scale = normalizer * std::pow(pscale, alpha);
+    hipLaunchKernelGGL(k_stage32, dim3(blocks_for(n4)), dim3(256), 0, s, D, K, n, perm, elem, x,
+                       q, v, scale, c->geom.as<double>(), c->pos32.as<float4>(),
+                       c->wgt32.as<float>());
+    PYR_CHECK_LAUNCH();
+  }
   return perm;
-}
-
-template <int D, int NP, int NPW>
-static void launch_gather(const GatherArgs &ga, int K, int precision, bool a2, bool count) {
-  constexpr int G = (NP + NPW - 1) / NPW;
-  dim3 grid(gather_grid(K)), block(64 * G);
-  hipStream_t s = stream();
-#define PYR_G(FP, AA, CC) \
-  hipLaunchKernelGGL((k_gather<D, NP, NPW, FP, AA, CC>), grid, block, 0, s, ga)
-  if (precision == PYR_FP64) {
-    if (a2) {
-      if (count) PYR_G(false, true, true); else PYR_G(false, true, false);
-    } else {
-      if (count) PYR_G(false, false, true); else PYR_G(false, false, false);
-    }
-  } else {
-    if (a2) {
-      if (count) PYR_G(true, true, true); else PYR_G(true, true, false);
-    } else {
-      if (count) PYR_G(true, false, true); else PYR_G(true, false, false);
-    }
-  }
-#undef PYR_G
-  PYR_CHECK_LAUNCH();
-}
-
-// Nodes per wave of the 3D gather (launch-shape tuning; the result does not
-// depend on it beyond summation order across waves, which is fixed).
-static int gather_npw(int dflt) {
-  const char *env = std::getenv("PYR_GATHER_NPW");
-  return env ? std::atoi(env) : dflt;
-}
-
-static void dispatch_gather(int D, int Np, const GatherArgs &ga, int K, int precision, bool a2,
-                            bool count) {
-  if (D == 2) {
-    switch (Np) {
-      case 3: return launch_gather<2, 3, 3>(ga, K, precision, a2, count);
-      case 6: return launch_gather<2, 6, 6>(ga, K, precision, a2, count);
-      case 10: return launch_gather<2, 10, 10>(ga, K, precision, a2, count);
-      case 15: return launch_gather<2, 15, 15>(ga, K, precision, a2, count);
-    }
-  } else {
-    switch (Np) {
-      case 4: return launch_gather<3, 4, 4>(ga, K, precision, a2, count);
-      case 10: return launch_gather<3, 10, 10>(ga, K, precision, a2, count);
-      case 20:
-        switch (gather_npw(precision == PYR_FP64 ? 10 : 5)) {
-          case 5: return launch_gather<3, 20, 5>(ga, K, precision, a2, count);
-          case 7: return launch_gather<3, 20, 7>(ga, K, precision, a2, count);
-          case 20: return launch_gather<3, 20, 20>(ga, K, precision, a2, count);
-          default: return launch_gather<3, 20, 10>(ga, K, precision, a2, count);
-        }
-      case 35:
-        switch (gather_npw(12)) {
-          case 7: return launch_gather<3, 35, 7>(ga, K, precision, a2, count);
-          case 9: return launch_gather<3, 35, 9>(ga, K, precision, a2, count);
-          default: return launch_gather<3, 35, 12>(ga, K, precision, a2, count);
-        }
-    }
-  }
-  throw Error(PYR_ERR_INVALID_ARG, "deposition supports P1..P4 (2D) and P1..P4 (3D) elements");
 }
 
 // candidate lists and node tables for radius R (cached per radius)
@@ -1061,41 +652,18 @@ static void build_transpose(pyr_mesh *mesh, DepositCache *c) {
   c->tr_radius = c->radius;
 }
 
-template <int D, int NP>
-static void launch_deposit_adj(const AdjArgs &ga, int K, int precision, bool a2) {
-  dim3 grid(gather_grid(K)), block(64);
-  hipStream_t s = stream();
-  if (precision == PYR_FP64) {
-    if (a2) hipLaunchKernelGGL((k_deposit_adj<D, NP, false, true>), grid, block, 0, s, ga);
-    else hipLaunchKernelGGL((k_deposit_adj<D, NP, false, false>), grid, block, 0, s, ga);
-  } else {
-    if (a2) hipLaunchKernelGGL((k_deposit_adj<D, NP, true, true>), grid, block, 0, s, ga);
-    else hipLaunchKernelGGL((k_deposit_adj<D, NP, true, false>), grid, block, 0, s, ga);
-  }
-  PYR_CHECK_LAUNCH();
-}
-
-static void dispatch_deposit_adj(int D, int Np, const AdjArgs &ga, int K, int precision,
-                                 bool a2) {
-  if (D == 2) {
-    switch (Np) {
-      case 3: return launch_deposit_adj<2, 3>(ga, K, precision, a2);
-      case 6: return launch_deposit_adj<2, 6>(ga, K, precision, a2);
-      case 10: return launch_deposit_adj<2, 10>(ga, K, precision, a2);
-      case 15: return launch_deposit_adj<2, 15>(ga, K, precision, a2);
-    }
-  } else {
-    switch (Np) {
-      case 4: return launch_deposit_adj<3, 4>(ga, K, precision, a2);
-      case 10: return launch_deposit_adj<3, 10>(ga, K, precision, a2);
-      case 20: return launch_deposit_adj<3, 20>(ga, K, precision, a2);
-      case 35: return launch_deposit_adj<3, 35>(ga, K, precision, a2);
-    }
-  }
-  throw Error(PYR_ERR_INVALID_ARG, "deposition supports P1..P4 (2D) and P1..P4 (3D) elements");
-}
 
 static int g_deposit_count_mode = 0;
+
+namespace pyr {
+bool gather_use_order() {
+  static const bool on = [] {
+    const char *env = std::getenv("PYR_GATHER_ORDER");
+    return !env || std::atoi(env) != 0;
+  }();
+  return on;
+}
+}  // namespace pyr
 
 extern "C" {
 
@@ -1116,13 +684,16 @@ int pyr_deposit(pyr_mesh *mesh, double radius, double alpha, double normalizer, 
     const int K = mesh->K, Np = mesh->Np, D = mesh->dim;
     hipStream_t s = stream();
     prepare_tables(mesh, c, radius);
-    c->pos.reserve(sizeof(double4) * (size_t)std::max<int64_t>(1, n));
-    c->wgt.reserve(sizeof(double4) * (size_t)std::max<int64_t>(1, n));
+    for (DevBuf *b : {&c->pos, &c->wgt}) b->reserve(sizeof(double4) * (size_t)std::max<int64_t>(1, n));
+    for (DevBuf *b : {&c->pos32, &c->wgt32})
+      b->reserve(sizeof(float4) * (size_t)std::max<int64_t>(4, (n + 3) & ~(int64_t)3));
     if (n > 0) PYR_REQUIRE(x && q && elem, "null pointer");
     {
       ProfScope prof(PROF_SORT);
       if (n > 0) {
-        sort_and_stage(mesh, c, n, x, q, j ? v : nullptr, elem, normalizer);
+        const bool mf = precision == PYR_FP32C && gather_fp32_uses_mfma();
+        sort_and_stage(mesh, c, n, x, q, j ? v : nullptr, elem, normalizer, !mf, mf, radius,
+                       alpha);
       } else {
         c->seg.reserve(sizeof(uint32_t) * ((size_t)K + 1));
         PYR_HIP(hipMemsetAsync(c->seg.ptr, 0, sizeof(uint32_t) * ((size_t)K + 1), s));
@@ -1143,6 +714,11 @@ int pyr_deposit(pyr_mesh *mesh, double radius, double alpha, double normalizer, 
     ga.seg = c->seg.as<uint32_t>();
     ga.pos = c->pos.as<double4>();
     ga.wgt = c->wgt.as<double4>();
+    ga.pos32 = c->pos32.as<float4>();
+    ga.wgt32 = c->wgt32.as<float4>();
+    // the Morton processing order helps the matrix-core gather's L2 reuse; the
+    // FP64 gather runs fastest in element-id order
+    ga.order = (precision == PYR_FP32C && gather_use_order()) ? c->order.as<uint32_t>() : nullptr;
     ga.geom = c->geom.as<double>();
     ga.face_normal = mesh->dev.face_normal;
     ga.face_rhs = mesh->dev.face_rhs;
@@ -1154,7 +730,8 @@ int pyr_deposit(pyr_mesh *mesh, double radius, double alpha, double normalizer, 
     const bool count = g_deposit_count_mode != 0;
     if (count) PYR_HIP(hipMemsetAsync(ga.stats, 0, 4 * sizeof(unsigned long long), s));
     ProfScope prof(PROF_GATHER);
-    dispatch_gather(D, Np, ga, K, precision, alpha == 2.0, count);
+    if (precision == PYR_FP64) launch_gather_fp64(D, Np, ga, K, alpha == 2.0, count);
+    else launch_gather_fp32(D, Np, ga, K, alpha == 2.0, count);
   });
 }
 
@@ -1208,7 +785,7 @@ int pyr_deposit_backward(pyr_mesh *mesh, double radius, double alpha, double nor
     ga.x_bar = x_bar;
     ga.v_bar = j_bar ? v_bar : nullptr;
     ProfScope prof(PROF_DEPOSIT_ADJ);
-    dispatch_deposit_adj(D, Np, ga, K, precision, alpha == 2.0);
+    launch_deposit_adj(D, Np, ga, K, precision, alpha == 2.0);
   });
 }
 

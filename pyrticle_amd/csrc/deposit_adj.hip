@@ -1,0 +1,39 @@
+// deposit_adj.hip -- instantiations of the deposition adjoint (k_deposit_adj).
+#include "deposit_kernels.h"
+
+namespace pyr {
+
+template <int D, int NP>
+static void launch(const AdjArgs &ga, int K, int precision, bool a2) {
+  dim3 grid(gather_grid(K)), block(64);
+  hipStream_t s = stream();
+  if (precision == PYR_FP64) {
+    if (a2) hipLaunchKernelGGL((k_deposit_adj<D, NP, false, true>), grid, block, 0, s, ga);
+    else hipLaunchKernelGGL((k_deposit_adj<D, NP, false, false>), grid, block, 0, s, ga);
+  } else {
+    if (a2) hipLaunchKernelGGL((k_deposit_adj<D, NP, true, true>), grid, block, 0, s, ga);
+    else hipLaunchKernelGGL((k_deposit_adj<D, NP, true, false>), grid, block, 0, s, ga);
+  }
+  PYR_CHECK_LAUNCH();
+}
+
+void launch_deposit_adj(int D, int Np, const AdjArgs &ga, int K, int precision, bool a2) {
+  if (D == 2) {
+    switch (Np) {
+      case 3: return launch<2, 3>(ga, K, precision, a2);
+      case 6: return launch<2, 6>(ga, K, precision, a2);
+      case 10: return launch<2, 10>(ga, K, precision, a2);
+      case 15: return launch<2, 15>(ga, K, precision, a2);
+    }
+  } else {
+    switch (Np) {
+      case 4: return launch<3, 4>(ga, K, precision, a2);
+      case 10: return launch<3, 10>(ga, K, precision, a2);
+      case 20: return launch<3, 20>(ga, K, precision, a2);
+      case 35: return launch<3, 35>(ga, K, precision, a2);
+    }
+  }
+  throw Error(PYR_ERR_INVALID_ARG, "deposition supports P1..P4 (2D) and P1..P4 (3D) elements");
+}
+
+}  // namespace pyr

@@ -1,0 +1,738 @@
+// deposit_kernels.h -- device code of the shape-function deposition, shared by
+// the translation units that instantiate it: gather_fp64.hip (FP64 gather),
+// gather_fp32.hip (FP32-compensated gather on the matrix cores) and
+// deposit_adj.hip (adjoint).  Host code and the sort live in deposit.hip.
+#pragma once
+
+#include <type_traits>
+
+#include "pyr_internal.h"
+
+namespace pyr {
+
+// ---------------------------------------------------------------------------
+// gather kernel
+// ---------------------------------------------------------------------------
+struct GatherArgs {
+  int K, Np;
+  double R, invR, R2;
+  double alpha;
+  double shift_len[3];              // periodic extents (shift code 1: +L, 2: -L)
+  const uint32_t *cand_start;       // K + 1
+  const uint32_t *cand;             // source element ids
+  const uint8_t *cand_shift;        // 2 bits per axis
+  const uint32_t *seg;              // K + 1 segment starts into pos/wgt
+  const double4 *pos;               // staged positions
+  const double4 *wgt;               // staged weights
+  const float4 *pos32;              // FP32 staging: x - c_src (k_stage32)
+  const float4 *wgt32;              //   (q N, q N v) in FP32, quads transposed
+  const uint32_t *order;            // processing order of target elements, or null
+  const double *geom;               // K x 8: centroid (3), (R + node radius)^2, vertex radius
+  const double *face_normal;        // mesh arrays
+  const double *face_rhs;
+  const void *node_tab;             // K x Np x 4 (T): n - c, |n - c|^2 / R
+  double *rho;                      // K*Np or null
+  double *j;                        // K*Np x D or null
+  unsigned long long *stats;        // [pairs evaluated, pairs inside R, candidates, passed]
+};
+
+template <int D>
+__device__ __forceinline__ void shift_vec(const GatherArgs &g, uint32_t code, double *s) {
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    uint32_t c = (code >> (2 * a)) & 3u;
+    s[a] = c == 1 ? g.shift_len[a] : (c == 2 ? -g.shift_len[a] : 0.0);
+  }
+}
+
+// Kahan-compensated accumulator for the FP32 form
+struct Kahan {
+  float s, c;
+  __device__ __forceinline__ void add(float y) {
+    float yy = y - c;
+    float t = s + yy;
+    c = (t - s) - yy;
+    s = t;
+  }
+};
+
+// Block -> element map.  The hardware deals consecutive blocks round-robin to
+// the 8 XCDs; chunks of XCD_CHUNK consecutive elements (spatial neighbours that
+// share candidate particles) go to one XCD and its L2, and the chunks rotate
+// over the XCDs so that a dense region is spread over all of them.
+constexpr int XCD_CHUNK = 32;
+__device__ __forceinline__ int xcd_element(int bid) {
+  const int xcd = bid & 7, slot = bid >> 3;
+  const int chunk = (slot / XCD_CHUNK) * 8 + xcd;
+  return chunk * XCD_CHUNK + slot % XCD_CHUNK;
+}
+static inline unsigned gather_grid(int K) {
+  const int64_t per = 8 * XCD_CHUNK;
+  return (unsigned)(((int64_t)K + per - 1) / per * per);
+}
+
+// Gather per target element.  A block holds G waves, each owning NPW of the
+// element's nodes.  The block streams the candidate particles, drops those
+// whose ball cannot reach the element (bounding sphere of the nodes inflated
+// by R, face planes moved out by R), queues the rest in an LDS ring buffer and
+// evaluates full batches of 64 queued particles one particle per lane against
+// each wave's nodes (node data in scalar registers), accumulating rho and j
+// per node in registers.
+template <int D, int NP, int NPW, bool FP32, bool A2, bool COUNT>
+__global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherArgs g) {
+  using T = typename std::conditional<FP32, float, double>::type;
+  constexpr int G = (NP + NPW - 1) / NPW;   // waves per block = node groups
+  constexpr int BS = 64 * G;
+  constexpr int QCAP = 64 + BS;             // ring buffer of queued particles
+  constexpr int NV = D + 1;                 // rho + j components
+  constexpr int REC = 8;                    // record: m (3), c, a, b (3)
+
+  const int slot_e = xcd_element(blockIdx.x);
+  if (slot_e >= g.K) return;
+  const int e = g.order ? (int)g.order[slot_e] : slot_e;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // uniform per wave
+  const int node0 = wave * NPW;
+  const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
+
+  __shared__ T q_rec[QCAP][REC];
+  __shared__ int wave_pass[G];
+
+  // target element geometry (uniform across the block)
+  const double cx = g.geom[(size_t)e * 8 + 0], cy = g.geom[(size_t)e * 8 + 1],
+               cz = g.geom[(size_t)e * 8 + 2], bound2 = g.geom[(size_t)e * 8 + 3];
+  double fnl[D + 1][D], frl[D + 1];
+#pragma unroll
+  for (int f = 0; f <= D; ++f) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) fnl[f][k] = g.face_normal[((size_t)e * (D + 1) + f) * D + k];
+    frl[f] = g.face_rhs[(size_t)e * (D + 1) + f] + g.R * (1.0 + 1e-12);
+  }
+  // node data through the constant address space: uniform -> scalar loads
+  using CT = const T __attribute__((address_space(4))) *;
+  const CT ntab = (CT)(static_cast<const T *>(g.node_tab)) + (size_t)e * NP * 4;
+
+  T acc[NPW][NV];
+  Kahan kacc[FP32 ? NPW : 1][FP32 ? NV : 1];
+#pragma unroll
+  for (int i = 0; i < NPW; ++i)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      acc[i][v] = T(0);
+      if constexpr (FP32) kacc[i][v] = Kahan{0.f, 0.f};
+    }
+  unsigned long long n_eval = 0, n_in = 0, n_cand = 0, n_pass = 0;
+  int q_head = 0, q_count = 0;   // identical in every thread of the block
+  bool any = false;
+  constexpr int FOLD = 8;        // FP32: batches per compensated fold
+  int n_fold = 0;
+
+  auto fold = [&]() {
+#pragma unroll
+    for (int i = 0; i < NPW; ++i)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        kacc[i][v].add(acc[i][v]);
+        acc[i][v] = T(0);
+      }
+  };
+
+  auto process_batch = [&](int count) {
+    // one queued particle per lane, all of this wave's nodes
+    if (lane < count) {
+      int slot = q_head + lane;
+      if (slot >= QCAP) slot -= QCAP;
+      T rec[REC];
+#pragma unroll
+      for (int k = 0; k < REC; ++k) rec[k] = q_rec[slot][k];
+#pragma unroll
+      for (int i = 0; i < NPW; ++i) {
+        // compile-time true when the waves tile the nodes exactly, so that
+        // the node loads are not split by branches
+        if (NP % NPW == 0 || node0 + i < NP) {
+          const CT nd = ntab + (size_t)(node0 + i) * 4;
+          T t = rec[3] - nd[3];
+#pragma unroll
+          for (int k = D - 1; k >= 0; --k) t = fma(nd[k], rec[k], t);
+          T w = t > T(0) ? t : T(0);
+          T s;
+          if constexpr (A2)
+            s = w * w;
+          else
+            s = (T)pow((double)w, g.alpha);
+          if constexpr (COUNT) n_in += t > T(0) ? 1 : 0;
+          acc[i][0] = fma(rec[4], s, acc[i][0]);
+#pragma unroll
+          for (int k = 0; k < D; ++k) acc[i][1 + k] = fma(rec[5 + k], s, acc[i][1 + k]);
+        }
+      }
+      if constexpr (COUNT) n_eval += (unsigned long long)min(NPW, NP - node0);
+    }
+    if constexpr (FP32) {
+      // plain FP32 partials over at most FOLD particles per lane, then a
+      // compensated fold
+      if (++n_fold == FOLD) {
+        n_fold = 0;
+        fold();
+      }
+    }
+  };
+
+  const uint32_t c_begin = g.cand_start[e], c_end = g.cand_start[e + 1];
+  for (uint32_t cw = c_begin; cw < c_end; cw += 64) {
+    // window of 64 candidate source elements: segment starts, lengths, shifts
+    uint32_t len = 0, start = 0, sc = 0;
+    if (cw + lane < c_end) {
+      const uint32_t src = g.cand[cw + lane];
+      start = g.seg[src];
+      len = g.seg[src + 1] - start;
+      sc = g.cand_shift[cw + lane];
+    }
+    uint64_t nonempty = __ballot(len > 0);
+    while (nonempty) {
+      const int k = __ffsll((unsigned long long)nonempty) - 1;
+      nonempty &= nonempty - 1;
+      const uint32_t s0 = __builtin_amdgcn_readlane(start, k);
+      const uint32_t sl = __builtin_amdgcn_readlane(len, k);
+      const uint32_t code = __builtin_amdgcn_readlane(sc, k);
+      double sh[3] = {0, 0, 0};
+      if (code) shift_vec<D>(g, code, sh);
+      for (uint32_t base = 0; base < sl; base += BS) {
+        const uint32_t idx = base + tid;
+        bool pass = false;
+        T rec[REC];
+        if (idx < sl) {
+          const double4 pp = g.pos[s0 + idx];
+          const double x[3] = {pp.x + sh[0], pp.y + sh[1], D == 3 ? pp.z + sh[2] : 0.0};
+          const double d[3] = {x[0] - cx, x[1] - cy, x[2] - cz};
+          const double dd = fma(d[0], d[0], fma(d[1], d[1], D == 3 ? d[2] * d[2] : 0.0));
+          // conservative: the ball of radius R misses the element if x lies
+          // beyond one of its face planes by more than R
+          double viol = -INFINITY;
+#pragma unroll
+          for (int f = 0; f <= D; ++f) {
+            double ip = fnl[f][0] * x[0];
+#pragma unroll
+            for (int kk = 1; kk < D; ++kk) ip = fma(fnl[f][kk], x[kk], ip);
+            viol = fmax(viol, ip - frl[f]);
+          }
+          pass = (dd <= bound2) & (viol <= 0.0);
+          if constexpr (COUNT) n_cand += 1;
+          if (pass) {
+            if constexpr (COUNT) n_pass += 1;
+            const double4 ww = g.wgt[s0 + idx];
+            const double two_inv_r = 2.0 * g.invR;
+            rec[0] = (T)(d[0] * two_inv_r);
+            rec[1] = (T)(d[1] * two_inv_r);
+            rec[2] = (T)(d[2] * two_inv_r);
+            rec[3] = (T)fma(-dd, g.invR, g.R);
+            rec[4] = (T)ww.x;
+            rec[5] = (T)ww.y;
+            rec[6] = (T)ww.z;
+            rec[7] = (T)ww.w;
+          }
+        }
+        const uint64_t bal = __ballot(pass);
+        if (G > 1) {
+          if (lane == 0) wave_pass[wave] = __popcll(bal);
+          __syncthreads();
+        }
+        int off = q_count, added = 0;
+        if (G > 1) {
+#pragma unroll
+          for (int w = 0; w < G; ++w) {
+            if (w < wave) off += wave_pass[w];
+            added += wave_pass[w];
+          }
+        } else {
+          added = __popcll(bal);
+        }
+        if (pass) {
+          int slot = q_head + off + __popcll(bal & lt_mask);
+          if (slot >= QCAP) slot -= QCAP;
+#pragma unroll
+          for (int kk = 0; kk < REC; ++kk) q_rec[slot][kk] = rec[kk];
+        }
+        q_count += added;
+        __syncthreads();
+        while (q_count >= 64) {
+          any = true;
+          process_batch(64);
+          q_head += 64;
+          if (q_head >= QCAP) q_head -= QCAP;
+          q_count -= 64;
+          __syncthreads();   // the freed slots may be refilled next
+        }
+      }
+    }
+  }
+  if (q_count > 0) {
+    any = true;
+    process_batch(q_count);
+  }
+  if constexpr (FP32) fold();
+
+  // reduce across lanes in a fixed tree and write this wave's nodes
+#pragma unroll
+  for (int i = 0; i < NPW; ++i) {
+    const int node = node0 + i;
+    if (node >= NP) continue;
+    double out[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      double val;
+      if constexpr (FP32)
+        val = any ? (double)kacc[i][v].s - (double)kacc[i][v].c : 0.0;
+      else
+        val = any ? (double)acc[i][v] : 0.0;
+      if (any)
+        for (int o = 32; o > 0; o >>= 1) val += __shfl_xor(val, o, 64);
+      out[v] = val;
+    }
+    if (lane == 0) {
+      const size_t gn = (size_t)e * NP + node;
+      if (g.rho) g.rho[gn] = out[0];
+      if (g.j)
+#pragma unroll
+        for (int k = 0; k < D; ++k) g.j[gn * D + k] = out[1 + k];
+    }
+  }
+  if constexpr (COUNT) {
+    unsigned long long vals[4] = {n_eval, n_in, n_cand, n_pass};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      unsigned long long v = vals[k];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0 && v) atomicAdd(g.stats + k, v);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP32-compensated gather on the matrix cores, one particle per lane
+// ---------------------------------------------------------------------------
+// Staging (k_stage32): particle j of the element-sorted order is stored as
+//   pos32[j]            = (x_j - c_src, 0)  in FP32 (offset from the centroid
+//                         of its own element, formed in FP64),
+//   wgtT[4 (j/4) + c]   = (w_{4(j/4)}c, w_{4(j/4)+1}c, w_{4(j/4)+2}c, w_{4(j/4)+3}c)
+// with w_j = (q N, q N v) in FP32: the weights of each aligned quad of
+// particles are stored transposed.  The offset of a candidate source element
+// from the target centroid, c_src + shift - c_e, is formed in FP64 once per
+// source element, so every FP32 offset d = x - c_e stays small.
+//
+// Streaming: the particle segments of the target's candidate source elements
+// (64 at a time, one per lane) are widened to aligned quads and concatenated
+// into one flat index range, walked in chunks of 64 (one particle per lane)
+// dealt round-robin to the block's waves.  A wave-uniform cursor finds the
+// segment of a chunk's first index; segment data reach the lanes through
+// readlane (scalar), with a short uniform loop over the segments that start
+// inside the chunk.  The loads of the next chunk are in flight while the
+// current one is evaluated.  Lanes outside their segment (quad padding) or
+// past the end get a particle that no node can reach.  There is no
+// prefilter: every candidate particle is evaluated against every node on the
+// matrix cores, and pairs farther apart than R give S = 0.
+//
+// Pairs, per group of four nodes n = 4 ng + (l % 4), with
+// v_mfma_f32_4x4x1_16b_f32 (16 independent 4x4 outer products, block b =
+// lanes 4b..4b+3):
+//   t[p][n]  = (R - |n - x_p|^2 / R) / P,   P = 2^ceil(log2 R) >= R (exact scaling)
+//            = -|n - c_e|^2 / (R P) + m_p . (n - c_e) / P + c_p,
+//              m_p = 2 d_p / R,  c_p = (R - |d_p|^2 / R) / P  (c_p enters as
+//              the last term: it carries the large cancelling part),
+//              as C + sum_k A[p][k] B[k][n] over A = (m, c_p), B = ((n - c_e)/P, 1);
+//              t <= R / P <= 1, so max(t, 0) is one clamp to [0, 1] (v_med3);
+//              A of block b, row i is particle 4b + i = the lane's own particle,
+//              so one instruction covers 64 particles x 4 nodes;
+//              t lands in VGPR i of lane 4b + j: particle 4b + i, node j;
+//   S        = max(t, 0)^alpha  (P^alpha is folded into the staged weights);
+//   acc[v][n] += sum_i wgtT_i S_i   (four K = 1 products, i = 0..3): the A
+//              operand of block b is w_{4b+i} spread over its four rows, which
+//              is component i of the lane's own wgtT load.
+// Each block accumulates its own particles; FP32 partial sums over FOLD steps
+// are folded into Kahan-compensated sums, and the sixteen blocks and the waves
+// are combined in FP64 in a fixed order, so the result is bitwise reproducible.
+typedef float pyr_v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ pyr_v4f mfma4x4(float a, float b, pyr_v4f c) {
+  return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 0);
+}
+
+template <int D, int NP, int W, bool A2>
+__global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
+  constexpr int NG = (NP + 3) / 4;   // node groups of four
+  constexpr int FOLD = 8;            // steps (four particles per block each) per fold
+  constexpr float FAR = 1e30f;       // -c of a particle no node can reach
+
+  const int slot_e = xcd_element(blockIdx.x);
+  if (slot_e >= g.K) return;
+  const int e = g.order ? (int)g.order[slot_e] : slot_e;
+  const uint32_t c_begin = g.cand_start[e], c_end = g.cand_start[e + 1];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+
+  __shared__ double red[W][NG * 4][4];
+
+  const double cx = g.geom[(size_t)e * 8 + 0], cy = g.geom[(size_t)e * 8 + 1],
+               cz = g.geom[(size_t)e * 8 + 2];
+  const float two_inv_r = (float)(2.0 * g.invR), inv_rf = (float)g.invR, rf = (float)g.R;
+  // P = 2^ceil(log2 R): scaling by 1/P is exact
+  int p_exp = 0;
+  const double p_mant = frexp(g.R, &p_exp);   // R = p_mant 2^p_exp, p_mant in [0.5, 1)
+  const float inv_p = (float)ldexp(1.0, -(p_mant == 0.5 ? p_exp - 1 : p_exp));
+
+  // node constants of this lane's node in every group
+  float bx[NG], by[NG], bz[NG], cc[NG], node_on[NG];
+#pragma unroll
+  for (int ng = 0; ng < NG; ++ng) {
+    const int n = 4 * ng + (lane & 3);
+    bx[ng] = by[ng] = bz[ng] = 0.f;
+    cc[ng] = -1.f;
+    node_on[ng] = n < NP ? 1.f : 0.f;
+    if (n < NP) {
+      const float *nd = static_cast<const float *>(g.node_tab) + ((size_t)e * NP + n) * 4;
+      bx[ng] = nd[0] * inv_p;
+      by[ng] = nd[1] * inv_p;
+      bz[ng] = D == 3 ? nd[2] * inv_p : 0.f;
+      cc[ng] = -nd[3] * inv_p;
+    }
+  }
+
+  pyr_v4f cinit[NG];   // C operand of the first product: -|n - c_e|^2 / (R P) in every row
+#pragma unroll
+  for (int ng = 0; ng < NG; ++ng) cinit[ng] = pyr_v4f{cc[ng], cc[ng], cc[ng], cc[ng]};
+  pyr_v4f acc[NG];
+  float ks[NG][4], kc[NG][4];
+#pragma unroll
+  for (int ng = 0; ng < NG; ++ng) {
+    acc[ng] = pyr_v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int v = 0; v < 4; ++v) ks[ng][v] = kc[ng][v] = 0.f;
+  }
+  int n_steps = 0;
+
+  auto fold = [&]() {
+#pragma unroll
+    for (int ng = 0; ng < NG; ++ng)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const float y = acc[ng][v] - kc[ng][v];
+        const float t = ks[ng][v] + y;
+        kc[ng][v] = (t - ks[ng][v]) - y;
+        ks[ng][v] = t;
+      }
+#pragma unroll
+    for (int ng = 0; ng < NG; ++ng) acc[ng] = pyr_v4f{0.f, 0.f, 0.f, 0.f};
+  };
+
+  // particle data of one chunk: m = 2 d / R (3), c = (R - |d|^2 / R) / P, and
+  // the quad weights
+  struct Part {
+    float mx, my, mz, cp;
+    float4 w;
+  };
+  // loads in flight for one chunk, before the offsets are applied
+  struct Raw {
+    float4 dp, w;
+    float ox, oy, oz;
+    bool in_seg;
+  };
+  auto derive = [&](const Raw &r) {
+    Part P;
+    const float dx = r.ox + r.dp.x, dy = r.oy + r.dp.y, dz = D == 3 ? r.oz + r.dp.z : 0.f;
+    const float dd = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    P.mx = dx * two_inv_r;
+    P.my = dy * two_inv_r;
+    P.mz = dz * two_inv_r;
+    // padding lanes keep finite m and get a c that no node can overcome
+    P.cp = r.in_seg ? fmaf(-dd, inv_rf, rf) * inv_p : -FAR;
+    P.w = r.w;
+    return P;
+  };
+  auto evaluate = [&](const Part &P) {
+#pragma unroll
+    for (int ng = 0; ng < NG; ++ng) {
+      pyr_v4f t = mfma4x4(P.mx, bx[ng], cinit[ng]);
+      t = mfma4x4(P.my, by[ng], t);
+      if (D == 3) t = mfma4x4(P.mz, bz[ng], t);
+      t = mfma4x4(P.cp, node_on[ng], t);
+      float s[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float tw = __builtin_amdgcn_fmed3f(t[i], 0.f, 1.f);
+        if constexpr (A2)
+          s[i] = tw * tw;
+        else
+          s[i] = tw > 0.f ? __powf(tw, (float)g.alpha) : 0.f;
+      }
+      acc[ng] = mfma4x4(P.w.x, s[0], acc[ng]);
+      acc[ng] = mfma4x4(P.w.y, s[1], acc[ng]);
+      acc[ng] = mfma4x4(P.w.z, s[2], acc[ng]);
+      acc[ng] = mfma4x4(P.w.w, s[3], acc[ng]);
+    }
+    if (++n_steps == FOLD) {
+      n_steps = 0;
+      fold();
+    }
+  };
+
+  for (uint32_t cw = c_begin; cw < c_end; cw += 64) {
+    // window of up to 64 candidate source elements, one per lane: aligned
+    // segment start a0, the segment [lo, hi), and the offset of the source
+    // centroid from the target centroid (every wave holds the same window)
+    uint32_t alen = 0, a0 = 0, lo = 0, hi = 0;
+    float ofx = 0.f, ofy = 0.f, ofz = 0.f;
+    if (cw + lane < c_end) {
+      const uint32_t src = g.cand[cw + lane];
+      lo = g.seg[src];
+      hi = g.seg[src + 1];
+      if (hi > lo) {
+        a0 = lo & ~3u;
+        alen = ((hi + 3u) & ~3u) - a0;
+        double sh[3] = {0, 0, 0};
+        const uint32_t code = g.cand_shift[cw + lane];
+        if (code) shift_vec<D>(g, code, sh);
+        ofx = (float)(g.geom[(size_t)src * 8 + 0] + sh[0] - cx);
+        ofy = (float)(g.geom[(size_t)src * 8 + 1] + sh[1] - cy);
+        ofz = D == 3 ? (float)(g.geom[(size_t)src * 8 + 2] + sh[2] - cz) : 0.f;
+      }
+    }
+    // inclusive scan of the aligned lengths; the flat range is their concatenation
+    uint32_t incl = alen;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += y;
+    }
+    const uint32_t excl = incl - alen;
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    auto rl = [](uint32_t v, int k) { return (uint32_t)__builtin_amdgcn_readlane((int)v, k); };
+    auto rlf = [](float v, int k) {
+      return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+    };
+
+    // chunks of 64 flat indices, dealt round-robin to the waves; kc is the
+    // segment holding the chunk's first index (wave-uniform, monotone)
+    int kc = 0;
+    auto issue = [&](uint32_t base) {
+      while (rl(incl, kc) <= base) ++kc;
+      const uint32_t last = min(base + 63u, total - 1u);
+      int kl = kc;
+      while (rl(incl, kl) <= last) ++kl;
+      // segment data of this lane's index: the first segment, then the ones
+      // that start inside the chunk (usually none)
+      const uint32_t f = base + (uint32_t)lane;
+      uint32_t sa = rl(a0, kc), se = rl(excl, kc), sl = rl(lo, kc), sh2 = rl(hi, kc);
+      Raw r;
+      r.ox = rlf(ofx, kc);
+      r.oy = rlf(ofy, kc);
+      r.oz = rlf(ofz, kc);
+      for (int kk = kc + 1; kk <= kl; ++kk) {
+        const bool in = f >= rl(excl, kk);
+        sa = in ? rl(a0, kk) : sa;
+        se = in ? rl(excl, kk) : se;
+        sl = in ? rl(lo, kk) : sl;
+        sh2 = in ? rl(hi, kk) : sh2;
+        r.ox = in ? rlf(ofx, kk) : r.ox;
+        r.oy = in ? rlf(ofy, kk) : r.oy;
+        r.oz = in ? rlf(ofz, kk) : r.oz;
+      }
+      const bool valid = f < total;
+      const uint32_t j = valid ? sa + (f - se) : (uint32_t)(lane & 3);
+      r.in_seg = valid && j >= sl && j < sh2;
+      r.dp = g.pos32[j];
+      r.w = g.wgt32[j];
+      return r;
+    };
+
+    uint32_t base = (uint32_t)wave * 64u;
+    if (base < total) {
+      Raw raw = issue(base);
+      for (;;) {
+        const Part cur = derive(raw);
+        const uint32_t nb = base + (uint32_t)(W * 64);
+        const bool more = nb < total;   // wave-uniform
+        if (more) raw = issue(nb);
+        evaluate(cur);
+        if (!more) break;
+        base = nb;
+      }
+    }
+  }
+  fold();
+
+  // combine: sixteen blocks by a fixed butterfly, then the waves in order
+#pragma unroll
+  for (int ng = 0; ng < NG; ++ng)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      double val = (double)ks[ng][v] - (double)kc[ng][v];
+#pragma unroll
+      for (int o = 4; o < 64; o <<= 1) val += __shfl_xor(val, o, 64);
+      if (lane < 4) red[wave][4 * ng + lane][v] = val;
+    }
+  __syncthreads();
+  if (tid < NP) {
+    double out[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) out[v] += red[w][tid][v];
+    const size_t gn = (size_t)e * NP + tid;
+    if (g.rho) g.rho[gn] = out[0];
+    if (g.j)
+#pragma unroll
+      for (int k = 0; k < D; ++k) g.j[gn * D + k] = out[1 + k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// adjoint of the deposition: per-particle gather of grad S . (rho_bar, j_bar)
+// ---------------------------------------------------------------------------
+// For particle p with a = q N and the nodes i within R:
+//   rho_i = sum_p a S_ip,  j_ia = sum_p a v_pa S_ip,  S = w^alpha, w = R - |n_i - x_p|^2 / R
+//   x_bar_p = (2/R) sum_i alpha w^(alpha-1) c_ip (n_i - x_p),  c_ip = a rho_bar_i + sum_a a v_pa j_bar_ia
+//   v_bar_pa = a sum_i j_bar_ia S_ip
+// Outputs are per particle, so a wave takes 64 particles of one source element
+// and walks the target elements that element can reach (the transpose of the
+// gather's candidate lists); target node data and adjoint densities are wave-
+// uniform and come through scalar loads.  No atomics.
+struct AdjArgs {
+  int K;
+  double R, invR, alpha;
+  double shift_len[3];
+  const uint32_t *tr_start;   // K + 1, per source element
+  const uint32_t *tr;         // target elements
+  const uint8_t *tr_shift;    // shift code applied to the source particle
+  const uint32_t *seg;        // K + 1 segment starts into pos/wgt
+  const uint32_t *perm;       // sorted position -> particle index
+  const double4 *pos, *wgt;
+  const double *geom;
+  const void *node_tab;
+  const double *rho_bar;      // K*Np or null
+  const double *j_bar;        // K*Np x D or null
+  double *x_bar, *v_bar;      // n x D (v_bar only with j_bar)
+};
+
+template <int D, int NP, bool FP32, bool A2>
+__global__ __launch_bounds__(64) void k_deposit_adj(AdjArgs g) {
+  using T = typename std::conditional<FP32, float, double>::type;
+  using CT = const T __attribute__((address_space(4))) *;
+  using CD = const double __attribute__((address_space(4))) *;
+  const int s = xcd_element(blockIdx.x);
+  if (s >= g.K) return;
+  const uint32_t p0 = g.seg[s], p1 = g.seg[s + 1];
+  if (p0 >= p1) return;
+  const int lane = threadIdx.x;
+  const uint32_t t_begin = g.tr_start[s], t_end = g.tr_start[s + 1];
+  const double two_inv_r = 2.0 * g.invR;
+  const CT ntab_all = (CT)(static_cast<const T *>(g.node_tab));
+  const bool has_rho = g.rho_bar != nullptr, has_j = g.j_bar != nullptr;
+  for (uint32_t base = p0; base < p1; base += 64) {
+    const uint32_t jj = base + lane;
+    const bool valid = jj < p1;
+    double4 pp = make_double4(0, 0, 0, 0), ww = make_double4(0, 0, 0, 0);
+    if (valid) {
+      pp = g.pos[jj];
+      ww = g.wgt[jj];
+    }
+    const T wa = (T)ww.x;
+    const T wv[3] = {(T)ww.y, (T)ww.z, (T)ww.w};
+    double gx[D], gv[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      gx[k] = 0;
+      gv[k] = 0;
+    }
+    for (uint32_t ti = t_begin; ti < t_end; ++ti) {
+      const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.tr[ti]);
+      const uint32_t code = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.tr_shift[ti]);
+      double sh[3] = {0, 0, 0};
+      if (code) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          const uint32_t cc = (code >> (2 * a)) & 3u;
+          sh[a] = cc == 1 ? g.shift_len[a] : (cc == 2 ? -g.shift_len[a] : 0.0);
+        }
+      }
+      const CD gt = (CD)g.geom + (size_t)t * 8;
+      const double d[3] = {pp.x + sh[0] - gt[0], pp.y + sh[1] - gt[1],
+                           D == 3 ? pp.z + sh[2] - gt[2] : 0.0};
+      const double dd = fma(d[0], d[0], fma(d[1], d[1], D == 3 ? d[2] * d[2] : 0.0));
+      const bool pass = valid && dd <= gt[3];
+      if (__ballot(pass) == 0) continue;
+      const T m[3] = {(T)(d[0] * two_inv_r), (T)(d[1] * two_inv_r), (T)(d[2] * two_inv_r)};
+      const T c0 = (T)fma(-dd, g.invR, g.R);
+      const CT nt = ntab_all + (size_t)t * NP * 4;
+      const CD rb = (CD)g.rho_bar + (size_t)t * NP;
+      const CD jb = (CD)g.j_bar + (size_t)t * NP * D;
+      T sD = T(0), sDn[D], sS[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        sDn[k] = T(0);
+        sS[k] = T(0);
+      }
+#pragma unroll
+      for (int i = 0; i < NP; ++i) {
+        T tt = c0 - nt[i * 4 + 3];
+#pragma unroll
+        for (int k = D - 1; k >= 0; --k) tt = fma(nt[i * 4 + k], m[k], tt);
+        const T w = (pass && tt > T(0)) ? tt : T(0);
+        T cp = has_rho ? wa * (T)rb[i] : T(0);
+        T jbi[D];
+        if (has_j) {
+#pragma unroll
+          for (int a = 0; a < D; ++a) {
+            jbi[a] = (T)jb[i * D + a];
+            cp = fma(wv[a], jbi[a], cp);
+          }
+        }
+        T dS, S;
+        if constexpr (A2) {
+          dS = w;   // dS/dw = 2w: the 2 is applied once at the end
+          S = w * w;
+        } else {
+          dS = w > T(0) ? (T)(g.alpha * pow((double)w, g.alpha - 1.0)) : T(0);
+          S = w > T(0) ? (T)pow((double)w, g.alpha) : T(0);
+        }
+        const T Dv = dS * cp;
+        sD += Dv;
+#pragma unroll
+        for (int k = 0; k < D; ++k) sDn[k] = fma(Dv, nt[i * 4 + k], sDn[k]);
+        if (has_j) {
+#pragma unroll
+          for (int a = 0; a < D; ++a) sS[a] = fma(jbi[a], S, sS[a]);
+        }
+      }
+      // fold the target's partial sums in FP64: sum_i D_i (n_i - d)
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        gx[k] += (double)sDn[k] - d[k] * (double)sD;
+        gv[k] += (double)sS[k];
+      }
+    }
+    if (valid) {
+      const uint32_t idx = g.perm[jj];
+      const double fx = two_inv_r * (A2 ? 2.0 : 1.0);
+#pragma unroll
+      for (int k = 0; k < D; ++k) g.x_bar[(size_t)idx * D + k] = fx * gx[k];
+      if (g.v_bar) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) g.v_bar[(size_t)idx * D + k] = ww.x * gv[k];
+      }
+    }
+  }
+}
+
+
+// Target-element processing order on (PYR_GATHER_ORDER, default 1), and
+// whether the FP32-compensated gather runs on the matrix cores (PYR_GATHER_MFMA,
+// default 1; it then reads the FP32 staging).
+bool gather_use_order();
+bool gather_fp32_uses_mfma();
+
+// Launchers, one per translation unit.  precision: PYR_FP64 or PYR_FP32C.
+void launch_gather_fp64(int D, int Np, const GatherArgs &ga, int K, bool a2, bool count);
+void launch_gather_fp32(int D, int Np, const GatherArgs &ga, int K, bool a2, bool count);
+void launch_deposit_adj(int D, int Np, const AdjArgs &ga, int K, int precision, bool a2);
+
+}  // namespace pyr
