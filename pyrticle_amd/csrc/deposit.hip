@@ -136,7 +136,7 @@ __global__ void k_stage(int D, int64_t n, const uint32_t *__restrict__ perm,
 }
 
 
-// FP32 staging of the FP32-compensated gather, same order as k_stage:
+// FP32 staging of the FP32-compensated gather, in element-sorted order:
 //   pos32[j]             = (x_j - c_e, 0)  offset from the vertex centroid of
 //                          the particle's own element e, formed in FP64 and
 //                          rounded once;
@@ -169,6 +169,8 @@ __global__ void k_stage32(int D, int K, int64_t n, const uint32_t *__restrict__ 
   const int64_t quad = j & ~(int64_t)3;
   for (int c = 0; c < 4; ++c) wgt_t[(quad + c) * 4 + (j & 3)] = w[c];
 }
+
+
 
 
 
@@ -239,7 +241,6 @@ struct DepositCache {
   // sort / staging scratch
   DevBuf keys_a, keys_b, vals_a, vals_b, hist, hist_scan, counts, seg, tmp, pos, wgt, stats;
   DevBuf pos32, wgt32;   // FP32 staging
-  DevBuf order;          // target elements in Morton order of their centroids
   DevBuf skeys;   // Morton sort keys
   std::vector<double> h_geom;   // K x 8
 };
@@ -251,7 +252,7 @@ void pyr_deposit_cache_free(pyr_mesh *mesh) {
                     &c->tab_R, &c->tr_start, &c->tr, &c->tr_shift, &c->keys_a, &c->keys_b,
                     &c->vals_a, &c->vals_b, &c->skeys, &c->hist,
                     &c->hist_scan, &c->counts, &c->seg, &c->tmp, &c->pos, &c->wgt, &c->stats,
-                    &c->pos32, &c->wgt32, &c->order})
+                    &c->pos32, &c->wgt32})
     b->release();
   delete c;
   mesh->dep = nullptr;
@@ -459,36 +460,49 @@ static void build_candidates(pyr_mesh *h, DepositCache *c, double R) {
   upload(c->cand_shift, flat_s.data(), std::max<size_t>(1, flat_s.size()));
   upload(c->geom, c->h_geom.data(), c->h_geom.size());
   c->n_cand = starts[K];
-  // processing order of the targets: Morton order of the centroids (10 bits
-  // per axis over the mesh's bounding box), so that the blocks resident on an
-  // XCD at one time are spatial neighbours sharing candidate particles in L2
-  {
-    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int e = 0; e < K; ++e)
-      for (int i = 0; i < D; ++i) {
-        lo[i] = std::min(lo[i], cen[(size_t)e * 3 + i]);
-        hi[i] = std::max(hi[i], cen[(size_t)e * 3 + i]);
-      }
-    std::vector<std::pair<uint64_t, uint32_t>> keyed((size_t)K);
-    for (int e = 0; e < K; ++e) {
-      uint64_t code = 0;
-      for (int i = 0; i < D; ++i) {
-        const double span = hi[i] > lo[i] ? hi[i] - lo[i] : 1.0;
-        const uint64_t qv = (uint64_t)std::min(1023.0, std::max(0.0, (cen[(size_t)e * 3 + i] - lo[i]) / span * 1024.0));
-        for (int b = 0; b < 10; ++b) code |= ((qv >> b) & 1u) << (b * D + i);
-      }
-      keyed[e] = {code, (uint32_t)e};
-    }
-    std::sort(keyed.begin(), keyed.end());
-    std::vector<uint32_t> ord((size_t)K);
-    for (int e = 0; e < K; ++e) ord[e] = keyed[e].second;
-    upload(c->order, ord.data(), ord.size());
-  }
   c->radius = R;
 }
 
 static unsigned blocks_for(int64_t n, int bs = 256) {
   return (unsigned)std::max<int64_t>(1, (n + bs - 1) / bs);
+}
+
+// Stable LSD radix sort of the indices 0..n-1 by keys clamped to [0, kmax]
+// (8-bit digits); returns the sorted indices (in va or vb).
+static const uint32_t *radix_sort(DepositCache *c, int64_t n, const uint32_t *keys,
+                                  uint32_t kmax, DevBuf &ka, DevBuf &kb, DevBuf &va,
+                                  DevBuf &vb) {
+  hipStream_t s = stream();
+  const int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
+  int bits = 1;
+  while (bits < 32 && ((uint64_t)1 << bits) <= (uint64_t)kmax) ++bits;
+  const int passes = (bits + 7) / 8;
+  for (DevBuf *b : {&ka, &kb, &va, &vb}) b->reserve(sizeof(uint32_t) * n);
+  c->hist.reserve(sizeof(uint32_t) * 256 * ntiles);
+  c->hist_scan.reserve(sizeof(uint32_t) * 256 * ntiles);
+  c->tmp.reserve(sizeof(uint32_t) * scan_tmp_words(256 * ntiles));
+  const uint32_t *kin = keys, *vin = nullptr;
+  uint32_t *kout = ka.as<uint32_t>(), *vout = va.as<uint32_t>();
+  for (int p = 0; p < passes; ++p) {
+    hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)ntiles), dim3(RS_BS), 0, s, kin, n, 8 * p,
+                       c->hist.as<uint32_t>(), (int)ntiles, kmax);
+    PYR_CHECK_LAUNCH();
+    scan_exclusive_u32(c->hist.as<uint32_t>(), c->hist_scan.as<uint32_t>(), 256 * ntiles,
+                       nullptr, c->tmp.as<uint32_t>(), s);
+    hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)ntiles), dim3(RS_BS), 0, s, kin, vin, kout,
+                       vout, n, 8 * p, c->hist_scan.as<uint32_t>(), (int)ntiles, kmax);
+    PYR_CHECK_LAUNCH();
+    kin = kout;
+    vin = vout;
+    if (kout == ka.as<uint32_t>()) {
+      kout = kb.as<uint32_t>();
+      vout = vb.as<uint32_t>();
+    } else {
+      kout = ka.as<uint32_t>();
+      vout = va.as<uint32_t>();
+    }
+  }
+  return vin;
 }
 
 // Stable radix sort of particle indices by keys (element ids, or element ids
@@ -514,42 +528,15 @@ static const uint32_t *sort_by_element(pyr_mesh *h, DepositCache *c, int64_t n,
     keys = elem;
     key_shift = 0;
   }
-  const uint32_t kmax = (uint32_t)K << key_shift;   // keys are clamped to [0, kmax]
-  int bits = 1;
-  while (bits < 32 && ((uint64_t)1 << bits) <= (uint64_t)kmax) ++bits;
-  const int passes = (bits + 7) / 8;
-  c->keys_a.reserve(sizeof(uint32_t) * n);
-  c->keys_b.reserve(sizeof(uint32_t) * n);
-  c->vals_a.reserve(sizeof(uint32_t) * n);
-  c->vals_b.reserve(sizeof(uint32_t) * n);
-  c->hist.reserve(sizeof(uint32_t) * 256 * ntiles);
-  c->hist_scan.reserve(sizeof(uint32_t) * 256 * ntiles);
-  const uint32_t *kin = keys, *vin = nullptr;
-  uint32_t *kout = c->keys_a.as<uint32_t>(), *vout = c->vals_a.as<uint32_t>();
-  for (int p = 0; p < passes; ++p) {
-    hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)ntiles), dim3(RS_BS), 0, s, kin, n, 8 * p,
-                       c->hist.as<uint32_t>(), (int)ntiles, kmax);
-    PYR_CHECK_LAUNCH();
-    scan_exclusive_u32(c->hist.as<uint32_t>(), c->hist_scan.as<uint32_t>(), 256 * ntiles,
-                       nullptr, c->tmp.as<uint32_t>(), s);
-    hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)ntiles), dim3(RS_BS), 0, s, kin, vin, kout,
-                       vout, n, 8 * p, c->hist_scan.as<uint32_t>(), (int)ntiles, kmax);
-    PYR_CHECK_LAUNCH();
-    kin = kout;
-    vin = vout;
-    if (kout == c->keys_a.as<uint32_t>()) {
-      kout = c->keys_b.as<uint32_t>();
-      vout = c->vals_b.as<uint32_t>();
-    } else {
-      kout = c->keys_a.as<uint32_t>();
-      vout = c->vals_a.as<uint32_t>();
-    }
-  }
-  return vin;
+  return radix_sort(c, n, keys, (uint32_t)K << key_shift, c->keys_a, c->keys_b, c->vals_a,
+                    c->vals_b);
 }
 
-// Sorts the particles by (element, Morton code inside the element) and stages
-// positions and weights in that order; returns the permutation.
+// Sorts the particles by element and stages them in that order; returns the
+// permutation.  The FP64 staging sorts by (element, Morton code inside the
+// element), so that the VALU gather's queued batches are spatially coherent;
+// the FP32 staging (matrix-core gather, which takes whole segments) sorts by
+// element alone, one radix pass fewer.
 static const uint32_t *sort_and_stage(pyr_mesh *mesh, DepositCache *c, int64_t n,
                                       const double *x, const double *q, const double *v,
                                       const uint32_t *elem, double normalizer,
@@ -563,7 +550,7 @@ static const uint32_t *sort_and_stage(pyr_mesh *mesh, DepositCache *c, int64_t n
     const char *env = std::getenv("PYR_DEPOSIT_MORTON");
     return !env || std::atoi(env) != 0;
   }();
-  if (morton && (uint64_t)K < ((uint64_t)1 << (32 - MORTON_BITS))) {
+  if (morton && stage64 && (uint64_t)K < ((uint64_t)1 << (32 - MORTON_BITS))) {
     c->skeys.reserve(sizeof(uint32_t) * (size_t)n);
     if (D == 2)
       hipLaunchKernelGGL(k_sort_keys<2>, dim3(blocks_for(n)), dim3(256), 0, s, n, K, elem, x,
@@ -655,15 +642,7 @@ static void build_transpose(pyr_mesh *mesh, DepositCache *c) {
 
 static int g_deposit_count_mode = 0;
 
-namespace pyr {
-bool gather_use_order() {
-  static const bool on = [] {
-    const char *env = std::getenv("PYR_GATHER_ORDER");
-    return !env || std::atoi(env) != 0;
-  }();
-  return on;
-}
-}  // namespace pyr
+
 
 extern "C" {
 
@@ -716,9 +695,6 @@ int pyr_deposit(pyr_mesh *mesh, double radius, double alpha, double normalizer, 
     ga.wgt = c->wgt.as<double4>();
     ga.pos32 = c->pos32.as<float4>();
     ga.wgt32 = c->wgt32.as<float4>();
-    // the Morton processing order helps the matrix-core gather's L2 reuse; the
-    // FP64 gather runs fastest in element-id order
-    ga.order = (precision == PYR_FP32C && gather_use_order()) ? c->order.as<uint32_t>() : nullptr;
     ga.geom = c->geom.as<double>();
     ga.face_normal = mesh->dev.face_normal;
     ga.face_rhs = mesh->dev.face_rhs;

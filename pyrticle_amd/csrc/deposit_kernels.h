@@ -26,7 +26,6 @@ struct GatherArgs {
   const double4 *wgt;               // staged weights
   const float4 *pos32;              // FP32 staging: x - c_src (k_stage32)
   const float4 *wgt32;              //   (q N, q N v) in FP32, quads transposed
-  const uint32_t *order;            // processing order of target elements, or null
   const double *geom;               // K x 8: centroid (3), (R + node radius)^2, vertex radius
   const double *face_normal;        // mesh arrays
   const double *face_rhs;
@@ -87,9 +86,8 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
   constexpr int NV = D + 1;                 // rho + j components
   constexpr int REC = 8;                    // record: m (3), c, a, b (3)
 
-  const int slot_e = xcd_element(blockIdx.x);
-  if (slot_e >= g.K) return;
-  const int e = g.order ? (int)g.order[slot_e] : slot_e;
+  const int e = xcd_element(blockIdx.x);
+  if (e >= g.K) return;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // uniform per wave
   const int node0 = wave * NPW;
@@ -349,8 +347,9 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
 //              operand of block b is w_{4b+i} spread over its four rows, which
 //              is component i of the lane's own wgtT load.
 // Each block accumulates its own particles; FP32 partial sums over FOLD steps
-// are folded into Kahan-compensated sums, and the sixteen blocks and the waves
-// are combined in FP64 in a fixed order, so the result is bitwise reproducible.
+// are folded into Kahan-compensated sums (kept in LDS, one pair per lane and
+// entry), and the sixteen blocks and the waves are combined in FP64 in a fixed
+// order, so the result is bitwise reproducible.
 typedef float pyr_v4f __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ pyr_v4f mfma4x4(float a, float b, pyr_v4f c) {
@@ -363,14 +362,14 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
   constexpr int FOLD = 8;            // steps (four particles per block each) per fold
   constexpr float FAR = 1e30f;       // -c of a particle no node can reach
 
-  const int slot_e = xcd_element(blockIdx.x);
-  if (slot_e >= g.K) return;
-  const int e = g.order ? (int)g.order[slot_e] : slot_e;
+  const int e = xcd_element(blockIdx.x);
+  if (e >= g.K) return;
   const uint32_t c_begin = g.cand_start[e], c_end = g.cand_start[e + 1];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
   __shared__ double red[W][NG * 4][4];
+  __shared__ float2 kahan[W][NG * 4][64];   // per-lane compensated sums (s, c)
 
   const double cx = g.geom[(size_t)e * 8 + 0], cy = g.geom[(size_t)e * 8 + 1],
                cz = g.geom[(size_t)e * 8 + 2];
@@ -381,13 +380,13 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
   const float inv_p = (float)ldexp(1.0, -(p_mant == 0.5 ? p_exp - 1 : p_exp));
 
   // node constants of this lane's node in every group
-  float bx[NG], by[NG], bz[NG], cc[NG], node_on[NG];
+  // (padded nodes: B = (0, 1), C = -1, so t = c_p - 1 <= 0 and S = 0)
+  float bx[NG], by[NG], bz[NG], cc[NG];
 #pragma unroll
   for (int ng = 0; ng < NG; ++ng) {
     const int n = 4 * ng + (lane & 3);
     bx[ng] = by[ng] = bz[ng] = 0.f;
     cc[ng] = -1.f;
-    node_on[ng] = n < NP ? 1.f : 0.f;
     if (n < NP) {
       const float *nd = static_cast<const float *>(g.node_tab) + ((size_t)e * NP + n) * 4;
       bx[ng] = nd[0] * inv_p;
@@ -401,12 +400,14 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
 #pragma unroll
   for (int ng = 0; ng < NG; ++ng) cinit[ng] = pyr_v4f{cc[ng], cc[ng], cc[ng], cc[ng]};
   pyr_v4f acc[NG];
-  float ks[NG][4], kc[NG][4];
+  // Kahan sums (s, c) of every lane's accumulator entries live in LDS, each
+  // lane touching only its own entries (no barriers needed)
+  float2 *kl = &kahan[wave][0][lane];
 #pragma unroll
   for (int ng = 0; ng < NG; ++ng) {
     acc[ng] = pyr_v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int v = 0; v < 4; ++v) ks[ng][v] = kc[ng][v] = 0.f;
+    for (int v = 0; v < 4; ++v) kl[(ng * 4 + v) * 64] = make_float2(0.f, 0.f);
   }
   int n_steps = 0;
 
@@ -415,10 +416,12 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
     for (int ng = 0; ng < NG; ++ng)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const float y = acc[ng][v] - kc[ng][v];
-        const float t = ks[ng][v] + y;
-        kc[ng][v] = (t - ks[ng][v]) - y;
-        ks[ng][v] = t;
+        float2 k = kl[(ng * 4 + v) * 64];
+        const float y = acc[ng][v] - k.y;
+        const float t = k.x + y;
+        k.y = (t - k.x) - y;
+        k.x = t;
+        kl[(ng * 4 + v) * 64] = k;
       }
 #pragma unroll
     for (int ng = 0; ng < NG; ++ng) acc[ng] = pyr_v4f{0.f, 0.f, 0.f, 0.f};
@@ -454,7 +457,7 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
       pyr_v4f t = mfma4x4(P.mx, bx[ng], cinit[ng]);
       t = mfma4x4(P.my, by[ng], t);
       if (D == 3) t = mfma4x4(P.mz, bz[ng], t);
-      t = mfma4x4(P.cp, node_on[ng], t);
+      t = mfma4x4(P.cp, 1.f, t);
       float s[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -565,7 +568,8 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
   for (int ng = 0; ng < NG; ++ng)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      double val = (double)ks[ng][v] - (double)kc[ng][v];
+      const float2 k = kl[(ng * 4 + v) * 64];
+      double val = (double)k.x - (double)k.y;
 #pragma unroll
       for (int o = 4; o < 64; o <<= 1) val += __shfl_xor(val, o, 64);
       if (lane < 4) red[wave][4 * ng + lane][v] = val;
@@ -724,10 +728,8 @@ __global__ __launch_bounds__(64) void k_deposit_adj(AdjArgs g) {
 }
 
 
-// Target-element processing order on (PYR_GATHER_ORDER, default 1), and
-// whether the FP32-compensated gather runs on the matrix cores (PYR_GATHER_MFMA,
-// default 1; it then reads the FP32 staging).
-bool gather_use_order();
+// Whether the FP32-compensated gather runs on the matrix cores
+// (PYR_GATHER_MFMA, default 1; it then reads the FP32 staging).
 bool gather_fp32_uses_mfma();
 
 // Launchers, one per translation unit.  precision: PYR_FP64 or PYR_FP32C.

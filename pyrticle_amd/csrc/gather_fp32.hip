@@ -63,10 +63,10 @@ void launch_gather_fp32(int D, int Np, const GatherArgs &ga, int K, bool a2, boo
       case 4: return launch_mf<3, 4, 4>(ga, K, a2);
       case 10: return launch_mf<3, 10, 4>(ga, K, a2);
       case 20:
-        switch (env_int("PYR_GATHER_WAVES", 4)) {
-          case 2: return launch_mf<3, 20, 2>(ga, K, a2);
-          case 8: return launch_mf<3, 20, 8>(ga, K, a2);
-          default: return launch_mf<3, 20, 4>(ga, K, a2);
+        switch (env_int("PYR_GATHER_WAVES", 2)) {
+          case 1: return launch_mf<3, 20, 1>(ga, K, a2);
+          case 4: return launch_mf<3, 20, 4>(ga, K, a2);
+          default: return launch_mf<3, 20, 2>(ga, K, a2);
         }
       case 35: return launch_mf<3, 35, 4>(ga, K, a2);
     }
