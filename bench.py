@@ -217,6 +217,83 @@ def adjoint_leg(wl, dm, device, torch, normalizer, n, nsteps, ckpt, fd_n):
     return res
 
 
+def dense_p4_leg(device, torch, steps, warmup, n_particles):
+    """Dense-tile interpolation leg: 49,896 P4 tets (Np=35), 16M electrons uniform
+    in the cylinder (about 320 per element), element-sorted.  Times the fused
+    LSRK4 push (5 stages, interpolation of 6 fields per stage) in FP64, in FP32C
+    with the VALU contraction and in FP32C with the matrix-core contraction
+    (full waves inside one element), and reports the FP32C errors against FP64."""
+    from pyrticle_amd import device as dv
+    from pyrticle_amd import workloads as W
+    t0 = time.perf_counter()
+    wl = W.cylinder_dense_p4(n_particles=n_particles)
+    dm = dv.DeviceMesh(wl.ma)
+    dm.set_static_fields([torch.from_numpy(f).to(device) for f in wl.fields])
+    st0, _ = build_state(wl, dm, device, torch)
+    occupied = int(torch.unique(st0["elem"]).numel())
+    log("dense P4 setup %.1fs: K=%d Np=%d n=%d, %.0f particles per occupied element"
+        % (time.perf_counter() - t0, wl.ma.K, wl.ma.Np, st0["n"], st0["n"] / occupied))
+    res = {"workload": "cylinder_dense_p4: %d P4 tets (Np=35), %d electrons uniform in the "
+                       "cylinder, gamma=10, Bz=0.5 T, Ez=1e6 V/m, dt=0.2*h_min/c; fused LSRK4 "
+                       "push only" % (wl.ma.K, st0["n"]),
+           "particles_per_occupied_element": st0["n"] / occupied, "steps": steps}
+    finals = {}
+    try:
+        for name, prec, mf in (("fp64", dv.FP64, False), ("fp32c_valu", dv.FP32C, False),
+                               ("fp32c_mfma", dv.FP32C, True)):
+            dv.set_push_mfma(mf)
+            if mf:   # untimed pass: share of particle-stages on the matrix cores
+                dv.set_push_mfma(True, count=True)
+                dv.push_mfma_waves()
+                st = clone_state(st0)
+                n_b = st["n"]
+                st["n"] = dm.lsrk4_steps(1, wl.dt, wl.c, st["n"], st["x"], st["p"], st["q"],
+                                         st["m"], st["elem"], precision=prec)
+                res["mfma_share_of_particle_stages"] = dv.push_mfma_waves() / (5.0 * n_b)
+                dv.set_push_mfma(True)
+            st = clone_state(st0)
+            for _ in range(warmup):
+                st["n"] = dm.lsrk4_steps(1, wl.dt, wl.c, st["n"], st["x"], st["p"], st["q"],
+                                         st["m"], st["elem"], precision=prec)
+            st = clone_state(st0)
+            torch.cuda.synchronize()
+            dv.profile_enable(True)
+            t1 = time.perf_counter()
+            alive = 0
+            for _ in range(steps):
+                st["n"] = dm.lsrk4_steps(1, wl.dt, wl.c, st["n"], st["x"], st["p"], st["q"],
+                                         st["m"], st["elem"], precision=prec)
+                alive += st["n"]
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t1
+            push_ms, push_n = dv.profile_read("push")
+            dv.profile_enable(False)
+            res[name] = {"particle_steps_per_s": alive / wall,
+                         "ms_per_step": 1e3 * wall / steps,
+                         "push_kernel_ms": push_ms / max(1, push_n)}
+            finals[name] = st
+            log("dense P4 %s: %.3f ms/step, push kernel %.3f ms" % (
+                name, 1e3 * wall / steps, push_ms / max(1, push_n)))
+    finally:
+        dv.set_push_mfma(True)
+    ref = finals["fp64"]
+    for name in ("fp32c_valu", "fp32c_mfma"):
+        st = finals[name]
+        if st["n"] == ref["n"]:
+            n = st["n"]
+            dp_ref = ref["p"][:n] - st0["p"][:n]
+            res[name]["x_rel_l2_vs_fp64"] = float(torch.linalg.norm(st["x"][:n] - ref["x"][:n])
+                                                  / torch.linalg.norm(ref["x"][:n]))
+            res[name]["dp_rel_l2_vs_fp64"] = float(torch.linalg.norm(
+                (st["p"][:n] - st0["p"][:n]) - dp_ref) / torch.linalg.norm(dp_ref))
+    res["mfma_speedup_vs_valu"] = (res["fp32c_mfma"]["particle_steps_per_s"]
+                                   / res["fp32c_valu"]["particle_steps_per_s"])
+    # the contraction the matrix cores take: 6 fields x 35 monomials per particle-stage
+    contraction = 2 * 6 * 35 * 5 * alive / steps
+    res["contraction_tflops_mfma"] = contraction / (res["fp32c_mfma"]["push_kernel_ms"] * 1e-3) / 1e12
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -232,6 +309,10 @@ def main(argv=None):
                     help="only run two 512 MiB device-to-device copies (the reference "
                          "kernel for scaling FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--no-dense", action="store_true",
+                    help="skip the dense P4 interpolation leg (16M particles, 49,896 tets)")
+    ap.add_argument("--dense-particles", type=int, default=16_000_000)
+    ap.add_argument("--dense-steps", type=int, default=5)
     args = ap.parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -337,6 +418,11 @@ def main(argv=None):
                               args.adjoint_steps, 20, 10_000)
         dm.set_static_fields([torch.from_numpy(f).to(device) for f in wl.fields])
 
+    # ---- dense P4 tiles: matrix-core interpolation ------------------------
+    dense = None
+    if not args.no_dense:
+        dense = dense_p4_leg(device, torch, args.dense_steps, 1, args.dense_particles)
+
     # ---- CPU baseline ------------------------------------------------------
     cpu = None
     visits_per_particle = None
@@ -402,6 +488,7 @@ def main(argv=None):
         "sort_ms": sort_ms / max(1, sort_n),
         "fp32c": fp32c,
         "adjoint": adjoint,
+        "dense_p4": dense,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -163,6 +163,17 @@ int pyr_lsrk4_steps(pyr_mesh *mesh, int precision, int nsteps, double dt, double
                     double *masses, uint32_t *containing_elements, int64_t *ids,
                     unsigned long long *counters);
 
+/* PYR_FP32C pyr_lsrk4_steps: evaluate the interpolation contraction (6 fields
+ * x Np monomials) of full waves on the matrix cores instead of the vector ALUs,
+ * one pass per element present in a block of four consecutive particles (one
+ * pass when particles are sorted by element).  mode: 0 off, 1 on (default),
+ * 2 on and count the particle-stages that take the matrix-core path
+ * (pyr_push_mfma_waves returns the count since the last call and resets it).
+ * No reference counterpart: a switch between two evaluations of the same FP32
+ * interpolant (push_monomial.hpp:99-111), kept for measurement and tests. */
+int pyr_push_set_mfma(int mode);
+int pyr_push_mfma_waves(int64_t *out);
+
 /* adjoint of the push ------------------------------------------------------ */
 /* No reference counterpart (pyrticle has no adjoint); these differentiate the
  * pyr_lsrk4_steps dynamics for torch.autograd (pyrticle_amd/adjoint.py).

@@ -60,6 +60,8 @@ _SIGNATURES = {
     "pyr_deposit": (C.c_int, [_P, _D, _D, _D, C.c_int, _I64, C.c_int, _P, _P, _P, _P, _P,
                               _P]),
     "pyr_deposit_set_counting": (C.c_int, [C.c_int]),
+    "pyr_push_set_mfma": (C.c_int, [C.c_int]),
+    "pyr_push_mfma_waves": (C.c_int, [C.POINTER(_I64)]),
     "pyr_deposit_stats": (C.c_int, [_P, C.POINTER(C.c_ulonglong * 4)]),
     "pyr_deposit_candidates": (C.c_int, [_P, _D, C.POINTER(_I64)]),
     "pyr_sort_by_element": (C.c_int, [_P, _I64, _P, _P, _P]),

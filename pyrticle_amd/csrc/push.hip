@@ -71,19 +71,88 @@ __device__ __forceinline__ const_ptr<T> as_constant(const T *p) {
   return (const_ptr<T>)p;
 }
 
+// Row stride of the modal coefficient tables [element][field][monomial]: the
+// FP32 table pads every row to a multiple of four monomials (16-byte aligned
+// rows for the matrix-core interpolation), the FP64 table is dense.
+template <class T>
+__host__ __device__ constexpr int coef_stride(int np) {
+  return sizeof(T) == 4 ? (np + 3) & ~3 : np;
+}
+
 // six interpolated field values at unit coordinates r in element en
 template <int D, int N, class T, class P>
 __device__ __forceinline__ void interp6(P coef, uint32_t en, const T *r, T *f6) {
-  constexpr int NP = Basis<D, N>::NP;
+  constexpr int NP = Basis<D, N>::NP, NS = coef_stride<T>(NP);
   T phi[NP];
   monomials<D, N, T>(r, phi);
-  const P c = coef + (size_t)en * 6 * NP;
+  const P c = coef + (size_t)en * 6 * NS;
 #pragma unroll
   for (int f = 0; f < 6; ++f) {
     T acc = T(0);
 #pragma unroll
-    for (int k = 0; k < NP; ++k) acc = fma(c[f * NP + k], phi[k], acc);
+    for (int k = 0; k < NP; ++k) acc = fma(c[f * NS + k], phi[k], acc);
     f6[f] = acc;
+  }
+}
+
+// The same six values on the matrix cores, for a full wave (all 64 lanes
+// active): f6^T = C^T Phi^T as Np rank-1 steps of v_mfma_f32_4x4x1_16b_f32.
+// Block b (lanes 4b..4b+3) covers four fields (rows i, the A operand: lane
+// 4b+i holds c[field i][k] of the block's element) for the block's four
+// particles (columns j, the B operand: lane 4b+j holds its own phi_k), so VGPR
+// i of every lane is field i at the lane's own particle; a second accumulator
+// takes fields 4 and 5 (rows 2 and 3 zero).  A block whose particles sit in
+// several elements takes one pass per element (the smallest pending element
+// id of the block first), every lane keeping the pass of its own element; a
+// wave runs as many passes as its most mixed block.  Each rank-1 step rounds
+// like an FMA, in the same monomial order as the VALU contraction, so both give
+// the same bits.
+typedef float pyr_push_v4f __attribute__((ext_vector_type(4)));
+
+template <int D, int N>
+__device__ __forceinline__ void interp6_mfma(const float *__restrict__ coef, uint32_t e,
+                                             const float *r, float *f6) {
+  constexpr int NP = Basis<D, N>::NP, NS = coef_stride<float>(NP);
+  float phi[NS];
+  monomials<D, N, float>(r, phi);
+#pragma unroll
+  for (int k = NP; k < NS; ++k) phi[k] = 0.f;
+  const int row = threadIdx.x & 3;
+  const bool rb = row < 2;
+  bool done = false;
+  for (;;) {
+    // element served in this pass: the smallest pending element of the block
+    uint32_t ce = done ? INVALID : e;
+    ce = min(ce, (uint32_t)__shfl_xor((int)ce, 1, 64));
+    ce = min(ce, (uint32_t)__shfl_xor((int)ce, 2, 64));
+    if (__ballot(ce != INVALID) == 0) break;
+    const uint32_t el = ce == INVALID ? 0u : ce;
+    const float4 *ca = reinterpret_cast<const float4 *>(coef + ((size_t)el * 6 + row) * NS);
+    const float4 *cb =
+        reinterpret_cast<const float4 *>(coef + ((size_t)el * 6 + 4 + (row & 1)) * NS);
+    pyr_push_v4f a = {0.f, 0.f, 0.f, 0.f}, b = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k4 = 0; k4 < NS / 4; ++k4) {
+      const float4 x = ca[k4];
+      const float4 y = rb ? cb[k4] : make_float4(0.f, 0.f, 0.f, 0.f);
+      a = __builtin_amdgcn_mfma_f32_4x4x1f32(x.x, phi[4 * k4 + 0], a, 0, 0, 0);
+      b = __builtin_amdgcn_mfma_f32_4x4x1f32(y.x, phi[4 * k4 + 0], b, 0, 0, 0);
+      a = __builtin_amdgcn_mfma_f32_4x4x1f32(x.y, phi[4 * k4 + 1], a, 0, 0, 0);
+      b = __builtin_amdgcn_mfma_f32_4x4x1f32(y.y, phi[4 * k4 + 1], b, 0, 0, 0);
+      a = __builtin_amdgcn_mfma_f32_4x4x1f32(x.z, phi[4 * k4 + 2], a, 0, 0, 0);
+      b = __builtin_amdgcn_mfma_f32_4x4x1f32(y.z, phi[4 * k4 + 2], b, 0, 0, 0);
+      a = __builtin_amdgcn_mfma_f32_4x4x1f32(x.w, phi[4 * k4 + 3], a, 0, 0, 0);
+      b = __builtin_amdgcn_mfma_f32_4x4x1f32(y.w, phi[4 * k4 + 3], b, 0, 0, 0);
+    }
+    if (!done && ce == e) {
+      f6[0] = a[0];
+      f6[1] = a[1];
+      f6[2] = a[2];
+      f6[3] = a[3];
+      f6[4] = b[0];
+      f6[5] = b[1];
+      done = true;
+    }
   }
 }
 
@@ -131,19 +200,21 @@ struct Fields6 {
   const double *f[6];
 };
 
+// (rows of coef_stride<T>(np) entries; the padding is zero)
 template <class T>
 __global__ void k_modal(int K, int np, Fields6 fl, const double *__restrict__ minv,
                         T *__restrict__ out) {
+  const int ns = coef_stride<T>(np);
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)K * np) return;
-  int e = (int)(t / np), k = (int)(t % np);
+  if (t >= (int64_t)K * ns) return;
+  int e = (int)(t / ns), k = (int)(t % ns);
   for (int f = 0; f < 6; ++f) {
     double acc = 0;
-    if (fl.f[f]) {
+    if (fl.f[f] && k < np) {
       const double *src = fl.f[f] + (size_t)e * np;
       for (int j = 0; j < np; ++j) acc = fma(minv[k * np + j], src[j], acc);
     }
-    out[((size_t)e * 6 + f) * np + k] = (T)acc;
+    out[((size_t)e * 6 + f) * ns + k] = (T)acc;
   }
 }
 
@@ -201,12 +272,16 @@ __constant__ double c_rk4b[5] = {
 // res = a_s res + dt k, y += b_s res, then relocalization.  Returns false if
 // the particle left the mesh.  Every forward variant (per-step, checkpointed,
 // and the adjoint's recompute) runs this one function, so they agree bitwise.
-template <int D, int N, class T>
+// MF (FP32 only): a full wave interpolates on the matrix cores, one pass per
+// element of its most mixed block of four particles.  Which waves qualify
+// depends on how particles share waves, so only the per-step push, which is
+// never recomputed, turns it on.
+template <int D, int N, class T, bool MF = false>
 __device__ __forceinline__ bool lsrk4_stage(const DevMesh &m, const T *__restrict__ coef, int s,
                                             double dt, double c, double qi, double mi,
                                             double *xi, double *pi, double *rx, double *rp,
                                             double *r, uint32_t &e, FindCounters &cnt,
-                                            int &e_flag) {
+                                            int &e_flag, unsigned &n_mf) {
   double v[D], F[D];
   velocity<D>(pi, mi, c, v);
   const uint32_t e_uni = wave_uniform_element(e);
@@ -221,7 +296,11 @@ __device__ __forceinline__ bool lsrk4_stage(const DevMesh &m, const T *__restric
       rf[k] = (float)r[k];
       vf[k] = (float)v[k];
     }
-    interp6_any<D, N, float>(coef, e, e_uni, rf, f6);
+    if (MF && __ballot(1) == ~0ull) {
+      interp6_mfma<D, N>(coef, e, rf, f6);
+      ++n_mf;
+    } else
+      interp6_any<D, N, float>(coef, e, e_uni, rf, f6);
     lorentz<D, float>((float)qi, f6, vf, Ff, el, mag);
 #pragma unroll
     for (int k = 0; k < D; ++k) F[k] = (double)Ff[k];
@@ -257,15 +336,16 @@ __device__ __forceinline__ bool lsrk4_stage(const DevMesh &m, const T *__restric
   return relocate<D>(m, xi, pi, e, cnt, e_flag, r);
 }
 
-template <int D, int N, class T>
+template <int D, int N, class T, bool MF>
 __global__ __launch_bounds__(256) void k_lsrk4_step(
     DevMesh m, const T *__restrict__ coef, int64_t n, double dt, double c,
     double *__restrict__ x, double *__restrict__ p, const double *__restrict__ q,
     const double *__restrict__ mass, uint32_t *__restrict__ elem,
     uint32_t *__restrict__ death, unsigned long long *counters, int *err,
-    unsigned int *n_dead) {
+    unsigned int *n_dead, unsigned long long *mf_waves) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   FindCounters cnt;
+  unsigned n_mf = 0;   // stages this particle was interpolated on the matrix cores
   if (i < n) {
     double xi[D], pi[D], r[D], rx[D], rp[D];
 #pragma unroll
@@ -283,7 +363,7 @@ __global__ __launch_bounds__(256) void k_lsrk4_step(
     else inverse_map<D>(m, e, xi, r);
 #pragma unroll 1
     for (int s = 0; s < 5 && !died; ++s)
-      if (!lsrk4_stage<D, N, T>(m, coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, cnt, e_flag))
+      if (!lsrk4_stage<D, N, T, MF>(m, coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, cnt, e_flag, n_mf))
         died = (uint32_t)s + 1;
     if (e_flag) atomicOr(err, 1);
 #pragma unroll
@@ -294,6 +374,12 @@ __global__ __launch_bounds__(256) void k_lsrk4_step(
     elem[i] = e;
     death[i] = died;
     if (died) atomicAdd(n_dead, 1u);
+  }
+  // particle-stages interpolated on the matrix cores, summed over the wave
+  if (MF && mf_waves) {
+    unsigned long long v = n_mf;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(mf_waves, v);
   }
   flush_counters(cnt, counters);
 }
@@ -315,6 +401,7 @@ __global__ __launch_bounds__(256) void k_lsrk4_ckpt(
     int *err) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   FindCounters cnt;
+  unsigned n_mf = 0;
   if (i < n) {
     double xi[D], pi[D], r[D], rx[D], rp[D];
 #pragma unroll
@@ -348,7 +435,7 @@ __global__ __launch_bounds__(256) void k_lsrk4_ckpt(
 #pragma unroll 1
       for (int s = 0; s < 5; ++s)
         if (!lsrk4_stage<D, N, T>(m, coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, cnt,
-                                  e_flag)) {
+                                  e_flag, n_mf)) {
           died = (uint32_t)(5 * step + s + 1);
           break;
         }
@@ -411,7 +498,8 @@ __device__ __forceinline__ void rhs_adjoint(const DevMesh &m, const T *__restric
 #pragma unroll
     for (int k = 1; k <= N; ++k) pw[ii][k] = pw[ii][k - 1] * rt[ii];
   }
-  const T *cf = coef + (size_t)e * 6 * NP;
+  constexpr int NS = coef_stride<T>(NP);
+  const T *cf = coef + (size_t)e * 6 * NS;
   T bf[3] = {T(0), T(0), T(0)}, g[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) g[k] = T(0);
@@ -421,10 +509,10 @@ __device__ __forceinline__ void rhs_adjoint(const DevMesh &m, const T *__restric
 #pragma unroll
     for (int ii = 1; ii < D; ++ii) phi = phi * pw[ii][B.e[k][ii]];
 #pragma unroll
-    for (int f = 0; f < 3; ++f) bf[f] = fma(cf[(3 + f) * NP + k], phi, bf[f]);
+    for (int f = 0; f < 3; ++f) bf[f] = fma(cf[(3 + f) * NS + k], phi, bf[f]);
     T wk = T(0);
 #pragma unroll
-    for (int f = 0; f < 6; ++f) wk = fma(ws[f], cf[f * NP + k], wk);
+    for (int f = 0; f < 6; ++f) wk = fma(ws[f], cf[f * NS + k], wk);
 #pragma unroll
     for (int jj = 0; jj < D; ++jj) {
       const int ej = B.e[k][jj];
@@ -498,6 +586,7 @@ __global__ __launch_bounds__(256) void k_lsrk4_adjoint(
   }
   const double qi = q[i], mi = mass[i];
   FindCounters cnt;
+  unsigned n_mf = 0;
   int e_flag = 0, lost = 0;
   const size_t nn = (size_t)n, ii = (size_t)i;
   const int nseg = (nsteps + ckpt_every - 1) / ckpt_every;
@@ -530,7 +619,7 @@ __global__ __launch_bounds__(256) void k_lsrk4_adjoint(
       }
 #pragma unroll 1
       for (int s = 0; s < 5; ++s)
-        if (!lsrk4_stage<D, N, T>(m, coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, cnt, e_flag))
+        if (!lsrk4_stage<D, N, T>(m, coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, cnt, e_flag, n_mf))
           lost = 1;
     }
 #pragma unroll 1
@@ -555,7 +644,7 @@ __global__ __launch_bounds__(256) void k_lsrk4_adjoint(
         }
         sebuf[(size_t)s * nn + ii] = e;
         if (s == 4) break;   // the last stage's output is not needed
-        if (!lsrk4_stage<D, N, T>(m, coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, cnt, e_flag))
+        if (!lsrk4_stage<D, N, T>(m, coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, cnt, e_flag, n_mf))
           lost = 1;
       }
       double rbx[D], rbp[D];
@@ -666,6 +755,12 @@ static void dispatch_order(int D, int N, F &&f) {
   throw Error(PYR_ERR_INVALID_ARG, "unsupported dimension/order");
 }
 
+// FP32C per-step push: interpolate on the matrix cores where a full wave sits
+// in one element (pyr_push_set_mfma: 0 off, 1 on (default), 2 on and count the
+// particle-stages that took the path in g_mf_count)
+static int g_push_mfma = 1;
+static DevBuf g_mf_count;
+
 static void modal_transform(pyr_mesh *h, const double *const fields[6], double *out64,
                             float *out32) {
   Fields6 fl;
@@ -677,7 +772,8 @@ static void modal_transform(pyr_mesh *h, const double *const fields[6], double *
     PYR_CHECK_LAUNCH();
   }
   if (out32) {
-    hipLaunchKernelGGL(k_modal<float>, dim3(blocks_for(work)), dim3(256), 0, stream(), h->K,
+    const int64_t work32 = (int64_t)h->K * coef_stride<float>(h->Np);
+    hipLaunchKernelGGL(k_modal<float>, dim3(blocks_for(work32)), dim3(256), 0, stream(), h->K,
                        h->Np, fl, h->minv.as<double>(), out32);
     PYR_CHECK_LAUNCH();
   }
@@ -718,7 +814,7 @@ int pyr_set_static_fields(pyr_mesh *mesh, const double *const fields[6]) {
     order_of(mesh);
     const size_t cnt = (size_t)mesh->K * 6 * mesh->Np;
     mesh->sf64.reserve(cnt * sizeof(double));
-    mesh->sf32.reserve(cnt * sizeof(float));
+    mesh->sf32.reserve((size_t)mesh->K * 6 * coef_stride<float>(mesh->Np) * sizeof(float));
     modal_transform(mesh, fields, mesh->sf64.as<double>(), mesh->sf32.as<float>());
     mesh->sf_set = true;
   });
@@ -746,6 +842,11 @@ int pyr_lsrk4_steps(pyr_mesh *mesh, int precision, int nsteps, double dt, double
     uint32_t *mv_to = mv_from + n;
     int *err = mesh->s_misc.as<int>();
     unsigned int *n_dead = reinterpret_cast<unsigned int *>(err + 1);
+    unsigned long long *mf_waves = nullptr;
+    if (g_push_mfma == 2) {
+      g_mf_count.reserve(sizeof(unsigned long long));
+      mf_waves = g_mf_count.as<unsigned long long>();
+    }
     for (int step = 0; step < nsteps && n > 0; ++step) {
       PYR_HIP(hipMemsetAsync(err, 0, 2 * sizeof(int), stream()));
       {
@@ -753,13 +854,17 @@ int pyr_lsrk4_steps(pyr_mesh *mesh, int precision, int nsteps, double dt, double
         dispatch_order(D, N, [&](auto DD, auto NN) {
           constexpr int D_ = decltype(DD)::value, N_ = decltype(NN)::value;
           if (precision == PYR_FP64)
-            hipLaunchKernelGGL((k_lsrk4_step<D_, N_, double>), dim3(blocks_for(n)), dim3(256),
-                               0, stream(), mesh->dev, mesh->sf64.as<double>(), n, dt, vacuum_c, x,
-                               p, q, mass, elem, death, counters, err, n_dead);
+            hipLaunchKernelGGL((k_lsrk4_step<D_, N_, double, false>), dim3(blocks_for(n)),
+                               dim3(256), 0, stream(), mesh->dev, mesh->sf64.as<double>(), n, dt,
+                               vacuum_c, x, p, q, mass, elem, death, counters, err, n_dead, mf_waves);
+          else if (g_push_mfma)
+            hipLaunchKernelGGL((k_lsrk4_step<D_, N_, float, true>), dim3(blocks_for(n)),
+                               dim3(256), 0, stream(), mesh->dev, mesh->sf32.as<float>(), n, dt,
+                               vacuum_c, x, p, q, mass, elem, death, counters, err, n_dead, mf_waves);
           else
-            hipLaunchKernelGGL((k_lsrk4_step<D_, N_, float>), dim3(blocks_for(n)), dim3(256),
-                               0, stream(), mesh->dev, mesh->sf32.as<float>(), n, dt, vacuum_c, x,
-                               p, q, mass, elem, death, counters, err, n_dead);
+            hipLaunchKernelGGL((k_lsrk4_step<D_, N_, float, false>), dim3(blocks_for(n)),
+                               dim3(256), 0, stream(), mesh->dev, mesh->sf32.as<float>(), n, dt,
+                               vacuum_c, x, p, q, mass, elem, death, counters, err, n_dead, mf_waves);
           PYR_CHECK_LAUNCH();
         });
       }
@@ -890,6 +995,30 @@ int pyr_lsrk4_adjoint(pyr_mesh *mesh, int precision, int nsteps, int ckpt_every,
       throw Error(PYR_ERR_INTERNAL,
                   "adjoint recompute diverged from the forward pass (checkpoints do not match "
                   "the static fields or the particle data)");
+  });
+}
+
+int pyr_push_set_mfma(int mode) {
+  return guarded([&] {
+    PYR_REQUIRE(mode >= 0 && mode <= 2, "mode: 0 off, 1 on, 2 on and counting");
+    if (mode == 2 && g_push_mfma != 2) {
+      g_mf_count.reserve(sizeof(unsigned long long));
+      PYR_HIP(hipMemsetAsync(g_mf_count.ptr, 0, sizeof(unsigned long long), stream()));
+    }
+    g_push_mfma = mode;
+  });
+}
+
+int pyr_push_mfma_waves(int64_t *out) {
+  return guarded([&] {
+    PYR_REQUIRE(out, "null argument");
+    *out = 0;
+    if (!g_mf_count.ptr) return;
+    unsigned long long v = 0;
+    PYR_HIP(hipMemcpyAsync(&v, g_mf_count.ptr, sizeof(v), hipMemcpyDeviceToHost, stream()));
+    PYR_HIP(hipMemsetAsync(g_mf_count.ptr, 0, sizeof(v), stream()));
+    PYR_HIP(hipStreamSynchronize(stream()));
+    *out = (int64_t)v;
   });
 }
 

@@ -219,6 +219,19 @@ def set_deposit_counting(on: bool):
     _lib.check(_lib.lib().pyr_deposit_set_counting(1 if on else 0))
 
 
+def set_push_mfma(on: bool, count: bool = False):
+    """FP32C push: interpolate on the matrix cores for waves inside one element;
+    with count, the particle-stages that take that path are counted."""
+    _lib.check(_lib.lib().pyr_push_set_mfma((2 if count else 1) if on else 0))
+
+
+def push_mfma_waves() -> int:
+    """Particle-stages interpolated on the matrix cores since the last call."""
+    out = C.c_int64(0)
+    _lib.check(_lib.lib().pyr_push_mfma_waves(C.byref(out)))
+    return int(out.value)
+
+
 def velocities(p: torch.Tensor, m: torch.Tensor, c: float) -> torch.Tensor:
     _lib.sync_stream()
     n, vd = p.shape

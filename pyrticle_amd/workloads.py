@@ -10,6 +10,10 @@
   0.3 x 0.2, z ~ N(2, 0.4^2) (re-drawn inside the mesh), gamma=10 along z with
   0.1% momentum spread; R=0.08, alpha=2; static solenoid B_z=0.5 T plus a
   uniform E_z=1e6 V/m; dt = 0.2*h_min/c; 200 steps.
+* ``cylinder_dense_p4``: the dense-tile variant: the same cylinder with 49,896
+  P4 tetrahedra (Np=35, 7 rings x 54 layers) and 16M electrons uniform in the
+  cylinder inscribed in the mesh (about 320 per element), same momenta, fields,
+  R and dt rule.
 
 Random streams use numpy.random.default_rng: seed 0 for meshes, seed 1 for
 particles.  Particles that land outside the mesh are re-drawn (the reference's
@@ -106,3 +110,25 @@ def cylinder_3d(n_particles: int = 4_000_000, n_rings: int = 12, n_layers: int =
     c = U.VACUUM_LIGHT_SPEED()
     dt = 0.2 * D.min_edge_length(discr.mesh) / c
     return Workload("cylinder_3d", discr, ma, x, p, q, m, 0.08, 2.0, fields, dt, 200, c)
+
+
+def cylinder_dense_p4(n_particles: int = 16_000_000, n_rings: int = 7, n_layers: int = 54,
+                      order: int = 4, seed: int = 1, mesh=None) -> Workload:
+    discr, ma = mesh if mesh is not None else cylinder_mesh_3d(n_rings, n_layers, order)
+    rng = np.random.default_rng(seed)
+    # uniform in the cylinder inscribed in the mesh's polygonal cross-section
+    # (the outer ring has round(2 pi n_rings) vertices on the unit circle)
+    r_in = 0.99 * np.cos(np.pi / max(6, int(round(2 * np.pi * n_rings))))
+    r = r_in * np.sqrt(rng.uniform(size=n_particles))
+    phi = rng.uniform(0, 2 * np.pi, size=n_particles)
+    z = rng.uniform(1e-3, 4.0 - 1e-3, size=n_particles)
+    x = np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
+    p = _momenta_gamma(10.0, n_particles, 2, 1e-3, rng, 3)
+    q = np.full(n_particles, -U.EL_CHARGE)
+    m = np.full(n_particles, U.EL_MASS)
+    nn = len(discr)
+    fields = [np.zeros(nn), np.zeros(nn), np.full(nn, 1e6),
+              np.zeros(nn), np.zeros(nn), np.full(nn, 0.5)]
+    c = U.VACUUM_LIGHT_SPEED()
+    dt = 0.2 * D.min_edge_length(discr.mesh) / c
+    return Workload("cylinder_dense_p4", discr, ma, x, p, q, m, 0.08, 2.0, fields, dt, 200, c)

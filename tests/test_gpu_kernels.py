@@ -271,3 +271,67 @@ def test_lsrk4_square_matches_oracle(square, oracle_square, dmesh2, cuda, dt_sca
     assert rel_l2(tx[:n].cpu().numpy(), ox[:n_o]) < FP64_TOL
     assert rel_l2(tp[:n].cpu().numpy(), op[:n_o]) < FP64_TOL
     assert tuple(int(c) for c in counters.cpu()) == cnt_o
+
+
+@pytest.fixture(scope="module")
+def dense_p4():
+    """Small P4 cylinder (1,368 tets, Np=35) filled with ~100 particles per
+    element, so that most waves of the element-sorted beam sit in one element."""
+    from pyrticle_amd import workloads as W
+    mesh = W.cylinder_mesh_3d(n_rings=3, n_layers=8, order=4)
+    return W.cylinder_dense_p4(n_particles=150_000, n_rings=3, mesh=mesh)
+
+
+def test_lsrk4_fp32c_mfma_interpolation(dense_p4, cuda):
+    """FP32C push of a dense P4 beam with non-uniform fields: the matrix-core
+    interpolation (full waves inside one element) against the FP64 oracle and
+    against the FP32 VALU interpolation; within the FP32 budget, bitwise run-to-
+    run stable, and the matrix-core path carries most particle-stages."""
+    from pyrticle_amd import device as dv
+    wl = dense_p4
+    om = O.OracleMesh(wl.ma)
+    dm = dv.DeviceMesh(wl.ma)
+    el = dv.element_numpy(dm.find_containing_element(_dev(wl.x, cuda)))
+    assert np.all(el != dv.INVALID_ELEMENT)
+    order = np.argsort(el, kind="stable")
+    x0, p0, q0, m0 = (np.ascontiguousarray(a[order]) for a in (wl.x, wl.p, wl.q, wl.m))
+    el0 = el[order]
+    assert np.bincount(el0).mean() > 64
+    steps = 2
+    # non-uniform static fields, polynomials of degree <= 4 (exact in P4), so
+    # that every monomial of the contraction contributes
+    X = np.asarray(wl.ma.nodes).reshape(-1, 3)
+    xx, yy, zz = X[:, 0], X[:, 1], X[:, 2]
+    fields = [1e5 * (xx + 0.5 * yy * zz), 1e5 * (yy - 0.3 * xx ** 2),
+              1e6 * (1 + 0.1 * zz - 0.02 * zz ** 2 + 0.05 * xx * yy),
+              0.05 * xx * zz, -0.05 * yy * zz, 0.5 * (1 + 0.1 * xx ** 2 - 0.05 * yy ** 3)]
+    ox, op, oq, om_, oel = x0.copy(), p0.copy(), q0.copy(), m0.copy(), el0.copy()
+    n_o, _ = O.lsrk4_run(om, fields, wl.c, ox, op, oq, om_, oel, wl.dt, steps)
+    dm.set_static_fields([_dev(f, cuda) for f in fields])
+    out = {}
+    try:
+        for name, on in (("mfma", True), ("valu", False), ("mfma2", True)):
+            dv.set_push_mfma(on, count=True)
+            dv.push_mfma_waves()
+            tx, tp, tq, tm = (_dev(a, cuda) for a in (x0, p0, q0, m0))
+            tel = dv.element_tensor(el0, cuda)
+            n = dm.lsrk4_steps(steps, wl.dt, wl.c, x0.shape[0], tx, tp, tq, tm, tel,
+                               precision=dv.FP32C)
+            assert n == n_o
+            assert np.array_equal(dv.element_numpy(tel[:n]), oel[:n_o])
+            out[name] = (tx[:n].cpu().numpy(), tp[:n].cpu().numpy(), dv.push_mfma_waves())
+    finally:
+        dv.set_push_mfma(True)
+    # displacement and momentum change over the run against the FP64 oracle:
+    # FP32 fields (relative rounding ~1e-7) enter through the momentum change
+    dx_ref, dp_ref = ox[:n_o] - x0[:n_o], op[:n_o] - p0[:n_o]
+    for name in ("mfma", "valu"):
+        xs, ps, _ = out[name]
+        assert rel_l2(xs - x0[:n_o], dx_ref) < 1e-6, name
+        assert rel_l2(ps - p0[:n_o], dp_ref) < 1e-5, name
+    assert np.array_equal(out["mfma"][1], out["mfma2"][1])      # deterministic
+    # full waves (all but the last, and waves with a particle that left the
+    # mesh at an earlier stage) take the matrix-core path, and each rank-1 step
+    # rounds like the FMA chain of the VALU contraction: identical bits
+    assert out["mfma"][2] >= 0.8 * 5 * steps * n_o and out["valu"][2] == 0
+    assert np.array_equal(out["mfma"][1], out["valu"][1])
