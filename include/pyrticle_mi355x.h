@@ -166,7 +166,7 @@ int pyr_lsrk4_steps(pyr_mesh *mesh, int precision, int nsteps, double dt, double
 /* PYR_FP32C pyr_lsrk4_steps: evaluate the interpolation contraction (6 fields
  * x Np monomials) of full waves on the matrix cores instead of the vector ALUs,
  * one pass per element present in a block of four consecutive particles (one
- * pass when particles are sorted by element).  mode: 0 off, 1 on (default),
+ * pass when particles are sorted by element).  mode: 0 off (default), 1 on,
  * 2 on and count the particle-stages that take the matrix-core path
  * (pyr_push_mfma_waves returns the count since the last call and resets it).
  * No reference counterpart: a switch between two evaluations of the same FP32

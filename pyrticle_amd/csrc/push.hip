@@ -755,10 +755,11 @@ static void dispatch_order(int D, int N, F &&f) {
   throw Error(PYR_ERR_INVALID_ARG, "unsupported dimension/order");
 }
 
-// FP32C per-step push: interpolate on the matrix cores where a full wave sits
-// in one element (pyr_push_set_mfma: 0 off, 1 on (default), 2 on and count the
-// particle-stages that took the path in g_mf_count)
-static int g_push_mfma = 1;
+// FP32C per-step push: interpolate full waves on the matrix cores
+// (pyr_push_set_mfma: 0 off (default: the VALU contraction measured faster
+// inside the register-bound push), 1 on, 2 on and count the particle-stages
+// that took the path in g_mf_count)
+static int g_push_mfma = 0;
 static DevBuf g_mf_count;
 
 static void modal_transform(pyr_mesh *h, const double *const fields[6], double *out64,

@@ -321,7 +321,7 @@ def test_lsrk4_fp32c_mfma_interpolation(dense_p4, cuda):
             assert np.array_equal(dv.element_numpy(tel[:n]), oel[:n_o])
             out[name] = (tx[:n].cpu().numpy(), tp[:n].cpu().numpy(), dv.push_mfma_waves())
     finally:
-        dv.set_push_mfma(True)
+        dv.set_push_mfma(False)
     # displacement and momentum change over the run against the FP64 oracle:
     # FP32 fields (relative rounding ~1e-7) enter through the momentum change
     dx_ref, dp_ref = ox[:n_o] - x0[:n_o], op[:n_o] - p0[:n_o]
