@@ -94,24 +94,22 @@ __global__ __launch_bounds__(RS_BS) void k_rs_scatter(const uint32_t *__restrict
   }
 }
 
-// counts[K] collects particles with an invalid element; they sort last and
-// no segment references them.
-// Runs of equal keys inside a wave add their length with one atomic (element-
-// sorted input has long runs; unsorted input degrades to one atomic per lane).
-__global__ void k_hist_keys(const uint32_t *__restrict__ keys, int64_t n,
-                            uint32_t *__restrict__ counts, uint32_t kmax) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  const bool valid = i < n;
-  const uint32_t k = valid ? min(keys[i], kmax) : 0xFFFFFFFFu;
-  const uint32_t left = __shfl_up(k, 1, 64);
-  const bool head = valid && (lane == 0 || left != k);
-  const uint64_t heads = __ballot(head) | (~__ballot(valid));   // invalid lanes end runs
-  if (head) {
-    const uint64_t after = lane == 63 ? 0ull : (heads >> (lane + 1)) << (lane + 1);
-    const int next = after ? __ffsll((unsigned long long)after) - 1 : 64;
-    atomicAdd(&counts[k], (uint32_t)(next - lane));
+// Segment starts from the sorted (clamped) keys: seg[e] = first index whose
+// key is >= e << shift, for e = 0..K (seg[K] = number of located particles;
+// keys of particles without an element are clamped to K << shift and sort
+// last).  One binary search per element, balanced whatever the occupancy.
+__global__ void k_seg_from_sorted(int K, int64_t n, const uint32_t *__restrict__ keys,
+                                  int shift, uint32_t *__restrict__ seg) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e > K) return;
+  const uint32_t target = (uint32_t)e << shift;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < target) lo = mid + 1;
+    else hi = mid;
   }
+  seg[e] = (uint32_t)lo;
 }
 
 // staged particles in element-sorted order, two arrays of double4:
@@ -136,7 +134,7 @@ __global__ void k_stage(int D, int64_t n, const uint32_t *__restrict__ perm,
 }
 
 
-// FP32 staging of the FP32-compensated gather, in element-sorted order:
+// FP32 staging of the FP32-compensated gather, in the same order as k_stage:
 //   pos32[j]             = (x_j - c_e, 0)  offset from the vertex centroid of
 //                          the particle's own element e, formed in FP64 and
 //                          rounded once;
@@ -239,7 +237,7 @@ struct DepositCache {
   DevBuf tr_start, tr, tr_shift;
   double tr_radius = -1;
   // sort / staging scratch
-  DevBuf keys_a, keys_b, vals_a, vals_b, hist, hist_scan, counts, seg, tmp, pos, wgt, stats;
+  DevBuf keys_a, keys_b, vals_a, vals_b, hist, hist_scan, seg, tmp, pos, wgt, stats;
   DevBuf pos32, wgt32;   // FP32 staging
   DevBuf skeys;   // Morton sort keys
   std::vector<double> h_geom;   // K x 8
@@ -251,7 +249,7 @@ void pyr_deposit_cache_free(pyr_mesh *mesh) {
   for (DevBuf *b : {&c->cand_start, &c->cand, &c->cand_shift, &c->geom, &c->tab64, &c->tab32,
                     &c->tab_R, &c->tr_start, &c->tr, &c->tr_shift, &c->keys_a, &c->keys_b,
                     &c->vals_a, &c->vals_b, &c->skeys, &c->hist,
-                    &c->hist_scan, &c->counts, &c->seg, &c->tmp, &c->pos, &c->wgt, &c->stats,
+                    &c->hist_scan, &c->seg, &c->tmp, &c->pos, &c->wgt, &c->stats,
                     &c->pos32, &c->wgt32})
     b->release();
   delete c;
@@ -471,7 +469,7 @@ static unsigned blocks_for(int64_t n, int bs = 256) {
 // (8-bit digits); returns the sorted indices (in va or vb).
 static const uint32_t *radix_sort(DepositCache *c, int64_t n, const uint32_t *keys,
                                   uint32_t kmax, DevBuf &ka, DevBuf &kb, DevBuf &va,
-                                  DevBuf &vb) {
+                                  DevBuf &vb, const uint32_t **sorted_keys = nullptr) {
   hipStream_t s = stream();
   const int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
   int bits = 1;
@@ -502,6 +500,7 @@ static const uint32_t *radix_sort(DepositCache *c, int64_t n, const uint32_t *ke
       vout = va.as<uint32_t>();
     }
   }
+  if (sorted_keys) *sorted_keys = kin;
   return vin;
 }
 
@@ -513,30 +512,25 @@ static const uint32_t *sort_by_element(pyr_mesh *h, DepositCache *c, int64_t n,
                                        int key_shift = 0) {
   const int K = h->K;
   hipStream_t s = stream();
-  c->counts.reserve(sizeof(uint32_t) * ((size_t)K + 2));
   c->seg.reserve(sizeof(uint32_t) * ((size_t)K + 2));
-  const int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
-  const int64_t hist_n = std::max<int64_t>(256 * ntiles, K + 1);
-  c->tmp.reserve(sizeof(uint32_t) * scan_tmp_words(hist_n));
-  PYR_HIP(hipMemsetAsync(c->counts.ptr, 0, sizeof(uint32_t) * ((size_t)K + 2), s));
-  hipLaunchKernelGGL(k_hist_keys, dim3(blocks_for(n)), dim3(256), 0, s, elem, n,
-                     c->counts.as<uint32_t>(), (uint32_t)K);
-  PYR_CHECK_LAUNCH();
-  scan_exclusive_u32(c->counts.as<uint32_t>(), c->seg.as<uint32_t>(), (int64_t)K + 1, nullptr,
-                     c->tmp.as<uint32_t>(), s);
   if (!keys) {
     keys = elem;
     key_shift = 0;
   }
-  return radix_sort(c, n, keys, (uint32_t)K << key_shift, c->keys_a, c->keys_b, c->vals_a,
-                    c->vals_b);
+  const uint32_t *sorted = nullptr;
+  const uint32_t *perm = radix_sort(c, n, keys, (uint32_t)K << key_shift, c->keys_a, c->keys_b,
+                                    c->vals_a, c->vals_b, &sorted);
+  hipLaunchKernelGGL(k_seg_from_sorted, dim3(blocks_for((int64_t)K + 1)), dim3(256), 0, s, K, n,
+                     sorted, key_shift, c->seg.as<uint32_t>());
+  PYR_CHECK_LAUNCH();
+  return perm;
 }
 
-// Sorts the particles by element and stages them in that order; returns the
-// permutation.  The FP64 staging sorts by (element, Morton code inside the
-// element), so that the VALU gather's queued batches are spatially coherent;
-// the FP32 staging (matrix-core gather, which takes whole segments) sorts by
-// element alone, one radix pass fewer.
+// Sorts the particles by (element, Morton code inside the element) and stages
+// them in that order; returns the permutation.  The Morton sub-order makes the
+// gathers' batches of consecutive particles spatially coherent (the FP64
+// gather's queued batches, the matrix-core gather's 64-particle steps, which
+// it skips when none of them can reach the target).
 static const uint32_t *sort_and_stage(pyr_mesh *mesh, DepositCache *c, int64_t n,
                                       const double *x, const double *q, const double *v,
                                       const uint32_t *elem, double normalizer,
@@ -550,7 +544,7 @@ static const uint32_t *sort_and_stage(pyr_mesh *mesh, DepositCache *c, int64_t n
     const char *env = std::getenv("PYR_DEPOSIT_MORTON");
     return !env || std::atoi(env) != 0;
   }();
-  if (morton && stage64 && (uint64_t)K < ((uint64_t)1 << (32 - MORTON_BITS))) {
+  if (morton && (uint64_t)K < ((uint64_t)1 << (32 - MORTON_BITS))) {
     c->skeys.reserve(sizeof(uint32_t) * (size_t)n);
     if (D == 2)
       hipLaunchKernelGGL(k_sort_keys<2>, dim3(blocks_for(n)), dim3(256), 0, s, n, K, elem, x,

@@ -328,7 +328,10 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
 // current one is evaluated.  Lanes outside their segment (quad padding) or
 // past the end get a particle that no node can reach.  There is no
 // prefilter: every candidate particle is evaluated against every node on the
-// matrix cores, and pairs farther apart than R give S = 0.
+// matrix cores, and pairs farther apart than R give S = 0; a step is skipped
+// when none of its 64 particles passes a conservative reach test (node sphere
+// and face planes of the target moved out by R), which the Morton sub-order of
+// the staging makes common.
 //
 // Pairs, per group of four nodes n = 4 ng + (l % 4), with
 // v_mfma_f32_4x4x1_16b_f32 (16 independent 4x4 outer products, block b =
@@ -356,10 +359,18 @@ __device__ __forceinline__ pyr_v4f mfma4x4(float a, float b, pyr_v4f c) {
   return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 0);
 }
 
+// relative margin of the FP32 reach test (it may only keep extra particles)
+constexpr float MF_MARGIN = 1e-4f;
+
+#ifndef PYR_MF_FOLD
+#define PYR_MF_FOLD 8
+#endif
+constexpr int FOLD_STEPS = PYR_MF_FOLD;
+
 template <int D, int NP, int W, bool A2>
 __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
   constexpr int NG = (NP + 3) / 4;   // node groups of four
-  constexpr int FOLD = 8;            // steps (four particles per block each) per fold
+  constexpr int FOLD = FOLD_STEPS;   // steps (four particles per block each) per fold
   constexpr float FAR = 1e30f;       // -c of a particle no node can reach
 
   const int e = xcd_element(blockIdx.x);
@@ -394,6 +405,24 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
       bz[ng] = D == 3 ? nd[2] * inv_p : 0.f;
       cc[ng] = -nd[3] * inv_p;
     }
+  }
+
+  // conservative reach test of a particle (FP32, with a relative margin): its
+  // ball of radius R meets the node sphere of the target and lies within R of
+  // every face plane.  A particle failing it has t <= 0 at every node.
+  auto uni = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
+  const float bound2 = uni((float)g.geom[(size_t)e * 8 + 3] * (1.f + 2.f * MF_MARGIN));
+  float fnl[D + 1][D], hf[D + 1];
+#pragma unroll
+  for (int f = 0; f <= D; ++f) {
+    double nc = 0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const double nk = g.face_normal[((size_t)e * (D + 1) + f) * D + k];
+      fnl[f][k] = uni((float)nk);
+      nc = fma(nk, k == 0 ? cx : (k == 1 ? cy : cz), nc);
+    }
+    hf[f] = uni((float)(g.face_rhs[(size_t)e * (D + 1) + f] + g.R - nc + MF_MARGIN * g.R));
   }
 
   pyr_v4f cinit[NG];   // C operand of the first product: -|n - c_e|^2 / (R P) in every row
@@ -432,6 +461,7 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
   struct Part {
     float mx, my, mz, cp;
     float4 w;
+    bool live;   // may reach a node of the target
   };
   // loads in flight for one chunk, before the offsets are applied
   struct Raw {
@@ -449,9 +479,20 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
     // padding lanes keep finite m and get a c that no node can overcome
     P.cp = r.in_seg ? fmaf(-dd, inv_rf, rf) * inv_p : -FAR;
     P.w = r.w;
+    float viol = -INFINITY;
+#pragma unroll
+    for (int f = 0; f <= D; ++f) {
+      float ip = fnl[f][0] * dx;
+      ip = fmaf(fnl[f][1], dy, ip);
+      if constexpr (D == 3) ip = fmaf(fnl[f][D - 1], dz, ip);
+      viol = fmaxf(viol, ip - hf[f]);
+    }
+    P.live = r.in_seg && dd <= bound2 && viol <= 0.f;
     return P;
   };
   auto evaluate = [&](const Part &P) {
+    // a step none of whose particles can reach the target adds nothing
+    if (__ballot(P.live) == 0) return;
 #pragma unroll
     for (int ng = 0; ng < NG; ++ng) {
       pyr_v4f t = mfma4x4(P.mx, bx[ng], cinit[ng]);

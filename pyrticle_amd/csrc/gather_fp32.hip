@@ -1,6 +1,7 @@
 // gather_fp32.hip -- instantiations of the FP32-compensated deposition gather:
-// the matrix-core form k_gather_mf (default; template W = waves per target
-// element) and the VALU form k_gather<.., FP32>
+// the matrix-core form k_gather_mf (default; two waves per target element,
+// the fastest of 1-8 measured on the cylinder workload) and the VALU form
+// k_gather<.., FP32>
 // (PYR_GATHER_MFMA=0, kept for comparison; it reads the FP64 staging).
 #include <cstdlib>
 
@@ -51,24 +52,21 @@ void launch_gather_fp32(int D, int Np, const GatherArgs &ga, int K, bool a2, boo
   // the matrix-core form evaluates every candidate pair; it has no pair
   // counters (count mode is served by the FP64 gather)
   if (count) throw Error(PYR_ERR_INVALID_ARG, "deposit counting needs the FP64 precision");
+
   if (D == 2) {
     switch (Np) {
-      case 3: return launch_mf<2, 3, 4>(ga, K, a2);
-      case 6: return launch_mf<2, 6, 4>(ga, K, a2);
-      case 10: return launch_mf<2, 10, 4>(ga, K, a2);
-      case 15: return launch_mf<2, 15, 4>(ga, K, a2);
+      case 3: return launch_mf<2, 3, 2>(ga, K, a2);
+      case 6: return launch_mf<2, 6, 2>(ga, K, a2);
+      case 10: return launch_mf<2, 10, 2>(ga, K, a2);
+      case 15: return launch_mf<2, 15, 2>(ga, K, a2);
     }
   } else {
     switch (Np) {
-      case 4: return launch_mf<3, 4, 4>(ga, K, a2);
-      case 10: return launch_mf<3, 10, 4>(ga, K, a2);
+      case 4: return launch_mf<3, 4, 2>(ga, K, a2);
+      case 10: return launch_mf<3, 10, 2>(ga, K, a2);
       case 20:
-        switch (env_int("PYR_GATHER_WAVES", 2)) {
-          case 1: return launch_mf<3, 20, 1>(ga, K, a2);
-          case 4: return launch_mf<3, 20, 4>(ga, K, a2);
-          default: return launch_mf<3, 20, 2>(ga, K, a2);
-        }
-      case 35: return launch_mf<3, 35, 4>(ga, K, a2);
+        return launch_mf<3, 20, 2>(ga, K, a2);
+      case 35: return launch_mf<3, 35, 2>(ga, K, a2);
     }
   }
   throw Error(PYR_ERR_INVALID_ARG, "deposition supports P1..P4 (2D) and P1..P4 (3D) elements");
