@@ -555,30 +555,44 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
     };
 
     // chunks of 64 flat indices, dealt round-robin to the waves; kc is the
-    // segment holding the chunk's first index (wave-uniform, monotone)
+    // segment holding the chunk's first index (wave-uniform, monotone), its
+    // data cached in scalar registers and re-read only when kc moves
     int kc = 0;
+    uint32_t c_end = rl(incl, 0), c_ex = rl(excl, 0), c_a0 = rl(a0, 0), c_lo = rl(lo, 0),
+             c_hi = rl(hi, 0);
+    float c_ox = rlf(ofx, 0), c_oy = rlf(ofy, 0), c_oz = rlf(ofz, 0);
     auto issue = [&](uint32_t base) {
-      while (rl(incl, kc) <= base) ++kc;
+      if (c_end <= base) {
+        do ++kc;
+        while (rl(incl, kc) <= base);
+        c_end = rl(incl, kc);
+        c_ex = rl(excl, kc);
+        c_a0 = rl(a0, kc);
+        c_lo = rl(lo, kc);
+        c_hi = rl(hi, kc);
+        c_ox = rlf(ofx, kc);
+        c_oy = rlf(ofy, kc);
+        c_oz = rlf(ofz, kc);
+      }
       const uint32_t last = min(base + 63u, total - 1u);
-      int kl = kc;
-      while (rl(incl, kl) <= last) ++kl;
-      // segment data of this lane's index: the first segment, then the ones
-      // that start inside the chunk (usually none)
       const uint32_t f = base + (uint32_t)lane;
-      uint32_t sa = rl(a0, kc), se = rl(excl, kc), sl = rl(lo, kc), sh2 = rl(hi, kc);
+      uint32_t sa = c_a0, se = c_ex, sl = c_lo, sh2 = c_hi;
       Raw r;
-      r.ox = rlf(ofx, kc);
-      r.oy = rlf(ofy, kc);
-      r.oz = rlf(ofz, kc);
-      for (int kk = kc + 1; kk <= kl; ++kk) {
-        const bool in = f >= rl(excl, kk);
-        sa = in ? rl(a0, kk) : sa;
-        se = in ? rl(excl, kk) : se;
-        sl = in ? rl(lo, kk) : sl;
-        sh2 = in ? rl(hi, kk) : sh2;
-        r.ox = in ? rlf(ofx, kk) : r.ox;
-        r.oy = in ? rlf(ofy, kk) : r.oy;
-        r.oz = in ? rlf(ofz, kk) : r.oz;
+      r.ox = c_ox;
+      r.oy = c_oy;
+      r.oz = c_oz;
+      if (last >= c_end) {
+        // segments that start inside the chunk (rare): per-lane selection
+        for (int kk = kc + 1; kk < 64 && rl(excl, kk) <= last; ++kk) {
+          const bool in = f >= rl(excl, kk);
+          sa = in ? rl(a0, kk) : sa;
+          se = in ? rl(excl, kk) : se;
+          sl = in ? rl(lo, kk) : sl;
+          sh2 = in ? rl(hi, kk) : sh2;
+          r.ox = in ? rlf(ofx, kk) : r.ox;
+          r.oy = in ? rlf(ofy, kk) : r.oy;
+          r.oz = in ? rlf(ofz, kk) : r.oz;
+        }
       }
       const bool valid = f < total;
       const uint32_t j = valid ? sa + (f - se) : (uint32_t)(lane & 3);
