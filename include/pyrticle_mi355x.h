@@ -221,6 +221,22 @@ int pyr_deposit(pyr_mesh *mesh, double radius, double alpha, double normalizer,
                 const double *charges, const double *velocities,
                 const uint32_t *containing_elements, double *rho_out, double *j_out);
 
+/* dep_normshape.hpp:53-245 normalized_shape_function_depositor (wrap_depositor.cpp
+ * :129-134, NormalizingInterpolatingDepositor; pyrticle/deposition/shape.py:58-98):
+ * like pyr_deposit, but every particle deposits (q / I) S, I = sum over the
+ * elements its shape reaches of |J_e| sum_i S(node_i - x) w_i with w = M 1
+ * (integral_weights: device, Np doubles, the reference element's mass matrix
+ * times ones), so that the deposited charge integrates to q exactly.  I is
+ * formed in FP64 for both precisions.  Located particles with I = 0 deposit
+ * nothing; their number is returned in *n_zero_out (host) for the caller's
+ * warning (the reference's WARN per particle).  scale_out (device, n doubles,
+ * may be NULL) receives q / I. */
+int pyr_deposit_normalized(pyr_mesh *mesh, double radius, double alpha, double normalizer,
+                           int precision, int64_t n, int vdim, const double *x, const double *q,
+                           const double *v, const uint32_t *containing_elements,
+                           const double *integral_weights, double *rho, double *j,
+                           double *scale_out, int64_t *n_zero_out);
+
 /* Adjoint of pyr_deposit (no reference counterpart): given rho_bar = dL/drho
  * (K*Np, may be NULL) and j_bar = dL/dj (K*Np x vdim, may be NULL), writes
  * x_bar = dL/dx (n x vdim) and, with j_bar, v_bar = dL/dv (n x vdim); both are

@@ -310,12 +310,52 @@ typedef struct {
   int64_t mark;
   uint64_t visits, elements;
   int depth, max_depth;
+  /* normalized deposition (dep_normshape.hpp:88-174): record instead of add */
+  const double *iw;       /* integral weights M 1 (Np), or NULL */
+  double integral;        /* sum over visited elements of |J| sum_i S_i w_i */
+  double *rec_s;          /* recorded shape values */
+  uint32_t *rec_el;       /* recorded elements */
+  size_t rec_n, rec_cap;  /* recorded elements, capacity in elements */
 } dfs_ctx;
+
+/* dep_normshape.hpp:124-165 add_shape_on_element: store the element's shape
+ * values and add |J| sum_i S_i w_i to the particle's integral. */
+static int record_shape_on_element(dfs_ctx *c, const double *center, uint32_t en) {
+  const orc_mesh *m = c->m;
+  const int d = m->dim, np = m->Np;
+  const size_t start = (size_t)en * np;
+  if (c->rec_n == c->rec_cap) {
+    size_t cap = c->rec_cap ? 2 * c->rec_cap : 64;
+    double *s2 = (double *)realloc(c->rec_s, sizeof(double) * cap * np);
+    uint32_t *e2 = (uint32_t *)realloc(c->rec_el, sizeof(uint32_t) * cap);
+    if (!s2 || !e2) return -1;
+    c->rec_s = s2;
+    c->rec_el = e2;
+    c->rec_cap = cap;
+  }
+  c->elements++;
+  double el_integral = 0;
+  double *sv = c->rec_s + c->rec_n * np;
+  for (int i = 0; i < np; ++i) {
+    double r[3];
+    for (int k = 0; k < d; ++k) r[k] = m->nodes[(start + i) * d + k] - center[k];
+    sv[i] = shape_eval(c->normalizer, c->radius, c->alpha, r, d);
+    c->visits++;
+    el_integral += sv[i] * c->iw[i];
+  }
+  c->integral += el_integral * m->jacobian[en];
+  c->rec_el[c->rec_n++] = en;
+  return 0;
+}
 
 static void add_shape_on_element(dfs_ctx *c, const double *center, uint32_t en) {
   const orc_mesh *m = c->m;
   const int d = m->dim, np = m->Np;
   const size_t start = (size_t)en * np;
+  if (c->iw) {
+    if (record_shape_on_element(c, center, en)) c->depth = -1000000;   /* out of memory */
+    return;
+  }
   c->elements++;
   for (int i = 0; i < np; ++i) {
     double r[3];
@@ -407,6 +447,71 @@ int orc_deposit(const orc_mesh *m, double radius, double alpha, int64_t n, int x
     visits_out[2] = (uint64_t)c.max_depth;
   }
   return 0;
+}
+
+/* dep_normshape.hpp:53-245 normalized_shape_function_depositor over the whole
+ * particle slice: per particle, the face walk records the shape values of
+ * every visited element and the integral I = sum_e |J_e| sum_i S_i w_i
+ * (w = M 1, dep_normshape.hpp:214-222); the particle deposits (q / I) S_i,
+ * and (q / I) v S_i into j (end_particle, :168-198).  A particle with I = 0
+ * deposits nothing and is counted in *n_zero (the reference's warning).
+ * scale_out (n, may be NULL) receives q / I (0 where I = 0).                  */
+int orc_deposit_normalized(const orc_mesh *m, double radius, double alpha, int64_t n, int xd,
+                           int vd, const double *x, const double *q, const double *v,
+                           const uint32_t *elem, const double *integral_weights, double *rho,
+                           double *j, double *scale_out, int64_t *n_zero) {
+  const int np = m->Np;
+  const size_t nn = (size_t)m->K * np;
+  if (rho) memset(rho, 0, nn * sizeof(double));
+  if (j) memset(j, 0, nn * vd * sizeof(double));
+  dfs_ctx c;
+  memset(&c, 0, sizeof c);
+  c.m = m;
+  c.radius = radius;
+  c.alpha = alpha;
+  c.normalizer = orc_shape_normalizer(radius, m->dim, alpha);
+  c.vd = vd;
+  c.iw = integral_weights;
+  c.stamp = (int64_t *)malloc(sizeof(int64_t) * (size_t)m->K);
+  if (!c.stamp) return -1;
+  for (int k = 0; k < m->K; ++k) c.stamp[k] = -1;
+  int64_t zeros = 0;
+  int rc = 0;
+  for (int64_t pn = 0; pn < n; ++pn) {
+    double pos[3];
+    for (int i = 0; i < m->dim; ++i) pos[i] = x[pn * xd + i];
+    c.mark = pn;
+    c.integral = 0;
+    c.rec_n = 0;
+    c.depth = 0;
+    recurse(&c, pos, elem[pn]);
+    if (c.depth < 0) {
+      rc = -1;
+      break;
+    }
+    if (c.integral == 0) {
+      ++zeros;
+      if (scale_out) scale_out[pn] = 0;
+      continue;
+    }
+    const double scale = q[pn] / c.integral;
+    if (scale_out) scale_out[pn] = scale;
+    for (size_t r = 0; r < c.rec_n; ++r) {
+      const size_t start = (size_t)c.rec_el[r] * np;
+      const double *sv = c.rec_s + r * np;
+      for (int i = 0; i < np; ++i) {
+        const double val = scale * sv[i];
+        if (rho) rho[start + i] += val;
+        if (j && v)
+          for (int a = 0; a < vd; ++a) j[(start + i) * vd + a] += v[pn * vd + a] * val;
+      }
+    }
+  }
+  free(c.stamp);
+  free(c.rec_s);
+  free(c.rec_el);
+  if (n_zero) *n_zero = zeros;
+  return rc;
 }
 
 /* ---------------------------------------------------------------------- */

@@ -180,6 +180,33 @@ def deposit(om, radius, alpha, x, q, v=None, elem=None, want_rho=True, want_j=Tr
     return rho, j, visits
 
 
+def deposit_normalized(om, radius, alpha, x, q, v, elem, integral_weights, want_rho=True,
+                       want_j=True):
+    """dep_normshape.hpp:53-245: returns (rho, j, scale, n_zero), scale = q / I per
+    particle (0 where the shape integral I vanishes)."""
+    ma = om.ma
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    elem = np.ascontiguousarray(elem, dtype=np.uint32)
+    w = np.ascontiguousarray(integral_weights, dtype=np.float64)
+    assert w.shape == (ma.Np,)
+    n, xd = x.shape
+    vd = v.shape[1] if v is not None else xd
+    if v is not None:
+        v = np.ascontiguousarray(v, dtype=np.float64)
+    rho = np.empty(ma.nnodes) if want_rho else None
+    j = np.empty((ma.nnodes, vd)) if (want_j and v is not None) else None
+    scale = np.empty(n)
+    nz = C.c_int64(0)
+    rc = lib().orc_deposit_normalized(om.ref, C.c_double(radius), C.c_double(alpha),
+                                      C.c_int64(n), xd, vd, _ptr(x), _ptr(q), _ptr(v),
+                                      _ptr(elem), _ptr(w), _ptr(rho), _ptr(j), _ptr(scale),
+                                      C.byref(nz))
+    if rc:
+        raise RuntimeError("normalized deposit failed")
+    return rho, j, scale, int(nz.value)
+
+
 def _field_ptrs(fields):
     arrs = [None if f is None else np.ascontiguousarray(f, dtype=np.float64) for f in fields]
     ptrs = (C.c_void_p * 6)(*[None if a is None else a.ctypes.data for a in arrs])

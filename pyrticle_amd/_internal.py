@@ -311,6 +311,101 @@ class InterpolatingDepositor3d3v(_InterpolatingDepositor):
     pass
 
 
+class StatsGatherer:
+    """tools.hpp:340-400 stats_gatherer (running mean and variance, Welford);
+    add_array merges a whole batch with the parallel combination formula."""
+
+    def __init__(self):
+        self.reset()
+
+    def reset(self):
+        self._count, self._mean, self._m2 = 0, 0.0, 0.0
+        self._min = self._max = None
+
+    def add(self, x):
+        x = float(x)
+        if self._count == 0 or x < self._min:
+            self._min = x
+        if self._count == 0 or x > self._max:
+            self._max = x
+        self._count += 1
+        delta = x - self._mean
+        self._mean += delta / self._count
+        self._m2 += delta * (x - self._mean)
+
+    def add_array(self, a):
+        a = np.asarray(a, dtype=np.float64).ravel()
+        nb = a.size
+        if nb == 0:
+            return
+        mb = float(a.mean())
+        m2b = float(((a - mb) ** 2).sum())
+        lo, hi = float(a.min()), float(a.max())
+        if self._count == 0:
+            self._min, self._max = lo, hi
+        else:
+            self._min, self._max = min(self._min, lo), max(self._max, hi)
+        n = self._count + nb
+        delta = mb - self._mean
+        self._mean += delta * nb / n
+        self._m2 += m2b + delta * delta * self._count * nb / n
+        self._count = n
+
+    def count(self):
+        return self._count
+
+    def minimum(self):
+        return self._min
+
+    def maximum(self):
+        return self._max
+
+    def mean(self):
+        return self._mean
+
+    def variance_sample(self):
+        return self._m2 / (self._count - 1)
+
+    def standard_deviation_sample(self):
+        return math.sqrt(self.variance_sample())
+
+    def variance(self):
+        return self._m2 / self._count
+
+    def standard_deviation(self):
+        return math.sqrt(self.variance())
+
+
+class _NormalizingInterpolatingDepositor(_InterpolatingDepositor):
+    """dep_normshape.hpp:53-245 (wrap_depositor.cpp:129-134): constructed from the
+    local discretization's mass matrix; the integral weights are M 1
+    (dep_normshape.hpp:214-222)."""
+
+    class DepositorState(_InterpolatingDepositor.DepositorState):
+        class _Stats:
+            def __init__(self):
+                self.normalization_stats = StatsGatherer()
+
+        def __init__(self):
+            self.stats = self._Stats()
+
+        def clear(self):
+            self.stats = self._Stats()
+
+    def __init__(self, mesh_data, mass_matrix):
+        _InterpolatingDepositor.__init__(self, mesh_data)
+        mm = np.asarray(mass_matrix, dtype=np.float64)
+        self.integral_weights = mm @ np.ones(mm.shape[1])
+
+
+class NormalizingInterpolatingDepositor2d2v(_NormalizingInterpolatingDepositor):
+    pass
+
+
+class NormalizingInterpolatingDepositor3d3v(_NormalizingInterpolatingDepositor):
+    pass
+
+
 def _slice_range(pslice, n):
     if pslice is None:
         return 0, n
@@ -320,7 +415,7 @@ def _slice_range(pslice, n):
     return start, stop
 
 
-def _deposit(dep, ps, node_count, velocities, pslice, want_rho, want_j):
+def _deposit(dep, ps, node_count, velocities, pslice, want_rho, want_j, ds=None):
     sf = dep.shape_function
     if sf is None:
         raise RuntimeError("shape function never set")
@@ -330,6 +425,19 @@ def _deposit(dep, ps, node_count, velocities, pslice, want_rho, want_j):
     a, b = _slice_range(pslice, ps.particle_count)
     x = ps.positions[a:b]
     v = velocities[a:b].contiguous() if velocities is not None else None
+    if isinstance(dep, _NormalizingInterpolatingDepositor):
+        scale = torch.empty(b - a, dtype=torch.float64, device=x.device)
+        rho, j, n_zero = md.device.deposit_normalized(
+            sf.radius, sf.exponent, sf.normalizer, x, ps.charges[a:b],
+            ps.containing_elements[a:b], dep.integral_weights, v=v, want_rho=want_rho,
+            want_j=want_j, precision=dep.precision, scale=scale)
+        if ds is not None:
+            sc = scale.cpu().numpy()
+            ds.stats.normalization_stats.add_array(sc[sc != 0])
+        if n_zero:
+            # dep_normshape.hpp:175-183 warns once per such particle
+            WarningListener.warn("deposited particle mass is zero (%d particles)" % n_zero)
+        return rho, j
     rho, j = md.device.deposit(sf.radius, sf.exponent, sf.normalizer, x, ps.charges[a:b],
                                ps.containing_elements[a:b], v=v, want_rho=want_rho,
                                want_j=want_j, precision=dep.precision)
@@ -337,16 +445,16 @@ def _deposit(dep, ps, node_count, velocities, pslice, want_rho, want_j):
 
 
 def deposit_rho(dep, ds, ps, node_count, pslice=slice(None)):
-    return _deposit(dep, ps, node_count, None, pslice, True, False)[0]
+    return _deposit(dep, ps, node_count, None, pslice, True, False, ds)[0]
 
 
 def deposit_j(dep, ds, ps, node_count, velocities, pslice=slice(None)):
     """j as node_count x vdim (node-major, as the reference returns it)."""
-    return _deposit(dep, ps, node_count, velocities, pslice, False, True)[1]
+    return _deposit(dep, ps, node_count, velocities, pslice, False, True, ds)[1]
 
 
 def deposit_densities(dep, ds, ps, node_count, velocities, pslice=slice(None)):
-    return _deposit(dep, ps, node_count, velocities, pslice, True, True)
+    return _deposit(dep, ps, node_count, velocities, pslice, True, True, ds)
 
 
 # ---------------------------------------------------------------------------

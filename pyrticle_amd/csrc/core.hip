@@ -432,6 +432,7 @@ int pyr_mesh_create(const pyr_mesh_desc *d, pyr_mesh **out) {
         return upload(h->owned.back(), p, count);
       };
       m.inv_map = up(d->inv_map, (size_t)K * (D * D + D));
+      m.jacobian = d->jacobian ? up(d->jacobian, (size_t)K) : nullptr;
       m.el_vertices = up(d->el_vertices, (size_t)K * F);
       m.face_normal = up(d->face_normal, (size_t)K * F * D);
       m.face_rhs = up(d->face_rhs, (size_t)K * F);

@@ -239,6 +239,7 @@ struct DepositCache {
   // sort / staging scratch
   DevBuf keys_a, keys_b, vals_a, vals_b, hist, hist_scan, seg, tmp, pos, wgt, stats;
   DevBuf pos32, wgt32;   // FP32 staging
+  DevBuf nrm;            // normalized deposition: integrals, scaled charges, counter
   DevBuf skeys;   // Morton sort keys
   std::vector<double> h_geom;   // K x 8
 };
@@ -250,7 +251,7 @@ void pyr_deposit_cache_free(pyr_mesh *mesh) {
                     &c->tab_R, &c->tr_start, &c->tr, &c->tr_shift, &c->keys_a, &c->keys_b,
                     &c->vals_a, &c->vals_b, &c->skeys, &c->hist,
                     &c->hist_scan, &c->seg, &c->tmp, &c->pos, &c->wgt, &c->stats,
-                    &c->pos32, &c->wgt32})
+                    &c->pos32, &c->wgt32, &c->nrm})
     b->release();
   delete c;
   mesh->dep = nullptr;
@@ -636,14 +637,28 @@ static void build_transpose(pyr_mesh *mesh, DepositCache *c) {
 
 static int g_deposit_count_mode = 0;
 
+// q / I per particle for the normalized deposition; located particles whose
+// shape integral vanishes deposit nothing and are counted (the reference warns
+// for each, dep_normshape.hpp:175-183).  scale (optional) receives q / I.
+__global__ void k_normalized_charges(int64_t n, int K, const uint32_t *__restrict__ elem,
+                                     const double *__restrict__ q,
+                                     const double *__restrict__ integral,
+                                     double *__restrict__ qs, double *scale,
+                                     unsigned long long *zeros) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bool located = elem[i] < (uint32_t)K;
+  const double I = integral[i];
+  const double v = (located && I != 0.0) ? q[i] / I : 0.0;
+  qs[i] = v;
+  if (scale) scale[i] = v;
+  if (located && I == 0.0) atomicAdd(zeros, 1ull);
+}
 
-
-extern "C" {
-
-int pyr_deposit(pyr_mesh *mesh, double radius, double alpha, double normalizer, int precision,
-                int64_t n, int vdim, const double *x, const double *q, const double *v,
-                const uint32_t *elem, double *rho, double *j) {
-  return guarded([&] {
+static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double normalizer,
+                         int precision, int64_t n, int vdim, const double *x, const double *q,
+                         const double *v, const uint32_t *elem, double *rho, double *j) {
+  {
     PYR_REQUIRE(mesh, "null mesh");
     PYR_REQUIRE(radius > 0 && std::isfinite(radius), "shape radius must be positive");
     PYR_REQUIRE(alpha > 0, "shape exponent must be positive");
@@ -702,6 +717,86 @@ int pyr_deposit(pyr_mesh *mesh, double radius, double alpha, double normalizer, 
     ProfScope prof(PROF_GATHER);
     if (precision == PYR_FP64) launch_gather_fp64(D, Np, ga, K, alpha == 2.0, count);
     else launch_gather_fp32(D, Np, ga, K, alpha == 2.0, count);
+  }
+}
+
+extern "C" {
+
+int pyr_deposit(pyr_mesh *mesh, double radius, double alpha, double normalizer, int precision,
+                int64_t n, int vdim, const double *x, const double *q, const double *v,
+                const uint32_t *elem, double *rho, double *j) {
+  return guarded([&] {
+    deposit_impl(mesh, radius, alpha, normalizer, precision, n, vdim, x, q, v, elem, rho, j);
+  });
+}
+
+int pyr_deposit_normalized(pyr_mesh *mesh, double radius, double alpha, double normalizer,
+                           int precision, int64_t n, int vdim, const double *x, const double *q,
+                           const double *v, const uint32_t *elem,
+                           const double *integral_weights, double *rho, double *j,
+                           double *scale_out, int64_t *n_zero_out) {
+  return guarded([&] {
+    PYR_REQUIRE(mesh && n_zero_out, "null argument");
+    *n_zero_out = 0;
+    PYR_REQUIRE(radius > 0 && std::isfinite(radius), "shape radius must be positive");
+    PYR_REQUIRE(alpha > 0, "shape exponent must be positive");
+    PYR_REQUIRE(precision == PYR_FP64 || precision == PYR_FP32C, "unknown precision");
+    PYR_REQUIRE(vdim == mesh->dim, "vdim must equal the mesh dimension");
+    PYR_REQUIRE(n >= 0 && n < (int64_t)0x7FFFFFFF, "invalid particle count");
+    PYR_REQUIRE(!j || v, "j requires velocities");
+    PYR_REQUIRE(mesh->dev.jacobian, "the mesh was created without element jacobians");
+    if (!rho && !j) return;
+    if (n > 0) PYR_REQUIRE(x && q && elem && integral_weights, "null pointer");
+    if (!mesh->dep) mesh->dep = new DepositCache();
+    DepositCache *c = mesh->dep;
+    const int K = mesh->K, Np = mesh->Np, D = mesh->dim;
+    hipStream_t s = stream();
+    const double *qn = q;
+    if (n > 0) {
+      prepare_tables(mesh, c, radius);
+      if (c->tr_radius != radius) build_transpose(mesh, c);
+      const uint32_t *perm;
+      {
+        ProfScope prof(PROF_SORT);
+        perm = sort_and_stage(mesh, c, n, x, q, nullptr, elem, normalizer);
+      }
+      // FP64 shape integral per particle (dep_normshape.hpp:124-165)
+      c->nrm.reserve(sizeof(double) * ((size_t)2 * n + 1) + sizeof(unsigned long long));
+      double *integral = c->nrm.as<double>();
+      double *qs = integral + n;
+      unsigned long long *zeros = reinterpret_cast<unsigned long long *>(qs + n);
+      PYR_HIP(hipMemsetAsync(integral, 0, sizeof(double) * (size_t)n, s));
+      PYR_HIP(hipMemsetAsync(zeros, 0, sizeof(unsigned long long), s));
+      IntegralArgs ia{};
+      ia.K = K;
+      ia.R = radius;
+      ia.invR = 1.0 / radius;
+      ia.alpha = alpha;
+      ia.normalizer = normalizer;
+      for (int a = 0; a < 3; ++a) ia.shift_len[a] = mesh->dev.per_max[a] - mesh->dev.per_min[a];
+      ia.tr_start = c->tr_start.as<uint32_t>();
+      ia.tr = c->tr.as<uint32_t>();
+      ia.tr_shift = c->tr_shift.as<uint8_t>();
+      ia.seg = c->seg.as<uint32_t>();
+      ia.perm = perm;
+      ia.pos = c->pos.as<double4>();
+      ia.geom = c->geom.as<double>();
+      ia.node_tab = c->tab64.as<double>();
+      ia.iw = integral_weights;
+      ia.jac = mesh->dev.jacobian;
+      ia.integral = integral;
+      launch_shape_integral(D, Np, ia, K, alpha == 2.0);
+      // charges q / I (dep_normshape.hpp:168-198); zero-mass particles drop out
+      hipLaunchKernelGGL(k_normalized_charges, dim3(blocks_for(n)), dim3(256), 0, s, n, K, elem,
+                         q, integral, qs, scale_out, zeros);
+      PYR_CHECK_LAUNCH();
+      unsigned long long hz = 0;
+      PYR_HIP(hipMemcpyAsync(&hz, zeros, sizeof(hz), hipMemcpyDeviceToHost, s));
+      PYR_HIP(hipStreamSynchronize(s));
+      *n_zero_out = (int64_t)hz;
+      qn = qs;
+    }
+    deposit_impl(mesh, radius, alpha, normalizer, precision, n, vdim, x, qn, v, elem, rho, j);
   });
 }
 

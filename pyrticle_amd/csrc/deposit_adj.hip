@@ -1,4 +1,7 @@
-// deposit_adj.hip -- instantiations of the deposition adjoint (k_deposit_adj).
+// deposit_adj.hip -- instantiations of the deposition adjoint (k_deposit_adj)
+// and of the per-particle shape integral of the normalized deposition
+// (k_shape_integral), both per-particle gathers over the transposed candidate
+// lists.
 #include "deposit_kernels.h"
 
 namespace pyr {
@@ -31,6 +34,33 @@ void launch_deposit_adj(int D, int Np, const AdjArgs &ga, int K, int precision, 
       case 10: return launch<3, 10>(ga, K, precision, a2);
       case 20: return launch<3, 20>(ga, K, precision, a2);
       case 35: return launch<3, 35>(ga, K, precision, a2);
+    }
+  }
+  throw Error(PYR_ERR_INVALID_ARG, "deposition supports P1..P4 (2D) and P1..P4 (3D) elements");
+}
+
+template <int D, int NP>
+static void launch_integral(const IntegralArgs &ga, int K, bool a2) {
+  dim3 grid(gather_grid(K)), block(64);
+  if (a2) hipLaunchKernelGGL((k_shape_integral<D, NP, true>), grid, block, 0, stream(), ga);
+  else hipLaunchKernelGGL((k_shape_integral<D, NP, false>), grid, block, 0, stream(), ga);
+  PYR_CHECK_LAUNCH();
+}
+
+void launch_shape_integral(int D, int Np, const IntegralArgs &ga, int K, bool a2) {
+  if (D == 2) {
+    switch (Np) {
+      case 3: return launch_integral<2, 3>(ga, K, a2);
+      case 6: return launch_integral<2, 6>(ga, K, a2);
+      case 10: return launch_integral<2, 10>(ga, K, a2);
+      case 15: return launch_integral<2, 15>(ga, K, a2);
+    }
+  } else {
+    switch (Np) {
+      case 4: return launch_integral<3, 4>(ga, K, a2);
+      case 10: return launch_integral<3, 10>(ga, K, a2);
+      case 20: return launch_integral<3, 20>(ga, K, a2);
+      case 35: return launch_integral<3, 35>(ga, K, a2);
     }
   }
   throw Error(PYR_ERR_INVALID_ARG, "deposition supports P1..P4 (2D) and P1..P4 (3D) elements");

@@ -787,9 +787,88 @@ __global__ __launch_bounds__(64) void k_deposit_adj(AdjArgs g) {
 // (PYR_GATHER_MFMA, default 1; it then reads the FP32 staging).
 bool gather_fp32_uses_mfma();
 
+// ---------------------------------------------------------------------------
+// per-particle shape integral of the normalized deposition
+// ---------------------------------------------------------------------------
+// dep_normshape.hpp:124-165: I_p = sum_t |J_t| sum_i w_i S(n_ti - x_p) over the
+// target elements t within reach of the particle (the transposed candidate
+// lists of its source element), w = M 1 the integral weights of the reference
+// element.  FP64 in both precisions (the charge normalization stays FP64).
+// One wave per 64 particles of a source element; target data through scalar
+// loads.  integral[perm[j]] receives I_p (including the shape normalizer).
+struct IntegralArgs {
+  int K;
+  double R, invR, alpha, normalizer;
+  double shift_len[3];
+  const uint32_t *tr_start;   // K + 1, per source element
+  const uint32_t *tr;         // target elements
+  const uint8_t *tr_shift;    // shift code applied to the source particle
+  const uint32_t *seg;        // K + 1 segment starts into pos
+  const uint32_t *perm;       // sorted position -> particle index
+  const double4 *pos;
+  const double *geom;
+  const double *node_tab;     // K x Np x 4 (FP64): n - c, |n - c|^2 / R
+  const double *iw;           // Np integral weights
+  const double *jac;          // K: |det| of the element maps
+  double *integral;           // n
+};
+
+template <int D, int NP, bool A2>
+__global__ __launch_bounds__(64) void k_shape_integral(IntegralArgs g) {
+  using CD = const double __attribute__((address_space(4))) *;
+  const int s = xcd_element(blockIdx.x);
+  if (s >= g.K) return;
+  const uint32_t p0 = g.seg[s], p1 = g.seg[s + 1];
+  if (p0 >= p1) return;
+  const int lane = threadIdx.x;
+  const uint32_t t_begin = g.tr_start[s], t_end = g.tr_start[s + 1];
+  const double two_inv_r = 2.0 * g.invR;
+  const CD ntab_all = (CD)g.node_tab, iw = (CD)g.iw, jac = (CD)g.jac;
+  for (uint32_t base = p0; base < p1; base += 64) {
+    const uint32_t jj = base + lane;
+    const bool valid = jj < p1;
+    const double4 pp = valid ? g.pos[jj] : make_double4(0, 0, 0, 0);
+    double integral = 0;
+    for (uint32_t ti = t_begin; ti < t_end; ++ti) {
+      const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.tr[ti]);
+      const uint32_t code = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.tr_shift[ti]);
+      double sh[3] = {0, 0, 0};
+      if (code) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          const uint32_t cc = (code >> (2 * a)) & 3u;
+          sh[a] = cc == 1 ? g.shift_len[a] : (cc == 2 ? -g.shift_len[a] : 0.0);
+        }
+      }
+      const CD gt = (CD)g.geom + (size_t)t * 8;
+      const double d[3] = {pp.x + sh[0] - gt[0], pp.y + sh[1] - gt[1],
+                           D == 3 ? pp.z + sh[2] - gt[2] : 0.0};
+      const double dd = fma(d[0], d[0], fma(d[1], d[1], D == 3 ? d[2] * d[2] : 0.0));
+      const bool pass = valid && dd <= gt[3];
+      if (__ballot(pass) == 0) continue;
+      const double m[3] = {d[0] * two_inv_r, d[1] * two_inv_r, d[2] * two_inv_r};
+      const double c0 = fma(-dd, g.invR, g.R);
+      const CD nt = ntab_all + (size_t)t * NP * 4;
+      double el_integral = 0;
+#pragma unroll
+      for (int i = 0; i < NP; ++i) {
+        double tt = c0 - nt[i * 4 + 3];
+#pragma unroll
+        for (int k = D - 1; k >= 0; --k) tt = fma(nt[i * 4 + k], m[k], tt);
+        const double w = (pass && tt > 0.0) ? tt : 0.0;
+        const double sv = A2 ? w * w : (w > 0.0 ? pow(w, g.alpha) : 0.0);
+        el_integral = fma(sv, iw[i], el_integral);
+      }
+      integral = fma(el_integral, jac[t], integral);
+    }
+    if (valid) g.integral[g.perm[jj]] = g.normalizer * integral;
+  }
+}
+
 // Launchers, one per translation unit.  precision: PYR_FP64 or PYR_FP32C.
 void launch_gather_fp64(int D, int Np, const GatherArgs &ga, int K, bool a2, bool count);
 void launch_gather_fp32(int D, int Np, const GatherArgs &ga, int K, bool a2, bool count);
 void launch_deposit_adj(int D, int Np, const AdjArgs &ga, int K, int precision, bool a2);
+void launch_shape_integral(int D, int Np, const IntegralArgs &ga, int K, bool a2);
 
 }  // namespace pyr

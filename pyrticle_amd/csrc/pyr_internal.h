@@ -114,6 +114,7 @@ T *upload(DevBuf &buf, const T *host, size_t count) {
 struct DevMesh {
   int dim, K, Np, nv;
   const double *inv_map;        // K x (d*d + d)
+  const double *jacobian;       // K: |det| of the forward maps (or null)
   const uint32_t *el_vertices;  // K x (d+1)
   const double *face_normal;    // K x (d+1) x d
   const double *face_rhs;       // K x (d+1)

@@ -38,3 +38,42 @@ class ShapeFunctionDepositor(Depositor):
 
     def note_change_size(self, state, count):
         pass
+
+
+class NormalizedShapeFunctionDepositor(Depositor):
+    """pyrticle/deposition/shape.py:55-98: every particle's deposit is scaled so
+    that its charge density integrates exactly to its charge over the elements
+    it reaches (dep_normshape.hpp:122-195).  Backed by
+    NormalizingInterpolatingDepositor{2d2v,3d3v}: per-particle shape integrals
+    in FP64 on the GPU, then the same gather deposition as ShapeFunctionDepositor
+    with the scaled charges.  The depositor state's
+    ``stats.normalization_stats`` gathers q / integral of every deposited
+    particle."""
+
+    name = "NormShape"
+
+    def __init__(self, precision="fp64"):
+        Depositor.__init__(self)
+        self.precision = dv.FP64 if precision == "fp64" else dv.FP32C
+
+    def initialize(self, method):
+        Depositor.initialize(self, method)
+        eg, = method.discretization.element_groups
+        ldis = eg.local_discretization
+        backend_class = _internal.class_for("NormalizingInterpolatingDepositor",
+                                            method.get_dimensionality_suffix())
+        self.backend = backend_class(method.mesh_data, ldis.mass_matrix())
+        self.backend.precision = self.precision
+
+    def make_state(self, state):
+        return self.backend.DepositorState()
+
+    def set_shape_function(self, state, sf):
+        Depositor.set_shape_function(self, state, sf)
+        self.backend.shape_function = sf
+
+    def note_move(self, state, orig, dest, size):
+        pass
+
+    def note_change_size(self, state, count):
+        pass

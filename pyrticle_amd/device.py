@@ -160,6 +160,26 @@ class DeviceMesh:
             _lib.ptr(j) if (want_j and v is not None) else None))
         return (rho if want_rho else None), (j if (want_j and v is not None) else None)
 
+    def deposit_normalized(self, radius, alpha, normalizer, x, q, elem, integral_weights,
+                           v=None, want_rho=True, want_j=True, precision=FP64, scale=None):
+        """Normalized shape deposition (each particle's deposit integrates to its
+        charge).  Returns (rho, j, n_zero): n_zero located particles had a zero
+        shape integral and deposited nothing.  scale (device, n) receives q / I."""
+        _lib.sync_stream()
+        n, d = x.shape
+        nn = self.K * self.Np
+        rho = torch.empty(nn, dtype=torch.float64, device=x.device) if want_rho else None
+        j = torch.empty((nn, d), dtype=torch.float64, device=x.device) \
+            if (want_j and v is not None) else None
+        w = torch.as_tensor(np.asarray(integral_weights, dtype=np.float64), device=x.device)
+        assert w.shape == (self.Np,), "integral weights need one entry per element node"
+        nz = C.c_int64(0)
+        _lib.check(self._lib.pyr_deposit_normalized(
+            self._h, radius, alpha, normalizer, precision, n, d, _lib.ptr(x), _lib.ptr(q),
+            _lib.ptr(v) if v is not None else None, _lib.ptr(elem), _lib.ptr(w),
+            _lib.ptr(rho), _lib.ptr(j), _lib.ptr(scale), C.byref(nz)))
+        return rho, j, int(nz.value)
+
     def deposit_backward(self, radius, alpha, normalizer, x, q, elem, rho_bar=None, j_bar=None,
                          v=None, precision=FP64):
         """(x_bar, v_bar) from the adjoint densities; v_bar is None without j_bar."""

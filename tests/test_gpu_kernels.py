@@ -241,6 +241,88 @@ def test_deposit_dense_beam_matches_oracle(small_cylinder, dmesh3, cuda, precisi
     assert rel_l2(j.cpu().numpy(), j_o) < tol
 
 
+def _normalized_case(dmesh, om, wl, x, q, v, el, radius, dim, prec, cuda):
+    from pyrticle_amd import device as dv
+    w = wl.discr.element_groups[0].local_discretization.integral_weights()
+    rho_o, j_o, s_o, nz_o = O.deposit_normalized(om, radius, wl.alpha, x, q, v, el, w)
+    nrm = O.shape_normalizer(radius, dim, wl.alpha)
+    scale = torch.empty(x.shape[0], dtype=torch.float64, device=cuda)
+    rho, j, nz = dmesh.deposit_normalized(radius, wl.alpha, nrm, _dev(x, cuda), _dev(q, cuda),
+                                          dv.element_tensor(el, cuda), w, v=_dev(v, cuda),
+                                          precision=prec, scale=scale)
+    s = scale.cpu().numpy()
+    assert nz == nz_o
+    assert np.array_equal(s == 0, s_o == 0)
+    live = s_o != 0
+    assert rel_l2(s, s_o) < FP64_TOL
+    # Per particle, the shape integral I = q / scale.  Each node term
+    # |J| w S(n - x) carries an error of a few ulps of |J| |w| max(S) (R - r^2/R
+    # cancels near the support's edge), so a support that barely reaches a node
+    # makes I small and its relative error large: the bound is that sum of
+    # |J| |w| max(S) over the nodes within R, not I itself.
+    from scipy.spatial import cKDTree
+    node_w = np.abs(wl.ma.jacobian[:, None] * w[None, :]).ravel()
+    near = cKDTree(wl.ma.nodes).query_ball_point(x[live], radius)
+    bound = np.array([node_w[i].sum() for i in near]) * nrm * radius ** wl.alpha
+    integ, integ_o = q[live] / s[live], q[live] / s_o[live]
+    assert (np.abs(integ - integ_o) <= FP64_TOL * bound).all()
+    return rho.cpu().numpy(), j.cpu().numpy(), rho_o, j_o, nz
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_deposit_normalized_2d_matches_oracle(square, oracle_square, dmesh2, cuda, precision):
+    """NormalizedShapeFunctionDepositor's deposit: the beam plus particles whose
+    shapes are cut off by the boundary."""
+    from pyrticle_amd import device as dv
+    wl = square
+    rng = np.random.default_rng(11)
+    x = np.concatenate([wl.x, rng.uniform(-0.99, 0.99, size=(3000, 2))])
+    el = dv.element_numpy(dmesh2.find_containing_element(_dev(x, cuda)))
+    n = x.shape[0]
+    q = np.full(n, wl.q[0])
+    v = O.velocities(np.resize(wl.p, (n, 2)), np.resize(wl.m, n), wl.c)
+    prec = dv.FP64 if precision == "fp64" else dv.FP32C
+    tol = FP64_TOL if precision == "fp64" else FP32C_TOL
+    rho, j, rho_o, j_o, nz = _normalized_case(dmesh2, oracle_square, wl, x, q, v, el,
+                                              wl.radius, 2, prec, cuda)
+    assert nz == 0
+    assert rel_l2(rho, rho_o) < tol and rel_l2(j, j_o) < tol
+    assert abs(wl.discr.integral(rho) / q.sum() - 1) < (1e-12 if precision == "fp64" else 1e-6)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_deposit_normalized_3d_matches_oracle(small_cylinder, dmesh3, cuda, precision):
+    from pyrticle_amd import device as dv
+    wl = small_cylinder
+    om = O.OracleMesh(wl.ma)
+    n = 5000
+    x, q = wl.x[:n], wl.q[:n]
+    el = dv.element_numpy(dmesh3.find_containing_element(_dev(x, cuda)))
+    v = O.velocities(wl.p[:n], wl.m[:n], wl.c)
+    prec = dv.FP64 if precision == "fp64" else dv.FP32C
+    tol = FP64_TOL if precision == "fp64" else FP32C_TOL
+    rho, j, rho_o, j_o, _ = _normalized_case(dmesh3, om, wl, x, q, v, el, wl.radius, 3, prec,
+                                             cuda)
+    assert rel_l2(rho, rho_o) < tol and rel_l2(j, j_o) < tol
+
+
+def test_deposit_normalized_zero_integral(square, oracle_square, dmesh2, cuda):
+    """A support far below the node spacing: most particles reach no node, deposit
+    nothing and are counted; the count and the zero pattern match the oracle."""
+    from pyrticle_amd import device as dv
+    wl = square
+    rng = np.random.default_rng(2)
+    x = np.concatenate([rng.uniform(-0.9, 0.9, size=(400, 2)),
+                        wl.ma.nodes[[17, 403, 999, 4000]] + 1e-5])
+    el = dv.element_numpy(dmesh2.find_containing_element(_dev(x, cuda)))
+    q = np.full(x.shape[0], wl.q[0])
+    v = np.zeros_like(x)
+    rho, _, rho_o, _, nz = _normalized_case(dmesh2, oracle_square, wl, x, q, v, el, 1e-3, 2,
+                                            dv.FP64, cuda)
+    assert 300 < nz < x.shape[0] - 4
+    assert rel_l2(rho, rho_o) < FP64_TOL
+
+
 @pytest.mark.parametrize("dt_scale", [1.0, 4.0])
 def test_lsrk4_square_matches_oracle(square, oracle_square, dmesh2, cuda, dt_scale):
     """Square workload, 10k electrons, 100 LSRK4 steps with relocalization; with a 4x

@@ -258,6 +258,115 @@ def test_deposit_linearity_and_single_particle_current(square, oracle_square):
 
 
 # ---------------------------------------------------------------------------
+# normalized deposition (dep_normshape.hpp)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dim", [2, 3])
+def test_integral_weights_from_mass_matrix(dim):
+    """M 1 (dep_normshape.hpp:214-222) equals the Vandermonde-solved node weights,
+    and they sum to the volume of the (-1,..,-1) unit simplex, 2^d / d!."""
+    ldis = D.LocalDiscretization(dim, 3)
+    w = ldis.mass_matrix() @ np.ones(ldis.node_count())
+    assert np.abs(w - ldis.integral_weights()).max() < 1e-13
+    assert abs(w.sum() - 2.0 ** dim / math.factorial(dim)) < 1e-13
+
+
+def test_jacobians_tile_the_domain(square):
+    """sum_e |J_e| sum_i w_i = area of [-1,1]^2 (pins the jacobian the normalized
+    depositor weights element integrals with, meshdata.py:71)."""
+    w = square.discr.element_groups[0].local_discretization.integral_weights()
+    assert abs(square.ma.jacobian.sum() * w.sum() - 4.0) < 1e-12
+
+
+def _brute_normalized(ma, w, x, q, v, R, alpha, dim):
+    """Normalization over ALL nodes of the mesh (no element finder):
+    I_p = sum_e |J_e| sum_i w_i S(n_ei - x_p), deposit (q_p / I_p) S."""
+    N = O.shape_normalizer(R, dim, alpha)
+    r2 = ((ma.nodes[:, None, :] - x[None, :, :]) ** 2).sum(-1)
+    t = np.where(r2 <= R * R, R - r2 / R, 0.0)
+    S = N * t ** alpha
+    node_w = (ma.jacobian[:, None] * w[None, :]).ravel()
+    integ = node_w @ S
+    scale = np.where(integ != 0, q / np.where(integ != 0, integ, 1.0), 0.0)
+    return S @ scale, S @ (scale[:, None] * v), scale, int((integ == 0).sum())
+
+
+def _located(om, x):
+    return np.array([O.find_containing_element(om, xi) for xi in x], dtype=np.uint32)
+
+
+def test_normalized_deposit_equals_brute_force_2d(square, oracle_square):
+    wl = square
+    n = 1500
+    w = wl.discr.element_groups[0].local_discretization.integral_weights()
+    # a disk reaching the outer boundary, so that shapes are cut off there
+    rng = np.random.default_rng(11)
+    x = np.concatenate([wl.x[:n // 2], rng.uniform(-0.98, 0.98, size=(n - n // 2, 2))])
+    el = _located(oracle_square, x)
+    v = O.velocities(wl.p[:n], wl.m[:n], wl.c)
+    rho, j, scale, nz = O.deposit_normalized(oracle_square, wl.radius, wl.alpha, x, wl.q[:n],
+                                             v, el, w)
+    rb, jb, sb, nzb = _brute_normalized(wl.ma, w, x, wl.q[:n], v, wl.radius, wl.alpha, 2)
+    assert nz == nzb == 0
+    assert np.abs(scale / sb - 1).max() < 1e-13
+    assert rel_l2(rho, rb) < 1e-13 and rel_l2(j, jb) < 1e-13
+    # boundary particles lose charge in the plain deposit but not here
+    assert abs(wl.discr.integral(rho) / wl.q[:n].sum() - 1) < 1e-13
+    rho0, _, _ = O.deposit(oracle_square, wl.radius, wl.alpha, x, wl.q[:n], v, el)
+    assert abs(wl.discr.integral(rho0) / wl.q[:n].sum() - 1) > 1e-3
+
+
+def test_normalized_deposit_equals_brute_force_3d(small_cylinder):
+    wl = small_cylinder
+    om = O.OracleMesh(wl.ma)
+    w = wl.discr.element_groups[0].local_discretization.integral_weights()
+    rng = np.random.default_rng(5)
+    idx = rng.choice(wl.x.shape[0], 300, replace=False)
+    x = wl.x[idx]
+    el = _located(om, x)
+    v = O.velocities(wl.p[idx], wl.m[idx], wl.c)
+    rho, j, scale, nz = O.deposit_normalized(om, wl.radius, 2.0, x, wl.q[idx], v, el, w)
+    rb, jb, sb, nzb = _brute_normalized(wl.ma, w, x, wl.q[idx], v, wl.radius, 2.0, 3)
+    assert nz == nzb == 0
+    assert np.abs(scale / sb - 1).max() < 1e-13
+    assert rel_l2(rho, rb) < 1e-13 and rel_l2(j, jb) < 1e-13
+    assert abs(wl.discr.integral(rho) / wl.q[idx].sum() - 1) < 1e-13
+
+
+def test_normalized_deposit_zero_integral_particles(square, oracle_square):
+    """A support smaller than the node spacing misses every node for most
+    particles: those deposit nothing and are counted (dep_normshape.hpp:178-184);
+    particles sitting on a node still normalize to their charge."""
+    wl = square
+    w = wl.discr.element_groups[0].local_discretization.integral_weights()
+    rng = np.random.default_rng(2)
+    x = np.concatenate([rng.uniform(-0.9, 0.9, size=(200, 2)), wl.ma.nodes[[17, 403, 999]]])
+    x[-3:] += 1e-5
+    el = _located(oracle_square, x)
+    q = np.full(x.shape[0], -U.EL_CHARGE)
+    v = np.zeros_like(x)
+    R = 1e-3
+    rho, _, scale, nz = O.deposit_normalized(oracle_square, R, 2.0, x, q, v, el, w)
+    _, _, sb, nzb = _brute_normalized(wl.ma, w, x, q, v, R, 2.0, 2)
+    assert nz == nzb and 150 < nz < x.shape[0] - 3
+    assert np.array_equal(scale == 0, sb == 0)
+    assert abs(wl.discr.integral(rho) / q[scale != 0].sum() - 1) < 1e-12
+
+
+def test_normalized_deposit_linearity_and_current(square, oracle_square):
+    wl = square
+    w = wl.discr.element_groups[0].local_discretization.integral_weights()
+    x = wl.x[:40]
+    el = _located(oracle_square, x)
+    v = O.velocities(wl.p[:40], wl.m[:40], wl.c)
+    r1, _, s1, _ = O.deposit_normalized(oracle_square, 0.1, 2.0, x, wl.q[:40], v, el, w)
+    r2, _, s2, _ = O.deposit_normalized(oracle_square, 0.1, 2.0, x, 3 * wl.q[:40], v, el, w)
+    assert rel_l2(r2, 3 * r1) < 1e-15 and rel_l2(s2, 3 * s1) < 1e-15
+    rho, j, _, _ = O.deposit_normalized(oracle_square, 0.1, 2.0, x[:1], wl.q[:1], v[:1],
+                                        el[:1], w)
+    assert rel_l2(j, rho[:, None] * v[0][None, :]) < 1e-15
+
+
+# ---------------------------------------------------------------------------
 # interpolation, LU, forces
 # ---------------------------------------------------------------------------
 def test_getrf_getrs_solve():
