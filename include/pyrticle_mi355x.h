@@ -237,6 +237,30 @@ int pyr_deposit_normalized(pyr_mesh *mesh, double radius, double alpha, double n
                            const double *integral_weights, double *rho, double *j,
                            double *scale_out, int64_t *n_zero_out);
 
+/* push_average.hpp:338-443 averaging_particle_pusher::forces (wrap_pusher.cpp
+ * :39-76, :124-140 AveragingPusher; pyrticle/pusher.py:167-207
+ * AverageParticlePusher):
+ * the force on each particle from the six nodal fields (device, K*Np each;
+ * NULL = ZeroVector) averaged with the particle's deposited shape as weight,
+ * <F> = sum_e |J_e| sum_i F_i S_i w_i / sum_e |J_e| sum_i S_i w_i over the nodes
+ * the shape reaches (w = M 1: integral_weights, device, Np doubles), giving
+ * F = q <E> + v x (q <B>) truncated to vdim (force_out, device, n x vdim).
+ * The same force results with the plain or the normalized shape depositor
+ * (the deposit's scale cancels in the ratio).  integral_out (device, n, may be
+ * NULL) receives the shape integral I = N sum_e |J_e| sum_i S_i w_i, from
+ * which the caller forms the reference's normalization statistics.
+ * Particles with I = 0 get no force; their count goes to *n_zero_out (host).
+ * vis_* (all or none, device): pt_e, pt_b (n x 3, the averages), pt_e_stddev,
+ * pt_b_stddev (n), el_force, mag_force (n x vdim), as push_average.hpp
+ * :430-438 stores them. */
+int pyr_average_forces(pyr_mesh *mesh, double radius, double alpha, double normalizer,
+                       int64_t n, int vdim, const double *const fields[6], const double *x,
+                       const uint32_t *containing_elements, const double *charges,
+                       const double *velocities, const double *integral_weights,
+                       double *force_out, double *integral_out, double *vis_e, double *vis_b,
+                       double *vis_e_stddev, double *vis_b_stddev, double *vis_el_force,
+                       double *vis_mag_force, int64_t *n_zero_out);
+
 /* Adjoint of pyr_deposit (no reference counterpart): given rho_bar = dL/drho
  * (K*Np, may be NULL) and j_bar = dL/dj (K*Np x vdim, may be NULL), writes
  * x_bar = dL/dx (n x vdim) and, with j_bar, v_bar = dL/dv (n x vdim); both are

@@ -514,6 +514,140 @@ int orc_deposit_normalized(const orc_mesh *m, double radius, double alpha, int64
   return rc;
 }
 
+/* push_average.hpp:38-443 averaging_particle_pusher::forces with the chained
+ * el_/mag_force_averaging_target over the deposition of either depositor
+ * (dep_shape.hpp:67-150 when normalized == 0: rho_contrib = q S;
+ * dep_normshape.hpp:168-198 when normalized == 1: rho_contrib = (q / I) S and
+ * a particle with I = 0 never reaches the targets).  fields[6] (ex..bz,
+ * K*Np nodal, NULL = zero_vector).  force (n x vd) = el_force + mag_force.
+ * charge_out (n, may be NULL): the reconstructed charge m_particle_charge
+ * (0 for particles the targets never saw).  vis_* (all or none):
+ * pt_e, pt_b (n x 3), pt_e_stddev, pt_b_stddev (n), el_force, mag_force
+ * (n x vd).  n_zero: particles whose targets saw zero charge, plus (normalized)
+ * those the depositor skipped.                                                */
+int orc_average_forces(const orc_mesh *m, double radius, double alpha, int normalized,
+                       int64_t n, int xd, int vd, const double *x, const double *q,
+                       const double *v, const uint32_t *elem, const double *integral_weights,
+                       const double *const *fields, double *force, double *charge_out,
+                       double *vis_e, double *vis_b, double *vis_e_std, double *vis_b_std,
+                       double *vis_el, double *vis_mag, int64_t *n_zero) {
+  const int np = m->Np;
+  dfs_ctx c;
+  memset(&c, 0, sizeof c);
+  c.m = m;
+  c.radius = radius;
+  c.alpha = alpha;
+  c.normalizer = orc_shape_normalizer(radius, m->dim, alpha);
+  c.vd = vd;
+  c.iw = integral_weights;
+  c.stamp = (int64_t *)malloc(sizeof(int64_t) * (size_t)m->K);
+  double *contrib = (double *)malloc(sizeof(double) * np);
+  if (!c.stamp || !contrib) return -1;
+  for (int k = 0; k < m->K; ++k) c.stamp[k] = -1;
+  int64_t zeros = 0;
+  int rc = 0;
+  for (int64_t pn = 0; pn < n; ++pn) {
+    double pos[3];
+    for (int i = 0; i < m->dim; ++i) pos[i] = x[pn * xd + i];
+    c.mark = pn;
+    c.integral = 0;
+    c.rec_n = 0;
+    c.depth = 0;
+    recurse(&c, pos, elem[pn]);
+    if (c.depth < 0) {
+      rc = -1;
+      break;
+    }
+    for (int a = 0; a < vd; ++a) force[pn * vd + a] = 0;
+    if (vis_e) {
+      for (int a = 0; a < 3; ++a) vis_e[pn * 3 + a] = vis_b[pn * 3 + a] = 0;
+      vis_e_std[pn] = vis_b_std[pn] = 0;
+      for (int a = 0; a < vd; ++a) vis_el[pn * vd + a] = vis_mag[pn * vd + a] = 0;
+    }
+    if (charge_out) charge_out[pn] = 0;
+    double dep_scale = q[pn];
+    if (normalized) {
+      if (c.integral == 0) {   /* dep_normshape.hpp:178-184: targets never see it */
+        ++zeros;
+        continue;
+      }
+      dep_scale = q[pn] / c.integral;
+    }
+    /* begin_particle (:125-131) for both targets; they accumulate alike */
+    double qf[2][3] = {{0, 0, 0}, {0, 0, 0}}, sq[2] = {0, 0}, charge = 0;
+    double ptf[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    for (size_t r = 0; r < c.rec_n; ++r) {
+      const uint32_t en = c.rec_el[r];
+      const size_t start = (size_t)en * np;
+      const double *sv = c.rec_s + r * np;
+      const double jac = m->jacobian[en];
+      for (int i = 0; i < np; ++i) contrib[i] = dep_scale * sv[i] * c.iw[i];
+      /* add_shape_on_element (:135-168) */
+      for (int h = 0; h < 2; ++h) {
+        double qfield[3];
+        for (int k = 0; k < 3; ++k) {
+          const double *f = fields[3 * h + k];
+          double ip = 0;
+          if (f)
+            for (int i = 0; i < np; ++i) ip += f[start + i] * contrib[i];
+          qfield[k] = jac * ip;
+          qf[h][k] += qfield[k];
+          ptf[h][k] += qfield[k];
+        }
+        double sqs = 0;
+        for (int k = 0; k < 3; ++k) {
+          const double *f = fields[3 * h + k];
+          double ip = 0;
+          if (f)
+            for (int i = 0; i < np; ++i) ip += f[start + i] * f[start + i] * contrib[i];
+          sqs += ip;
+        }
+        sq[h] += jac * sqs;
+      }
+      double ipc = 0;
+      for (int i = 0; i < np; ++i) ipc += contrib[i];
+      charge += jac * ipc;
+    }
+    if (charge_out) charge_out[pn] = charge;
+    /* end_particle (:170-215, :249-265, :309-330) */
+    if (charge == 0) {
+      ++zeros;
+      continue;
+    }
+    if (vis_e) {
+      for (int h = 0; h < 2; ++h) {
+        double *pf = h ? vis_b + pn * 3 : vis_e + pn * 3;
+        for (int k = 0; k < 3; ++k) pf[k] = ptf[h][k] / charge;
+        const double squared_mean = sq[h] / charge;
+        const double mean_squared = pf[0] * pf[0] + pf[1] * pf[1] + pf[2] * pf[2];
+        const double variance = squared_mean - mean_squared;
+        (h ? vis_b_std : vis_e_std)[pn] = variance < 0 ? 0 : sqrt(variance);
+      }
+    }
+    const double scale = q[pn] / charge;
+    double vv[3] = {0, 0, 0};
+    for (int a = 0; a < vd; ++a) vv[a] = v[pn * vd + a];
+    const double *b = qf[1];
+    const double cr[3] = {vv[1] * b[2] - vv[2] * b[1], vv[2] * b[0] - vv[0] * b[2],
+                          vv[0] * b[1] - vv[1] * b[0]};
+    for (int a = 0; a < vd; ++a) {
+      const double el = scale * qf[0][a];
+      const double mag = scale * cr[a];
+      force[pn * vd + a] = el + mag;
+      if (vis_e) {
+        vis_el[pn * vd + a] = el;
+        vis_mag[pn * vd + a] = mag;
+      }
+    }
+  }
+  free(c.stamp);
+  free(c.rec_s);
+  free(c.rec_el);
+  free(contrib);
+  if (n_zero) *n_zero = zeros;
+  return rc;
+}
+
 /* ---------------------------------------------------------------------- */
 /* LAPACK dgetf2 (netlib): LU with partial pivoting of a column-major n x n
  * matrix, as wrap_tools.cpp:126-145 (lu -> getrf) uses it.  ipiv 0-based.  */

@@ -233,6 +233,37 @@ def forces(om, x, elem, q, v, fields, vis=False):
     return f
 
 
+def average_forces(om, radius, alpha, x, q, v, elem, integral_weights, fields,
+                   normalized=False, vis=False):
+    """push_average.hpp:38-443 (averaging pusher over the plain or the
+    normalized shape deposit): returns (force, charge, n_zero[, vis dict]);
+    charge is the reconstructed charge per particle (0 where the targets saw
+    none)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    elem = np.ascontiguousarray(elem, dtype=np.uint32)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    w = np.ascontiguousarray(integral_weights, dtype=np.float64)
+    n, xd = x.shape
+    vd = v.shape[1]
+    keep, ptrs = _field_ptrs(fields)
+    f = np.empty((n, vd))
+    charge = np.empty(n)
+    names = ("pt_e", "pt_b", "pt_e_stddev", "pt_b_stddev", "el_force", "mag_force")
+    shapes = ((n, 3), (n, 3), (n,), (n,), (n, vd), (n, vd))
+    vis_arrays = [np.empty(sh) for sh in shapes] if vis else [None] * 6
+    nz = C.c_int64(0)
+    rc = lib().orc_average_forces(om.ref, C.c_double(radius), C.c_double(alpha),
+                                  int(normalized), C.c_int64(n), xd, vd, _ptr(x), _ptr(q),
+                                  _ptr(v), _ptr(elem), _ptr(w), ptrs, _ptr(f), _ptr(charge),
+                                  *[_ptr(a) for a in vis_arrays], C.byref(nz))
+    if rc:
+        raise RuntimeError("average forces failed")
+    if vis:
+        return f, charge, int(nz.value), dict(zip(names, vis_arrays))
+    return f, charge, int(nz.value)
+
+
 def lagrange_weights(om, en, x):
     x = np.ascontiguousarray(x, dtype=np.float64)
     c = np.empty(om.ma.Np)

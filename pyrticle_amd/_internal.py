@@ -334,13 +334,24 @@ class StatsGatherer:
         self._m2 += delta * (x - self._mean)
 
     def add_array(self, a):
-        a = np.asarray(a, dtype=np.float64).ravel()
-        nb = a.size
-        if nb == 0:
-            return
-        mb = float(a.mean())
-        m2b = float(((a - mb) ** 2).sum())
-        lo, hi = float(a.min()), float(a.max())
+        """Adds every entry of a (numpy array or device tensor: reduced where
+        it lives)."""
+        if isinstance(a, torch.Tensor):
+            a = a.reshape(-1).to(torch.float64)
+            nb = a.numel()
+            if nb == 0:
+                return
+            mean_t = a.mean()
+            red = torch.stack([mean_t, ((a - mean_t) ** 2).sum(), a.min(), a.max()]).cpu()
+            mb, m2b, lo, hi = (float(t) for t in red)
+        else:
+            a = np.asarray(a, dtype=np.float64).ravel()
+            nb = a.size
+            if nb == 0:
+                return
+            mb = float(a.mean())
+            m2b = float(((a - mb) ** 2).sum())
+            lo, hi = float(a.min()), float(a.max())
         if self._count == 0:
             self._min, self._max = lo, hi
         else:
@@ -432,8 +443,7 @@ def _deposit(dep, ps, node_count, velocities, pslice, want_rho, want_j, ds=None)
             ps.containing_elements[a:b], dep.integral_weights, v=v, want_rho=want_rho,
             want_j=want_j, precision=dep.precision, scale=scale)
         if ds is not None:
-            sc = scale.cpu().numpy()
-            ds.stats.normalization_stats.add_array(sc[sc != 0])
+            ds.stats.normalization_stats.add_array(scale[scale != 0])
         if n_zero:
             # dep_normshape.hpp:175-183 warns once per such particle
             WarningListener.warn("deposited particle mass is zero (%d particles)" % n_zero)
@@ -514,6 +524,73 @@ class MonomialPusher2d2v(_MonomialPusher):
 
 
 class MonomialPusher3d3v(_MonomialPusher):
+    pass
+
+
+class _AveragingPusher:
+    """push_average.hpp:338-443 averaging_particle_pusher (wrap_pusher.cpp:124-140):
+    forces from E and B averaged over each particle's deposited shape, with the
+    depositor's shape function and element reach; integral weights M 1."""
+
+    class PusherState:
+        def __init__(self):
+            self.e_normalization_stats = StatsGatherer()
+            self.b_normalization_stats = StatsGatherer()
+
+    def __init__(self, mesh_data, mass_matrix):
+        self.mesh_data = mesh_data
+        mm = np.asarray(mass_matrix, dtype=np.float64)
+        self.integral_weights = mm @ np.ones(mm.shape[1])
+
+    def forces(self, ex, ey, ez, bx, by, bz, particle_state, pusher_state, depositor,
+               depositor_state, velocities, vis_listener=None):
+        md = self.mesh_data
+        sf = depositor.shape_function
+        if sf is None:
+            raise RuntimeError("shape function never set")
+        fields = [None if (f is None or isinstance(f, ZeroVector)) else f
+                  for f in (ex, ey, ez, bx, by, bz)]
+        if md.dimensions == 2:
+            # the 2D binding is the TE case: ez, bx, by are zero_vector
+            # (wrap_pusher.cpp:57-73)
+            fields[2] = fields[3] = fields[4] = None
+        ps = particle_state
+        n = ps.particle_count
+        q = ps.charges[:n]
+        res = md.device.average_forces(
+            sf.radius, sf.exponent, sf.normalizer, fields, ps.positions[:n],
+            ps.containing_elements[:n], q, velocities[:n].contiguous(), self.integral_weights,
+            vis=vis_listener is not None)
+        F, integral, n_zero = res[:3]
+        live = integral != 0
+        if isinstance(depositor, _NormalizingInterpolatingDepositor):
+            # the normalized deposit runs underneath (dep_normshape.hpp:168-198):
+            # its statistics and warnings, and charges q / I reaching the pusher
+            dscale = q[live] / integral[live]
+            if depositor_state is not None:
+                depositor_state.stats.normalization_stats.add_array(dscale)
+            if n_zero:
+                WarningListener.warn("deposited particle mass is zero (%d particles)" % n_zero)
+            pscale = q[live] / (dscale * integral[live])
+        else:
+            # reconstructed charge q I: scale q / (q I) (push_average.hpp:258-262)
+            pscale = q[live] / (q[live] * integral[live])
+            if n_zero and vis_listener is not None:
+                WarningListener.warn("average pusher: %d particles had zero reconstructed "
+                                     "charge" % n_zero)
+        pusher_state.e_normalization_stats.add_array(pscale)
+        pusher_state.b_normalization_stats.add_array(pscale)
+        if vis_listener is not None:
+            for name, arr in res[3].items():
+                vis_listener.store_particle_vis_vector(name, arr)
+        return F
+
+
+class AveragingPusher2d2v(_AveragingPusher):
+    pass
+
+
+class AveragingPusher3d3v(_AveragingPusher):
     pass
 
 

@@ -61,6 +61,8 @@ _SIGNATURES = {
                               _P]),
     "pyr_deposit_normalized": (C.c_int, [_P, _D, _D, _D, C.c_int, _I64, C.c_int, _P, _P, _P,
                                          _P, _P, _P, _P, _P, C.POINTER(_I64)]),
+    "pyr_average_forces": (C.c_int, [_P, _D, _D, _D, _I64, C.c_int, _P, _P, _P, _P, _P, _P,
+                                     _P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(_I64)]),
     "pyr_deposit_set_counting": (C.c_int, [C.c_int]),
     "pyr_push_set_mfma": (C.c_int, [C.c_int]),
     "pyr_push_mfma_waves": (C.c_int, [C.POINTER(_I64)]),

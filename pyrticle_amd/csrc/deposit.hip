@@ -800,6 +800,92 @@ int pyr_deposit_normalized(pyr_mesh *mesh, double radius, double alpha, double n
   });
 }
 
+int pyr_average_forces(pyr_mesh *mesh, double radius, double alpha, double normalizer,
+                       int64_t n, int vdim, const double *const fields[6], const double *x,
+                       const uint32_t *elem, const double *q, const double *v,
+                       const double *integral_weights, double *force, double *integral_out,
+                       double *vis_e, double *vis_b, double *vis_e_std, double *vis_b_std,
+                       double *vis_el, double *vis_mag, int64_t *n_zero_out) {
+  return guarded([&] {
+    PYR_REQUIRE(mesh && n_zero_out && fields, "null argument");
+    *n_zero_out = 0;
+    PYR_REQUIRE(radius > 0 && std::isfinite(radius), "shape radius must be positive");
+    PYR_REQUIRE(alpha > 0, "shape exponent must be positive");
+    PYR_REQUIRE(vdim == mesh->dim, "vdim must equal the mesh dimension");
+    PYR_REQUIRE(n >= 0 && n < (int64_t)0x7FFFFFFF, "invalid particle count");
+    PYR_REQUIRE(mesh->dev.jacobian, "the mesh was created without element jacobians");
+    const bool vis = vis_e != nullptr;
+    PYR_REQUIRE(!vis || (vis_b && vis_e_std && vis_b_std && vis_el && vis_mag),
+                "all six vis arrays or none");
+    if (n == 0) return;
+    PYR_REQUIRE(x && q && v && elem && integral_weights && force, "null pointer");
+    if (!mesh->dep) mesh->dep = new DepositCache();
+    DepositCache *c = mesh->dep;
+    const int K = mesh->K, Np = mesh->Np, D = mesh->dim;
+    hipStream_t s = stream();
+    prepare_tables(mesh, c, radius);
+    if (c->tr_radius != radius) build_transpose(mesh, c);
+    const uint32_t *perm;
+    {
+      ProfScope prof(PROF_SORT);
+      perm = sort_and_stage(mesh, c, n, x, q, nullptr, elem, normalizer);
+    }
+    // particles without an element keep zero outputs
+    PYR_HIP(hipMemsetAsync(force, 0, sizeof(double) * (size_t)n * vdim, s));
+    if (integral_out) PYR_HIP(hipMemsetAsync(integral_out, 0, sizeof(double) * (size_t)n, s));
+    if (vis) {
+      for (double *b : {vis_e, vis_b})
+        PYR_HIP(hipMemsetAsync(b, 0, sizeof(double) * (size_t)n * 3, s));
+      for (double *b : {vis_e_std, vis_b_std})
+        PYR_HIP(hipMemsetAsync(b, 0, sizeof(double) * (size_t)n, s));
+      for (double *b : {vis_el, vis_mag})
+        PYR_HIP(hipMemsetAsync(b, 0, sizeof(double) * (size_t)n * vdim, s));
+    }
+    c->nrm.reserve(sizeof(double) * ((size_t)2 * n + 1) + sizeof(unsigned long long));
+    unsigned long long *zeros = c->nrm.as<unsigned long long>();
+    PYR_HIP(hipMemsetAsync(zeros, 0, sizeof(unsigned long long), s));
+    AverageArgs aa{};
+    IntegralArgs &ia = aa.b;
+    ia.K = K;
+    ia.R = radius;
+    ia.invR = 1.0 / radius;
+    ia.alpha = alpha;
+    ia.normalizer = normalizer;
+    for (int a = 0; a < 3; ++a) ia.shift_len[a] = mesh->dev.per_max[a] - mesh->dev.per_min[a];
+    ia.tr_start = c->tr_start.as<uint32_t>();
+    ia.tr = c->tr.as<uint32_t>();
+    ia.tr_shift = c->tr_shift.as<uint8_t>();
+    ia.seg = c->seg.as<uint32_t>();
+    ia.perm = perm;
+    ia.pos = c->pos.as<double4>();
+    ia.geom = c->geom.as<double>();
+    ia.node_tab = c->tab64.as<double>();
+    ia.iw = integral_weights;
+    ia.jac = mesh->dev.jacobian;
+    ia.integral = integral_out;
+    for (int f = 0; f < 6; ++f) aa.f[f] = fields[f];
+    aa.q = q;
+    aa.v = v;
+    aa.vdim = vdim;
+    aa.force = force;
+    aa.vis_e = vis_e;
+    aa.vis_b = vis_b;
+    aa.vis_e_std = vis_e_std;
+    aa.vis_b_std = vis_b_std;
+    aa.vis_el = vis_el;
+    aa.vis_mag = vis_mag;
+    aa.zeros = zeros;
+    {
+      ProfScope prof(PROF_GATHER);
+      launch_shape_average(D, Np, aa, K, alpha == 2.0, vis);
+    }
+    unsigned long long hz = 0;
+    PYR_HIP(hipMemcpyAsync(&hz, zeros, sizeof(hz), hipMemcpyDeviceToHost, s));
+    PYR_HIP(hipStreamSynchronize(s));
+    *n_zero_out = (int64_t)hz;
+  });
+}
+
 int pyr_deposit_backward(pyr_mesh *mesh, double radius, double alpha, double normalizer,
                          int precision, int64_t n, int vdim, const double *x, const double *q,
                          const double *v, const uint32_t *elem, const double *rho_bar,

@@ -865,10 +865,153 @@ __global__ __launch_bounds__(64) void k_shape_integral(IntegralArgs g) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Shape-weighted field averages (push_average.hpp:38-443, the averaging
+// pusher): per particle, over the same nodes the deposition reaches,
+//   I   = sum_e |J_e| sum_i S_i w_i,
+//   <F> = sum_e |J_e| sum_i F_i S_i w_i / I      (F: ex..ez, bx..bz),
+// and the force q <E> + v x (q <B>), truncated to vdim.  The reference forms
+// the same ratio with q S (or the normalized (q / I) S) as the deposit and
+// divides by the reconstructed charge (force_averaging_target
+// ::add_shape_on_element :135-168, el_/mag_force_averaging_target::end_particle
+// :249-265, :309-330); the
+// common factor cancels.  A particle whose I is 0 gets no force, zero
+// averages and zero deviations.
+// ---------------------------------------------------------------------------
+struct AverageArgs {
+  IntegralArgs b;                   // traversal; b.integral (may be null) receives N I
+  const double *f[6];               // ex ey ez bx by bz, K x Np nodal; null = zero
+  const double *q, *v;              // per particle (caller's order), v: n x vdim
+  int vdim;
+  double *force;                    // n x vdim
+  double *vis_e, *vis_b;            // n x 3 averages, or null
+  double *vis_e_std, *vis_b_std;    // n, or null
+  double *vis_el, *vis_mag;         // n x vdim, or null
+  unsigned long long *zeros;        // count of located particles with I = 0
+};
+
+template <int D, int NP, bool A2, bool VIS>
+__global__ __launch_bounds__(64) void k_shape_average(AverageArgs g) {
+  using CD = const double __attribute__((address_space(4))) *;
+  const int s = xcd_element(blockIdx.x);
+  if (s >= g.b.K) return;
+  const uint32_t p0 = g.b.seg[s], p1 = g.b.seg[s + 1];
+  if (p0 >= p1) return;
+  const int lane = threadIdx.x;
+  const uint32_t t_begin = g.b.tr_start[s], t_end = g.b.tr_start[s + 1];
+  const double two_inv_r = 2.0 * g.b.invR;
+  const CD ntab_all = (CD)g.b.node_tab, iw = (CD)g.b.iw, jac = (CD)g.b.jac;
+  for (uint32_t base = p0; base < p1; base += 64) {
+    const uint32_t jj = base + lane;
+    const bool valid = jj < p1;
+    const double4 pp = valid ? g.b.pos[jj] : make_double4(0, 0, 0, 0);
+    double aI = 0, aF[6] = {0, 0, 0, 0, 0, 0}, aS[2] = {0, 0};
+    for (uint32_t ti = t_begin; ti < t_end; ++ti) {
+      const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.b.tr[ti]);
+      const uint32_t code = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.b.tr_shift[ti]);
+      double sh[3] = {0, 0, 0};
+      if (code) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          const uint32_t cc = (code >> (2 * a)) & 3u;
+          sh[a] = cc == 1 ? g.b.shift_len[a] : (cc == 2 ? -g.b.shift_len[a] : 0.0);
+        }
+      }
+      const CD gt = (CD)g.b.geom + (size_t)t * 8;
+      const double d[3] = {pp.x + sh[0] - gt[0], pp.y + sh[1] - gt[1],
+                           D == 3 ? pp.z + sh[2] - gt[2] : 0.0};
+      const double dd = fma(d[0], d[0], fma(d[1], d[1], D == 3 ? d[2] * d[2] : 0.0));
+      const bool pass = valid && dd <= gt[3];
+      if (__ballot(pass) == 0) continue;
+      const double m[3] = {d[0] * two_inv_r, d[1] * two_inv_r, d[2] * two_inv_r};
+      const double c0 = fma(-dd, g.b.invR, g.b.R);
+      const CD nt = ntab_all + (size_t)t * NP * 4;
+      // c_i = S_i w_i for this element's nodes
+      double cw[NP];
+      double eI = 0;
+#pragma unroll
+      for (int i = 0; i < NP; ++i) {
+        double tt = c0 - nt[i * 4 + 3];
+#pragma unroll
+        for (int k = D - 1; k >= 0; --k) tt = fma(nt[i * 4 + k], m[k], tt);
+        const double w = (pass && tt > 0.0) ? tt : 0.0;
+        const double sv = A2 ? w * w : (w > 0.0 ? pow(w, g.b.alpha) : 0.0);
+        cw[i] = sv * iw[i];
+        eI += cw[i];
+      }
+      const double J = jac[t];
+      aI = fma(eI, J, aI);
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        if (!g.f[f]) continue;                      // uniform: a zero component
+        const CD fv = (CD)g.f[f] + (size_t)t * NP;
+        double eF = 0, eS = 0;
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+          eF = fma(fv[i], cw[i], eF);
+          if (VIS) eS = fma(fv[i] * fv[i], cw[i], eS);
+        }
+        aF[f] = fma(eF, J, aF[f]);
+        if (VIS) aS[f / 3] = fma(eS, J, aS[f / 3]);
+      }
+    }
+    const bool zero = valid && aI == 0.0;
+    if (zero) {
+      const unsigned long long zb = __ballot(1);
+      if (lane == __builtin_ffsll((long long)zb) - 1)
+        atomicAdd(g.zeros, (unsigned long long)__popcll(zb));
+    }
+    if (!valid) continue;
+    const uint32_t pi = g.b.perm[jj];
+    const double qv = g.q[pi];
+    if (g.b.integral) g.b.integral[pi] = g.b.normalizer * aI;
+    double avg[6] = {0, 0, 0, 0, 0, 0}, el[3] = {0, 0, 0}, mag[3] = {0, 0, 0};
+    if (!zero) {
+#pragma unroll
+      for (int f = 0; f < 6; ++f) avg[f] = aF[f] / aI;
+      double vv[3] = {0, 0, 0};
+      for (int k = 0; k < g.vdim; ++k) vv[k] = g.v[(size_t)pi * g.vdim + k];
+      double qb[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        el[k] = qv * avg[k];
+        qb[k] = qv * avg[3 + k];
+      }
+      mag[0] = vv[1] * qb[2] - vv[2] * qb[1];
+      mag[1] = vv[2] * qb[0] - vv[0] * qb[2];
+      mag[2] = vv[0] * qb[1] - vv[1] * qb[0];
+    }
+    for (int k = 0; k < g.vdim; ++k) g.force[(size_t)pi * g.vdim + k] = el[k] + mag[k];
+    if (VIS) {
+      for (int k = 0; k < 3; ++k) {
+        g.vis_e[(size_t)pi * 3 + k] = avg[k];
+        g.vis_b[(size_t)pi * 3 + k] = avg[3 + k];
+      }
+      for (int k = 0; k < g.vdim; ++k) {
+        g.vis_el[(size_t)pi * g.vdim + k] = el[k];
+        g.vis_mag[(size_t)pi * g.vdim + k] = mag[k];
+      }
+      // push_average.hpp:190-212: sqrt(mean of |F|^2 - |mean F|^2), 0 if negative
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        double sd = 0;
+        if (!zero) {
+          const double mean_sq = avg[3 * h] * avg[3 * h] + avg[3 * h + 1] * avg[3 * h + 1] +
+                                 avg[3 * h + 2] * avg[3 * h + 2];
+          const double var = aS[h] / aI - mean_sq;
+          sd = var < 0 ? 0.0 : sqrt(var);
+        }
+        (h ? g.vis_b_std : g.vis_e_std)[pi] = sd;
+      }
+    }
+  }
+}
+
 // Launchers, one per translation unit.  precision: PYR_FP64 or PYR_FP32C.
 void launch_gather_fp64(int D, int Np, const GatherArgs &ga, int K, bool a2, bool count);
 void launch_gather_fp32(int D, int Np, const GatherArgs &ga, int K, bool a2, bool count);
 void launch_deposit_adj(int D, int Np, const AdjArgs &ga, int K, int precision, bool a2);
 void launch_shape_integral(int D, int Np, const IntegralArgs &ga, int K, bool a2);
+void launch_shape_average(int D, int Np, const AverageArgs &ga, int K, bool a2, bool vis);
 
 }  // namespace pyr

@@ -129,6 +129,32 @@ class DeviceMesh:
             return F, dict(zip(("pt_e", "pt_b", "el_force", "mag_force"), vis_t))
         return F
 
+    def average_forces(self, radius, alpha, normalizer, fields, x, elem, q, v, integral_weights,
+                       vis=False):
+        """Shape-averaged Lorentz forces (averaging pusher).  Returns
+        (F, integral, n_zero[, vis dict]); integral is the per-particle shape
+        integral N sum_e |J_e| sum_i S_i w_i (0 for particles with no mass)."""
+        _lib.sync_stream()
+        n, vd = v.shape
+        dev = x.device
+        F = torch.empty((n, vd), dtype=torch.float64, device=dev)
+        integral = torch.empty(n, dtype=torch.float64, device=dev)
+        names = ("pt_e", "pt_b", "pt_e_stddev", "pt_b_stddev", "el_force", "mag_force")
+        shapes = ((n, 3), (n, 3), (n,), (n,), (n, vd), (n, vd))
+        vis_t = [torch.empty(sh, dtype=torch.float64, device=dev) for sh in shapes] \
+            if vis else [None] * 6
+        w = torch.as_tensor(np.asarray(integral_weights, dtype=np.float64), device=dev)
+        assert w.shape == (self.Np,), "integral weights need one entry per element node"
+        fp = (C.c_void_p * 6)(*[None if f is None else _lib.ptr(f) for f in fields])
+        nz = C.c_int64(0)
+        _lib.check(self._lib.pyr_average_forces(
+            self._h, radius, alpha, normalizer, n, vd, fp, _lib.ptr(x), _lib.ptr(elem),
+            _lib.ptr(q), _lib.ptr(v), _lib.ptr(w), _lib.ptr(F), _lib.ptr(integral),
+            *[_lib.ptr(t) for t in vis_t], C.byref(nz)))
+        if vis:
+            return F, integral, int(nz.value), dict(zip(names, vis_t))
+        return F, integral, int(nz.value)
+
     def set_static_fields(self, fields):
         _lib.sync_stream()
         fp = (C.c_void_p * 6)(*[None if f is None else _lib.ptr(f) for f in fields])

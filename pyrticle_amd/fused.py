@@ -22,6 +22,10 @@ class FusedParticleStepper:
     static fields (E nodal, B = mu H nodal; None for zero components)."""
 
     def __init__(self, method, e_fields, b_fields, precision="fp64"):
+        from .pusher import MonomialParticlePusher
+        if not isinstance(method.pusher, MonomialParticlePusher):
+            raise ValueError("the fused step interpolates like MonomialParticlePusher; "
+                             "step other pushers through the per-stage RHS")
         self.method = method
         self.precision = dv.FP64 if precision == "fp64" else dv.FP32C
         fields = [None if f is None else f.contiguous() for f in list(e_fields) + list(b_fields)]

@@ -67,3 +67,29 @@ class MonomialParticlePusher(Pusher):
             lmd.lu_vandermonde_t, lmd.lu_piv_vandermonde_t = _internal.lu(vdm.T)
             self.backend.local_discretizations.append(lmd)
             self.backend.ldis_indices.extend([i] * len(eg.members))
+
+
+class AverageParticlePusher(Pusher):
+    """pusher.py:167-207: forces from the fields averaged over each particle's
+    deposited shape (AveragingPusher{2d2v,3d3v}, one shape-weighted gather on
+    the GPU per call).  Uses the method's depositor for the shape function and
+    must run with a shape depositor (plain or normalized)."""
+
+    def initialize(self, method):
+        Pusher.initialize(self, method)
+        eg, = method.mesh_data.discr.element_groups
+        ldis = eg.local_discretization
+        backend_class = _internal.class_for("AveragingPusher",
+                                            method.get_dimensionality_suffix())
+        self.backend = backend_class(method.mesh_data, ldis.mass_matrix())
+
+    def make_state(self, state):
+        return self.backend.PusherState()
+
+    def _forces(self, state, velocities, *field_args):
+        state.pusher_state.e_normalization_stats.reset()
+        state.pusher_state.b_normalization_stats.reset()
+        return self.backend.forces(
+            *field_args, particle_state=state.particle_state, pusher_state=state.pusher_state,
+            depositor=self.method.depositor.backend, depositor_state=state.depositor_state,
+            velocities=velocities, vis_listener=state.vis_listener)

@@ -323,6 +323,84 @@ def test_deposit_normalized_zero_integral(square, oracle_square, dmesh2, cuda):
     assert rel_l2(rho, rho_o) < FP64_TOL
 
 
+def _poly_fields(nodes, dim):
+    x = nodes
+    z = x[:, 2] if dim == 3 else 0.0
+    return [1e3 * (1 + x[:, 0] ** 2), -2e3 * x[:, 1] * x[:, 0], 5e2 * (1 + z),
+            0.3 * x[:, 1], 0.2 + 0.1 * z, 0.5 - 0.4 * x[:, 0] * x[:, 1]]
+
+
+def _average_case(dmesh, om, wl, x, q, v, el, fields, radius, dim, cuda, vis):
+    from pyrticle_amd import device as dv
+    w = wl.discr.element_groups[0].local_discretization.integral_weights()
+    ref = O.average_forces(om, radius, wl.alpha, x, q, v, el, w, fields, vis=vis)
+    nrm = O.shape_normalizer(radius, dim, wl.alpha)
+    dfields = [None if f is None else _dev(f, cuda) for f in fields]
+    out = dmesh.average_forces(radius, wl.alpha, nrm, dfields, _dev(x, cuda),
+                               dv.element_tensor(el, cuda), _dev(q, cuda), _dev(v, cuda), w,
+                               vis=vis)
+    f, integ = out[0].cpu().numpy(), out[1].cpu().numpy()
+    assert out[2] == ref[2]
+    assert np.array_equal(integ == 0, ref[1] == 0)
+    assert rel_l2(q * integ, ref[1]) < FP64_TOL
+    assert rel_l2(f, ref[0]) < FP64_TOL
+    return out, ref
+
+
+def test_average_forces_2d_matches_oracle(square, oracle_square, dmesh2, cuda):
+    """Averaging pusher, TE fields (ex, ey, bz), polynomial in x, with particles
+    whose shapes the boundary cuts: forces, averages, deviations."""
+    from pyrticle_amd import device as dv
+    wl = square
+    rng = np.random.default_rng(4)
+    x = np.concatenate([wl.x, rng.uniform(-0.98, 0.98, size=(3000, 2))])
+    el = dv.element_numpy(dmesh2.find_containing_element(_dev(x, cuda)))
+    n = x.shape[0]
+    q = np.full(n, wl.q[0])
+    v = O.velocities(np.resize(wl.p, (n, 2)), np.resize(wl.m, n), wl.c)
+    fields = _poly_fields(wl.ma.nodes, 2)
+    fields[2] = fields[3] = fields[4] = None
+    out, ref = _average_case(dmesh2, oracle_square, wl, x, q, v, el, fields, wl.radius, 2, cuda,
+                             vis=True)
+    vis, vref = out[3], ref[3]
+    for name in ("pt_e", "pt_b", "el_force", "mag_force"):
+        assert rel_l2(vis[name].cpu().numpy(), vref[name]) < FP64_TOL, name
+    # deviations: sqrt of a difference of nearly equal means, so an absolute
+    # bound relative to the field scale
+    for name, ref_scale in (("pt_e_stddev", vref["pt_e"]), ("pt_b_stddev", vref["pt_b"])):
+        err = np.abs(vis[name].cpu().numpy() - vref[name]).max()
+        assert err < 1e-6 * np.abs(ref_scale).max(), name
+
+
+def test_average_forces_3d_matches_oracle(small_cylinder, dmesh3, cuda):
+    from pyrticle_amd import device as dv
+    wl = small_cylinder
+    om = O.OracleMesh(wl.ma)
+    n = 4000
+    x, q = wl.x[:n], wl.q[:n]
+    el = dv.element_numpy(dmesh3.find_containing_element(_dev(x, cuda)))
+    v = O.velocities(wl.p[:n], wl.m[:n], wl.c)
+    _average_case(dmesh3, om, wl, x, q, v, el, _poly_fields(wl.ma.nodes, 3), wl.radius, 3, cuda,
+                  vis=False)
+
+
+def test_average_forces_zero_charge(square, oracle_square, dmesh2, cuda):
+    from pyrticle_amd import device as dv
+    wl = square
+    rng = np.random.default_rng(2)
+    x = np.concatenate([rng.uniform(-0.9, 0.9, size=(400, 2)),
+                        wl.ma.nodes[[17, 403, 999]] + 1e-5])
+    el = dv.element_numpy(dmesh2.find_containing_element(_dev(x, cuda)))
+    q = np.full(x.shape[0], wl.q[0])
+    v = np.full_like(x, 1e6)
+    out, ref = _average_case(dmesh2, oracle_square, wl, x, q, v, el, wl.fields, 1e-3, 2, cuda,
+                             vis=True)
+    assert 300 < out[2] < x.shape[0] - 3
+    dead = (ref[1] == 0)
+    for name, arr in out[3].items():
+        assert not arr.cpu().numpy()[dead].any(), name
+
+
 @pytest.mark.parametrize("dt_scale", [1.0, 4.0])
 def test_lsrk4_square_matches_oracle(square, oracle_square, dmesh2, cuda, dt_scale):
     """Square workload, 10k electrons, 100 LSRK4 steps with relocalization; with a 4x

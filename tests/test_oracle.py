@@ -367,6 +367,125 @@ def test_normalized_deposit_linearity_and_current(square, oracle_square):
 
 
 # ---------------------------------------------------------------------------
+# averaging pusher (push_average.hpp)
+# ---------------------------------------------------------------------------
+def _brute_average(ma, w, x, fields, R, alpha, dim):
+    """Shape-weighted averages over ALL nodes: sum |J| w S F / sum |J| w S, and
+    the per-particle deviations sqrt(<|F|^2> - |<F>|^2) of E and of B."""
+    N = O.shape_normalizer(R, dim, alpha)
+    r2 = ((ma.nodes[:, None, :] - x[None, :, :]) ** 2).sum(-1)
+    t = np.where(r2 <= R * R, R - r2 / R, 0.0)
+    wS = (ma.jacobian[:, None] * w[None, :]).ravel()[:, None] * (N * t ** alpha)
+    integ = wS.sum(0)
+    F = np.stack([np.zeros(ma.nodes.shape[0]) if f is None else f for f in fields])
+    avg = (F @ wS) / integ
+    sq = np.stack([(F[3 * h:3 * h + 3] ** 2).sum(0) @ wS for h in range(2)]) / integ
+    std = np.sqrt(np.maximum(sq - np.stack([(avg[3 * h:3 * h + 3] ** 2).sum(0)
+                                            for h in range(2)]), 0))
+    return avg.T, std.T, integ
+
+
+def _lorentz(q, e, b, v):
+    vv = np.zeros((v.shape[0], 3))
+    vv[:, :v.shape[1]] = v
+    return (q[:, None] * (e + np.cross(vv, b)))[:, :v.shape[1]]
+
+
+def _poly_fields(nodes, dim):
+    x = nodes
+    z = x[:, 2] if dim == 3 else 0.0
+    return [1e3 * (1 + x[:, 0] ** 2), -2e3 * x[:, 1] * x[:, 0], 5e2 * (1 + z),
+            0.3 * x[:, 1], 0.2 + 0.1 * z, 0.5 - 0.4 * x[:, 0] * x[:, 1]]
+
+
+def test_average_forces_uniform_fields_closed_form(square, oracle_square):
+    """The average of a constant field is the field: F = q (E + v x B), equal to
+    the interpolating pusher's force, with zero deviations."""
+    wl = square
+    w = wl.discr.element_groups[0].local_discretization.integral_weights()
+    n = 500
+    x, q = wl.x[:n], wl.q[:n]
+    el = _located(oracle_square, x)
+    v = O.velocities(wl.p[:n], wl.m[:n], wl.c)
+    f, charge, nz, vis = O.average_forces(oracle_square, wl.radius, wl.alpha, x, q, v, el, w,
+                                          wl.fields, vis=True)
+    e = np.tile([0.0, 1e3, 0.0], (n, 1))
+    b = np.tile([0.0, 0.0, 0.01], (n, 1))
+    assert nz == 0
+    assert np.abs(f - _lorentz(q, e, b, v)).max() < 1e-13 * np.abs(f).max()
+    assert np.abs(vis["pt_e"] - e).max() < 1e-12 and np.abs(vis["pt_b"] - b).max() < 1e-16
+    assert vis["pt_e_stddev"].max() < 1e-7 * 1e3 and vis["pt_b_stddev"].max() < 1e-7 * 0.01
+    assert rel_l2(vis["el_force"] + vis["mag_force"], f) < 1e-15
+    fm = O.forces(oracle_square, x, el, q, v, wl.fields)
+    assert np.abs(f - fm).max() < 1e-12 * np.abs(fm).max()
+
+
+@pytest.mark.parametrize("normalized", [False, True])
+def test_average_forces_equal_brute_force_2d(square, oracle_square, normalized):
+    """Polynomial fields, particles reaching the boundary: averages, deviations
+    and forces against the all-node sums; the plain and the normalized deposit
+    give the same force."""
+    wl = square
+    w = wl.discr.element_groups[0].local_discretization.integral_weights()
+    rng = np.random.default_rng(4)
+    x = np.concatenate([wl.x[:400], rng.uniform(-0.98, 0.98, size=(400, 2))])
+    n = x.shape[0]
+    q = np.full(n, wl.q[0])
+    v = O.velocities(np.resize(wl.p, (n, 2)), np.resize(wl.m, n), wl.c)
+    el = _located(oracle_square, x)
+    fields = _poly_fields(wl.ma.nodes, 2)
+    fields[2] = fields[3] = fields[4] = None          # the TE binding
+    f, charge, nz, vis = O.average_forces(oracle_square, wl.radius, wl.alpha, x, q, v, el, w,
+                                          fields, normalized=normalized, vis=True)
+    avg, std, integ = _brute_average(wl.ma, w, x, fields, wl.radius, wl.alpha, 2)
+    assert nz == 0
+    assert np.abs(vis["pt_e"] - avg[:, :3]).max() < 1e-12 * np.abs(avg[:, :3]).max()
+    assert np.abs(vis["pt_b"] - avg[:, 3:]).max() < 1e-12 * np.abs(avg[:, 3:]).max()
+    assert np.abs(vis["pt_e_stddev"] - std[:, 0]).max() < 1e-7 * np.abs(avg[:, :3]).max()
+    assert np.abs(vis["pt_b_stddev"] - std[:, 1]).max() < 1e-7 * np.abs(avg[:, 3:]).max()
+    fb = _lorentz(q, avg[:, :3], avg[:, 3:], v)
+    assert rel_l2(f, fb) < 1e-13
+    expect_charge = q if normalized else q * integ
+    assert np.abs(charge / expect_charge - 1).max() < 1e-12
+    other = O.average_forces(oracle_square, wl.radius, wl.alpha, x, q, v, el, w, fields,
+                             normalized=not normalized)[0]
+    assert rel_l2(other, f) < 1e-14
+
+
+def test_average_forces_equal_brute_force_3d(small_cylinder):
+    wl = small_cylinder
+    om = O.OracleMesh(wl.ma)
+    w = wl.discr.element_groups[0].local_discretization.integral_weights()
+    rng = np.random.default_rng(8)
+    idx = rng.choice(wl.x.shape[0], 200, replace=False)
+    x, q = wl.x[idx], wl.q[idx]
+    el = _located(om, x)
+    v = O.velocities(wl.p[idx], wl.m[idx], wl.c)
+    fields = _poly_fields(wl.ma.nodes, 3)
+    f, charge, nz = O.average_forces(om, wl.radius, 2.0, x, q, v, el, w, fields)
+    avg, _, integ = _brute_average(wl.ma, w, x, fields, wl.radius, 2.0, 3)
+    assert nz == 0
+    assert rel_l2(f, _lorentz(q, avg[:, :3], avg[:, 3:], v)) < 1e-13
+
+
+def test_average_forces_zero_charge(square, oracle_square):
+    """Supports that reach no node: no force, counted, with either depositor."""
+    wl = square
+    w = wl.discr.element_groups[0].local_discretization.integral_weights()
+    rng = np.random.default_rng(2)
+    x = np.concatenate([rng.uniform(-0.9, 0.9, size=(100, 2)), wl.ma.nodes[[17, 403]] + 1e-5])
+    el = _located(oracle_square, x)
+    q = np.full(x.shape[0], -U.EL_CHARGE)
+    v = np.full_like(x, 1e6)
+    for normalized in (False, True):
+        f, charge, nz = O.average_forces(oracle_square, 1e-3, 2.0, x, q, v, el, w, wl.fields,
+                                         normalized=normalized)
+        dead = charge == 0
+        assert nz == dead.sum() and 60 < nz < x.shape[0] - 2
+        assert not f[dead].any() and np.abs(f[~dead]).min() > 0
+
+
+# ---------------------------------------------------------------------------
 # interpolation, LU, forces
 # ---------------------------------------------------------------------------
 def test_getrf_getrs_solve():
