@@ -261,6 +261,40 @@ int pyr_average_forces(pyr_mesh *mesh, double radius, double alpha, double norma
                        double *vis_e_stddev, double *vis_b_stddev, double *vis_el_force,
                        double *vis_mag_force, int64_t *n_zero_out);
 
+/* Particle-state diagnostics (diagnostics.hpp:37-183, wrap_pic.cpp:38-47).
+ * Device inputs (positions n x xdim, momenta n x vdim, charges, masses,
+ * velocities n x vdim); scalar and vector results go to host memory.  Sums are
+ * formed on the device in a fixed order (deterministic; the reference sums
+ * sequentially).  An empty state gives 0 (the reference's early returns). */
+
+/* diagnostics.hpp:37-57 kinetic_energies: out (device, n) = (|p| / v - m) c^2,
+ * v = c |p| / sqrt(m^2 c^2 + |p|^2). */
+int pyr_kinetic_energies(int64_t n, int vdim, double vacuum_c, const double *momenta,
+                         const double *masses, double *out);
+
+/* diagnostics.hpp:61-70 total_charge: *out = sum q. */
+int pyr_total_charge(int64_t n, const double *charges, double *out);
+
+/* diagnostics.hpp:73-87 particle_momentum: out[vdim] = sum p. */
+int pyr_particle_momentum(int64_t n, int vdim, const double *momenta, double *out);
+
+/* diagnostics.hpp:171-183 particle_current: out[vdim] = sum q v / length. */
+int pyr_particle_current(int64_t n, int vdim, const double *charges, const double *velocities,
+                         double length, double *out);
+
+/* diagnostics.hpp:90-101 rms_beam_size: *out = sqrt(mean x_axis^2). */
+int pyr_rms_beam_size(int64_t n, int xdim, const double *positions, int axis, double *out);
+
+/* diagnostics.hpp:106-155 rms_beam_emittance along axis with x' = p_axis /
+ * p_beam_axis (0 where p_beam_axis = 0), by the reference's moment expansion. */
+int pyr_rms_beam_emittance(int64_t n, int xdim, int vdim, const double *positions,
+                           const double *momenta, int axis, int beam_axis, double *out);
+
+/* diagnostics.hpp:159-167 rms_energy_spread with tools.hpp:303-322 std_dev:
+ * *out = sqrt(<E^2> - <E>^2) over the kinetic energies. */
+int pyr_rms_energy_spread(int64_t n, int vdim, double vacuum_c, const double *momenta,
+                          const double *masses, double *out);
+
 /* Adjoint of pyr_deposit (no reference counterpart): given rho_bar = dL/drho
  * (K*Np, may be NULL) and j_bar = dL/dj (K*Np x vdim, may be NULL), writes
  * x_bar = dL/dx (n x vdim) and, with j_bar, v_bar = dL/dv (n x vdim); both are

@@ -649,6 +649,83 @@ int orc_average_forces(const orc_mesh *m, double radius, double alpha, int norma
 }
 
 /* ---------------------------------------------------------------------- */
+/* diagnostics.hpp:37-183 (sequential sums in particle order, as there).    */
+static double kinetic_energy(const double *p, int vd, double m, double c) {
+  double s = 0;
+  for (int k = 0; k < vd; ++k) s += p[k] * p[k];
+  const double pn = sqrt(s);
+  const double v = c * pn / sqrt(m * m * c * c + pn * pn);
+  return (pn / v - m) * (c * c);
+}
+
+void orc_kinetic_energies(int64_t n, int vd, double c, const double *p, const double *m,
+                          double *out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = kinetic_energy(p + i * vd, vd, m[i], c);
+}
+
+double orc_total_charge(int64_t n, const double *q) {
+  double r = 0;
+  for (int64_t i = 0; i < n; ++i) r += q[i];
+  return r;
+}
+
+void orc_particle_momentum(int64_t n, int vd, const double *p, double *out) {
+  for (int k = 0; k < vd; ++k) out[k] = 0;
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < vd; ++k) out[k] += p[i * vd + k];
+}
+
+void orc_particle_current(int64_t n, int vd, const double *q, const double *v, double length,
+                          double *out) {
+  for (int k = 0; k < vd; ++k) out[k] = 0;
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < vd; ++k) out[k] += q[i] * v[i * vd + k];
+  for (int k = 0; k < vd; ++k) out[k] /= length;
+}
+
+double orc_rms_beam_size(int64_t n, int xd, const double *x, int axis) {
+  if (n == 0) return 0;
+  double r = 0;
+  for (int64_t i = 0; i < n; ++i) r += x[i * xd + axis] * x[i * xd + axis];
+  return sqrt(r / n);
+}
+
+double orc_rms_beam_emittance(int64_t n, int xd, int vd, const double *x, const double *p,
+                              int axis, int beam_axis) {
+  if (n == 0) return 0;
+  double mx = 0, mxp = 0, mx2 = 0, mxp2 = 0, mxxp = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const double xi = x[i * xd + axis];
+    mx += xi;
+    mx2 += xi * xi;
+    const double px = p[i * vd + axis], pz = p[i * vd + beam_axis];
+    const double xp = pz != 0 ? px / pz : 0;
+    mxp += xp;
+    mxp2 += xp * xp;
+    mxxp += xi * xp;
+  }
+  mx /= n;
+  mxp /= n;
+  mx2 /= n;
+  mxp2 /= n;
+  mxxp /= n;
+  return sqrt(mx2 * mxp2 - mx * mx * mxp2 - mx2 * (mxp * mxp) - (mxxp * mxxp - 2 * mxxp * mx * mxp));
+}
+
+/* with tools.hpp:303-322 std_dev */
+double orc_rms_energy_spread(int64_t n, int vd, double c, const double *p, const double *m) {
+  if (n == 0) return 0;
+  double sum = 0, sq = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const double e = kinetic_energy(p + i * vd, vd, m[i], c);
+    sum += e;
+    sq += e * e;
+  }
+  const double mean = sum / n;
+  return sqrt(sq / n - mean * mean);
+}
+
+/* ---------------------------------------------------------------------- */
 /* LAPACK dgetf2 (netlib): LU with partial pivoting of a column-major n x n
  * matrix, as wrap_tools.cpp:126-145 (lu -> getrf) uses it.  ipiv 0-based.  */
 int orc_getrf(int n, double *a, int *ipiv) {

@@ -54,6 +54,9 @@ def lib():
         _lib.orc_find_containing_element.restype = C.c_uint32
         _lib.orc_find_new_containing_element.restype = C.c_uint32
         _lib.orc_last_error.restype = C.c_char_p
+        for name in ("orc_total_charge", "orc_rms_beam_size", "orc_rms_beam_emittance",
+                     "orc_rms_energy_spread"):
+            getattr(_lib, name).restype = C.c_double
     return _lib
 
 
@@ -291,3 +294,55 @@ def lsrk4_run(om, fields, c_light, x, p, q, m, elem, dt, nsteps, ids=None):
     if rc:
         raise RuntimeError(lib().orc_last_error().decode() or "lsrk4 failed")
     return int(n.value), cnt.as_tuple()
+
+
+# ---------------------------------------------------------------------------
+# diagnostics (diagnostics.hpp:37-183)
+# ---------------------------------------------------------------------------
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def kinetic_energies(p, m, c):
+    p, m = _f64(p), _f64(m)
+    out = np.empty(p.shape[0])
+    lib().orc_kinetic_energies(C.c_int64(p.shape[0]), p.shape[1], C.c_double(c), _ptr(p),
+                               _ptr(m), _ptr(out))
+    return out
+
+
+def total_charge(q):
+    q = _f64(q)
+    return lib().orc_total_charge(C.c_int64(q.shape[0]), _ptr(q))
+
+
+def particle_momentum(p):
+    p = _f64(p)
+    out = np.empty(p.shape[1])
+    lib().orc_particle_momentum(C.c_int64(p.shape[0]), p.shape[1], _ptr(p), _ptr(out))
+    return out
+
+
+def particle_current(q, v, length):
+    q, v = _f64(q), _f64(v)
+    out = np.empty(v.shape[1])
+    lib().orc_particle_current(C.c_int64(v.shape[0]), v.shape[1], _ptr(q), _ptr(v),
+                               C.c_double(length), _ptr(out))
+    return out
+
+
+def rms_beam_size(x, axis):
+    x = _f64(x)
+    return lib().orc_rms_beam_size(C.c_int64(x.shape[0]), x.shape[1], _ptr(x), axis)
+
+
+def rms_beam_emittance(x, p, axis, beam_axis):
+    x, p = _f64(x), _f64(p)
+    return lib().orc_rms_beam_emittance(C.c_int64(x.shape[0]), x.shape[1], p.shape[1], _ptr(x),
+                                        _ptr(p), axis, beam_axis)
+
+
+def rms_energy_spread(p, m, c):
+    p, m = _f64(p), _f64(m)
+    return lib().orc_rms_energy_spread(C.c_int64(p.shape[0]), p.shape[1], C.c_double(c),
+                                       _ptr(p), _ptr(m))

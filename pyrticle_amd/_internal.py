@@ -594,6 +594,46 @@ class AveragingPusher3d3v(_AveragingPusher):
     pass
 
 
+# ---------------------------------------------------------------------------
+# diagnostics (diagnostics.hpp:37-183, wrap_pic.cpp:38-47)
+# ---------------------------------------------------------------------------
+def _live(ps):
+    n = ps.particle_count
+    return n, ps.positions[:n], ps.momenta[:n]
+
+
+def kinetic_energies(ps, vacuum_c):
+    n, _, p = _live(ps)
+    return dv.kinetic_energies(p, ps.masses[:n], vacuum_c)
+
+
+def total_charge(ps):
+    return dv.total_charge(ps.charges[:ps.particle_count])
+
+
+def particle_momentum(ps):
+    return dv.particle_momentum(_live(ps)[2])
+
+
+def particle_current(ps, velocities, length):
+    n = ps.particle_count
+    return dv.particle_current(ps.charges[:n], velocities[:n].contiguous(), length)
+
+
+def rms_beam_size(ps, axis):
+    return dv.rms_beam_size(_live(ps)[1], axis)
+
+
+def rms_beam_emittance(ps, axis, beam_axis):
+    _, x, p = _live(ps)
+    return dv.rms_beam_emittance(x, p, axis, beam_axis)
+
+
+def rms_energy_spread(ps, vacuum_c):
+    n, _, p = _live(ps)
+    return dv.rms_energy_spread(p, ps.masses[:n], vacuum_c)
+
+
 _SUFFIXES = {"2d2v": 2, "3d3v": 3}
 
 

@@ -63,6 +63,13 @@ _SIGNATURES = {
                                          _P, _P, _P, _P, _P, C.POINTER(_I64)]),
     "pyr_average_forces": (C.c_int, [_P, _D, _D, _D, _I64, C.c_int, _P, _P, _P, _P, _P, _P,
                                      _P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(_I64)]),
+    "pyr_kinetic_energies": (C.c_int, [_I64, C.c_int, _D, _P, _P, _P]),
+    "pyr_total_charge": (C.c_int, [_I64, _P, _P]),
+    "pyr_particle_momentum": (C.c_int, [_I64, C.c_int, _P, _P]),
+    "pyr_particle_current": (C.c_int, [_I64, C.c_int, _P, _P, _D, _P]),
+    "pyr_rms_beam_size": (C.c_int, [_I64, C.c_int, _P, C.c_int, _P]),
+    "pyr_rms_beam_emittance": (C.c_int, [_I64, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int, _P]),
+    "pyr_rms_energy_spread": (C.c_int, [_I64, C.c_int, _D, _P, _P, _P]),
     "pyr_deposit_set_counting": (C.c_int, [C.c_int]),
     "pyr_push_set_mfma": (C.c_int, [C.c_int]),
     "pyr_push_mfma_waves": (C.c_int, [C.POINTER(_I64)]),

@@ -160,6 +160,19 @@ class FieldToParticleRhsCalculator:
             0])
 
 
+class ParticleToFieldRhsCalculator:
+    """cloud.py:225-234: the particles' contribution to the field RHS,
+    dE/dt = -j / epsilon, with j deposited by the method's depositor."""
+
+    def __init__(self, method, maxwell_op):
+        self.method = method
+        self.maxwell_op = maxwell_op
+
+    def __call__(self, t, fields_f, state_f):
+        j = self.method.deposit_j(state_f())
+        return self.maxwell_op.assemble_fields(e=-1 / self.maxwell_op.epsilon * j)
+
+
 class ParticleRhsCalculator:
     """cloud.py:293-311: dx/dt = v."""
 

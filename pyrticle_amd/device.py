@@ -286,6 +286,70 @@ def velocities(p: torch.Tensor, m: torch.Tensor, c: float) -> torch.Tensor:
     return v
 
 
+# particle-state diagnostics (device inputs; sums formed on the device)
+def kinetic_energies(p: torch.Tensor, m: torch.Tensor, c: float) -> torch.Tensor:
+    _lib.sync_stream()
+    n, vd = p.shape
+    out = torch.empty(n, dtype=torch.float64, device=p.device)
+    _lib.check(_lib.lib().pyr_kinetic_energies(n, vd, c, _lib.ptr(p), _lib.ptr(m),
+                                               _lib.ptr(out)))
+    return out
+
+
+def _host_out(k):
+    return np.zeros(k, dtype=np.float64)
+
+
+def total_charge(q: torch.Tensor) -> float:
+    _lib.sync_stream()
+    out = _host_out(1)
+    _lib.check(_lib.lib().pyr_total_charge(q.shape[0], _lib.ptr(q), out.ctypes.data))
+    return float(out[0])
+
+
+def particle_momentum(p: torch.Tensor) -> np.ndarray:
+    _lib.sync_stream()
+    n, vd = p.shape
+    out = _host_out(vd)
+    _lib.check(_lib.lib().pyr_particle_momentum(n, vd, _lib.ptr(p), out.ctypes.data))
+    return out
+
+
+def particle_current(q: torch.Tensor, v: torch.Tensor, length: float) -> np.ndarray:
+    _lib.sync_stream()
+    n, vd = v.shape
+    out = _host_out(vd)
+    _lib.check(_lib.lib().pyr_particle_current(n, vd, _lib.ptr(q), _lib.ptr(v), length,
+                                               out.ctypes.data))
+    return out
+
+
+def rms_beam_size(x: torch.Tensor, axis: int) -> float:
+    _lib.sync_stream()
+    n, xd = x.shape
+    out = _host_out(1)
+    _lib.check(_lib.lib().pyr_rms_beam_size(n, xd, _lib.ptr(x), axis, out.ctypes.data))
+    return float(out[0])
+
+
+def rms_beam_emittance(x: torch.Tensor, p: torch.Tensor, axis: int, beam_axis: int) -> float:
+    _lib.sync_stream()
+    n, xd = x.shape
+    out = _host_out(1)
+    _lib.check(_lib.lib().pyr_rms_beam_emittance(n, xd, p.shape[1], _lib.ptr(x), _lib.ptr(p),
+                                                 axis, beam_axis, out.ctypes.data))
+    return float(out[0])
+
+
+def rms_energy_spread(p: torch.Tensor, m: torch.Tensor, c: float) -> float:
+    _lib.sync_stream()
+    n, vd = p.shape
+    out = _host_out(1)
+    _lib.check(_lib.lib().pyr_rms_energy_spread(n, vd, c, _lib.ptr(p), _lib.ptr(m),
+                                                out.ctypes.data))
+    return float(out[0])
+
+
 def velocities_backward(p: torch.Tensor, m: torch.Tensor, c: float,
                         v_bar: torch.Tensor) -> torch.Tensor:
     """p_bar = (dv/dp)^T v_bar."""

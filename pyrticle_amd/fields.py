@@ -42,6 +42,12 @@ class StaticFieldOperator:
         return out
 
     def assemble_fields(self, e=None, h=None):
+        """Like assemble_eh, with an absent part filled with zeros (scalar 0
+        entries, which add to nodal vectors as zero fields)."""
+        ne = self.count_subset(self.eh_subset[:3])
+        nh = self.count_subset(self.eh_subset[3:])
+        e = [0] * ne if e is None else list(e)
+        h = [0] * nh if h is None else list(h)
         return self.assemble_eh(e, h)
 
     def split_eh(self, fields):

@@ -389,3 +389,42 @@ def test_average_pusher_per_stage_lsrk4(square, oracle_square, cuda):
     assert np.array_equal(element_numpy(st.containing_elements), oel)
     assert rel_l2(st.positions.cpu().numpy(), ox) < 1e-12
     assert rel_l2(st.momenta.cpu().numpy(), op_) < 1e-11
+
+
+def test_diagnostics_and_field_rhs_dropin(square, oracle_square, cuda):
+    """_internal's diagnostics on a PicState (wrap_pic.cpp:38-47) against the
+    oracle, and ParticleToFieldRhsCalculator (cloud.py:225-234): -j / epsilon
+    into the E components, zeros into H."""
+    from pyrticle_amd import _internal
+    from pyrticle_amd.cloud import ParticleToFieldRhsCalculator
+    from pyrticle_amd.fields import StaticFieldOperator
+    wl = square
+    method = _method(wl.discr, 2)
+    state = _load(method, wl)
+    ps = state.particle_state
+    n = len(state)
+    p, m, q = wl.p[:n], wl.m[:n], wl.q[:n]
+    x = wl.x[:n]
+    e = _internal.kinetic_energies(ps, wl.c).cpu().numpy()
+    assert np.abs(e / O.kinetic_energies(p, m, wl.c) - 1).max() < 1e-13
+    assert _internal.total_charge(ps) == pytest.approx(O.total_charge(q), rel=1e-12)
+    assert np.allclose(_internal.particle_momentum(ps), O.particle_momentum(p), rtol=1e-12)
+    vel = method.velocities(state)
+    assert np.allclose(_internal.particle_current(ps, vel, 2.0),
+                       O.particle_current(q, vel.cpu().numpy(), 2.0), rtol=1e-12)
+    assert _internal.rms_beam_size(ps, 1) == pytest.approx(O.rms_beam_size(x, 1), rel=1e-12)
+    assert _internal.rms_beam_emittance(ps, 1, 0) == pytest.approx(
+        O.rms_beam_emittance(x, p, 1, 0), rel=1e-9)
+    assert _internal.rms_energy_spread(ps, wl.c) == pytest.approx(
+        O.rms_energy_spread(p, m, wl.c), rel=1e-9)
+
+    method.depositor.set_shape_function(
+        state, method.get_shape_function_class()(wl.radius, 2, wl.alpha))
+    op = StaticFieldOperator(eh_subset=(True, True, False, False, False, True))
+    rhs = ParticleToFieldRhsCalculator(method, op)(0.0, lambda: None, lambda: state)
+    assert len(rhs) == 3 and isinstance(rhs[2], int) and rhs[2] == 0
+    from pyrticle_amd.device import element_numpy
+    el = element_numpy(state.containing_elements)
+    _, j_o, _ = O.deposit(oracle_square, wl.radius, wl.alpha, x, q, vel.cpu().numpy(), el)
+    for k in range(2):
+        assert rel_l2(rhs[k].cpu().numpy(), -j_o[:, k] / op.epsilon) < 1e-12

@@ -401,6 +401,35 @@ def test_average_forces_zero_charge(square, oracle_square, dmesh2, cuda):
         assert not arr.cpu().numpy()[dead].any(), name
 
 
+def test_diagnostics_match_oracle(cuda):
+    """The seven diagnostics on 1M particles against the oracle's sequential
+    sums (summation order differs: sums within 1e-12, the cancelling emittance
+    and one-pass energy spread within their conditioning)."""
+    from pyrticle_amd import device as dv
+    from tests.test_oracle import _diag_beam
+    x, p, m, q, c = _diag_beam(n=1_000_000, seed=9)
+    v = O.velocities(p, m, c)
+    X, P, M, Q, V = (_dev(a, cuda) for a in (x, p, m, q, v))
+    e = dv.kinetic_energies(P, M, c).cpu().numpy()
+    e_o = O.kinetic_energies(p, m, c)
+    assert np.abs(e / e_o - 1).max() < 1e-13
+    assert dv.total_charge(Q) == pytest.approx(O.total_charge(q), rel=1e-12)
+    assert np.allclose(dv.particle_momentum(P), O.particle_momentum(p), rtol=1e-12, atol=0)
+    assert np.allclose(dv.particle_current(Q, V, 4.0), O.particle_current(q, v, 4.0),
+                       rtol=1e-12, atol=0)
+    for axis in range(3):
+        assert dv.rms_beam_size(X, axis) == pytest.approx(O.rms_beam_size(x, axis), rel=1e-12)
+    for axis in (0, 1):
+        assert dv.rms_beam_emittance(X, P, axis, 2) == pytest.approx(
+            O.rms_beam_emittance(x, p, axis, 2), rel=1e-9)
+    cond = (e_o.mean() / e_o.std()) ** 2
+    assert dv.rms_energy_spread(P, M, c) == pytest.approx(O.rms_energy_spread(p, m, c),
+                                                          rel=1e-14 * cond)
+    # deterministic, and an empty state gives zeros
+    assert dv.rms_energy_spread(P, M, c) == dv.rms_energy_spread(P, M, c)
+    assert dv.total_charge(Q[:0]) == 0 and dv.rms_beam_size(X[:0], 0) == 0
+
+
 @pytest.mark.parametrize("dt_scale", [1.0, 4.0])
 def test_lsrk4_square_matches_oracle(square, oracle_square, dmesh2, cuda, dt_scale):
     """Square workload, 10k electrons, 100 LSRK4 steps with relocalization; with a 4x

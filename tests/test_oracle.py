@@ -486,6 +486,63 @@ def test_average_forces_zero_charge(square, oracle_square):
 
 
 # ---------------------------------------------------------------------------
+# diagnostics (diagnostics.hpp)
+# ---------------------------------------------------------------------------
+def _diag_beam(n=20000, seed=6):
+    rng = np.random.default_rng(seed)
+    c = U.VACUUM_LIGHT_SPEED()
+    x = np.stack([0.1 + 0.3 * rng.normal(size=n), -0.05 + 0.2 * rng.normal(size=n),
+                  rng.uniform(0, 4, size=n)], axis=1)
+    p0 = 10 * U.EL_MASS * c * math.sqrt(1 - 1e-2)
+    p = np.stack([p0 * (0.01 * rng.normal(size=n) + 0.3 * x[:, 0]),
+                  p0 * 0.01 * rng.normal(size=n), p0 * (1 + 1e-3 * rng.normal(size=n))], axis=1)
+    m = np.full(n, U.EL_MASS)
+    q = np.full(n, -U.EL_CHARGE)
+    return x, p, m, q, c
+
+
+def test_diagnostics_closed_forms():
+    """Kinetic energy (gamma - 1) m c^2, exact sums (math.fsum), the emittance
+    as sqrt(det) of the centred (x, x') covariance, the energy spread as a
+    two-pass standard deviation."""
+    x, p, m, q, c = _diag_beam()
+    n = x.shape[0]
+    e = O.kinetic_energies(p, m, c)
+    p2c2 = (p ** 2).sum(1) * c * c
+    mc2 = m * c * c
+    e_ref = p2c2 / (np.sqrt(mc2 ** 2 + p2c2) + mc2)
+    assert np.abs(e / e_ref - 1).max() < 1e-12
+    assert O.total_charge(q) == pytest.approx(math.fsum(q), rel=1e-14)
+    mom = O.particle_momentum(p)
+    assert all(mom[k] == pytest.approx(math.fsum(p[:, k]), rel=1e-12) for k in range(3))
+    v = O.velocities(p, m, c)
+    cur = O.particle_current(q, v, 4.0)
+    assert all(cur[k] == pytest.approx(math.fsum(q * v[:, k]) / 4.0, rel=1e-12)
+               for k in range(3))
+    for axis in range(3):
+        assert O.rms_beam_size(x, axis) == pytest.approx(
+            math.sqrt(math.fsum(x[:, axis] ** 2) / n), rel=1e-13)
+    xp = p[:, 0] / p[:, 2]
+    cov = np.cov(np.stack([x[:, 0], xp]), bias=True)
+    eps = O.rms_beam_emittance(x, p, 0, 2)
+    assert eps == pytest.approx(math.sqrt(np.linalg.det(cov)), rel=1e-9)
+    # the one-pass variance cancels about log10(<E>^2 / var) digits
+    spread = O.rms_energy_spread(p, m, c)
+    assert spread == pytest.approx(np.std(e_ref), rel=1e-16 * (e_ref.mean() / np.std(e_ref)) ** 2
+                                   * 100)
+
+
+def test_diagnostics_edge_cases():
+    x = np.zeros((0, 3))
+    assert O.rms_beam_size(x, 0) == 0 and O.rms_beam_emittance(x, x, 0, 2) == 0
+    assert O.rms_energy_spread(x, np.zeros(0), 3e8) == 0 and O.total_charge(np.zeros(0)) == 0
+    # x' = 0 where the beam-axis momentum vanishes (diagnostics.hpp:129)
+    x = np.array([[1.0, 0, 0], [-1.0, 0, 0]])
+    p = np.array([[5.0, 0, 0], [-5.0, 0, 0]])
+    assert O.rms_beam_emittance(x, p, 0, 2) == 0
+
+
+# ---------------------------------------------------------------------------
 # interpolation, LU, forces
 # ---------------------------------------------------------------------------
 def test_getrf_getrs_solve():
