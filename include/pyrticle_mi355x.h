@@ -175,6 +175,20 @@ int pyr_push_set_mfma(int mode);
 int pyr_push_mfma_waves(int64_t *out);
 
 /* adjoint of the push ------------------------------------------------------ */
+/* pyr_lsrk4_steps with the averaging pusher's force (push_average.hpp:338-443,
+ * pyrticle/pusher.py:167-207 AverageParticlePusher, driven by hedge LSRK4 as
+ * test/test_pyrticle.py:381-384 runs it): the static fields of
+ * pyr_set_static_fields (nodal) averaged over each particle's shape of radius,
+ * alpha, normalizer with integral_weights (device, Np doubles, M 1).  Per
+ * stage: velocities, the shape-weighted gather of pyr_average_forces, the RK
+ * update and relocalization; kills compact the arrays in the per-stage path's
+ * order.  Arguments otherwise as pyr_lsrk4_steps (FP64 only). */
+int pyr_lsrk4_steps_average(pyr_mesh *mesh, int nsteps, double dt, double vacuum_c,
+                            double radius, double alpha, double normalizer,
+                            const double *integral_weights, int64_t *n_inout, double *x,
+                            double *p, double *q, double *mass, uint32_t *containing_elements,
+                            int64_t *ids, unsigned long long *counters);
+
 /* No reference counterpart (pyrticle has no adjoint); these differentiate the
  * pyr_lsrk4_steps dynamics for torch.autograd (pyrticle_amd/adjoint.py).
  *

@@ -720,6 +720,91 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
   }
 }
 
+
+// The averaging pusher's forces (push_average.hpp:338-443): one sort of the
+// particles by element, then the shape-weighted gather k_shape_average.
+// Stream-ordered, no host synchronization; returns the device counter of
+// located particles with a zero shape integral (null when n = 0).  vis: the
+// six vis arrays (pt_e, pt_b, pt_e_stddev, pt_b_stddev, el_force, mag_force)
+// or null.
+namespace pyr {
+unsigned long long *average_forces_impl(pyr_mesh *mesh, double radius, double alpha,
+                                        double normalizer, int64_t n, int vdim,
+                                        const double *const fields[6], const double *x,
+                                        const uint32_t *elem, const double *q, const double *v,
+                                        const double *integral_weights, double *force,
+                                        double *integral_out, double *const *vis) {
+  PYR_REQUIRE(radius > 0 && std::isfinite(radius), "shape radius must be positive");
+  PYR_REQUIRE(alpha > 0, "shape exponent must be positive");
+  PYR_REQUIRE(vdim == mesh->dim, "vdim must equal the mesh dimension");
+  PYR_REQUIRE(n >= 0 && n < (int64_t)0x7FFFFFFF, "invalid particle count");
+  PYR_REQUIRE(mesh->dev.jacobian, "the mesh was created without element jacobians");
+  if (vis)
+    for (int k = 0; k < 6; ++k) PYR_REQUIRE(vis[k], "all six vis arrays or none");
+  if (n == 0) return nullptr;
+  PYR_REQUIRE(x && q && v && elem && integral_weights && force, "null pointer");
+  if (!mesh->dep) mesh->dep = new DepositCache();
+  DepositCache *c = mesh->dep;
+  const int K = mesh->K, Np = mesh->Np, D = mesh->dim;
+  hipStream_t s = stream();
+  prepare_tables(mesh, c, radius);
+  if (c->tr_radius != radius) build_transpose(mesh, c);
+  const uint32_t *perm;
+  {
+    ProfScope prof(PROF_SORT);
+    perm = sort_and_stage(mesh, c, n, x, q, nullptr, elem, normalizer);
+  }
+  // particles without an element keep zero outputs
+  PYR_HIP(hipMemsetAsync(force, 0, sizeof(double) * (size_t)n * vdim, s));
+  if (integral_out) PYR_HIP(hipMemsetAsync(integral_out, 0, sizeof(double) * (size_t)n, s));
+  if (vis) {
+    const size_t w[6] = {3, 3, 1, 1, (size_t)vdim, (size_t)vdim};
+    for (int k = 0; k < 6; ++k)
+      PYR_HIP(hipMemsetAsync(vis[k], 0, sizeof(double) * (size_t)n * w[k], s));
+  }
+  c->nrm.reserve(sizeof(double) * ((size_t)2 * n + 1) + sizeof(unsigned long long));
+  unsigned long long *zeros = c->nrm.as<unsigned long long>();
+  PYR_HIP(hipMemsetAsync(zeros, 0, sizeof(unsigned long long), s));
+  AverageArgs aa{};
+  IntegralArgs &ia = aa.b;
+  ia.K = K;
+  ia.R = radius;
+  ia.invR = 1.0 / radius;
+  ia.alpha = alpha;
+  ia.normalizer = normalizer;
+  for (int a = 0; a < 3; ++a) ia.shift_len[a] = mesh->dev.per_max[a] - mesh->dev.per_min[a];
+  ia.tr_start = c->tr_start.as<uint32_t>();
+  ia.tr = c->tr.as<uint32_t>();
+  ia.tr_shift = c->tr_shift.as<uint8_t>();
+  ia.seg = c->seg.as<uint32_t>();
+  ia.perm = perm;
+  ia.pos = c->pos.as<double4>();
+  ia.geom = c->geom.as<double>();
+  ia.node_tab = c->tab64.as<double>();
+  ia.iw = integral_weights;
+  ia.jac = mesh->dev.jacobian;
+  ia.integral = integral_out;
+  for (int f = 0; f < 6; ++f) aa.f[f] = fields[f];
+  aa.q = q;
+  aa.v = v;
+  aa.vdim = vdim;
+  aa.force = force;
+  if (vis) {
+    aa.vis_e = vis[0];
+    aa.vis_b = vis[1];
+    aa.vis_e_std = vis[2];
+    aa.vis_b_std = vis[3];
+    aa.vis_el = vis[4];
+    aa.vis_mag = vis[5];
+  }
+  aa.zeros = zeros;
+  ProfScope prof(PROF_GATHER);
+  launch_shape_average(D, Np, aa, K, alpha == 2.0, vis != nullptr);
+  return zeros;
+}
+
+}  // namespace pyr
+
 extern "C" {
 
 int pyr_deposit(pyr_mesh *mesh, double radius, double alpha, double normalizer, int precision,
@@ -809,79 +894,14 @@ int pyr_average_forces(pyr_mesh *mesh, double radius, double alpha, double norma
   return guarded([&] {
     PYR_REQUIRE(mesh && n_zero_out && fields, "null argument");
     *n_zero_out = 0;
-    PYR_REQUIRE(radius > 0 && std::isfinite(radius), "shape radius must be positive");
-    PYR_REQUIRE(alpha > 0, "shape exponent must be positive");
-    PYR_REQUIRE(vdim == mesh->dim, "vdim must equal the mesh dimension");
-    PYR_REQUIRE(n >= 0 && n < (int64_t)0x7FFFFFFF, "invalid particle count");
-    PYR_REQUIRE(mesh->dev.jacobian, "the mesh was created without element jacobians");
-    const bool vis = vis_e != nullptr;
-    PYR_REQUIRE(!vis || (vis_b && vis_e_std && vis_b_std && vis_el && vis_mag),
-                "all six vis arrays or none");
-    if (n == 0) return;
-    PYR_REQUIRE(x && q && v && elem && integral_weights && force, "null pointer");
-    if (!mesh->dep) mesh->dep = new DepositCache();
-    DepositCache *c = mesh->dep;
-    const int K = mesh->K, Np = mesh->Np, D = mesh->dim;
-    hipStream_t s = stream();
-    prepare_tables(mesh, c, radius);
-    if (c->tr_radius != radius) build_transpose(mesh, c);
-    const uint32_t *perm;
-    {
-      ProfScope prof(PROF_SORT);
-      perm = sort_and_stage(mesh, c, n, x, q, nullptr, elem, normalizer);
-    }
-    // particles without an element keep zero outputs
-    PYR_HIP(hipMemsetAsync(force, 0, sizeof(double) * (size_t)n * vdim, s));
-    if (integral_out) PYR_HIP(hipMemsetAsync(integral_out, 0, sizeof(double) * (size_t)n, s));
-    if (vis) {
-      for (double *b : {vis_e, vis_b})
-        PYR_HIP(hipMemsetAsync(b, 0, sizeof(double) * (size_t)n * 3, s));
-      for (double *b : {vis_e_std, vis_b_std})
-        PYR_HIP(hipMemsetAsync(b, 0, sizeof(double) * (size_t)n, s));
-      for (double *b : {vis_el, vis_mag})
-        PYR_HIP(hipMemsetAsync(b, 0, sizeof(double) * (size_t)n * vdim, s));
-    }
-    c->nrm.reserve(sizeof(double) * ((size_t)2 * n + 1) + sizeof(unsigned long long));
-    unsigned long long *zeros = c->nrm.as<unsigned long long>();
-    PYR_HIP(hipMemsetAsync(zeros, 0, sizeof(unsigned long long), s));
-    AverageArgs aa{};
-    IntegralArgs &ia = aa.b;
-    ia.K = K;
-    ia.R = radius;
-    ia.invR = 1.0 / radius;
-    ia.alpha = alpha;
-    ia.normalizer = normalizer;
-    for (int a = 0; a < 3; ++a) ia.shift_len[a] = mesh->dev.per_max[a] - mesh->dev.per_min[a];
-    ia.tr_start = c->tr_start.as<uint32_t>();
-    ia.tr = c->tr.as<uint32_t>();
-    ia.tr_shift = c->tr_shift.as<uint8_t>();
-    ia.seg = c->seg.as<uint32_t>();
-    ia.perm = perm;
-    ia.pos = c->pos.as<double4>();
-    ia.geom = c->geom.as<double>();
-    ia.node_tab = c->tab64.as<double>();
-    ia.iw = integral_weights;
-    ia.jac = mesh->dev.jacobian;
-    ia.integral = integral_out;
-    for (int f = 0; f < 6; ++f) aa.f[f] = fields[f];
-    aa.q = q;
-    aa.v = v;
-    aa.vdim = vdim;
-    aa.force = force;
-    aa.vis_e = vis_e;
-    aa.vis_b = vis_b;
-    aa.vis_e_std = vis_e_std;
-    aa.vis_b_std = vis_b_std;
-    aa.vis_el = vis_el;
-    aa.vis_mag = vis_mag;
-    aa.zeros = zeros;
-    {
-      ProfScope prof(PROF_GATHER);
-      launch_shape_average(D, Np, aa, K, alpha == 2.0, vis);
-    }
+    double *vis[6] = {vis_e, vis_b, vis_e_std, vis_b_std, vis_el, vis_mag};
+    unsigned long long *zeros =
+        average_forces_impl(mesh, radius, alpha, normalizer, n, vdim, fields, x, elem, q, v,
+                            integral_weights, force, integral_out, vis_e ? vis : nullptr);
+    if (!zeros) return;
     unsigned long long hz = 0;
-    PYR_HIP(hipMemcpyAsync(&hz, zeros, sizeof(hz), hipMemcpyDeviceToHost, s));
-    PYR_HIP(hipStreamSynchronize(s));
+    PYR_HIP(hipMemcpyAsync(&hz, zeros, sizeof(hz), hipMemcpyDeviceToHost, stream()));
+    PYR_HIP(hipStreamSynchronize(stream()));
     *n_zero_out = (int64_t)hz;
   });
 }

@@ -713,6 +713,53 @@ __global__ void k_stage_flags(int64_t n, const uint32_t *__restrict__ death, uin
   if (i < n) flags[i] = death[i] == s ? 1 : 0;
 }
 
+// One LSRK4 stage of the averaging pusher's step for the particles still
+// alive (death == 0), in the per-stage path's arithmetic: residual = a
+// residual + dt k, y = y + b residual (timestep.py, cloud.py:554-603), then
+// relocalization; a particle that leaves is frozen with death = stage + 1 and
+// elem = INVALID.
+template <int D>
+__global__ __launch_bounds__(256) void k_avg_stage(DevMesh m, int64_t n, int s, double dt,
+                                                   double *__restrict__ x, double *__restrict__ p,
+                                                   double *__restrict__ rx, double *__restrict__ rp,
+                                                   const double *__restrict__ v,
+                                                   const double *__restrict__ F,
+                                                   uint32_t *__restrict__ elem,
+                                                   uint32_t *__restrict__ death,
+                                                   unsigned long long *counters, int *err,
+                                                   unsigned int *n_dead) {
+  FindCounters cnt{};
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && death[i] == 0) {
+    const double a = c_rk4a[s], b = c_rk4b[s];
+    double xi[D], pi[D], r[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const double rxk = a * rx[i * D + k] + dt * v[i * D + k];
+      const double rpk = a * rp[i * D + k] + dt * F[i * D + k];
+      rx[i * D + k] = rxk;
+      rp[i * D + k] = rpk;
+      xi[k] = x[i * D + k] + b * rxk;
+      pi[k] = p[i * D + k] + b * rpk;
+    }
+    uint32_t e = elem[i];
+    int e_flag = 0;
+    const bool alive = relocate<D>(m, xi, pi, e, cnt, e_flag, r);
+    if (e_flag) atomicOr(err, 1);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      x[i * D + k] = xi[k];
+      p[i * D + k] = pi[k];
+    }
+    elem[i] = alive ? e : INVALID;
+    if (!alive) {
+      death[i] = (uint32_t)s + 1;
+      atomicAdd(n_dead, 1u);
+    }
+  }
+  flush_counters(cnt, counters);
+}
+
 static inline unsigned blocks_for(int64_t n, int bs = 256) {
   return (unsigned)((n + bs - 1) / bs > 0 ? (n + bs - 1) / bs : 1);
 }
@@ -817,6 +864,14 @@ int pyr_set_static_fields(pyr_mesh *mesh, const double *const fields[6]) {
     mesh->sf64.reserve(cnt * sizeof(double));
     mesh->sf32.reserve((size_t)mesh->K * 6 * coef_stride<float>(mesh->Np) * sizeof(float));
     modal_transform(mesh, fields, mesh->sf64.as<double>(), mesh->sf32.as<float>());
+    const size_t nodes = (size_t)mesh->K * mesh->Np;
+    mesh->sf_nodal.reserve(6 * nodes * sizeof(double));
+    for (int f = 0; f < 6; ++f) {
+      mesh->sf_has[f] = fields[f] != nullptr;
+      if (fields[f])
+        PYR_HIP(hipMemcpyAsync(mesh->sf_nodal.as<double>() + f * nodes, fields[f],
+                               nodes * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+    }
     mesh->sf_set = true;
   });
 }
@@ -876,6 +931,92 @@ int pyr_lsrk4_steps(pyr_mesh *mesh, int precision, int nsteps, double dt, double
       if (h[1] == 0) continue;
       // the reference compacts after each stage (cloud.py:587-592); replay
       // the five compactions in stage order on the stage-of-death tags
+      for (uint32_t s = 1; s <= 5 && n > 0; ++s) {
+        hipLaunchKernelGGL(k_stage_flags, dim3(blocks_for(n)), dim3(256), 0, stream(), n,
+                           death, s, flags);
+        PYR_CHECK_LAUNCH();
+        int64_t n_new = 0, n_moves = 0;
+        int rc = pyr_compact_plan(n, flags, mv_from, mv_to, &n_new, &n_moves);
+        if (rc) throw Error(rc, pyr_last_error());
+        if (n_moves) {
+          struct {
+            void *buf;
+            int64_t bytes;
+          } arrays[] = {{x, 8 * D}, {p, 8 * D}, {q, 8}, {mass, 8}, {elem, 4},
+                        {death, 4}, {ids, 8}};
+          for (auto &a : arrays)
+            if (a.buf) {
+              rc = pyr_apply_moves(a.buf, a.bytes, n_moves, mv_from, mv_to);
+              if (rc) throw Error(rc, pyr_last_error());
+            }
+        }
+        n = n_new;
+      }
+    }
+    *n_inout = n;
+  });
+}
+
+int pyr_lsrk4_steps_average(pyr_mesh *mesh, int nsteps, double dt, double vacuum_c,
+                            double radius, double alpha, double normalizer,
+                            const double *integral_weights, int64_t *n_inout, double *x,
+                            double *p, double *q, double *mass, uint32_t *elem, int64_t *ids,
+                            unsigned long long *counters) {
+  return guarded([&] {
+    PYR_REQUIRE(mesh && n_inout && nsteps >= 0, "invalid argument");
+    PYR_REQUIRE(mesh->sf_set, "pyr_set_static_fields has not been called for this mesh");
+    int64_t n = *n_inout;
+    PYR_REQUIRE(n >= 0 && n < (int64_t)0x7FFFFFFF, "invalid particle count");
+    if (n == 0 || nsteps == 0) return;
+    PYR_REQUIRE(x && p && q && mass && elem && integral_weights, "null pointer");
+    const int D = mesh->dim;
+    const size_t nodes = (size_t)mesh->K * mesh->Np;
+    const double *fields[6];
+    for (int f = 0; f < 6; ++f)
+      fields[f] = mesh->sf_has[f] ? mesh->sf_nodal.as<double>() + f * nodes : nullptr;
+    // scratch: residuals rx, rp, velocities, forces (n x D each); death
+    // stage, flags, moves, error flag, dead count
+    const size_t nd = (size_t)n * D;
+    mesh->s_avg.reserve(4 * nd * sizeof(double));
+    double *rx = mesh->s_avg.as<double>(), *rp = rx + nd, *v = rp + nd, *F = v + nd;
+    mesh->s_flags.reserve((size_t)n * (sizeof(uint32_t) + 1));
+    mesh->s_holes.reserve((size_t)n * 2 * sizeof(uint32_t));
+    mesh->s_misc.reserve(64);
+    uint32_t *death = mesh->s_flags.as<uint32_t>();
+    uint8_t *flags = reinterpret_cast<uint8_t *>(death + n);
+    uint32_t *mv_from = mesh->s_holes.as<uint32_t>();
+    uint32_t *mv_to = mv_from + n;
+    int *err = mesh->s_misc.as<int>();
+    unsigned int *n_dead = reinterpret_cast<unsigned int *>(err + 1);
+    for (int step = 0; step < nsteps && n > 0; ++step) {
+      // the residual starts as 0 * rhs (timestep.py); a_0 = 0 keeps it there
+      PYR_HIP(hipMemsetAsync(rx, 0, 2 * (size_t)n * D * sizeof(double), stream()));
+      PYR_HIP(hipMemsetAsync(death, 0, (size_t)n * sizeof(uint32_t), stream()));
+      PYR_HIP(hipMemsetAsync(err, 0, 2 * sizeof(int), stream()));
+      for (int s = 0; s < 5; ++s) {
+        int rc = pyr_velocities(n, D, vacuum_c, p, mass, v);
+        if (rc) throw Error(rc, pyr_last_error());
+        {
+          ProfScope prof(PROF_PUSH);
+          average_forces_impl(mesh, radius, alpha, normalizer, n, D, fields, x, elem, q, v,
+                              integral_weights, F, nullptr, nullptr);
+        }
+        if (D == 2)
+          hipLaunchKernelGGL(k_avg_stage<2>, dim3(blocks_for(n)), dim3(256), 0, stream(),
+                             mesh->dev, n, s, dt, x, p, rx, rp, v, F, elem, death, counters, err,
+                             n_dead);
+        else
+          hipLaunchKernelGGL(k_avg_stage<3>, dim3(blocks_for(n)), dim3(256), 0, stream(),
+                             mesh->dev, n, s, dt, x, p, rx, rp, v, F, elem, death, counters, err,
+                             n_dead);
+        PYR_CHECK_LAUNCH();
+      }
+      int h[2] = {0, 0};
+      PYR_HIP(hipMemcpyAsync(h, err, 2 * sizeof(int), hipMemcpyDeviceToHost, stream()));
+      PYR_HIP(hipStreamSynchronize(stream()));
+      if (h[0]) throw Error(PYR_ERR_NO_BEST_NORMAL, "no best normal found--weird");
+      if (h[1] == 0) continue;
+      // compactions in stage order, as after each stage of the per-stage path
       for (uint32_t s = 1; s <= 5 && n > 0; ++s) {
         hipLaunchKernelGGL(k_stage_flags, dim3(blocks_for(n)), dim3(256), 0, stream(), n,
                            death, s, flags);

@@ -339,6 +339,20 @@ void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *
 }  // namespace pyr
 
 // Opaque handle behind the ABI's pyr_mesh.
+struct pyr_mesh;
+namespace pyr {
+// deposit.hip: the averaging pusher's shape-weighted forces, stream-ordered
+// (returns the device count of particles with a zero shape integral, or null
+// when n = 0).  vis: six arrays (pt_e, pt_b, pt_e_stddev, pt_b_stddev,
+// el_force, mag_force) or null.
+unsigned long long *average_forces_impl(pyr_mesh *mesh, double radius, double alpha,
+                                        double normalizer, int64_t n, int vdim,
+                                        const double *const fields[6], const double *x,
+                                        const uint32_t *elem, const double *q, const double *v,
+                                        const double *integral_weights, double *force,
+                                        double *integral_out, double *const *vis);
+}  // namespace pyr
+
 struct pyr_mesh {
   pyr::DevMesh dev{};
   int dim = 0, K = 0, Np = 0, nv = 0;
@@ -355,5 +369,10 @@ struct pyr_mesh {
   // static fields as modal coefficients, K x 6 x Np (pyr_set_static_fields)
   pyr::DevBuf sf64, sf32;
   bool sf_set = false;
+  // the same fields nodal, 6 x K x Np, for the averaging pusher's step;
+  // sf_has[f]: field f was given (else zero)
+  pyr::DevBuf sf_nodal;
+  bool sf_has[6] = {false, false, false, false, false, false};
+  pyr::DevBuf s_avg;                // averaging step scratch (push.hip)
   ~pyr_mesh();
 };

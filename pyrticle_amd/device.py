@@ -170,6 +170,19 @@ class DeviceMesh:
                                              _lib.ptr(counters)))
         return int(nn.value)
 
+    def lsrk4_steps_average(self, nsteps, dt, c, radius, alpha, normalizer, integral_weights,
+                            n, x, p, q, m, elem, ids=None, counters=None) -> int:
+        """LSRK4 steps with the averaging pusher in the static fields of
+        set_static_fields; returns the new particle count."""
+        _lib.sync_stream()
+        w = torch.as_tensor(np.asarray(integral_weights, dtype=np.float64), device=x.device)
+        nn = C.c_int64(n)
+        _lib.check(self._lib.pyr_lsrk4_steps_average(
+            self._h, nsteps, dt, c, radius, alpha, normalizer, _lib.ptr(w), C.byref(nn),
+            _lib.ptr(x), _lib.ptr(p), _lib.ptr(q), _lib.ptr(m), _lib.ptr(elem), _lib.ptr(ids),
+            _lib.ptr(counters)))
+        return int(nn.value)
+
     def deposit(self, radius, alpha, normalizer, x, q, elem, v=None, want_rho=True,
                 want_j=True, precision=FP64, rho=None, j=None):
         _lib.sync_stream()

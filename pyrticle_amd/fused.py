@@ -22,10 +22,14 @@ class FusedParticleStepper:
     static fields (E nodal, B = mu H nodal; None for zero components)."""
 
     def __init__(self, method, e_fields, b_fields, precision="fp64"):
-        from .pusher import MonomialParticlePusher
-        if not isinstance(method.pusher, MonomialParticlePusher):
-            raise ValueError("the fused step interpolates like MonomialParticlePusher; "
-                             "step other pushers through the per-stage RHS")
+        from .pusher import AverageParticlePusher, MonomialParticlePusher
+        self.average = isinstance(method.pusher, AverageParticlePusher)
+        if not (self.average or isinstance(method.pusher, MonomialParticlePusher)):
+            raise ValueError("the fused step supports MonomialParticlePusher and "
+                             "AverageParticlePusher; step other pushers through the "
+                             "per-stage RHS")
+        if self.average and precision != "fp64":
+            raise ValueError("the averaging pusher's step is FP64 only")
         self.method = method
         self.precision = dv.FP64 if precision == "fp64" else dv.FP32C
         fields = [None if f is None else f.contiguous() for f in list(e_fields) + list(b_fields)]
@@ -39,10 +43,26 @@ class FusedParticleStepper:
         ps = state.particle_state
         n = ps.particle_count
         ids = torch.arange(n, dtype=torch.int64, device="cuda")
-        n_new = self.method.mesh_data.device.lsrk4_steps(
-            nsteps, dt, self.method.units.VACUUM_LIGHT_SPEED(), n, ps.positions[:n],
-            ps.momenta[:n], ps.charges[:n], ps.masses[:n], ps.containing_elements[:n],
-            ids=ids, counters=self.counters, precision=self.precision)
+        dev = self.method.mesh_data.device
+        c = self.method.units.VACUUM_LIGHT_SPEED()
+        if self.average:
+            # the depositor's shape function and element reach, as the
+            # averaging pusher's per-stage forces use them (statistics and vis
+            # vectors are left to the per-stage path)
+            dep = self.method.depositor.backend
+            sf = dep.shape_function
+            if sf is None:
+                raise RuntimeError("shape function never set")
+            n_new = dev.lsrk4_steps_average(
+                nsteps, dt, c, sf.radius, sf.exponent, sf.normalizer,
+                self.method.pusher.backend.integral_weights, n, ps.positions[:n],
+                ps.momenta[:n], ps.charges[:n], ps.masses[:n], ps.containing_elements[:n],
+                ids=ids, counters=self.counters)
+        else:
+            n_new = dev.lsrk4_steps(
+                nsteps, dt, c, n, ps.positions[:n], ps.momenta[:n], ps.charges[:n],
+                ps.masses[:n], ps.containing_elements[:n], ids=ids, counters=self.counters,
+                precision=self.precision)
         ps.particle_count = n_new
         sig = state.particle_number_shift_signaller
         moved = (ids[:n_new] != torch.arange(n_new, device="cuda")).nonzero().squeeze(1)

@@ -253,17 +253,22 @@ class _Setup:
         return x, v
 
 
+@pytest.mark.parametrize("pusher", ["monomial", "average"])
 @pytest.mark.parametrize("kind", ["screw", "epb"])
-def test_static_field_kat_on_gpu(periodic_cylinder, cuda, kind):
-    """test_pyrticle.py:331-384 through the GPU path: 1e4 LSRK4 steps, relative
-    position error < 2e-12 and velocity error < 2e-13 (static_field.py:413-414),
-    checked every nsteps/300 steps."""
+def test_static_field_kat_on_gpu(periodic_cylinder, cuda, kind, pusher):
+    """test_pyrticle.py:331-384 through the GPU path, for both pushers as the
+    reference runs them (:378-384): 1e4 LSRK4 steps, relative position error
+    < 2e-12 and velocity error < 2e-13 (static_field.py:413-414), checked every
+    nsteps/300 steps.  The averaging pusher runs with a shape of radius 0.3:
+    the reference leaves the depositor's default (radius 1, whose support
+    spans the whole 2 m period of this mesh, which the GPU path rejects)."""
     from pyrticle_amd.cloud import TimesteppablePicState
     from pyrticle_amd.fused import FusedParticleStepper
+    from pyrticle_amd.pusher import AverageParticlePusher
     from pyrticle_amd.units import SIUnitsWithNaturalConstants as U
     discr, ma = periodic_cylinder
     setup = _Setup(kind)
-    method = _method(discr, 3)
+    method = _method(discr, 3, pusher=AverageParticlePusher() if pusher == "average" else None)
     x0, v0 = setup.state(0.0)
     c = U.VACUUM_LIGHT_SPEED()
     gam = 1 / np.sqrt(1 - np.sum(v0 ** 2, axis=1) / c ** 2)
@@ -272,6 +277,8 @@ def test_static_field_kat_on_gpu(periodic_cylinder, cuda, kind):
     n = len(x0)
     method.add_particle_arrays(state, x0, p0, np.full(n, setup.charge), np.full(n, U.EL_MASS))
     assert len(state) == n
+    if pusher == "average":
+        method.depositor.set_shape_function(state, method.get_shape_function_class()(0.3, 3))
     nn = ma.nnodes
     full = lambda v: torch.full((nn,), v, dtype=torch.float64, device=cuda)  # noqa: E731
     stepper = FusedParticleStepper(method, [None, None, full(setup.ez)],
@@ -428,3 +435,60 @@ def test_diagnostics_and_field_rhs_dropin(square, oracle_square, cuda):
     _, j_o, _ = O.deposit(oracle_square, wl.radius, wl.alpha, x, q, vel.cpu().numpy(), el)
     for k in range(2):
         assert rel_l2(rhs[k].cpu().numpy(), -j_o[:, k] / op.epsilon) < 1e-12
+
+
+@pytest.mark.parametrize("dt_scale", [1.0, 4.0])
+def test_average_pusher_fused_equals_per_stage(square, cuda, dt_scale):
+    """The averaging pusher's fused step (pyr_lsrk4_steps_average) against the
+    per-stage RHS path: bit-identical states, element ids and find counters,
+    with kills at the larger step."""
+    from pyrticle_amd.cloud import (FieldToParticleRhsCalculator, ParticleRhsCalculator,
+                                    TimesteppablePicState)
+    from pyrticle_amd.fused import FusedParticleStepper
+    from pyrticle_amd.pusher import AverageParticlePusher
+    from pyrticle_amd.timestep import LSRK4TimeStepper
+    wl = square
+    dt = wl.dt * dt_scale
+    nsteps = 10 if dt_scale == 1 else 40
+    n = 3000
+    nodes = wl.ma.nodes
+    ex = torch.from_numpy(1e3 * (1 + nodes[:, 0] ** 2)).to(cuda)
+    ey = torch.from_numpy(-2e3 * nodes[:, 0] * nodes[:, 1]).to(cuda)
+    bz = torch.from_numpy(0.01 * (1 - 0.4 * nodes[:, 0] * nodes[:, 1])).to(cuda)
+
+    def make():
+        method = _method(wl.discr, 2, pusher=AverageParticlePusher())
+        state = method.make_state()
+        method.add_particle_arrays(state, wl.x[:n], wl.p[:n], wl.q[:n], wl.m[:n])
+        method.depositor.set_shape_function(
+            state, method.get_shape_function_class()(wl.radius, 2, wl.alpha))
+        return method, state
+
+    method, state = make()
+    from pyrticle_amd.fields import StaticFieldOperator
+    from pyrticle_amd.units import SIUnitsWithNaturalConstants as U
+    op = StaticFieldOperator(eh_subset=(True, True, False, False, False, True))
+    fields = op.assemble_eh(e=[ex, ey], h=[bz / U.MU0])
+    p_rhs = ParticleRhsCalculator(method, op)
+    f_rhs = FieldToParticleRhsCalculator(method, op)
+
+    def rhs(t, ts):
+        return p_rhs(t, lambda: fields, lambda: ts.state) + f_rhs(t, lambda: fields,
+                                                                   lambda: ts.state)
+
+    stepper = LSRK4TimeStepper()
+    ts = TimesteppablePicState(method, state)
+    for k in range(nsteps):
+        ts = stepper(ts, k * dt, dt, rhs)
+    st = ts.state
+    method2, state2 = make()
+    # B as the per-stage path forms it, mu H (cloud.py:265-274)
+    fused = FusedParticleStepper(method2, [ex, ey, None], [None, None, op.mu * fields[2]])
+    ts2 = fused(TimesteppablePicState(method2, state2), 0.0, dt, nsteps)
+    if dt_scale > 1:
+        assert len(st) < n
+    assert len(ts2.state) == len(st)
+    assert torch.equal(ts2.state.positions, st.positions)
+    assert torch.equal(ts2.state.momenta, st.momenta)
+    assert torch.equal(ts2.state.containing_elements, st.containing_elements)
+    assert torch.equal(fused.counters.cpu(), method.find_counters.tensor.cpu())
