@@ -298,6 +298,108 @@ def dense_p4_leg(device, torch, steps, warmup, n_particles):
     return res
 
 
+def extensions_leg(wl, dm, device, torch, st, normalizer, reps, avg_steps, pairs_passed):
+    """The components beside the headline path, on the same 4M-particle state:
+    the normalized deposition (per-particle shape integrals, FP64, then the
+    deposit with the charges q / I), the averaging pusher's shape-weighted force
+    gather and its LSRK4 step, and the diagnostics reductions."""
+    from pyrticle_amd import device as dv
+    n = st["n"]
+    x, q, el, p, m = (st[k][:n] for k in ("x", "q", "elem", "p", "m"))
+    v = dv.velocities(p, m, wl.c)
+    w = wl.discr.element_groups[0].local_discretization.integral_weights()
+    Np = wl.ma.Np
+    res = {}
+
+    def timed(fn, region=None):
+        fn()
+        torch.cuda.synchronize()
+        if region:
+            dv.profile_enable(True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = fn()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / reps
+        kernel = None
+        if region:
+            ms, cnt = dv.profile_read(region)
+            dv.profile_enable(False)
+            kernel = ms / max(1, cnt)
+        return out, wall, kernel
+
+    # normalized deposition against the plain one
+    norm = {}
+    rho_n = {}
+    for name, prec in (("fp64", dv.FP64), ("fp32c", dv.FP32C)):
+        _, t_plain, _ = timed(lambda: dm.deposit(wl.radius, wl.alpha, normalizer, x, q, el, v=v,
+                                                 precision=prec))
+        (rho, j, nz), t_norm, k_int = timed(
+            lambda: dm.deposit_normalized(wl.radius, wl.alpha, normalizer, x, q, el, w, v=v,
+                                          precision=prec), "integral")
+        rho_n[name] = rho
+        norm[name] = {"ms_per_call": 1e3 * t_norm, "plain_deposit_ms_per_call": 1e3 * t_plain,
+                      "shape_integral_kernel_ms": k_int, "zero_mass_particles": nz}
+    qsum = float(q.sum())
+    norm["charge_rel_err_fp64"] = abs(wl.discr.integral(rho_n["fp64"].cpu().numpy())
+                                      / qsum - 1)
+    norm["rho_rel_l2_fp32c_vs_fp64"] = float(torch.linalg.norm(rho_n["fp32c"] - rho_n["fp64"])
+                                             / torch.linalg.norm(rho_n["fp64"]))
+    res["normalized_deposit"] = norm
+    log("normalized deposit: fp64 %.2f ms (plain %.2f), fp32c %.2f ms (plain %.2f)"
+        % (norm["fp64"]["ms_per_call"], norm["fp64"]["plain_deposit_ms_per_call"],
+           norm["fp32c"]["ms_per_call"], norm["fp32c"]["plain_deposit_ms_per_call"]))
+
+    # averaging pusher: forces, then LSRK4 steps
+    fields = [torch.from_numpy(f).to(device) for f in wl.fields]
+    (F, integ, nz), t_avg, k_avg = timed(
+        lambda: dm.average_forces(wl.radius, wl.alpha, normalizer, fields, x, el, q, v, w),
+        "average")
+    # per (particle, target element) pair that passes the bound-sphere test:
+    # Np nodes x (t: D FMAs + 1 sub, square, x w, + I: 10 flops) + 6 fields x
+    # Np FMAs (12 flops) = 22 Np flops
+    flops = 22.0 * Np * pairs_passed
+    tf = flops / (k_avg * 1e-3) / 1e12 if k_avg else 0.0
+    avg = {"forces_ms_per_call": 1e3 * t_avg, "gather_kernel_ms": k_avg,
+           "roofline": {"bound": "valu", "kernel": "k_shape_average (FP64)", "achieved": tf,
+                        "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": tf / FP64_PEAK_TFLOPS, "flops_per_launch": flops,
+                        "pairs_passed": pairs_passed, "flops_per_pair": 22 * Np}}
+    st2 = clone_state(st)
+    dm.lsrk4_steps_average(1, wl.dt, wl.c, wl.radius, wl.alpha, normalizer, w, st2["n"],
+                           st2["x"], st2["p"], st2["q"], st2["m"], st2["elem"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    alive = 0
+    for _ in range(avg_steps):
+        st2["n"] = dm.lsrk4_steps_average(1, wl.dt, wl.c, wl.radius, wl.alpha, normalizer, w,
+                                          st2["n"], st2["x"], st2["p"], st2["q"], st2["m"],
+                                          st2["elem"])
+        alive += st2["n"]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    avg["lsrk4_step"] = {"particle_steps_per_s": alive / wall, "ms_per_step": 1e3 * wall /
+                         avg_steps, "steps": avg_steps}
+    res["average_pusher"] = avg
+    log("average pusher: forces %.2f ms (gather %.2f ms, %.2f TFLOP/s), step %.2f ms"
+        % (1e3 * t_avg, k_avg or 0, tf, 1e3 * wall / avg_steps))
+
+    # diagnostics (each call: one reduction pass over the state, host result)
+    diag = {}
+    for name, fn, row_bytes in (
+            ("rms_energy_spread", lambda: dv.rms_energy_spread(p, m, wl.c), 32),
+            ("rms_beam_emittance", lambda: dv.rms_beam_emittance(x, p, 0, 2), 48),
+            ("total_charge", lambda: dv.total_charge(q), 8)):
+        _, wall_d, k_d = timed(fn, "diagnostics")
+        gbs = n * row_bytes / (k_d * 1e-3) / 1e9 if k_d else 0.0
+        diag[name] = {"ms_per_call": 1e3 * wall_d, "kernel_ms": k_d, "bytes_per_particle":
+                      row_bytes, "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+    res["diagnostics"] = diag
+    log("diagnostics: energy spread %.3f ms kernel (%.0f GB/s)"
+        % (diag["rms_energy_spread"]["kernel_ms"] or 0, diag["rms_energy_spread"]["hbm_gbs"]))
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -317,6 +419,11 @@ def main(argv=None):
                     help="skip the dense P4 interpolation leg (16M particles, 49,896 tets)")
     ap.add_argument("--dense-particles", type=int, default=16_000_000)
     ap.add_argument("--dense-steps", type=int, default=5)
+    ap.add_argument("--no-extensions", action="store_true",
+                    help="skip the normalized-deposition, averaging-pusher and diagnostics "
+                         "legs")
+    ap.add_argument("--extension-reps", type=int, default=5)
+    ap.add_argument("--average-steps", type=int, default=3)
     args = ap.parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -414,6 +521,12 @@ def main(argv=None):
                  "gather_ms": g32_ms / max(1, g32_n), "push_ms": p32_ms / max(1, p32_n)}
         log("fp32c: %.3f ms/step (x%.2f), rho err %.2e" % (fp32c["ms_per_step"],
                                                            fp32c["speedup_vs_fp64"], rho_err))
+
+    # ---- normalized deposition, averaging pusher, diagnostics -------------
+    extensions = None
+    if not args.no_extensions:
+        extensions = extensions_leg(wl, dm, device, torch, st, normalizer, args.extension_reps,
+                                    args.average_steps, n_pass)
 
     # ---- adjoint (gradients of a density-matching loss) -------------------
     adjoint = None
@@ -521,6 +634,7 @@ def main(argv=None):
         "fp32c": fp32c,
         "adjoint": adjoint,
         "dense_p4": dense,
+        "extensions": extensions,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -6,8 +6,8 @@
 namespace pyr {
 
 template <int D, int NP>
-static void launch_average(const AverageArgs &ga, int K, bool a2, bool vis) {
-  dim3 grid(gather_grid(K)), block(64);
+static void launch_average(const AverageArgs &ga, unsigned grid, bool a2, bool vis) {
+  dim3 block(64);
   hipStream_t s = stream();
   if (vis) {
     if (a2) hipLaunchKernelGGL((k_shape_average<D, NP, true, true>), grid, block, 0, s, ga);
@@ -19,20 +19,21 @@ static void launch_average(const AverageArgs &ga, int K, bool a2, bool vis) {
   PYR_CHECK_LAUNCH();
 }
 
-void launch_shape_average(int D, int Np, const AverageArgs &ga, int K, bool a2, bool vis) {
+void launch_shape_average(int D, int Np, const AverageArgs &ga, unsigned grid, bool a2,
+                          bool vis) {
   if (D == 2) {
     switch (Np) {
-      case 3: return launch_average<2, 3>(ga, K, a2, vis);
-      case 6: return launch_average<2, 6>(ga, K, a2, vis);
-      case 10: return launch_average<2, 10>(ga, K, a2, vis);
-      case 15: return launch_average<2, 15>(ga, K, a2, vis);
+      case 3: return launch_average<2, 3>(ga, grid, a2, vis);
+      case 6: return launch_average<2, 6>(ga, grid, a2, vis);
+      case 10: return launch_average<2, 10>(ga, grid, a2, vis);
+      case 15: return launch_average<2, 15>(ga, grid, a2, vis);
     }
   } else {
     switch (Np) {
-      case 4: return launch_average<3, 4>(ga, K, a2, vis);
-      case 10: return launch_average<3, 10>(ga, K, a2, vis);
-      case 20: return launch_average<3, 20>(ga, K, a2, vis);
-      case 35: return launch_average<3, 35>(ga, K, a2, vis);
+      case 4: return launch_average<3, 4>(ga, grid, a2, vis);
+      case 10: return launch_average<3, 10>(ga, grid, a2, vis);
+      case 20: return launch_average<3, 20>(ga, grid, a2, vis);
+      case 35: return launch_average<3, 35>(ga, grid, a2, vis);
     }
   }
   throw Error(PYR_ERR_INVALID_ARG, "the averaging pusher supports P1..P4 elements");

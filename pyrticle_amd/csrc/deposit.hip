@@ -241,6 +241,8 @@ struct DepositCache {
   DevBuf pos32, wgt32;   // FP32 staging
   DevBuf nrm;            // normalized deposition: integrals, scaled charges, counter
   DevBuf skeys;   // Morton sort keys
+  DevBuf chunks;         // per-particle gathers: 64-particle chunks (element, start),
+                         // per-element chunk counts/offsets, total
   std::vector<double> h_geom;   // K x 8
 };
 
@@ -251,7 +253,7 @@ void pyr_deposit_cache_free(pyr_mesh *mesh) {
                     &c->tab_R, &c->tr_start, &c->tr, &c->tr_shift, &c->keys_a, &c->keys_b,
                     &c->vals_a, &c->vals_b, &c->skeys, &c->hist,
                     &c->hist_scan, &c->seg, &c->tmp, &c->pos, &c->wgt, &c->stats,
-                    &c->pos32, &c->wgt32, &c->nrm})
+                    &c->pos32, &c->wgt32, &c->nrm, &c->chunks})
     b->release();
   delete c;
   mesh->dep = nullptr;
@@ -635,6 +637,49 @@ static void build_transpose(pyr_mesh *mesh, DepositCache *c) {
 }
 
 
+// Work list of the per-particle gathers (shape integral, averaging, adjoint):
+// the particles of each source element in chunks of up to 64 (one wave each),
+// so that a dense element spreads over many waves.
+__global__ void k_chunk_counts(int K, const uint32_t *__restrict__ seg,
+                               uint32_t *__restrict__ nch) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < K) nch[s] = (seg[s + 1] - seg[s] + 63u) / 64u;
+}
+
+__global__ void k_chunk_fill(int K, const uint32_t *__restrict__ seg,
+                             const uint32_t *__restrict__ nch, const uint32_t *__restrict__ off,
+                             uint2 *__restrict__ chunks) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= K) return;
+  const uint32_t o = off[s], p0 = seg[s];
+  for (uint32_t k = 0; k < nch[s]; ++k) chunks[o + k] = make_uint2((uint32_t)s, p0 + 64u * k);
+}
+
+// Builds the chunk list of the current sort (c->seg); returns the launch grid
+// (a multiple of the XCD grouping, an upper bound on the chunk count).
+static unsigned build_chunks(pyr_mesh *mesh, DepositCache *c, int64_t n, ChunkList &cl) {
+  const int K = mesh->K;
+  hipStream_t s = stream();
+  const int64_t bound = (n + 63) / 64 + std::min<int64_t>(K, n);
+  const size_t words = (size_t)2 * bound + 2 * (size_t)K + 1;
+  c->chunks.reserve(sizeof(uint32_t) * words);
+  uint2 *chunks = c->chunks.as<uint2>();
+  uint32_t *nch = reinterpret_cast<uint32_t *>(chunks + bound);
+  uint32_t *off = nch + K;
+  uint32_t *total = off + K;
+  c->tmp.reserve(sizeof(uint32_t) * scan_tmp_words(K));
+  hipLaunchKernelGGL(k_chunk_counts, dim3(blocks_for(K)), dim3(256), 0, s, K,
+                     c->seg.as<uint32_t>(), nch);
+  PYR_CHECK_LAUNCH();
+  scan_exclusive_u32(nch, off, K, total, c->tmp.as<uint32_t>(), s);
+  hipLaunchKernelGGL(k_chunk_fill, dim3(blocks_for(K)), dim3(256), 0, s, K,
+                     c->seg.as<uint32_t>(), nch, off, chunks);
+  PYR_CHECK_LAUNCH();
+  cl.chunks = chunks;
+  cl.n_chunks = total;
+  return gather_grid((int)bound);
+}
+
 static int g_deposit_count_mode = 0;
 
 // q / I per particle for the normalized deposition; located particles whose
@@ -798,8 +843,9 @@ unsigned long long *average_forces_impl(pyr_mesh *mesh, double radius, double al
     aa.vis_mag = vis[5];
   }
   aa.zeros = zeros;
-  ProfScope prof(PROF_GATHER);
-  launch_shape_average(D, Np, aa, K, alpha == 2.0, vis != nullptr);
+  const unsigned grid = build_chunks(mesh, c, n, ia.cl);
+  ProfScope prof(PROF_AVERAGE);
+  launch_shape_average(D, Np, aa, grid, alpha == 2.0, vis != nullptr);
   return zeros;
 }
 
@@ -870,7 +916,11 @@ int pyr_deposit_normalized(pyr_mesh *mesh, double radius, double alpha, double n
       ia.iw = integral_weights;
       ia.jac = mesh->dev.jacobian;
       ia.integral = integral;
-      launch_shape_integral(D, Np, ia, K, alpha == 2.0);
+      const unsigned grid = build_chunks(mesh, c, n, ia.cl);
+      {
+        ProfScope prof(PROF_INTEGRAL);
+        launch_shape_integral(D, Np, ia, grid, alpha == 2.0);
+      }
       // charges q / I (dep_normshape.hpp:168-198); zero-mass particles drop out
       hipLaunchKernelGGL(k_normalized_charges, dim3(blocks_for(n)), dim3(256), 0, s, n, K, elem,
                          q, integral, qs, scale_out, zeros);
@@ -955,8 +1005,9 @@ int pyr_deposit_backward(pyr_mesh *mesh, double radius, double alpha, double nor
     ga.j_bar = j_bar;
     ga.x_bar = x_bar;
     ga.v_bar = j_bar ? v_bar : nullptr;
+    const unsigned grid = build_chunks(mesh, c, n, ga.cl);
     ProfScope prof(PROF_DEPOSIT_ADJ);
-    launch_deposit_adj(D, Np, ga, K, precision, alpha == 2.0);
+    launch_deposit_adj(D, Np, ga, grid, precision, alpha == 2.0);
   });
 }
 

@@ -70,6 +70,24 @@ static inline unsigned gather_grid(int K) {
   return (unsigned)(((int64_t)K + per - 1) / per * per);
 }
 
+// 64-particle work chunks of the per-particle gathers: (source element, first
+// sorted index); n_chunks (device) holds the count.
+struct ChunkList {
+  const uint2 *chunks;
+  const uint32_t *n_chunks;
+};
+
+// The chunk a block of a per-particle gather handles (XCD-grouped: nearby
+// chunks, and so nearby elements, share an L2), or false past the end.
+__device__ __forceinline__ bool chunk_of(const ChunkList &cl, uint32_t &s, uint32_t &base) {
+  const uint32_t k = (uint32_t)xcd_element(blockIdx.x);
+  if (k >= *cl.n_chunks) return false;
+  const uint2 ch = cl.chunks[k];
+  s = ch.x;
+  base = ch.y;
+  return true;
+}
+
 // Gather per target element.  A block holds G waves, each owning NPW of the
 // element's nodes.  The block streams the candidate particles, drops those
 // whose ball cannot reach the element (bounding sphere of the nodes inflated
@@ -656,6 +674,7 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
 // gather's candidate lists); target node data and adjoint densities are wave-
 // uniform and come through scalar loads.  No atomics.
 struct AdjArgs {
+  ChunkList cl;
   int K;
   double R, invR, alpha;
   double shift_len[3];
@@ -677,16 +696,15 @@ __global__ __launch_bounds__(64) void k_deposit_adj(AdjArgs g) {
   using T = typename std::conditional<FP32, float, double>::type;
   using CT = const T __attribute__((address_space(4))) *;
   using CD = const double __attribute__((address_space(4))) *;
-  const int s = xcd_element(blockIdx.x);
-  if (s >= g.K) return;
-  const uint32_t p0 = g.seg[s], p1 = g.seg[s + 1];
-  if (p0 >= p1) return;
+  uint32_t s, base;
+  if (!chunk_of(g.cl, s, base)) return;
+  const uint32_t p1 = g.seg[s + 1];
   const int lane = threadIdx.x;
   const uint32_t t_begin = g.tr_start[s], t_end = g.tr_start[s + 1];
   const double two_inv_r = 2.0 * g.invR;
   const CT ntab_all = (CT)(static_cast<const T *>(g.node_tab));
   const bool has_rho = g.rho_bar != nullptr, has_j = g.j_bar != nullptr;
-  for (uint32_t base = p0; base < p1; base += 64) {
+  {
     const uint32_t jj = base + lane;
     const bool valid = jj < p1;
     double4 pp = make_double4(0, 0, 0, 0), ww = make_double4(0, 0, 0, 0);
@@ -797,6 +815,7 @@ bool gather_fp32_uses_mfma();
 // One wave per 64 particles of a source element; target data through scalar
 // loads.  integral[perm[j]] receives I_p (including the shape normalizer).
 struct IntegralArgs {
+  ChunkList cl;
   int K;
   double R, invR, alpha, normalizer;
   double shift_len[3];
@@ -816,15 +835,14 @@ struct IntegralArgs {
 template <int D, int NP, bool A2>
 __global__ __launch_bounds__(64) void k_shape_integral(IntegralArgs g) {
   using CD = const double __attribute__((address_space(4))) *;
-  const int s = xcd_element(blockIdx.x);
-  if (s >= g.K) return;
-  const uint32_t p0 = g.seg[s], p1 = g.seg[s + 1];
-  if (p0 >= p1) return;
+  uint32_t s, base;
+  if (!chunk_of(g.cl, s, base)) return;
+  const uint32_t p1 = g.seg[s + 1];
   const int lane = threadIdx.x;
   const uint32_t t_begin = g.tr_start[s], t_end = g.tr_start[s + 1];
   const double two_inv_r = 2.0 * g.invR;
   const CD ntab_all = (CD)g.node_tab, iw = (CD)g.iw, jac = (CD)g.jac;
-  for (uint32_t base = p0; base < p1; base += 64) {
+  {
     const uint32_t jj = base + lane;
     const bool valid = jj < p1;
     const double4 pp = valid ? g.pos[jj] : make_double4(0, 0, 0, 0);
@@ -893,15 +911,14 @@ struct AverageArgs {
 template <int D, int NP, bool A2, bool VIS>
 __global__ __launch_bounds__(64) void k_shape_average(AverageArgs g) {
   using CD = const double __attribute__((address_space(4))) *;
-  const int s = xcd_element(blockIdx.x);
-  if (s >= g.b.K) return;
-  const uint32_t p0 = g.b.seg[s], p1 = g.b.seg[s + 1];
-  if (p0 >= p1) return;
+  uint32_t s, base;
+  if (!chunk_of(g.b.cl, s, base)) return;
+  const uint32_t p1 = g.b.seg[s + 1];
   const int lane = threadIdx.x;
   const uint32_t t_begin = g.b.tr_start[s], t_end = g.b.tr_start[s + 1];
   const double two_inv_r = 2.0 * g.b.invR;
   const CD ntab_all = (CD)g.b.node_tab, iw = (CD)g.b.iw, jac = (CD)g.b.jac;
-  for (uint32_t base = p0; base < p1; base += 64) {
+  {
     const uint32_t jj = base + lane;
     const bool valid = jj < p1;
     const double4 pp = valid ? g.b.pos[jj] : make_double4(0, 0, 0, 0);
@@ -961,7 +978,7 @@ __global__ __launch_bounds__(64) void k_shape_average(AverageArgs g) {
       if (lane == __builtin_ffsll((long long)zb) - 1)
         atomicAdd(g.zeros, (unsigned long long)__popcll(zb));
     }
-    if (!valid) continue;
+    if (!valid) return;
     const uint32_t pi = g.b.perm[jj];
     const double qv = g.q[pi];
     if (g.b.integral) g.b.integral[pi] = g.b.normalizer * aI;
@@ -1010,8 +1027,10 @@ __global__ __launch_bounds__(64) void k_shape_average(AverageArgs g) {
 // Launchers, one per translation unit.  precision: PYR_FP64 or PYR_FP32C.
 void launch_gather_fp64(int D, int Np, const GatherArgs &ga, int K, bool a2, bool count);
 void launch_gather_fp32(int D, int Np, const GatherArgs &ga, int K, bool a2, bool count);
-void launch_deposit_adj(int D, int Np, const AdjArgs &ga, int K, int precision, bool a2);
-void launch_shape_integral(int D, int Np, const IntegralArgs &ga, int K, bool a2);
-void launch_shape_average(int D, int Np, const AverageArgs &ga, int K, bool a2, bool vis);
+void launch_deposit_adj(int D, int Np, const AdjArgs &ga, unsigned grid, int precision,
+                        bool a2);
+void launch_shape_integral(int D, int Np, const IntegralArgs &ga, unsigned grid, bool a2);
+void launch_shape_average(int D, int Np, const AverageArgs &ga, unsigned grid, bool a2,
+                          bool vis);
 
 }  // namespace pyr

@@ -132,8 +132,11 @@ void reduce(const RedArgs &a, double *sums) {
   g_partials.reserve(sizeof(double) * ((size_t)RED_BLOCKS * MAX_Q + MAX_Q));
   double *part = g_partials.as<double>();
   hipStream_t s = stream();
-  hipLaunchKernelGGL(k_reduce, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, a, part);
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, s, a.nq, part);
+  {
+    ProfScope prof(PROF_DIAG);
+    hipLaunchKernelGGL(k_reduce, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, a, part);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, s, a.nq, part);
+  }
   PYR_CHECK_LAUNCH();
   PYR_HIP(hipMemcpyAsync(sums, part + (size_t)RED_BLOCKS * MAX_Q, sizeof(double) * a.nq,
                          hipMemcpyDeviceToHost, s));

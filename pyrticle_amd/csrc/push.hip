@@ -996,11 +996,8 @@ int pyr_lsrk4_steps_average(pyr_mesh *mesh, int nsteps, double dt, double vacuum
       for (int s = 0; s < 5; ++s) {
         int rc = pyr_velocities(n, D, vacuum_c, p, mass, v);
         if (rc) throw Error(rc, pyr_last_error());
-        {
-          ProfScope prof(PROF_PUSH);
-          average_forces_impl(mesh, radius, alpha, normalizer, n, D, fields, x, elem, q, v,
-                              integral_weights, F, nullptr, nullptr);
-        }
+        average_forces_impl(mesh, radius, alpha, normalizer, n, D, fields, x, elem, q, v,
+                            integral_weights, F, nullptr, nullptr);
         if (D == 2)
           hipLaunchKernelGGL(k_avg_stage<2>, dim3(blocks_for(n)), dim3(256), 0, stream(),
                              mesh->dev, n, s, dt, x, p, rx, rp, v, F, elem, death, counters, err,

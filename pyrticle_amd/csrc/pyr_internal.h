@@ -55,7 +55,10 @@ enum {
   PROF_COMPACT = 3,
   PROF_ADJOINT = 4,
   PROF_DEPOSIT_ADJ = 5,
-  PROF_NREGIONS = 6
+  PROF_INTEGRAL = 6,     // shape integrals of the normalized deposition
+  PROF_AVERAGE = 7,      // shape-weighted field gather of the averaging pusher
+  PROF_DIAG = 8,         // diagnostics reductions
+  PROF_NREGIONS = 9
 };
 struct ProfScope {
   int region;

@@ -400,7 +400,7 @@ def apply_moves(buf: torch.Tensor, frm: torch.Tensor, to: torch.Tensor):
 
 
 PROFILE_REGIONS = {"push": 0, "gather": 1, "sort": 2, "compact": 3, "adjoint": 4,
-                   "deposit_adjoint": 5}
+                   "deposit_adjoint": 5, "integral": 6, "average": 7, "diagnostics": 8}
 
 
 def profile_enable(on: bool = True):
