@@ -340,6 +340,16 @@ def extensions_leg(wl, dm, device, torch, st, normalizer, reps, avg_steps, pairs
         rho_n[name] = rho
         norm[name] = {"ms_per_call": 1e3 * t_norm, "plain_deposit_ms_per_call": 1e3 * t_plain,
                       "shape_integral_kernel_ms": k_int, "zero_mass_particles": nz}
+    # shape integral: per passing (particle, target) pair, Np x (t: D FMAs + 1
+    # sub, square, x w + add) = 8 Np flops at D = 3
+    flops_int = (5.0 + wl.ma.dim) * Np * pairs_passed
+    k_int = norm["fp64"]["shape_integral_kernel_ms"]
+    tf_int = flops_int / (k_int * 1e-3) / 1e12 if k_int else 0.0
+    norm["shape_integral_roofline"] = {
+        "bound": "valu", "kernel": "k_shape_integral (FP64)", "achieved": tf_int,
+        "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tf_int / FP64_PEAK_TFLOPS,
+        "traffic": measured_traffic("k_shape_integral"), "flops_per_launch": flops_int,
+        "flops_per_pair": (5 + wl.ma.dim) * Np}
     qsum = float(q.sum())
     norm["charge_rel_err_fp64"] = abs(wl.discr.integral(rho_n["fp64"].cpu().numpy())
                                       / qsum - 1)
@@ -363,8 +373,10 @@ def extensions_leg(wl, dm, device, torch, st, normalizer, reps, avg_steps, pairs
     avg = {"forces_ms_per_call": 1e3 * t_avg, "gather_kernel_ms": k_avg,
            "roofline": {"bound": "valu", "kernel": "k_shape_average (FP64)", "achieved": tf,
                         "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                        "frac": tf / FP64_PEAK_TFLOPS, "flops_per_launch": flops,
-                        "pairs_passed": pairs_passed, "flops_per_pair": 22 * Np}}
+                        "frac": tf / FP64_PEAK_TFLOPS,
+                        "traffic": measured_traffic("k_shape_average"),
+                        "flops_per_launch": flops, "pairs_passed": pairs_passed,
+                        "flops_per_pair": 22 * Np}}
     st2 = clone_state(st)
     dm.lsrk4_steps_average(1, wl.dt, wl.c, wl.radius, wl.alpha, normalizer, w, st2["n"],
                            st2["x"], st2["p"], st2["q"], st2["m"], st2["elem"])
@@ -394,6 +406,7 @@ def extensions_leg(wl, dm, device, torch, st, normalizer, reps, avg_steps, pairs
         gbs = n * row_bytes / (k_d * 1e-3) / 1e9 if k_d else 0.0
         diag[name] = {"ms_per_call": 1e3 * wall_d, "kernel_ms": k_d, "bytes_per_particle":
                       row_bytes, "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+    diag["traffic_k_reduce"] = measured_traffic("k_reduce")
     res["diagnostics"] = diag
     log("diagnostics: energy spread %.3f ms kernel (%.0f GB/s)"
         % (diag["rms_energy_spread"]["kernel_ms"] or 0, diag["rms_energy_spread"]["hbm_gbs"]))
