@@ -333,25 +333,28 @@ class StatsGatherer:
         self._mean += delta / self._count
         self._m2 += delta * (x - self._mean)
 
-    def add_array(self, a):
-        """Adds every entry of a (numpy array or device tensor: reduced where
-        it lives)."""
+    @staticmethod
+    def moments(a):
+        """(count, mean, sum of squared deviations, min, max) of a numpy array
+        or device tensor, reduced where it lives; None when empty."""
         if isinstance(a, torch.Tensor):
             a = a.reshape(-1).to(torch.float64)
-            nb = a.numel()
-            if nb == 0:
-                return
+            if a.numel() == 0:
+                return None
             mean_t = a.mean()
             red = torch.stack([mean_t, ((a - mean_t) ** 2).sum(), a.min(), a.max()]).cpu()
-            mb, m2b, lo, hi = (float(t) for t in red)
-        else:
-            a = np.asarray(a, dtype=np.float64).ravel()
-            nb = a.size
-            if nb == 0:
-                return
-            mb = float(a.mean())
-            m2b = float(((a - mb) ** 2).sum())
-            lo, hi = float(a.min()), float(a.max())
+            return (a.numel(),) + tuple(float(t) for t in red)
+        a = np.asarray(a, dtype=np.float64).ravel()
+        if a.size == 0:
+            return None
+        mb = float(a.mean())
+        return a.size, mb, float(((a - mb) ** 2).sum()), float(a.min()), float(a.max())
+
+    def add_moments(self, mom):
+        """Merges a batch given by its moments (parallel variance formula)."""
+        if mom is None:
+            return
+        nb, mb, m2b, lo, hi = mom
         if self._count == 0:
             self._min, self._max = lo, hi
         else:
@@ -361,6 +364,10 @@ class StatsGatherer:
         self._mean += delta * nb / n
         self._m2 += m2b + delta * delta * self._count * nb / n
         self._count = n
+
+    def add_array(self, a):
+        """Adds every entry of a (numpy array or device tensor)."""
+        self.add_moments(self.moments(a))
 
     def count(self):
         return self._count
@@ -578,8 +585,10 @@ class _AveragingPusher:
             if n_zero and vis_listener is not None:
                 WarningListener.warn("average pusher: %d particles had zero reconstructed "
                                      "charge" % n_zero)
-        pusher_state.e_normalization_stats.add_array(pscale)
-        pusher_state.b_normalization_stats.add_array(pscale)
+        # both targets see the same reconstructed charge
+        mom = StatsGatherer.moments(pscale)
+        pusher_state.e_normalization_stats.add_moments(mom)
+        pusher_state.b_normalization_stats.add_moments(mom)
         if vis_listener is not None:
             for name, arr in res[3].items():
                 vis_listener.store_particle_vis_vector(name, arr)

@@ -79,3 +79,30 @@ def test_mesh_arrays_face_geometry():
         for vi in ma.el_vertices[e]:
             lst = ma.vadj_elements[ma.vadj_starts[vi]:ma.vadj_starts[vi + 1]]
             assert e in lst
+
+
+def test_stats_gatherer_matches_numpy():
+    """_internal.StatsGatherer (tools.hpp:340-400 stats_gatherer): one-at-a-time
+    adds and batch merges give numpy's moments, extremes and count."""
+    import numpy as np
+    import torch
+    from pyrticle_amd._internal import StatsGatherer
+    rng = np.random.default_rng(0)
+    a = rng.normal(3.0, 2.0, size=1000)
+    one = StatsGatherer()
+    for v in a:
+        one.add(v)
+    batched = StatsGatherer()
+    batched.add_array(a[:10])
+    batched.add_array(torch.from_numpy(a[10:600]))
+    batched.add_array(a[600:])
+    batched.add_array(np.zeros(0))
+    for st in (one, batched):
+        assert st.count() == a.size
+        assert abs(st.mean() - a.mean()) < 1e-13
+        assert abs(st.variance() / a.var() - 1) < 1e-12
+        assert abs(st.variance_sample() / a.var(ddof=1) - 1) < 1e-12
+        assert abs(st.standard_deviation_sample() / a.std(ddof=1) - 1) < 1e-12
+        assert st.minimum() == a.min() and st.maximum() == a.max()
+    one.reset()
+    assert one.count() == 0
