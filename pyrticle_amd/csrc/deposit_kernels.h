@@ -59,7 +59,10 @@ struct Kahan {
 // the 8 XCDs; chunks of XCD_CHUNK consecutive elements (spatial neighbours that
 // share candidate particles) go to one XCD and its L2, and the chunks rotate
 // over the XCDs so that a dense region is spread over all of them.
-constexpr int XCD_CHUNK = 32;
+#ifndef PYR_XCD_CHUNK
+#define PYR_XCD_CHUNK 32
+#endif
+constexpr int XCD_CHUNK = PYR_XCD_CHUNK;
 __device__ __forceinline__ int xcd_element(int bid) {
   const int xcd = bid & 7, slot = bid >> 3;
   const int chunk = (slot / XCD_CHUNK) * 8 + xcd;
