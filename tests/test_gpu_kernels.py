@@ -401,6 +401,77 @@ def test_average_forces_zero_charge(square, oracle_square, dmesh2, cuda):
         assert not arr.cpu().numpy()[dead].any(), name
 
 
+@pytest.fixture(scope="module")
+def periodic_beam(periodic_cylinder):
+    """Particles in the periodic cylinder (z period 2), half of them within R of
+    the periodic faces, so that their shapes wrap around."""
+    from pyrticle_amd.units import SIUnitsWithNaturalConstants as U
+    discr, ma = periodic_cylinder
+    om = O.OracleMesh(ma)
+    rng = np.random.default_rng(21)
+    n = 1500
+    r = 0.7 * np.sqrt(rng.uniform(size=n))
+    ph = rng.uniform(0, 2 * np.pi, n)
+    z = np.where(rng.uniform(size=n) < 0.5, rng.uniform(0.001, 0.25, n),
+                 rng.uniform(1.75, 1.999, n))
+    x = np.stack([r * np.cos(ph), r * np.sin(ph), z], axis=1)
+    el = np.array([O.find_containing_element(om, xi) for xi in x], dtype=np.uint32)
+    assert (el != 0xFFFFFFFF).all()
+    c = U.VACUUM_LIGHT_SPEED()
+    v = rng.normal(0, 0.1 * c, size=(n, 3))
+    q = np.full(n, -U.EL_CHARGE)
+    return discr, ma, om, x, el, v, q
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_deposits_periodic_match_oracle(periodic_beam, cuda, precision):
+    """Plain and normalized deposition with shapes that wrap through the
+    periodic faces (shift codes on the candidate lists)."""
+    from pyrticle_amd import device as dv
+    discr, ma, om, x, el, v, q = periodic_beam
+    dm = dv.DeviceMesh(ma)
+    R, alpha = 0.3, 2.0
+    nrm = O.shape_normalizer(R, 3, alpha)
+    prec = dv.FP64 if precision == "fp64" else dv.FP32C
+    tol = FP64_TOL if precision == "fp64" else FP32C_TOL
+    rho_o, j_o, _ = O.deposit(om, R, alpha, x, q, v, el)
+    rho, j = dm.deposit(R, alpha, nrm, _dev(x, cuda), _dev(q, cuda), dv.element_tensor(el, cuda),
+                        v=_dev(v, cuda), precision=prec)
+    assert rel_l2(rho.cpu().numpy(), rho_o) < tol and rel_l2(j.cpu().numpy(), j_o) < tol
+    w = discr.element_groups[0].local_discretization.integral_weights()
+    rho_o, j_o, s_o, nz_o = O.deposit_normalized(om, R, alpha, x, q, v, el, w)
+    scale = torch.empty(x.shape[0], dtype=torch.float64, device=cuda)
+    rho, j, nz = dm.deposit_normalized(R, alpha, nrm, _dev(x, cuda), _dev(q, cuda),
+                                       dv.element_tensor(el, cuda), w, v=_dev(v, cuda),
+                                       precision=prec, scale=scale)
+    assert nz == nz_o == 0
+    assert rel_l2(scale.cpu().numpy(), s_o) < FP64_TOL
+    assert rel_l2(rho.cpu().numpy(), rho_o) < tol and rel_l2(j.cpu().numpy(), j_o) < tol
+    # the periodic mesh has no boundary: the normalized deposit holds the charge
+    assert abs(discr.integral(rho.cpu().numpy()) / q.sum() - 1) < (1e-12 if precision == "fp64"
+                                                                   else 1e-6)
+
+
+def test_average_forces_periodic_match_oracle(periodic_beam, cuda):
+    from pyrticle_amd import device as dv
+    discr, ma, om, x, el, v, q = periodic_beam
+    dm = dv.DeviceMesh(ma)
+    R, alpha = 0.3, 2.0
+    w = discr.element_groups[0].local_discretization.integral_weights()
+    nodes = ma.nodes
+    fields = [1e3 * (1 + nodes[:, 0]), 2e2 * nodes[:, 1] * nodes[:, 0], 5e2 * np.sin(np.pi * nodes[:, 2]),
+              0.1 * nodes[:, 1], 0.2 + 0.05 * np.cos(np.pi * nodes[:, 2]), 0.5 - 0.3 * nodes[:, 0]]
+    f_o, charge_o, nz_o, vis_o = O.average_forces(om, R, alpha, x, q, v, el, w, fields, vis=True)
+    out = dm.average_forces(R, alpha, O.shape_normalizer(R, 3, alpha),
+                            [_dev(f, cuda) for f in fields], _dev(x, cuda),
+                            dv.element_tensor(el, cuda), _dev(q, cuda), _dev(v, cuda), w, vis=True)
+    assert out[2] == nz_o == 0
+    assert rel_l2(out[0].cpu().numpy(), f_o) < FP64_TOL
+    assert rel_l2(q * out[1].cpu().numpy(), charge_o) < FP64_TOL
+    for name in ("pt_e", "pt_b", "el_force", "mag_force"):
+        assert rel_l2(out[3][name].cpu().numpy(), vis_o[name]) < FP64_TOL, name
+
+
 def test_diagnostics_match_oracle(cuda):
     """The seven diagnostics on 1M particles against the oracle's sequential
     sums (summation order differs: sums within 1e-12, the cancelling emittance
