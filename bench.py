@@ -355,6 +355,9 @@ def extensions_leg(wl, dm, device, torch, st, normalizer, reps, avg_steps, pairs
                                       / qsum - 1)
     norm["rho_rel_l2_fp32c_vs_fp64"] = float(torch.linalg.norm(rho_n["fp32c"] - rho_n["fp64"])
                                              / torch.linalg.norm(rho_n["fp64"]))
+    # the face-walk statistics pass (instrumentation of the normalized depositor)
+    _, t_walk, _ = timed(lambda: dm.face_walk_stats(wl.radius, x, el))
+    norm["face_walk_stats_ms_per_call"] = 1e3 * t_walk
     res["normalized_deposit"] = norm
     log("normalized deposit: fp64 %.2f ms (plain %.2f), fp32c %.2f ms (plain %.2f)"
         % (norm["fp64"]["ms_per_call"], norm["fp64"]["plain_deposit_ms_per_call"],

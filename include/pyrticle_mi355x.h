@@ -309,6 +309,20 @@ int pyr_rms_beam_emittance(int64_t n, int xdim, int vdim, const double *position
 int pyr_rms_energy_spread(int64_t n, int vdim, double vacuum_c, const double *momenta,
                           const double *masses, double *out);
 
+/* The reference's face walk (element_finder.hpp:203-294 face_based_element_finder,
+ * as dep_normshape.hpp:131-136 and :175-177 gather statistics along it), for
+ * the normalized depositor's centroid_distance_stats and el_per_particle_stats:
+ * per particle (device arrays of n), the number of elements the walk visits
+ * (n_elements) and the sum, sum of squares, minimum and maximum of the
+ * distance from each visited element's vertex centroid to the shape centre
+ * (the periodic image the walk uses).  Deposition itself does not walk; this
+ * entry point exists for those statistics.  Fails if a walk visits more than
+ * 384 elements. */
+int pyr_face_walk_stats(pyr_mesh *mesh, double radius, int64_t n, const double *x,
+                        const uint32_t *containing_elements, uint32_t *n_elements,
+                        double *dist_sum, double *dist_sumsq, double *dist_min,
+                        double *dist_max);
+
 /* Adjoint of pyr_deposit (no reference counterpart): given rho_bar = dL/drho
  * (K*Np, may be NULL) and j_bar = dL/dj (K*Np x vdim, may be NULL), writes
  * x_bar = dL/dx (n x vdim) and, with j_bar, v_bar = dL/dv (n x vdim); both are

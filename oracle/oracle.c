@@ -316,6 +316,10 @@ typedef struct {
   double *rec_s;          /* recorded shape values */
   uint32_t *rec_el;       /* recorded elements */
   size_t rec_n, rec_cap;  /* recorded elements, capacity in elements */
+  /* walk statistics (dep_normshape.hpp:131-136): visits and centroid distances */
+  int walk;
+  uint64_t w_n;
+  double w_sum, w_sq, w_min, w_max;
 } dfs_ctx;
 
 /* dep_normshape.hpp:124-165 add_shape_on_element: store the element's shape
@@ -348,10 +352,32 @@ static int record_shape_on_element(dfs_ctx *c, const double *center, uint32_t en
   return 0;
 }
 
+/* meshdata.hpp:169-180 element_centroid: the vertex average               */
+static void element_centroid(const orc_mesh *m, uint32_t en, double *out) {
+  const int d = m->dim;
+  for (int i = 0; i < d; ++i) out[i] = m->vertices[(size_t)m->el_vertices[(size_t)en * (d + 1)] * d + i];
+  for (int k = 1; k <= d; ++k)
+    for (int i = 0; i < d; ++i)
+      out[i] += m->vertices[(size_t)m->el_vertices[(size_t)en * (d + 1) + k] * d + i];
+  for (int i = 0; i < d; ++i) out[i] /= d + 1;
+}
+
 static void add_shape_on_element(dfs_ctx *c, const double *center, uint32_t en) {
   const orc_mesh *m = c->m;
   const int d = m->dim, np = m->Np;
   const size_t start = (size_t)en * np;
+  if (c->walk) {
+    double cen[3], s = 0;
+    element_centroid(m, en, cen);
+    for (int i = 0; i < d; ++i) s += (cen[i] - center[i]) * (cen[i] - center[i]);
+    const double dist = sqrt(s);
+    if (c->w_n == 0 || dist < c->w_min) c->w_min = dist;
+    if (c->w_n == 0 || dist > c->w_max) c->w_max = dist;
+    c->w_n++;
+    c->w_sum += dist;
+    c->w_sq += dist * dist;
+    return;
+  }
   if (c->iw) {
     if (record_shape_on_element(c, center, en)) c->depth = -1000000;   /* out of memory */
     return;
@@ -512,6 +538,39 @@ int orc_deposit_normalized(const orc_mesh *m, double radius, double alpha, int64
   free(c.rec_el);
   if (n_zero) *n_zero = zeros;
   return rc;
+}
+
+/* The face walk of the normalized depositor's statistics (dep_normshape.hpp
+ * :131-136 centroid_distance_stats, :175-177 el_per_particle_stats): per
+ * particle, the number of elements the walk visits (n_el) and the sum, sum of
+ * squares, minimum and maximum of |element centroid - shape centre| over those
+ * visits (0 when n_el = 0).                                                   */
+int orc_face_walk_stats(const orc_mesh *m, double radius, int64_t n, int xd, const double *x,
+                        const uint32_t *elem, uint32_t *n_el, double *dsum, double *dsq,
+                        double *dmin, double *dmax) {
+  dfs_ctx c;
+  memset(&c, 0, sizeof c);
+  c.m = m;
+  c.radius = radius;
+  c.walk = 1;
+  c.stamp = (int64_t *)malloc(sizeof(int64_t) * (size_t)m->K);
+  if (!c.stamp) return -1;
+  for (int k = 0; k < m->K; ++k) c.stamp[k] = -1;
+  for (int64_t pn = 0; pn < n; ++pn) {
+    double pos[3];
+    for (int i = 0; i < m->dim; ++i) pos[i] = x[pn * xd + i];
+    c.mark = pn;
+    c.w_n = 0;
+    c.w_sum = c.w_sq = c.w_min = c.w_max = 0;
+    recurse(&c, pos, elem[pn]);
+    n_el[pn] = (uint32_t)c.w_n;
+    dsum[pn] = c.w_sum;
+    dsq[pn] = c.w_sq;
+    dmin[pn] = c.w_min;
+    dmax[pn] = c.w_max;
+  }
+  free(c.stamp);
+  return 0;
 }
 
 /* push_average.hpp:38-443 averaging_particle_pusher::forces with the chained

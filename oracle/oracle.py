@@ -236,6 +236,22 @@ def forces(om, x, elem, q, v, fields, vis=False):
     return f
 
 
+def face_walk_stats(om, radius, x, elem):
+    """dep_normshape.hpp:131-136, 175-177 with element_finder.hpp:203-294:
+    per particle (n_el, sum, sum of squares, min, max of the centroid
+    distances of the visited elements)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    elem = np.ascontiguousarray(elem, dtype=np.uint32)
+    n, xd = x.shape
+    n_el = np.empty(n, dtype=np.uint32)
+    out = [np.empty(n) for _ in range(4)]
+    rc = lib().orc_face_walk_stats(om.ref, C.c_double(radius), C.c_int64(n), xd, _ptr(x),
+                                   _ptr(elem), _ptr(n_el), *[_ptr(a) for a in out])
+    if rc:
+        raise RuntimeError("face walk failed")
+    return (n_el, *out)
+
+
 def average_forces(om, radius, alpha, x, q, v, elem, integral_weights, fields,
                    normalized=False, vis=False):
     """push_average.hpp:38-443 (averaging pusher over the plain or the

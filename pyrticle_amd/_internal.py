@@ -401,8 +401,12 @@ class _NormalizingInterpolatingDepositor(_InterpolatingDepositor):
 
     class DepositorState(_InterpolatingDepositor.DepositorState):
         class _Stats:
+            """dep_normshape.hpp:42-47 normalized_deposition_stats."""
+
             def __init__(self):
                 self.normalization_stats = StatsGatherer()
+                self.centroid_distance_stats = StatsGatherer()
+                self.el_per_particle_stats = StatsGatherer()
 
         def __init__(self):
             self.stats = self._Stats()
@@ -414,6 +418,9 @@ class _NormalizingInterpolatingDepositor(_InterpolatingDepositor):
         _InterpolatingDepositor.__init__(self, mesh_data)
         mm = np.asarray(mass_matrix, dtype=np.float64)
         self.integral_weights = mm @ np.ones(mm.shape[1])
+        # the face-walk statistics (centroid distances, elements per particle)
+        # need a pass of their own; gathered when set
+        self.walk_stats = False
 
 
 class NormalizingInterpolatingDepositor2d2v(_NormalizingInterpolatingDepositor):
@@ -422,6 +429,22 @@ class NormalizingInterpolatingDepositor2d2v(_NormalizingInterpolatingDepositor):
 
 class NormalizingInterpolatingDepositor3d3v(_NormalizingInterpolatingDepositor):
     pass
+
+
+def _walk_stats(md, radius, x, elem, stats):
+    """dep_normshape.hpp:131-136, 175-177: the face walk's elements per particle
+    and centroid distances, merged into the depositor state's statistics."""
+    n_el, dsum, dsq, dmin, dmax = md.device.face_walk_stats(radius, x, elem)
+    stats.el_per_particle_stats.add_array(n_el)
+    visited = n_el > 0
+    count = int(n_el.sum())
+    if count == 0:
+        return
+    red = torch.stack([dsum.sum(), dsq.sum(), dmin[visited].min(), dmax[visited].max()]).cpu()
+    total, total_sq, lo, hi = (float(t) for t in red)
+    mean = total / count
+    stats.centroid_distance_stats.add_moments((count, mean, max(total_sq - count * mean * mean,
+                                                                0.0), lo, hi))
 
 
 def _slice_range(pslice, n):
@@ -451,6 +474,8 @@ def _deposit(dep, ps, node_count, velocities, pslice, want_rho, want_j, ds=None)
             want_j=want_j, precision=dep.precision, scale=scale)
         if ds is not None:
             ds.stats.normalization_stats.add_array(scale[scale != 0])
+            if dep.walk_stats:
+                _walk_stats(md, sf.radius, x, ps.containing_elements[a:b], ds.stats)
         if n_zero:
             # dep_normshape.hpp:175-183 warns once per such particle
             WarningListener.warn("deposited particle mass is zero (%d particles)" % n_zero)

@@ -72,6 +72,7 @@ _SIGNATURES = {
     "pyr_rms_energy_spread": (C.c_int, [_I64, C.c_int, _D, _P, _P, _P]),
     "pyr_lsrk4_steps_average": (C.c_int, [_P, C.c_int, _D, _D, _D, _D, _D, _P,
                                           C.POINTER(_I64), _P, _P, _P, _P, _P, _P, _P]),
+    "pyr_face_walk_stats": (C.c_int, [_P, _D, _I64, _P, _P, _P, _P, _P, _P, _P]),
     "pyr_deposit_set_counting": (C.c_int, [C.c_int]),
     "pyr_push_set_mfma": (C.c_int, [C.c_int]),
     "pyr_push_mfma_waves": (C.c_int, [C.POINTER(_I64)]),

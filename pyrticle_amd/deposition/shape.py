@@ -48,13 +48,16 @@ class NormalizedShapeFunctionDepositor(Depositor):
     in FP64 on the GPU, then the same gather deposition as ShapeFunctionDepositor
     with the scaled charges.  The depositor state's
     ``stats.normalization_stats`` gathers q / integral of every deposited
-    particle."""
+    particle; ``centroid_distance_stats`` and ``el_per_particle_stats`` follow
+    the reference's face walk, run as a pass of its own when ``walk_stats`` is
+    set (and from add_instrumentation on)."""
 
     name = "NormShape"
 
-    def __init__(self, precision="fp64"):
+    def __init__(self, precision="fp64", walk_stats=False):
         Depositor.__init__(self)
         self.precision = dv.FP64 if precision == "fp64" else dv.FP32C
+        self.walk_stats = walk_stats
 
     def initialize(self, method):
         Depositor.initialize(self, method)
@@ -64,6 +67,13 @@ class NormalizedShapeFunctionDepositor(Depositor):
                                             method.get_dimensionality_suffix())
         self.backend = backend_class(method.mesh_data, ldis.mass_matrix())
         self.backend.precision = self.precision
+        self.backend.walk_stats = self.walk_stats
+
+    def add_instrumentation(self, mgr, observer):
+        """shape.py:71-88 logs all three statistics: the face-walk ones
+        (centroid distances, elements per particle) are gathered from here on."""
+        Depositor.add_instrumentation(self, mgr, observer)
+        self.walk_stats = self.backend.walk_stats = True
 
     def make_state(self, state):
         return self.backend.DepositorState()

@@ -219,6 +219,17 @@ class DeviceMesh:
             _lib.ptr(rho), _lib.ptr(j), _lib.ptr(scale), C.byref(nz)))
         return rho, j, int(nz.value)
 
+    def face_walk_stats(self, radius, x, elem):
+        """Per particle: (elements visited by the reference's face walk, and
+        sum, sum of squares, min, max of their centroid distances)."""
+        _lib.sync_stream()
+        n = x.shape[0]
+        n_el = torch.empty(n, dtype=torch.int32, device=x.device)
+        out = [torch.empty(n, dtype=torch.float64, device=x.device) for _ in range(4)]
+        _lib.check(self._lib.pyr_face_walk_stats(self._h, radius, n, _lib.ptr(x), _lib.ptr(elem),
+                                                 _lib.ptr(n_el), *[_lib.ptr(t) for t in out]))
+        return (n_el, *out)
+
     def deposit_backward(self, radius, alpha, normalizer, x, q, elem, rho_bar=None, j_bar=None,
                          v=None, precision=FP64):
         """(x_bar, v_bar) from the adjoint densities; v_bar is None without j_bar."""

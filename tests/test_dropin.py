@@ -94,6 +94,19 @@ def test_normalized_depositor_dropin(square, oracle_square, cuda):
     assert rel_l2(j.cpu().numpy(), j_o.T) < 1e-12
     st = state.depositor_state.stats.normalization_stats
     assert st.count() == n
+    # the face-walk statistics, gathered once instrumentation asks for them
+    walk = state.depositor_state.stats
+    assert walk.el_per_particle_stats.count() == 0
+    dep.add_instrumentation(None, None)
+    dep.deposit_rho(state)
+    n_el, dsum, _, dmin, dmax = O.face_walk_stats(oracle_square, wl.radius, wl.x[:n], el)
+    assert walk.el_per_particle_stats.count() == n
+    assert walk.el_per_particle_stats.mean() == pytest.approx(n_el.mean(), rel=1e-12)
+    assert walk.centroid_distance_stats.count() == n_el.sum()
+    assert walk.centroid_distance_stats.mean() == pytest.approx(dsum.sum() / n_el.sum(),
+                                                                rel=1e-12)
+    assert walk.centroid_distance_stats.minimum() == pytest.approx(dmin.min(), rel=1e-13)
+    assert walk.centroid_distance_stats.maximum() == pytest.approx(dmax.max(), rel=1e-13)
     assert abs(st.mean() / s_o.mean() - 1) < 1e-12
     assert st.minimum() == pytest.approx(s_o.min(), rel=1e-12)
     assert st.standard_deviation() == pytest.approx(s_o.std(), rel=1e-9)

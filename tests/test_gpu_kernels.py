@@ -472,6 +472,30 @@ def test_average_forces_periodic_match_oracle(periodic_beam, cuda):
         assert rel_l2(out[3][name].cpu().numpy(), vis_o[name]) < FP64_TOL, name
 
 
+@pytest.mark.parametrize("case", ["square", "cylinder", "periodic"])
+def test_face_walk_stats_match_oracle(case, square, oracle_square, small_cylinder,
+                                      periodic_beam, dmesh2, dmesh3, cuda):
+    """The reference's face walk for the normalized depositor's statistics:
+    elements per particle bit-exact, centroid-distance sums, squares and
+    extremes within 1e-13 (periodic images included)."""
+    from pyrticle_amd import device as dv
+    if case == "square":
+        om, dm, x, R = oracle_square, dmesh2, square.x, square.radius
+        el = np.array([O.find_containing_element(om, xi) for xi in x], dtype=np.uint32)
+    elif case == "cylinder":
+        wl = small_cylinder
+        om, dm, x, R = O.OracleMesh(wl.ma), dmesh3, wl.x[:3000], wl.radius
+        el = dv.element_numpy(dm.find_containing_element(_dev(x, cuda)))
+    else:
+        _, ma, om, x, el, _, _ = periodic_beam
+        dm, R = dv.DeviceMesh(ma), 0.3
+    ref = O.face_walk_stats(om, R, x, el)
+    got = [t.cpu().numpy() for t in dm.face_walk_stats(R, _dev(x, cuda), dv.element_tensor(el, cuda))]
+    assert np.array_equal(got[0].astype(np.uint32), ref[0])
+    for k in range(1, 5):
+        assert np.allclose(got[k], ref[k], rtol=1e-13, atol=0), k
+
+
 def test_diagnostics_match_oracle(cuda):
     """The seven diagnostics on 1M particles against the oracle's sequential
     sums (summation order differs: sums within 1e-12, the cancelling emittance

@@ -366,6 +366,48 @@ def test_normalized_deposit_linearity_and_current(square, oracle_square):
     assert rel_l2(j, rho[:, None] * v[0][None, :]) < 1e-15
 
 
+def _bfs_walk(ma, R, x, e0):
+    """Elements reachable from e0 through faces that pass the walk's two tests
+    (plane distance <= R, face circle within R): breadth first, a set."""
+    d = ma.dim
+    seen = {int(e0)}
+    todo = [int(e0)]
+    while todo:
+        e = todo.pop()
+        for f in range(d + 1):
+            nb = int(ma.face_neighbor[e, f])
+            if nb == 0xFFFFFFFF or nb in seen:
+                continue
+            if abs(ma.face_normal[e, f] @ x - ma.face_rhs[e, f]) > R:
+                continue
+            if np.linalg.norm(ma.face_centroid[e, f] - x) > R + ma.face_radius[e, f]:
+                continue
+            seen.add(nb)
+            todo.append(nb)
+    return sorted(seen)
+
+
+@pytest.mark.parametrize("R", [0.1, 0.25])
+def test_face_walk_stats_equal_breadth_first_component(square, oracle_square, R):
+    """The depth-first face walk (element_finder.hpp:203-294) visits exactly the
+    face-connected set that passes its tests (order-free on a mesh without
+    periodicity): counts and centroid distances against a BFS over sets."""
+    wl = square
+    x = wl.x[:60]
+    el = _located(oracle_square, x)
+    n_el, dsum, dsq, dmin, dmax = O.face_walk_stats(oracle_square, R, x, el)
+    ma = wl.ma
+    cen = ma.vertices[ma.el_vertices].mean(1)
+    for i in range(x.shape[0]):
+        els = _bfs_walk(ma, R, x[i], el[i])
+        dist = np.linalg.norm(cen[els] - x[i], axis=1)
+        assert n_el[i] == len(els)
+        assert abs(dsum[i] - dist.sum()) < 1e-13 * dist.sum()
+        assert abs(dsq[i] - (dist ** 2).sum()) < 1e-13 * (dist ** 2).sum()
+        assert dmin[i] == pytest.approx(dist.min(), rel=1e-14)
+        assert dmax[i] == pytest.approx(dist.max(), rel=1e-14)
+
+
 # ---------------------------------------------------------------------------
 # averaging pusher (push_average.hpp)
 # ---------------------------------------------------------------------------
