@@ -107,6 +107,44 @@ def cpu_baseline(wl, st0, sample, steps):
     return alive / dt, dt, int(visits[0]) / max(1, n), n
 
 
+def cpu_baseline_threads(wl, st0, sample, steps, threads):
+    """The same bounded sample on all host cores: particle chunks stepped and
+    deposited by the oracle in parallel threads (ctypes releases the GIL),
+    per-thread rho/j buffers summed afterwards.  Particles are independent in
+    static fields, so the chunks need no communication."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle as O
+    from pyrticle_amd import device as dv
+    om = O.OracleMesh(wl.ma)
+    stride = max(1, st0["n"] // sample)
+    sl = slice(0, stride * sample, stride)
+    arrays = [np.ascontiguousarray(st0[k][sl].cpu().numpy()) for k in ("x", "p", "q", "m")]
+    el = np.ascontiguousarray(dv.element_numpy(st0["elem"][sl]))
+    bounds = np.linspace(0, arrays[0].shape[0], threads + 1).astype(int)
+    chunks = [[a[lo:hi].copy() for a in arrays] + [el[lo:hi].copy()]
+              for lo, hi in zip(bounds[:-1], bounds[1:])]
+
+    def work(ch):
+        x, p, q, m, e = ch
+        alive = 0
+        rho = j = None
+        for _ in range(steps):
+            n, _ = O.lsrk4_run(om, wl.fields, wl.c, x, p, q, m, e, wl.dt, 1)
+            x, p, q, m, e = x[:n], p[:n], q[:n], m[:n], e[:n]
+            v = O.velocities(p, m, wl.c)
+            rho, j, _ = O.deposit(om, wl.radius, wl.alpha, x, q, v, e)
+            alive += n
+        return alive, rho, j
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        results = list(ex.map(work, chunks))
+    rho = sum(r[1] for r in results)
+    dt = time.perf_counter() - t0
+    del rho
+    return sum(r[0] for r in results) / dt, dt
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of a kernel from the committed counter profile
     (FETCH_SIZE and WRITE_SIZE of rocprofv3 --pmc runs, each scaled by what the
@@ -569,6 +607,14 @@ def main(argv=None):
                          "on the full 200,910-element mesh, %.1f s" % (
                              args.cpu_sample, max(1, st0["n"] // args.cpu_sample),
                              args.cpu_steps, secs)}
+        threads = max(1, min(16, os.cpu_count() or 1))
+        rate_mt, secs_mt = cpu_baseline_threads(wl, st0, args.cpu_sample, args.cpu_steps,
+                                                threads)
+        cpu["all_cores"] = {"value": rate_mt, "cores": threads, "seconds": secs_mt,
+                            "note": "the same sample in %d particle chunks, one oracle "
+                                    "thread each, per-thread rho/j buffers" % threads}
+        log("cpu baseline: %.3e particle-steps/s on 1 core, %.3e on %d threads"
+            % (rate, rate_mt, threads))
 
     # ---- rooflines ----------------------------------------------------------
     gather_avg_s = gather_ms / max(1, gather_n) / 1e3
