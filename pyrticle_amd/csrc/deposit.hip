@@ -534,32 +534,38 @@ static const uint32_t *sort_by_element(pyr_mesh *h, DepositCache *c, int64_t n,
 // gathers' batches of consecutive particles spatially coherent (the FP64
 // gather's queued batches, the matrix-core gather's 64-particle steps, which
 // it skips when none of them can reach the target).
+// sorted: a permutation (and c->seg) left by an earlier sort of the same
+// positions and elements, which is then only restaged.
 static const uint32_t *sort_and_stage(pyr_mesh *mesh, DepositCache *c, int64_t n,
                                       const double *x, const double *q, const double *v,
                                       const uint32_t *elem, double normalizer,
                                       bool stage64 = true, bool stage32 = false,
-                                      double radius = 1.0, double alpha = 1.0) {
+                                      double radius = 1.0, double alpha = 1.0,
+                                      const uint32_t *sorted = nullptr) {
   const int K = mesh->K, D = mesh->dim;
   hipStream_t s = stream();
-  const uint32_t *keys = nullptr;
-  int shift = 0;
-  static const bool morton = [] {
-    const char *env = std::getenv("PYR_DEPOSIT_MORTON");
-    return !env || std::atoi(env) != 0;
-  }();
-  if (morton && (uint64_t)K < ((uint64_t)1 << (32 - MORTON_BITS))) {
-    c->skeys.reserve(sizeof(uint32_t) * (size_t)n);
-    if (D == 2)
-      hipLaunchKernelGGL(k_sort_keys<2>, dim3(blocks_for(n)), dim3(256), 0, s, n, K, elem, x,
-                         c->geom.as<double>(), c->skeys.as<uint32_t>());
-    else
-      hipLaunchKernelGGL(k_sort_keys<3>, dim3(blocks_for(n)), dim3(256), 0, s, n, K, elem, x,
-                         c->geom.as<double>(), c->skeys.as<uint32_t>());
-    PYR_CHECK_LAUNCH();
-    keys = c->skeys.as<uint32_t>();
-    shift = MORTON_BITS;
+  const uint32_t *perm = sorted;
+  if (!perm) {
+    const uint32_t *keys = nullptr;
+    int shift = 0;
+    static const bool morton = [] {
+      const char *env = std::getenv("PYR_DEPOSIT_MORTON");
+      return !env || std::atoi(env) != 0;
+    }();
+    if (morton && (uint64_t)K < ((uint64_t)1 << (32 - MORTON_BITS))) {
+      c->skeys.reserve(sizeof(uint32_t) * (size_t)n);
+      if (D == 2)
+        hipLaunchKernelGGL(k_sort_keys<2>, dim3(blocks_for(n)), dim3(256), 0, s, n, K, elem, x,
+                           c->geom.as<double>(), c->skeys.as<uint32_t>());
+      else
+        hipLaunchKernelGGL(k_sort_keys<3>, dim3(blocks_for(n)), dim3(256), 0, s, n, K, elem, x,
+                           c->geom.as<double>(), c->skeys.as<uint32_t>());
+      PYR_CHECK_LAUNCH();
+      keys = c->skeys.as<uint32_t>();
+      shift = MORTON_BITS;
+    }
+    perm = sort_by_element(mesh, c, n, elem, keys, shift);
   }
-  const uint32_t *perm = sort_by_element(mesh, c, n, elem, keys, shift);
   if (stage64) {
     c->pos.reserve(sizeof(double4) * (size_t)n);
     c->wgt.reserve(sizeof(double4) * (size_t)n);
@@ -702,7 +708,8 @@ __global__ void k_normalized_charges(int64_t n, int K, const uint32_t *__restric
 
 static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double normalizer,
                          int precision, int64_t n, int vdim, const double *x, const double *q,
-                         const double *v, const uint32_t *elem, double *rho, double *j) {
+                         const double *v, const uint32_t *elem, double *rho, double *j,
+                         const uint32_t *sorted = nullptr) {
   {
     PYR_REQUIRE(mesh, "null mesh");
     PYR_REQUIRE(radius > 0 && std::isfinite(radius), "shape radius must be positive");
@@ -726,7 +733,7 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
       if (n > 0) {
         const bool mf = precision == PYR_FP32C && gather_fp32_uses_mfma();
         sort_and_stage(mesh, c, n, x, q, j ? v : nullptr, elem, normalizer, !mf, mf, radius,
-                       alpha);
+                       alpha, sorted);
       } else {
         c->seg.reserve(sizeof(uint32_t) * ((size_t)K + 1));
         PYR_HIP(hipMemsetAsync(c->seg.ptr, 0, sizeof(uint32_t) * ((size_t)K + 1), s));
@@ -883,6 +890,7 @@ int pyr_deposit_normalized(pyr_mesh *mesh, double radius, double alpha, double n
     const int K = mesh->K, Np = mesh->Np, D = mesh->dim;
     hipStream_t s = stream();
     const double *qn = q;
+    const uint32_t *perm_kept = nullptr;
     if (n > 0) {
       prepare_tables(mesh, c, radius);
       if (c->tr_radius != radius) build_transpose(mesh, c);
@@ -930,8 +938,11 @@ int pyr_deposit_normalized(pyr_mesh *mesh, double radius, double alpha, double n
       PYR_HIP(hipStreamSynchronize(s));
       *n_zero_out = (int64_t)hz;
       qn = qs;
+      perm_kept = perm;
     }
-    deposit_impl(mesh, radius, alpha, normalizer, precision, n, vdim, x, qn, v, elem, rho, j);
+    // the deposit restages with the scaled charges on the same sort
+    deposit_impl(mesh, radius, alpha, normalizer, precision, n, vdim, x, qn, v, elem, rho, j,
+                 n > 0 ? perm_kept : nullptr);
   });
 }
 
