@@ -717,7 +717,7 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
     PYR_REQUIRE(precision == PYR_FP64 || precision == PYR_FP32C, "unknown precision");
     PYR_REQUIRE(vdim == mesh->dim, "vdim must equal the mesh dimension");
     PYR_REQUIRE(n >= 0 && n < (int64_t)0x7FFFFFFF, "invalid particle count");
-    PYR_REQUIRE(!j || v, "j requires velocities");
+    PYR_REQUIRE(!j || v || n == 0, "j requires velocities");
     if (!rho && !j) return;
     if (!mesh->dep) mesh->dep = new DepositCache();
     DepositCache *c = mesh->dep;
@@ -881,7 +881,7 @@ int pyr_deposit_normalized(pyr_mesh *mesh, double radius, double alpha, double n
     PYR_REQUIRE(precision == PYR_FP64 || precision == PYR_FP32C, "unknown precision");
     PYR_REQUIRE(vdim == mesh->dim, "vdim must equal the mesh dimension");
     PYR_REQUIRE(n >= 0 && n < (int64_t)0x7FFFFFFF, "invalid particle count");
-    PYR_REQUIRE(!j || v, "j requires velocities");
+    PYR_REQUIRE(!j || v || n == 0, "j requires velocities");
     PYR_REQUIRE(mesh->dev.jacobian, "the mesh was created without element jacobians");
     if (!rho && !j) return;
     if (n > 0) PYR_REQUIRE(x && q && elem && integral_weights, "null pointer");

@@ -496,6 +496,38 @@ def test_face_walk_stats_match_oracle(case, square, oracle_square, small_cylinde
         assert np.allclose(got[k], ref[k], rtol=1e-13, atol=0), k
 
 
+def test_new_entry_points_edge_cases(square, oracle_square, dmesh2, cuda):
+    """Empty inputs and particles without an element: zero outputs, no counts;
+    a radius that breaks the face walk's bound fails loudly."""
+    from pyrticle_amd import device as dv
+    wl = square
+    w = wl.discr.element_groups[0].local_discretization.integral_weights()
+    nrm = O.shape_normalizer(wl.radius, 2, wl.alpha)
+    x0 = torch.zeros((0, 2), dtype=torch.float64, device=cuda)
+    q0 = torch.zeros(0, dtype=torch.float64, device=cuda)
+    e0 = dv.element_tensor(np.zeros(0, dtype=np.uint32), cuda)
+    rho, j, nz = dmesh2.deposit_normalized(wl.radius, wl.alpha, nrm, x0, q0, e0, w, v=x0)
+    assert nz == 0 and not rho.abs().max() > 0 and not j.abs().max() > 0
+    f, integ, nz = dmesh2.average_forces(wl.radius, wl.alpha, nrm, [None] * 6, x0, e0, q0, x0, w)
+    assert f.shape == (0, 2) and nz == 0
+    assert dmesh2.face_walk_stats(wl.radius, x0, e0)[0].numel() == 0
+    # one located particle and one outside the mesh (element id INVALID)
+    x = np.array([[0.1, 0.2], [5.0, 5.0]])
+    el = np.array([O.find_containing_element(oracle_square, x[0]), 0xFFFFFFFF], dtype=np.uint32)
+    fields = [_dev(f_, cuda) if f_ is not None else None for f_ in wl.fields]
+    f, integ, nz = dmesh2.average_forces(wl.radius, wl.alpha, nrm, fields, _dev(x, cuda),
+                                         dv.element_tensor(el, cuda),
+                                         _dev(np.full(2, wl.q[0]), cuda),
+                                         _dev(np.full((2, 2), 1e6), cuda), w)
+    f = f.cpu().numpy()
+    assert nz == 0 and np.abs(f[0]).max() > 0 and not f[1].any()
+    assert integ.cpu().numpy()[1] == 0
+    n_el = dmesh2.face_walk_stats(wl.radius, _dev(x, cuda), dv.element_tensor(el, cuda))[0]
+    assert n_el.cpu().numpy()[1] == 0 and n_el.cpu().numpy()[0] > 0
+    with pytest.raises(ValueError, match="face walk"):
+        dmesh2.face_walk_stats(3.0, _dev(x[:1], cuda), dv.element_tensor(el[:1], cuda))
+
+
 def test_diagnostics_match_oracle(cuda):
     """The seven diagnostics on 1M particles against the oracle's sequential
     sums (summation order differs: sums within 1e-12, the cancelling emittance
