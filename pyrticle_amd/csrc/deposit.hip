@@ -978,8 +978,8 @@ int pyr_deposit_backward(pyr_mesh *mesh, double radius, double alpha, double nor
     PYR_REQUIRE(precision == PYR_FP64 || precision == PYR_FP32C, "unknown precision");
     PYR_REQUIRE(vdim == mesh->dim, "vdim must equal the mesh dimension");
     PYR_REQUIRE(n >= 0 && n < (int64_t)0x7FFFFFFF, "invalid particle count");
-    PYR_REQUIRE(!j_bar || (v && v_bar), "j_bar requires velocities and v_bar");
     if (n == 0) return;
+    PYR_REQUIRE(!j_bar || (v && v_bar), "j_bar requires velocities and v_bar");
     PYR_REQUIRE(x && q && elem && x_bar, "null pointer");
     if (!mesh->dep) mesh->dep = new DepositCache();
     DepositCache *c = mesh->dep;
