@@ -523,11 +523,15 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
       float s[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const float tw = __builtin_amdgcn_fmed3f(t[i], 0.f, 1.f);
-        if constexpr (A2)
-          s[i] = tw * tw;
-        else
+        if constexpr (A2) {
+          // max(t, 0)^2 as clamp(t |t|) to [0, 1]: one v_mul_f32 with an abs
+          // source and the clamp bit, the same bits as squaring the clamped t
+          // (t <= 1 up to rounding, and t > 1 gives 1 either way)
+          s[i] = __builtin_amdgcn_fmed3f(t[i] * fabsf(t[i]), 0.f, 1.f);
+        } else {
+          const float tw = __builtin_amdgcn_fmed3f(t[i], 0.f, 1.f);
           s[i] = tw > 0.f ? __powf(tw, (float)g.alpha) : 0.f;
+        }
       }
       acc[ng] = mfma4x4(P.w.x, s[0], acc[ng]);
       acc[ng] = mfma4x4(P.w.y, s[1], acc[ng]);
