@@ -767,7 +767,8 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
     const bool count = g_deposit_count_mode != 0;
     if (count) PYR_HIP(hipMemsetAsync(ga.stats, 0, 4 * sizeof(unsigned long long), s));
     ProfScope prof(PROF_GATHER);
-    if (precision == PYR_FP64) launch_gather_fp64(D, Np, ga, K, alpha == 2.0, count);
+    if (precision == PYR_FP64)
+      launch_gather_fp64(D, Np, ga, K, shape_unit_square(alpha, radius), count);
     else launch_gather_fp32(D, Np, ga, K, alpha == 2.0, count);
   }
 }
@@ -852,7 +853,7 @@ unsigned long long *average_forces_impl(pyr_mesh *mesh, double radius, double al
   aa.zeros = zeros;
   const unsigned grid = build_chunks(mesh, c, n, ia.cl);
   ProfScope prof(PROF_AVERAGE);
-  launch_shape_average(D, Np, aa, grid, alpha == 2.0, vis != nullptr);
+  launch_shape_average(D, Np, aa, grid, shape_unit_square(alpha, radius), vis != nullptr);
   return zeros;
 }
 
@@ -927,7 +928,7 @@ int pyr_deposit_normalized(pyr_mesh *mesh, double radius, double alpha, double n
       const unsigned grid = build_chunks(mesh, c, n, ia.cl);
       {
         ProfScope prof(PROF_INTEGRAL);
-        launch_shape_integral(D, Np, ia, grid, alpha == 2.0);
+        launch_shape_integral(D, Np, ia, grid, shape_unit_square(alpha, radius));
       }
       // charges q / I (dep_normshape.hpp:168-198); zero-mass particles drop out
       hipLaunchKernelGGL(k_normalized_charges, dim3(blocks_for(n)), dim3(256), 0, s, n, K, elem,

@@ -44,6 +44,20 @@ __device__ __forceinline__ void shift_vec(const GatherArgs &g, uint32_t code, do
   }
 }
 
+// max(t, 0)^2 as one multiply with an abs source and the clamp bit
+// (v_mul_f64 / v_mul_f32 ... |t| clamp): clamp(t |t|) to [0, 1] equals
+// max(t, 0)^2 bit for bit whenever t <= 1, and gives 0 for t = -inf.
+template <class T>
+__device__ __forceinline__ T sq_pos_unit(T t) {
+  return fmin(fmax(t * fabs(t), T(0)), T(1));
+}
+// The FP64 shape kernels take that form for alpha = 2 when R <= 1/2: there
+// t = R - |n - x|^2 / R <= R up to rounding, far below 1.  Otherwise they
+// square (alpha = 2) or call pow, with the same bits as before.
+inline bool shape_unit_square(double alpha, double radius) {
+  return alpha == 2.0 && radius <= 0.5;
+}
+
 // Kahan-compensated accumulator for the FP32 form
 struct Kahan {
   float s, c;
@@ -98,6 +112,7 @@ __device__ __forceinline__ bool chunk_of(const ChunkList &cl, uint32_t &s, uint3
 // evaluates full batches of 64 queued particles one particle per lane against
 // each wave's nodes (node data in scalar registers), accumulating rho and j
 // per node in registers.
+// A2: alpha = 2 with R <= 1/2 (shape_unit_square), squares by sq_pos_unit
 template <int D, int NP, int NPW, bool FP32, bool A2, bool COUNT>
 __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherArgs g) {
   using T = typename std::conditional<FP32, float, double>::type;
@@ -173,12 +188,13 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
           T t = rec[3] - nd[3];
 #pragma unroll
           for (int k = D - 1; k >= 0; --k) t = fma(nd[k], rec[k], t);
-          T w = t > T(0) ? t : T(0);
           T s;
-          if constexpr (A2)
-            s = w * w;
-          else
-            s = (T)pow((double)w, g.alpha);
+          if constexpr (A2) {
+            s = sq_pos_unit(t);
+          } else {
+            const T w = t > T(0) ? t : T(0);
+            s = g.alpha == 2.0 ? w * w : (T)pow((double)w, g.alpha);
+          }
           if constexpr (COUNT) n_in += t > T(0) ? 1 : 0;
           acc[i][0] = fma(rec[4], s, acc[i][0]);
 #pragma unroll
@@ -839,6 +855,7 @@ struct IntegralArgs {
   double *integral;           // n
 };
 
+// A2: alpha = 2 with R <= 1/2 (shape_unit_square), squares by sq_pos_unit
 template <int D, int NP, bool A2>
 __global__ __launch_bounds__(64) void k_shape_integral(IntegralArgs g) {
   using CD = const double __attribute__((address_space(4))) *;
@@ -872,7 +889,9 @@ __global__ __launch_bounds__(64) void k_shape_integral(IntegralArgs g) {
       const bool pass = valid && dd <= gt[3];
       if (__ballot(pass) == 0) continue;
       const double m[3] = {d[0] * two_inv_r, d[1] * two_inv_r, d[2] * two_inv_r};
-      const double c0 = fma(-dd, g.invR, g.R);
+      // A2: a lane that does not pass gets c0 = -inf, so every t is -inf and
+      // its square term 0
+      const double c0 = A2 && !pass ? -HUGE_VAL : fma(-dd, g.invR, g.R);
       const CD nt = ntab_all + (size_t)t * NP * 4;
       double el_integral = 0;
 #pragma unroll
@@ -880,8 +899,13 @@ __global__ __launch_bounds__(64) void k_shape_integral(IntegralArgs g) {
         double tt = c0 - nt[i * 4 + 3];
 #pragma unroll
         for (int k = D - 1; k >= 0; --k) tt = fma(nt[i * 4 + k], m[k], tt);
-        const double w = (pass && tt > 0.0) ? tt : 0.0;
-        const double sv = A2 ? w * w : (w > 0.0 ? pow(w, g.alpha) : 0.0);
+        double sv;
+        if constexpr (A2) {
+          sv = sq_pos_unit(tt);
+        } else {
+          const double w = (pass && tt > 0.0) ? tt : 0.0;
+          sv = w > 0.0 ? (g.alpha == 2.0 ? w * w : pow(w, g.alpha)) : 0.0;
+        }
         el_integral = fma(sv, iw[i], el_integral);
       }
       integral = fma(el_integral, jac[t], integral);
@@ -915,6 +939,7 @@ struct AverageArgs {
   unsigned long long *zeros;        // count of located particles with I = 0
 };
 
+// A2: alpha = 2 with R <= 1/2 (shape_unit_square), squares by sq_pos_unit
 template <int D, int NP, bool A2, bool VIS>
 __global__ __launch_bounds__(64) void k_shape_average(AverageArgs g) {
   using CD = const double __attribute__((address_space(4))) *;
@@ -948,7 +973,9 @@ __global__ __launch_bounds__(64) void k_shape_average(AverageArgs g) {
       const bool pass = valid && dd <= gt[3];
       if (__ballot(pass) == 0) continue;
       const double m[3] = {d[0] * two_inv_r, d[1] * two_inv_r, d[2] * two_inv_r};
-      const double c0 = fma(-dd, g.b.invR, g.b.R);
+      // A2: a lane that does not pass gets c0 = -inf, so every t is -inf and
+      // its square term 0
+      const double c0 = A2 && !pass ? -HUGE_VAL : fma(-dd, g.b.invR, g.b.R);
       const CD nt = ntab_all + (size_t)t * NP * 4;
       // c_i = S_i w_i for this element's nodes
       double cw[NP];
@@ -958,8 +985,13 @@ __global__ __launch_bounds__(64) void k_shape_average(AverageArgs g) {
         double tt = c0 - nt[i * 4 + 3];
 #pragma unroll
         for (int k = D - 1; k >= 0; --k) tt = fma(nt[i * 4 + k], m[k], tt);
-        const double w = (pass && tt > 0.0) ? tt : 0.0;
-        const double sv = A2 ? w * w : (w > 0.0 ? pow(w, g.b.alpha) : 0.0);
+        double sv;
+        if constexpr (A2) {
+          sv = sq_pos_unit(tt);
+        } else {
+          const double w = (pass && tt > 0.0) ? tt : 0.0;
+          sv = w > 0.0 ? (g.b.alpha == 2.0 ? w * w : pow(w, g.b.alpha)) : 0.0;
+        }
         cw[i] = sv * iw[i];
         eI += cw[i];
       }

@@ -45,8 +45,10 @@ bool gather_fp32_uses_mfma() {
 
 void launch_gather_fp32(int D, int Np, const GatherArgs &ga, int K, bool a2, bool count) {
   if (!gather_fp32_uses_mfma()) {
-    if (D == 3 && Np == 20) return launch_valu<3, 20, 5>(ga, K, a2, count);
-    if (D == 2 && Np == 10) return launch_valu<2, 10, 10>(ga, K, a2, count);
+    // the VALU form's squares take t <= 1 (shape_unit_square)
+    const bool unit = a2 && ga.R <= 0.5;
+    if (D == 3 && Np == 20) return launch_valu<3, 20, 5>(ga, K, unit, count);
+    if (D == 2 && Np == 10) return launch_valu<2, 10, 10>(ga, K, unit, count);
     throw Error(PYR_ERR_INVALID_ARG, "the VALU FP32 gather is built for P3 elements only");
   }
   // the matrix-core form evaluates every candidate pair; it has no pair
