@@ -372,6 +372,45 @@ def test_average_forces_2d_matches_oracle(square, oracle_square, dmesh2, cuda):
         assert err < 1e-6 * np.abs(ref_scale).max(), name
 
 
+@pytest.mark.parametrize("radius,alpha", [(0.6, 2.0), (0.15, 3.0), (0.15, 1.5)])
+def test_shape_radius_and_exponent(square, oracle_square, dmesh2, cuda, radius, alpha):
+    """Shapes beside the FP64 kernels' alpha = 2, R <= 1/2 form (one clamped
+    multiply per node): R > 1/2 squares explicitly, alpha != 2 calls pow.
+    Plain deposit (FP64, and FP32C, whose general-alpha path uses the FP32
+    pow: within 1e-5), normalized deposit and averaging forces (FP64)."""
+    from pyrticle_amd import device as dv
+    wl = square
+    n = 1500
+    x, q = wl.x[:n], wl.q[:n]
+    el = np.array([O.find_containing_element(oracle_square, xi) for xi in x], dtype=np.uint32)
+    v = O.velocities(wl.p[:n], wl.m[:n], wl.c)
+    nrm = O.shape_normalizer(radius, 2, alpha)
+    rho_o, j_o, _ = O.deposit(oracle_square, radius, alpha, x, q, v, el)
+    for prec, tol in ((dv.FP64, FP64_TOL), (dv.FP32C, 1e-6 if alpha == 2.0 else 1e-5)):
+        rho, j = dmesh2.deposit(radius, alpha, nrm, _dev(x, cuda), _dev(q, cuda),
+                                dv.element_tensor(el, cuda), v=_dev(v, cuda), precision=prec)
+        assert rel_l2(rho.cpu().numpy(), rho_o) < tol, prec
+        assert rel_l2(j.cpu().numpy(), j_o) < tol, prec
+    w = wl.discr.element_groups[0].local_discretization.integral_weights()
+    rho_o, j_o, s_o, nz_o = O.deposit_normalized(oracle_square, radius, alpha, x, q, v, el, w)
+    rho, j, nz = dmesh2.deposit_normalized(radius, alpha, nrm, _dev(x, cuda), _dev(q, cuda),
+                                           dv.element_tensor(el, cuda), w, v=_dev(v, cuda),
+                                           precision=dv.FP64)
+    assert nz == nz_o
+    assert rel_l2(rho.cpu().numpy(), rho_o) < FP64_TOL
+    assert rel_l2(j.cpu().numpy(), j_o) < FP64_TOL
+    fields = _poly_fields(wl.ma.nodes, 2)
+    fields[2] = fields[3] = fields[4] = None
+    ref = O.average_forces(oracle_square, radius, alpha, x, q, v, el, w, fields, vis=False)
+    out = dmesh2.average_forces(radius, alpha, nrm, [None if f is None else _dev(f, cuda)
+                                                     for f in fields],
+                                _dev(x, cuda), dv.element_tensor(el, cuda), _dev(q, cuda),
+                                _dev(v, cuda), w, vis=False)
+    assert out[2] == ref[2]
+    assert rel_l2(q * out[1].cpu().numpy(), ref[1]) < FP64_TOL
+    assert rel_l2(out[0].cpu().numpy(), ref[0]) < FP64_TOL
+
+
 def test_average_forces_3d_matches_oracle(small_cylinder, dmesh3, cuda):
     from pyrticle_amd import device as dv
     wl = small_cylinder
