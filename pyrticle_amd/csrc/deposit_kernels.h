@@ -416,8 +416,9 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
-  __shared__ double red[W][NG * 4][4];
-  __shared__ float2 kahan[W][NG * 4][64];   // per-lane compensated sums (s, c)
+  // per-lane compensated sums (s, c); at the end each wave's per-node results
+  // overwrite its own slots (20 KB per two-wave block: 8 blocks per CU)
+  __shared__ float2 kahan[W][NG * 4][64];
 
   const double cx = g.geom[(size_t)e * 8 + 0], cy = g.geom[(size_t)e * 8 + 1],
                cz = g.geom[(size_t)e * 8 + 2];
@@ -659,7 +660,10 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
   }
   fold();
 
-  // combine: sixteen blocks by a fixed butterfly, then the waves in order
+  // combine: sixteen blocks by a fixed butterfly, then the waves in order.
+  // The result of node 4 ng + l, component v, goes (as the bits of a double)
+  // into slot [ng * 4 + v][l] of the wave's own Kahan array, which every lane
+  // of this wave has just read in the same instruction stream.
 #pragma unroll
   for (int ng = 0; ng < NG; ++ng)
 #pragma unroll
@@ -668,7 +672,7 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
       double val = (double)k.x - (double)k.y;
 #pragma unroll
       for (int o = 4; o < 64; o <<= 1) val += __shfl_xor(val, o, 64);
-      if (lane < 4) red[wave][4 * ng + lane][v] = val;
+      if (lane < 4) kl[(ng * 4 + v) * 64] = __builtin_bit_cast(float2, val);
     }
   __syncthreads();
   if (tid < NP) {
@@ -676,7 +680,8 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
 #pragma unroll
     for (int w = 0; w < W; ++w)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) out[v] += red[w][tid][v];
+      for (int v = 0; v < 4; ++v)
+        out[v] += __builtin_bit_cast(double, kahan[w][(tid >> 2) * 4 + v][tid & 3]);
     const size_t gn = (size_t)e * NP + tid;
     if (g.rho) g.rho[gn] = out[0];
     if (g.j)
