@@ -243,6 +243,9 @@ struct DepositCache {
   DevBuf skeys;   // Morton sort keys
   DevBuf chunks;         // per-particle gathers: 64-particle chunks (element, start),
                          // per-element chunk counts/offsets, total
+  // the last sort's permutation (in vals_a or vals_b) and its particle count
+  const uint32_t *last_perm = nullptr;
+  int64_t last_perm_n = -1;
   std::vector<double> h_geom;   // K x 8
 };
 
@@ -521,13 +524,27 @@ static const uint32_t *sort_by_element(pyr_mesh *h, DepositCache *c, int64_t n,
     key_shift = 0;
   }
   const uint32_t *sorted = nullptr;
+  c->last_perm_n = -1;
   const uint32_t *perm = radix_sort(c, n, keys, (uint32_t)K << key_shift, c->keys_a, c->keys_b,
                                     c->vals_a, c->vals_b, &sorted);
   hipLaunchKernelGGL(k_seg_from_sorted, dim3(blocks_for((int64_t)K + 1)), dim3(256), 0, s, K, n,
                      sorted, key_shift, c->seg.as<uint32_t>());
   PYR_CHECK_LAUNCH();
+  c->last_perm = perm;
+  c->last_perm_n = n;
   return perm;
 }
+
+namespace pyr {
+const uint32_t *deposit_last_order(pyr_mesh *mesh, int64_t n) {
+  static const bool on = [] {
+    const char *env = std::getenv("PYR_PUSH_ORDER");
+    return !env || std::atoi(env) != 0;
+  }();
+  if (!on || !mesh || !mesh->dep || n <= 0 || mesh->dep->last_perm_n != n) return nullptr;
+  return mesh->dep->last_perm;
+}
+}  // namespace pyr
 
 // Sorts the particles by (element, Morton code inside the element) and stages
 // them in that order; returns the permutation.  The Morton sub-order makes the

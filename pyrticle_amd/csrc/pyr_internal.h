@@ -344,6 +344,13 @@ void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *
 // Opaque handle behind the ABI's pyr_mesh.
 struct pyr_mesh;
 namespace pyr {
+// deposit.hip: the permutation (sorted position -> particle index) left by the
+// last sort of particles by element on this mesh, if it sorted n particles,
+// else null (also when PYR_PUSH_ORDER=0).  Any permutation of [0, n) is a
+// valid processing order for the fused push, whose particles are independent;
+// the last deposition's sort groups the particles of each element into
+// consecutive lanes, so a wave mostly sees one element (the FP64 push uses it).
+const uint32_t *deposit_last_order(pyr_mesh *mesh, int64_t n);
 // deposit.hip: the averaging pusher's shape-weighted forces, stream-ordered
 // (returns the device count of particles with a zero shape integral, or null
 // when n = 0).  vis: six arrays (pt_e, pt_b, pt_e_stddev, pt_b_stddev,

@@ -628,6 +628,44 @@ def test_lsrk4_square_matches_oracle(square, oracle_square, dmesh2, cuda, dt_sca
     assert tuple(int(c) for c in counters.cpu()) == cnt_o
 
 
+@pytest.mark.parametrize("dt_scale", [1.0, 4.0])
+def test_lsrk4_in_deposition_order_matches_oracle(square, oracle_square, cuda, dt_scale):
+    """A deposit before every step, as in a PIC loop: the fused push then steps
+    the particles in the order of the deposition's element sort (the square's
+    particles are not element-sorted, so the order is a real shuffle; a step
+    with kills drops it).  Ids, kill order, element ids, trajectories and find
+    counters as the oracle's."""
+    import dataclasses
+    from pyrticle_amd import device as dv
+    wl = dataclasses.replace(square, dt=square.dt * dt_scale)
+    dm = dv.DeviceMesh(wl.ma)
+    el = np.array([O.find_containing_element(oracle_square, xi) for xi in wl.x],
+                  dtype=np.uint32)
+    steps, n0 = (30 if dt_scale == 1.0 else 100), wl.x.shape[0]
+    ox, op, oq, om_, oel = wl.x.copy(), wl.p.copy(), wl.q.copy(), wl.m.copy(), el.copy()
+    oids = np.arange(n0, dtype=np.int64)
+    n_o, cnt_o = O.lsrk4_run(oracle_square, wl.fields, wl.c, ox, op, oq, om_, oel, wl.dt,
+                             steps, ids=oids)
+    dm.set_static_fields([None if f is None else _dev(f, cuda) for f in wl.fields])
+    tx, tp, tq, tm = (_dev(a, cuda) for a in (wl.x, wl.p, wl.q, wl.m))
+    tel = dv.element_tensor(el, cuda)
+    tids = torch.arange(n0, dtype=torch.int64, device=cuda)
+    counters = torch.zeros(4, dtype=torch.int64, device=cuda)
+    nrm = O.shape_normalizer(wl.radius, 2, wl.alpha)
+    n = n0
+    for _ in range(steps):
+        dm.deposit(wl.radius, wl.alpha, nrm, tx[:n], tq[:n], tel[:n])
+        n = dm.lsrk4_steps(1, wl.dt, wl.c, n, tx, tp, tq, tm, tel, ids=tids, counters=counters)
+    assert n == n_o
+    if dt_scale > 1:
+        assert n < n0
+    assert np.array_equal(tids[:n].cpu().numpy(), oids[:n_o])
+    assert np.array_equal(dv.element_numpy(tel[:n]), oel[:n_o])
+    assert rel_l2(tx[:n].cpu().numpy(), ox[:n_o]) < FP64_TOL
+    assert rel_l2(tp[:n].cpu().numpy(), op[:n_o]) < FP64_TOL
+    assert tuple(int(c) for c in counters.cpu()) == cnt_o
+
+
 @pytest.fixture(scope="module")
 def dense_p4():
     """Small P4 cylinder (1,368 tets, Np=35) filled with ~100 particles per
