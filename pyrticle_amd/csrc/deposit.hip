@@ -112,6 +112,17 @@ __global__ void k_seg_from_sorted(int K, int64_t n, const uint32_t *__restrict__
   seg[e] = (uint32_t)lo;
 }
 
+// Block -> range of sorted positions for the staging kernels: chunks of
+// XCD_CHUNK consecutive ranges go to one XCD (the hardware deals consecutive
+// blocks round-robin to the 8 XCDs), so the shuffled reads of one element's
+// particles come through one L2 instead of all eight.  A bijection on
+// [0, nblocks): the blocks past the last whole group of 8 chunks map to
+// themselves.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks) {
+  const int64_t full = nblocks / (8 * XCD_CHUNK) * (8 * XCD_CHUNK);
+  return b < full ? (int64_t)xcd_element((int)b) : b;
+}
+
 // staged particles in element-sorted order, two arrays of double4:
 //   pos[j] = (x0, x1, x2, 0)          read for every candidate visit
 //   wgt[j] = (a, a*v0, a*v1, a*v2)    read only for particles that pass,
@@ -120,7 +131,7 @@ __global__ void k_stage(int D, int64_t n, const uint32_t *__restrict__ perm,
                         const double *__restrict__ x, const double *__restrict__ q,
                         const double *__restrict__ v, double normalizer,
                         double4 *__restrict__ pos, double4 *__restrict__ wgt) {
-  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t j = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const uint32_t i = perm[j];
   double xs[3], bs[3];
@@ -150,7 +161,7 @@ __global__ void k_stage32(int D, int K, int64_t n, const uint32_t *__restrict__ 
                           const double *__restrict__ q, const double *__restrict__ v,
                           double scale, const double *__restrict__ geom,
                           float4 *__restrict__ pos, float *__restrict__ wgt_t) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t j = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (j >= ((n + 3) & ~(int64_t)3)) return;
   float xs[3] = {0.f, 0.f, 0.f}, w[4] = {0.f, 0.f, 0.f, 0.f};
   if (j < n) {
