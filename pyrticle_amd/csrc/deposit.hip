@@ -112,17 +112,6 @@ __global__ void k_seg_from_sorted(int K, int64_t n, const uint32_t *__restrict__
   seg[e] = (uint32_t)lo;
 }
 
-// Block -> range of sorted positions for the staging kernels: chunks of
-// XCD_CHUNK consecutive ranges go to one XCD (the hardware deals consecutive
-// blocks round-robin to the 8 XCDs), so the shuffled reads of one element's
-// particles come through one L2 instead of all eight.  A bijection on
-// [0, nblocks): the blocks past the last whole group of 8 chunks map to
-// themselves.
-__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks) {
-  const int64_t full = nblocks / (8 * XCD_CHUNK) * (8 * XCD_CHUNK);
-  return b < full ? (int64_t)xcd_element((int)b) : b;
-}
-
 // staged particles in element-sorted order, two arrays of double4:
 //   pos[j] = (x0, x1, x2, 0)          read for every candidate visit
 //   wgt[j] = (a, a*v0, a*v1, a*v2)    read only for particles that pass,

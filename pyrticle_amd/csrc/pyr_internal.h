@@ -342,6 +342,23 @@ void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *
 }  // namespace pyr
 
 // Opaque handle behind the ABI's pyr_mesh.
+namespace pyr {
+// Block b of a grid of nblocks -> the block index it handles, such that chunks
+// of 32 consecutive indices go to one XCD (the hardware deals consecutive
+// blocks round-robin to the 8 XCDs).  Kernels that read particles in a
+// shuffled order (the staging after the element sort, the push in that order)
+// then fetch each element's particles through one L2 instead of all eight.  A
+// bijection on [0, nblocks): the blocks past the last whole group of 8 chunks
+// map to themselves.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks) {
+  constexpr int64_t C = 32;
+  const int64_t full = nblocks / (8 * C) * (8 * C);
+  if (b >= full) return b;
+  const int64_t xcd = b & 7, slot = b >> 3;
+  return ((slot / C) * 8 + xcd) * C + slot % C;
+}
+}  // namespace pyr
+
 struct pyr_mesh;
 namespace pyr {
 // deposit.hip: the permutation (sorted position -> particle index) left by the
@@ -349,7 +366,7 @@ namespace pyr {
 // else null (also when PYR_PUSH_ORDER=0).  Any permutation of [0, n) is a
 // valid processing order for the fused push, whose particles are independent;
 // the last deposition's sort groups the particles of each element into
-// consecutive lanes, so a wave mostly sees one element (the FP64 push uses it).
+// consecutive lanes, so a wave mostly sees one element.
 const uint32_t *deposit_last_order(pyr_mesh *mesh, int64_t n);
 // deposit.hip: the averaging pusher's shape-weighted forces, stream-ordered
 // (returns the device count of particles with a zero shape integral, or null
