@@ -341,7 +341,6 @@ void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *
 
 }  // namespace pyr
 
-// Opaque handle behind the ABI's pyr_mesh.
 namespace pyr {
 // Block b of a grid of nblocks -> the block index it handles, such that chunks
 // of 32 consecutive indices go to one XCD (the hardware deals consecutive
@@ -359,6 +358,7 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks) {
 }
 }  // namespace pyr
 
+// Opaque handle behind the ABI's pyr_mesh.
 struct pyr_mesh;
 namespace pyr {
 // deposit.hip: the permutation (sorted position -> particle index) left by the
