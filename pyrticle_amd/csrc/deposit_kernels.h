@@ -399,8 +399,11 @@ __device__ __forceinline__ pyr_v4f mfma4x4(float a, float b, pyr_v4f c) {
 // relative margin of the FP32 reach test (it may only keep extra particles)
 constexpr float MF_MARGIN = 1e-4f;
 
+// 16 steps (64 particles per 4x4 block) per fold: against 8, the same error on
+// the coarse test mesh (5.55e-7, dominated by the FP32 distance evaluation)
+// and 1.4% less gather time on the cylinder workload
 #ifndef PYR_MF_FOLD
-#define PYR_MF_FOLD 8
+#define PYR_MF_FOLD 16
 #endif
 constexpr int FOLD_STEPS = PYR_MF_FOLD;
 
