@@ -471,22 +471,22 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
   for (int ng = 0; ng < NG; ++ng) cinit[ng] = pyr_v4f{cc[ng], cc[ng], cc[ng], cc[ng]};
   pyr_v4f acc[NG];
   // Kahan sums (s, c) of every lane's accumulator entries live in LDS, each
-  // lane touching only its own entries (no barriers needed)
+  // lane touching only its own entries (no barriers needed).  They start at
+  // zero: the first fold reads (0, 0) instead of LDS, so a wave that never
+  // evaluates a step (most targets hold no candidate particle) touches no
+  // Kahan slot.
   float2 *kl = &kahan[wave][0][lane];
 #pragma unroll
-  for (int ng = 0; ng < NG; ++ng) {
-    acc[ng] = pyr_v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int v = 0; v < 4; ++v) kl[(ng * 4 + v) * 64] = make_float2(0.f, 0.f);
-  }
+  for (int ng = 0; ng < NG; ++ng) acc[ng] = pyr_v4f{0.f, 0.f, 0.f, 0.f};
   int n_steps = 0;
+  bool evaluated = false, folded = false;   // wave-uniform
 
   auto fold = [&]() {
 #pragma unroll
     for (int ng = 0; ng < NG; ++ng)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        float2 k = kl[(ng * 4 + v) * 64];
+        float2 k = folded ? kl[(ng * 4 + v) * 64] : make_float2(0.f, 0.f);
         const float y = acc[ng][v] - k.y;
         const float t = k.x + y;
         k.y = (t - k.x) - y;
@@ -495,6 +495,7 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
       }
 #pragma unroll
     for (int ng = 0; ng < NG; ++ng) acc[ng] = pyr_v4f{0.f, 0.f, 0.f, 0.f};
+    folded = true;
   };
 
   // particle data of one chunk: m = 2 d / R (3), c = (R - |d|^2 / R) / P, and
@@ -534,6 +535,7 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
   auto evaluate = [&](const Part &P) {
     // a step none of whose particles can reach the target adds nothing
     if (__ballot(P.live) == 0) return;
+    evaluated = true;
 #pragma unroll
     for (int ng = 0; ng < NG; ++ng) {
       pyr_v4f t = mfma4x4(P.mx, bx[ng], cinit[ng]);
@@ -661,22 +663,27 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
       }
     }
   }
-  fold();
-
   // combine: sixteen blocks by a fixed butterfly, then the waves in order.
   // The result of node 4 ng + l, component v, goes (as the bits of a double)
   // into slot [ng * 4 + v][l] of the wave's own Kahan array, which every lane
-  // of this wave has just read in the same instruction stream.
+  // of this wave has just read in the same instruction stream.  A wave that
+  // evaluated nothing has zero sums and writes +0.0 there directly.
+  if (evaluated) {
+    fold();
 #pragma unroll
-  for (int ng = 0; ng < NG; ++ng)
+    for (int ng = 0; ng < NG; ++ng)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const float2 k = kl[(ng * 4 + v) * 64];
-      double val = (double)k.x - (double)k.y;
+      for (int v = 0; v < 4; ++v) {
+        const float2 k = kl[(ng * 4 + v) * 64];
+        double val = (double)k.x - (double)k.y;
 #pragma unroll
-      for (int o = 4; o < 64; o <<= 1) val += __shfl_xor(val, o, 64);
-      if (lane < 4) kl[(ng * 4 + v) * 64] = __builtin_bit_cast(float2, val);
-    }
+        for (int o = 4; o < 64; o <<= 1) val += __shfl_xor(val, o, 64);
+        if (lane < 4) kl[(ng * 4 + v) * 64] = __builtin_bit_cast(float2, val);
+      }
+  } else if (lane < 4) {
+#pragma unroll
+    for (int r = 0; r < NG * 4; ++r) kl[r * 64] = __builtin_bit_cast(float2, 0.0);
+  }
   __syncthreads();
   if (tid < NP) {
     double out[4] = {0.0, 0.0, 0.0, 0.0};
