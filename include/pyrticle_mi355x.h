@@ -132,6 +132,12 @@ int pyr_compact_plan(int64_t n, const uint8_t *dead, uint32_t *move_from,
 int pyr_apply_moves(void *buffer, int64_t elem_bytes, int64_t n_moves,
                     const uint32_t *move_from, const uint32_t *move_to);
 
+/* Frees the library-level scratch that pyr_compact_plan grows on demand (the
+ * per-mesh scratch goes with pyr_mesh_destroy).  No counterpart in the
+ * reference, whose kill loop allocates nothing on a device; call it when no
+ * other library call is in flight. */
+int pyr_release_scratch(void);
+
 /* push_monomial.hpp:219-325 monomial_particle_pusher::forces
  * (wrap_pusher.cpp:101-119) with make_interpolator (:155-210): interpolates
  * the six nodal fields (device, K*Np each; NULL = ZeroVector) at the particle

@@ -447,13 +447,17 @@ def _walk_stats(md, radius, x, elem, stats):
                                                                 0.0), lo, hi))
 
 
-def _slice_range(pslice, n):
-    if pslice is None:
-        return 0, n
-    start, stop, step = pslice.indices(n)
-    if step != 1:
-        raise ValueError("only contiguous particle slices are supported")
-    return start, stop
+def _sliced(pslice, n, arrays):
+    """The particles of a Python slice over [0, n) (tools.hpp:38-61 iterates
+    start, start + step, ... as the slice's indices() give them): contiguous
+    views for step 1, else the selected rows gathered into contiguous device
+    buffers.  The deposited sums do not depend on the particle order."""
+    start, stop, step = (0, n, 1) if pslice is None else pslice.indices(n)
+    if step == 1:
+        stop = max(start, stop)
+        return [None if t is None else t[start:stop] for t in arrays]
+    idx = torch.arange(start, stop, step, dtype=torch.int64, device=arrays[0].device)
+    return [None if t is None else t.index_select(0, idx).contiguous() for t in arrays]
 
 
 def _deposit(dep, ps, node_count, velocities, pslice, want_rho, want_j, ds=None):
@@ -463,26 +467,25 @@ def _deposit(dep, ps, node_count, velocities, pslice, want_rho, want_j, ds=None)
     md = dep.mesh_data
     if node_count != md.node_count():
         raise ValueError("node count does not match the mesh")
-    a, b = _slice_range(pslice, ps.particle_count)
-    x = ps.positions[a:b]
-    v = velocities[a:b].contiguous() if velocities is not None else None
+    x, q, el, v = _sliced(pslice, ps.particle_count,
+                          [ps.positions, ps.charges, ps.containing_elements, velocities])
+    if v is not None:
+        v = v.contiguous()
     if isinstance(dep, _NormalizingInterpolatingDepositor):
-        scale = torch.empty(b - a, dtype=torch.float64, device=x.device)
+        scale = torch.empty(x.shape[0], dtype=torch.float64, device=x.device)
         rho, j, n_zero = md.device.deposit_normalized(
-            sf.radius, sf.exponent, sf.normalizer, x, ps.charges[a:b],
-            ps.containing_elements[a:b], dep.integral_weights, v=v, want_rho=want_rho,
-            want_j=want_j, precision=dep.precision, scale=scale)
+            sf.radius, sf.exponent, sf.normalizer, x, q, el, dep.integral_weights, v=v,
+            want_rho=want_rho, want_j=want_j, precision=dep.precision, scale=scale)
         if ds is not None:
             ds.stats.normalization_stats.add_array(scale[scale != 0])
             if dep.walk_stats:
-                _walk_stats(md, sf.radius, x, ps.containing_elements[a:b], ds.stats)
+                _walk_stats(md, sf.radius, x, el, ds.stats)
         if n_zero:
             # dep_normshape.hpp:175-183 warns once per such particle
             WarningListener.warn("deposited particle mass is zero (%d particles)" % n_zero)
         return rho, j
-    rho, j = md.device.deposit(sf.radius, sf.exponent, sf.normalizer, x, ps.charges[a:b],
-                               ps.containing_elements[a:b], v=v, want_rho=want_rho,
-                               want_j=want_j, precision=dep.precision)
+    rho, j = md.device.deposit(sf.radius, sf.exponent, sf.normalizer, x, q, el, v=v,
+                               want_rho=want_rho, want_j=want_j, precision=dep.precision)
     return rho, j
 
 

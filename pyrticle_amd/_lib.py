@@ -53,6 +53,7 @@ _SIGNATURES = {
     "pyr_locate": (C.c_int, [_P, _I64, _P, _P, _P, _P, _P]),
     "pyr_compact_plan": (C.c_int, [_I64, _P, _P, _P, C.POINTER(_I64), C.POINTER(_I64)]),
     "pyr_apply_moves": (C.c_int, [_P, _I64, _I64, _P, _P]),
+    "pyr_release_scratch": (C.c_int, []),
     "pyr_forces": (C.c_int, [_P, _I64, C.c_int, _P * 6, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "pyr_set_static_fields": (C.c_int, [_P, _P * 6]),
     "pyr_lsrk4_steps": (C.c_int, [_P, C.c_int, C.c_int, _D, _D, C.POINTER(_I64), _P, _P, _P,
@@ -151,6 +152,39 @@ def ptr(t) -> int | None:
     if not t.is_contiguous():
         raise ValueError("expected a contiguous tensor")
     return t.data_ptr()
+
+
+# element types the C ABI reads, by kind; u32 buffers may be int32 tensors
+# holding the same bits
+_KINDS = {"f64": ("float64",), "u32": ("int32", "uint32"), "i64": ("int64",),
+          "u8": ("uint8", "bool")}
+
+
+def arg(t, kind: str, rows: int | None = None, cols: int | None = None,
+        name: str = "array") -> int | None:
+    """Checked device address of a particle or mesh array (None stays None).
+
+    kind: element type the C side reads ("f64", "u32", "i64", "u8"); rows: the
+    entries the call touches along the first axis (the tensor may hold more);
+    cols: the required second dimension (None: any shape whose total size
+    covers rows).  A mismatch raises ValueError before any device code runs,
+    so the kernel never reads a buffer with the wrong element size or length."""
+    if t is None:
+        return None
+    want = _KINDS[kind]
+    got = str(t.dtype).replace("torch.", "")
+    if got not in want:
+        raise ValueError("%s: expected dtype %s, got %s" % (name, " or ".join(want), got))
+    if cols is not None:
+        if t.dim() != 2 or t.shape[1] != cols:
+            raise ValueError("%s: expected shape (n, %d), got %s" % (name, cols,
+                                                                   tuple(t.shape)))
+        if rows is not None and t.shape[0] < rows:
+            raise ValueError("%s: expected at least %d rows, got %d" % (name, rows,
+                                                                        t.shape[0]))
+    elif rows is not None and t.numel() < rows:
+        raise ValueError("%s: expected at least %d entries, got %d" % (name, rows, t.numel()))
+    return ptr(t)
 
 
 def sync_stream():

@@ -160,19 +160,6 @@ class FieldToParticleRhsCalculator:
             0])
 
 
-class ParticleToFieldRhsCalculator:
-    """cloud.py:225-234: the particles' contribution to the field RHS,
-    dE/dt = -j / epsilon, with j deposited by the method's depositor."""
-
-    def __init__(self, method, maxwell_op):
-        self.method = method
-        self.maxwell_op = maxwell_op
-
-    def __call__(self, t, fields_f, state_f):
-        j = self.method.deposit_j(state_f())
-        return self.maxwell_op.assemble_fields(e=-1 / self.maxwell_op.epsilon * j)
-
-
 class ParticleRhsCalculator:
     """cloud.py:293-311: dx/dt = v."""
 
@@ -189,7 +176,9 @@ class ParticleRhsCalculator:
 
 
 class ParticleToFieldRhsCalculator:
-    """cloud.py:225-233: the field RHS contribution -j/epsilon."""
+    """cloud.py:225-233: the particles' contribution to the field RHS,
+    dE/dt = -j / epsilon, with j deposited by the method's depositor (one
+    device array per component)."""
 
     def __init__(self, method, maxwell_op):
         self.method = method

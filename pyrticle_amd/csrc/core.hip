@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "pyr_internal.h"
 
@@ -577,6 +578,21 @@ int pyr_locate(pyr_mesh *mesh, int64_t n, double *x, const double *p, uint32_t *
   });
 }
 
+// Scratch of pyr_compact_plan (scan of the dead flags, holes, dead count): one
+// library-level set, grown on demand, serialized by its mutex, freed by
+// pyr_release_scratch.
+static struct {
+  std::mutex mu;
+  DevBuf excl, tmp, holes, small;
+} g_plan;
+
+int pyr_release_scratch(void) {
+  return guarded([&] {
+    std::lock_guard<std::mutex> lock(g_plan.mu);
+    for (DevBuf *b : {&g_plan.excl, &g_plan.tmp, &g_plan.holes, &g_plan.small}) b->release();
+  });
+}
+
 int pyr_compact_plan(int64_t n, const uint8_t *dead, uint32_t *move_from, uint32_t *move_to,
                      int64_t *n_new_out, int64_t *n_moves_out) {
   return guarded([&] {
@@ -588,7 +604,8 @@ int pyr_compact_plan(int64_t n, const uint8_t *dead, uint32_t *move_from, uint32
       return;
     }
     PYR_REQUIRE(dead && move_from && move_to, "null pointer");
-    static thread_local DevBuf excl, tmp, holes, small;
+    std::lock_guard<std::mutex> lock(g_plan.mu);
+    DevBuf &excl = g_plan.excl, &tmp = g_plan.tmp, &holes = g_plan.holes, &small = g_plan.small;
     excl.reserve(sizeof(uint32_t) * (size_t)n);
     tmp.reserve(sizeof(uint32_t) * scan_tmp_words(n));
     holes.reserve(sizeof(uint32_t) * (size_t)n);

@@ -537,11 +537,7 @@ static const uint32_t *sort_by_element(pyr_mesh *h, DepositCache *c, int64_t n,
 
 namespace pyr {
 const uint32_t *deposit_last_order(pyr_mesh *mesh, int64_t n) {
-  static const bool on = [] {
-    const char *env = std::getenv("PYR_PUSH_ORDER");
-    return !env || std::atoi(env) != 0;
-  }();
-  if (!on || !mesh || !mesh->dep || n <= 0 || mesh->dep->last_perm_n != n) return nullptr;
+  if (!mesh || !mesh->dep || n <= 0 || mesh->dep->last_perm_n != n) return nullptr;
   return mesh->dep->last_perm;
 }
 }  // namespace pyr
@@ -565,11 +561,7 @@ static const uint32_t *sort_and_stage(pyr_mesh *mesh, DepositCache *c, int64_t n
   if (!perm) {
     const uint32_t *keys = nullptr;
     int shift = 0;
-    static const bool morton = [] {
-      const char *env = std::getenv("PYR_DEPOSIT_MORTON");
-      return !env || std::atoi(env) != 0;
-    }();
-    if (morton && (uint64_t)K < ((uint64_t)1 << (32 - MORTON_BITS))) {
+    if ((uint64_t)K < ((uint64_t)1 << (32 - MORTON_BITS))) {
       c->skeys.reserve(sizeof(uint32_t) * (size_t)n);
       if (D == 2)
         hipLaunchKernelGGL(k_sort_keys<2>, dim3(blocks_for(n)), dim3(256), 0, s, n, K, elem, x,
@@ -748,7 +740,7 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
     {
       ProfScope prof(PROF_SORT);
       if (n > 0) {
-        const bool mf = precision == PYR_FP32C && gather_fp32_uses_mfma();
+        const bool mf = precision == PYR_FP32C;
         sort_and_stage(mesh, c, n, x, q, j ? v : nullptr, elem, normalizer, !mf, mf, radius,
                        alpha, sorted);
       } else {

@@ -58,17 +58,6 @@ inline bool shape_unit_square(double alpha, double radius) {
   return alpha == 2.0 && radius <= 0.5;
 }
 
-// Kahan-compensated accumulator for the FP32 form
-struct Kahan {
-  float s, c;
-  __device__ __forceinline__ void add(float y) {
-    float yy = y - c;
-    float t = s + yy;
-    c = (t - s) - yy;
-    s = t;
-  }
-};
-
 // Block -> element map.  The hardware deals consecutive blocks round-robin to
 // the 8 XCDs; chunks of XCD_CHUNK consecutive elements (spatial neighbours that
 // share candidate particles) go to one XCD and its L2, and the chunks rotate
@@ -105,7 +94,7 @@ __device__ __forceinline__ bool chunk_of(const ChunkList &cl, uint32_t &s, uint3
   return true;
 }
 
-// Gather per target element.  A block holds G waves, each owning NPW of the
+// FP64 gather per target element.  A block holds G waves, each owning NPW of the
 // element's nodes.  The block streams the candidate particles, drops those
 // whose ball cannot reach the element (bounding sphere of the nodes inflated
 // by R, face planes moved out by R), queues the rest in an LDS ring buffer and
@@ -113,9 +102,9 @@ __device__ __forceinline__ bool chunk_of(const ChunkList &cl, uint32_t &s, uint3
 // each wave's nodes (node data in scalar registers), accumulating rho and j
 // per node in registers.
 // A2: alpha = 2 with R <= 1/2 (shape_unit_square), squares by sq_pos_unit
-template <int D, int NP, int NPW, bool FP32, bool A2, bool COUNT>
+template <int D, int NP, int NPW, bool A2, bool COUNT>
 __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherArgs g) {
-  using T = typename std::conditional<FP32, float, double>::type;
+  using T = double;
   constexpr int G = (NP + NPW - 1) / NPW;   // waves per block = node groups
   constexpr int BS = 64 * G;
   constexpr int QCAP = 64 + BS;             // ring buffer of queued particles
@@ -147,29 +136,13 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
   const CT ntab = (CT)(static_cast<const T *>(g.node_tab)) + (size_t)e * NP * 4;
 
   T acc[NPW][NV];
-  Kahan kacc[FP32 ? NPW : 1][FP32 ? NV : 1];
 #pragma unroll
   for (int i = 0; i < NPW; ++i)
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      acc[i][v] = T(0);
-      if constexpr (FP32) kacc[i][v] = Kahan{0.f, 0.f};
-    }
+    for (int v = 0; v < NV; ++v) acc[i][v] = T(0);
   unsigned long long n_eval = 0, n_in = 0, n_cand = 0, n_pass = 0;
   int q_head = 0, q_count = 0;   // identical in every thread of the block
   bool any = false;
-  constexpr int FOLD = 8;        // FP32: batches per compensated fold
-  int n_fold = 0;
-
-  auto fold = [&]() {
-#pragma unroll
-    for (int i = 0; i < NPW; ++i)
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        kacc[i][v].add(acc[i][v]);
-        acc[i][v] = T(0);
-      }
-  };
 
   auto process_batch = [&](int count) {
     // one queued particle per lane, all of this wave's nodes
@@ -202,14 +175,6 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
         }
       }
       if constexpr (COUNT) n_eval += (unsigned long long)min(NPW, NP - node0);
-    }
-    if constexpr (FP32) {
-      // plain FP32 partials over at most FOLD particles per lane, then a
-      // compensated fold
-      if (++n_fold == FOLD) {
-        n_fold = 0;
-        fold();
-      }
     }
   };
 
@@ -305,7 +270,6 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
     any = true;
     process_batch(q_count);
   }
-  if constexpr (FP32) fold();
 
   // reduce across lanes in a fixed tree and write this wave's nodes
 #pragma unroll
@@ -315,11 +279,7 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
     double out[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-      double val;
-      if constexpr (FP32)
-        val = any ? (double)kacc[i][v].s - (double)kacc[i][v].c : 0.0;
-      else
-        val = any ? (double)acc[i][v] : 0.0;
+      double val = any ? acc[i][v] : 0.0;
       if (any)
         for (int o = 32; o > 0; o >>= 1) val += __shfl_xor(val, o, 64);
       out[v] = val;
@@ -838,10 +798,6 @@ __global__ __launch_bounds__(64) void k_deposit_adj(AdjArgs g) {
   }
 }
 
-
-// Whether the FP32-compensated gather runs on the matrix cores
-// (PYR_GATHER_MFMA, default 1; it then reads the FP32 staging).
-bool gather_fp32_uses_mfma();
 
 // ---------------------------------------------------------------------------
 // per-particle shape integral of the normalized deposition

@@ -363,7 +363,7 @@ struct pyr_mesh;
 namespace pyr {
 // deposit.hip: the permutation (sorted position -> particle index) left by the
 // last sort of particles by element on this mesh, if it sorted n particles,
-// else null (also when PYR_PUSH_ORDER=0).  Any permutation of [0, n) is a
+// else null.  Any permutation of [0, n) is a
 // valid processing order for the fused push, whose particles are independent;
 // the last deposition's sort groups the particles of each element into
 // consecutive lanes, so a wave mostly sees one element.

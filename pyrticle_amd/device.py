@@ -89,19 +89,31 @@ class DeviceMesh:
     def handle(self):
         return self._h
 
+    def _field_ptrs(self, fields):
+        nn = self.K * self.Np
+        return (C.c_void_p * 6)(*[None if f is None else _lib.arg(f, "f64", nn, None, "field")
+                                  for f in fields])
+
+    def _state_ptrs(self, n, x, p, q, m, elem):
+        return (_lib.arg(x, "f64", n, self.dim, "x"), _lib.arg(p, "f64", n, self.dim, "p"),
+                _lib.arg(q, "f64", n, None, "q"), _lib.arg(m, "f64", n, None, "m"),
+                _lib.arg(elem, "u32", n, None, "elem"))
+
     # -- lookup -----------------------------------------------------------
     def find_containing_element(self, x: torch.Tensor) -> torch.Tensor:
         _lib.sync_stream()
         n = x.shape[0]
         out = torch.empty(n, dtype=torch.int32, device=x.device)
-        _lib.check(self._lib.pyr_find_containing_element(self._h, n, _lib.ptr(x), _lib.ptr(out)))
+        _lib.check(self._lib.pyr_find_containing_element(
+            self._h, n, _lib.arg(x, "f64", n, self.dim, "x"), _lib.ptr(out)))
         return out
 
     def is_in_element(self, elem: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
         _lib.sync_stream()
         n = x.shape[0]
         out = torch.empty(n, dtype=torch.uint8, device=x.device)
-        _lib.check(self._lib.pyr_is_in_element(self._h, n, _lib.ptr(elem), _lib.ptr(x),
+        _lib.check(self._lib.pyr_is_in_element(self._h, n, _lib.arg(elem, "u32", n, None, "elem"),
+                                               _lib.arg(x, "f64", n, self.dim, "x"),
                                                _lib.ptr(out)))
         return out
 
@@ -111,8 +123,11 @@ class DeviceMesh:
         _lib.sync_stream()
         n = x.shape[0]
         dead = torch.empty(n, dtype=torch.uint8, device=x.device)
-        _lib.check(self._lib.pyr_locate(self._h, n, _lib.ptr(x), _lib.ptr(p), _lib.ptr(elem),
-                                        _lib.ptr(dead), _lib.ptr(counters)))
+        _lib.check(self._lib.pyr_locate(self._h, n, _lib.arg(x, "f64", n, self.dim, "x"),
+                                        _lib.arg(p, "f64", n, self.dim, "p"),
+                                        _lib.arg(elem, "u32", n, None, "elem"),
+                                        _lib.ptr(dead), _lib.arg(counters, "i64", 4, None,
+                                                                 "counters")))
         return dead
 
     def forces(self, fields, x, elem, q, v, vis=False):
@@ -121,9 +136,10 @@ class DeviceMesh:
         F = torch.empty((n, vd), dtype=torch.float64, device=x.device)
         vis_t = [torch.empty((n, 3), dtype=torch.float64, device=x.device)
                  for _ in range(4)] if vis else [None] * 4
-        fp = (C.c_void_p * 6)(*[None if f is None else _lib.ptr(f) for f in fields])
-        _lib.check(self._lib.pyr_forces(self._h, n, vd, fp, _lib.ptr(x), _lib.ptr(elem),
-                                        _lib.ptr(q), _lib.ptr(v), _lib.ptr(F),
+        fp = self._field_ptrs(fields)
+        _lib.check(self._lib.pyr_forces(self._h, n, vd, fp, _lib.arg(x, "f64", n, self.dim, "x"),
+                                        _lib.arg(elem, "u32", n, None, "elem"),
+                                        _lib.arg(q, "f64", n, None, "q"), _lib.ptr(v), _lib.ptr(F),
                                         *[_lib.ptr(t) for t in vis_t]))
         if vis:
             return F, dict(zip(("pt_e", "pt_b", "el_force", "mag_force"), vis_t))
@@ -145,11 +161,12 @@ class DeviceMesh:
             if vis else [None] * 6
         w = torch.as_tensor(np.asarray(integral_weights, dtype=np.float64), device=dev)
         assert w.shape == (self.Np,), "integral weights need one entry per element node"
-        fp = (C.c_void_p * 6)(*[None if f is None else _lib.ptr(f) for f in fields])
+        fp = self._field_ptrs(fields)
         nz = C.c_int64(0)
         _lib.check(self._lib.pyr_average_forces(
-            self._h, radius, alpha, normalizer, n, vd, fp, _lib.ptr(x), _lib.ptr(elem),
-            _lib.ptr(q), _lib.ptr(v), _lib.ptr(w), _lib.ptr(F), _lib.ptr(integral),
+            self._h, radius, alpha, normalizer, n, vd, fp, _lib.arg(x, "f64", n, self.dim, "x"),
+            _lib.arg(elem, "u32", n, None, "elem"), _lib.arg(q, "f64", n, None, "q"),
+            _lib.ptr(v), _lib.ptr(w), _lib.ptr(F), _lib.ptr(integral),
             *[_lib.ptr(t) for t in vis_t], C.byref(nz)))
         if vis:
             return F, integral, int(nz.value), dict(zip(names, vis_t))
@@ -157,17 +174,16 @@ class DeviceMesh:
 
     def set_static_fields(self, fields):
         _lib.sync_stream()
-        fp = (C.c_void_p * 6)(*[None if f is None else _lib.ptr(f) for f in fields])
-        _lib.check(self._lib.pyr_set_static_fields(self._h, fp))
+        _lib.check(self._lib.pyr_set_static_fields(self._h, self._field_ptrs(fields)))
 
     def lsrk4_steps(self, nsteps, dt, c, n, x, p, q, m, elem, ids=None, counters=None,
                     precision=FP64) -> int:
         _lib.sync_stream()
         nn = C.c_int64(n)
         _lib.check(self._lib.pyr_lsrk4_steps(self._h, precision, nsteps, dt, c, C.byref(nn),
-                                             _lib.ptr(x), _lib.ptr(p), _lib.ptr(q),
-                                             _lib.ptr(m), _lib.ptr(elem), _lib.ptr(ids),
-                                             _lib.ptr(counters)))
+                                             *self._state_ptrs(n, x, p, q, m, elem),
+                                             _lib.arg(ids, "i64", n, None, "ids"),
+                                             _lib.arg(counters, "i64", 4, None, "counters")))
         return int(nn.value)
 
     def lsrk4_steps_average(self, nsteps, dt, c, radius, alpha, normalizer, integral_weights,
@@ -179,8 +195,8 @@ class DeviceMesh:
         nn = C.c_int64(n)
         _lib.check(self._lib.pyr_lsrk4_steps_average(
             self._h, nsteps, dt, c, radius, alpha, normalizer, _lib.ptr(w), C.byref(nn),
-            _lib.ptr(x), _lib.ptr(p), _lib.ptr(q), _lib.ptr(m), _lib.ptr(elem), _lib.ptr(ids),
-            _lib.ptr(counters)))
+            *self._state_ptrs(n, x, p, q, m, elem), _lib.arg(ids, "i64", n, None, "ids"),
+            _lib.arg(counters, "i64", 4, None, "counters")))
         return int(nn.value)
 
     def deposit(self, radius, alpha, normalizer, x, q, elem, v=None, want_rho=True,
@@ -193,10 +209,12 @@ class DeviceMesh:
         if want_j and v is not None and j is None:
             j = torch.empty((nn, d), dtype=torch.float64, device=x.device)
         _lib.check(self._lib.pyr_deposit(
-            self._h, radius, alpha, normalizer, precision, n, d, _lib.ptr(x), _lib.ptr(q),
-            _lib.ptr(v) if v is not None else None, _lib.ptr(elem),
-            _lib.ptr(rho) if want_rho else None,
-            _lib.ptr(j) if (want_j and v is not None) else None))
+            self._h, radius, alpha, normalizer, precision, n, d,
+            _lib.arg(x, "f64", n, self.dim, "x"), _lib.arg(q, "f64", n, None, "q"),
+            _lib.arg(v, "f64", n, d, "v") if v is not None else None,
+            _lib.arg(elem, "u32", n, None, "elem"),
+            _lib.arg(rho, "f64", nn, None, "rho") if want_rho else None,
+            _lib.arg(j, "f64", nn, d, "j") if (want_j and v is not None) else None))
         return (rho if want_rho else None), (j if (want_j and v is not None) else None)
 
     def deposit_normalized(self, radius, alpha, normalizer, x, q, elem, integral_weights,
@@ -214,9 +232,11 @@ class DeviceMesh:
         assert w.shape == (self.Np,), "integral weights need one entry per element node"
         nz = C.c_int64(0)
         _lib.check(self._lib.pyr_deposit_normalized(
-            self._h, radius, alpha, normalizer, precision, n, d, _lib.ptr(x), _lib.ptr(q),
-            _lib.ptr(v) if v is not None else None, _lib.ptr(elem), _lib.ptr(w),
-            _lib.ptr(rho), _lib.ptr(j), _lib.ptr(scale), C.byref(nz)))
+            self._h, radius, alpha, normalizer, precision, n, d,
+            _lib.arg(x, "f64", n, self.dim, "x"), _lib.arg(q, "f64", n, None, "q"),
+            _lib.arg(v, "f64", n, d, "v") if v is not None else None,
+            _lib.arg(elem, "u32", n, None, "elem"), _lib.ptr(w),
+            _lib.ptr(rho), _lib.ptr(j), _lib.arg(scale, "f64", n, None, "scale"), C.byref(nz)))
         return rho, j, int(nz.value)
 
     def face_walk_stats(self, radius, x, elem):
@@ -226,7 +246,9 @@ class DeviceMesh:
         n = x.shape[0]
         n_el = torch.empty(n, dtype=torch.int32, device=x.device)
         out = [torch.empty(n, dtype=torch.float64, device=x.device) for _ in range(4)]
-        _lib.check(self._lib.pyr_face_walk_stats(self._h, radius, n, _lib.ptr(x), _lib.ptr(elem),
+        _lib.check(self._lib.pyr_face_walk_stats(self._h, radius, n,
+                                                 _lib.arg(x, "f64", n, self.dim, "x"),
+                                                 _lib.arg(elem, "u32", n, None, "elem"),
                                                  _lib.ptr(n_el), *[_lib.ptr(t) for t in out]))
         return (n_el, *out)
 
@@ -239,8 +261,11 @@ class DeviceMesh:
         v_bar = torch.empty((n, d), dtype=torch.float64, device=x.device) \
             if j_bar is not None else None
         _lib.check(self._lib.pyr_deposit_backward(
-            self._h, radius, alpha, normalizer, precision, n, d, _lib.ptr(x), _lib.ptr(q),
-            _lib.ptr(v), _lib.ptr(elem), _lib.ptr(rho_bar), _lib.ptr(j_bar), _lib.ptr(x_bar),
+            self._h, radius, alpha, normalizer, precision, n, d,
+            _lib.arg(x, "f64", n, self.dim, "x"), _lib.arg(q, "f64", n, None, "q"),
+            _lib.arg(v, "f64", n, d, "v"), _lib.arg(elem, "u32", n, None, "elem"),
+            _lib.arg(rho_bar, "f64", self.K * self.Np, None, "rho_bar"),
+            _lib.arg(j_bar, "f64", self.K * self.Np, d, "j_bar"), _lib.ptr(x_bar),
             _lib.ptr(v_bar)))
         return x_bar, v_bar
 
@@ -251,9 +276,10 @@ class DeviceMesh:
         _lib.sync_stream()
         n = x.shape[0]
         _lib.check(self._lib.pyr_lsrk4_forward(
-            self._h, precision, nsteps, ckpt_every, dt, c, n, _lib.ptr(x), _lib.ptr(p),
-            _lib.ptr(q), _lib.ptr(m), _lib.ptr(elem), _lib.ptr(death), _lib.ptr(ckpt_xp),
-            _lib.ptr(ckpt_elem), _lib.ptr(counters)))
+            self._h, precision, nsteps, ckpt_every, dt, c, n,
+            *self._state_ptrs(n, x, p, q, m, elem), _lib.arg(death, "u32", n, None, "death"),
+            _lib.ptr(ckpt_xp), _lib.ptr(ckpt_elem),
+            _lib.arg(counters, "i64", 4, None, "counters")))
 
     def lsrk4_adjoint(self, nsteps, ckpt_every, dt, c, q, m, death, ckpt_xp, ckpt_elem, x_bar,
                       p_bar, precision=FP64):
@@ -280,7 +306,8 @@ class DeviceMesh:
         n = elem.shape[0]
         perm = torch.empty(n, dtype=torch.int32, device=elem.device)
         seg = torch.empty(self.K + 1, dtype=torch.int32, device=elem.device)
-        _lib.check(self._lib.pyr_sort_by_element(self._h, n, _lib.ptr(elem), _lib.ptr(perm),
+        _lib.check(self._lib.pyr_sort_by_element(self._h, n, _lib.arg(elem, "u32", n, None,
+                                                                      "elem"), _lib.ptr(perm),
                                                  _lib.ptr(seg)))
         return perm, seg
 
@@ -397,6 +424,11 @@ def compact_plan(dead: torch.Tensor):
                                            C.byref(n_new), C.byref(n_moves)))
     k = int(n_moves.value)
     return int(n_new.value), frm[:k], to[:k]
+
+
+def release_scratch():
+    """Frees the library-level scratch of compact_plan (device memory)."""
+    _lib.check(_lib.lib().pyr_release_scratch())
 
 
 def apply_moves(buf: torch.Tensor, frm: torch.Tensor, to: torch.Tensor):

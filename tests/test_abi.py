@@ -42,3 +42,24 @@ def test_error_reporting_without_gpu():
 def test_mesh_desc_layout_matches_header():
     # five int32 then sixteen pointers
     assert C.sizeof(_lib.MeshDesc) == 5 * 4 + 4 + 16 * 8
+
+
+def test_argument_types_checked_before_the_call():
+    """Wrong element types or short arrays never reach a kernel: the checked
+    addresses raise ValueError first (the dtype and length checks run before
+    the device check, so CPU tensors exercise them)."""
+    import pytest
+    import torch
+    with pytest.raises(ValueError, match="dtype"):
+        _lib.arg(torch.zeros(8, dtype=torch.float32), "f64", 8)
+    with pytest.raises(ValueError, match="dtype"):
+        _lib.arg(torch.zeros(8, dtype=torch.int64), "u32", 8)
+    with pytest.raises(ValueError, match="shape"):
+        _lib.arg(torch.zeros((8, 2), dtype=torch.float64), "f64", 8, 3)
+    with pytest.raises(ValueError, match="rows"):
+        _lib.arg(torch.zeros((4, 3), dtype=torch.float64), "f64", 8, 3)
+    with pytest.raises(ValueError, match="entries"):
+        _lib.arg(torch.zeros(4, dtype=torch.float64), "f64", 8)
+    with pytest.raises(ValueError, match="device"):
+        _lib.arg(torch.zeros((8, 3), dtype=torch.float64), "f64", 8, 3)
+    assert _lib.arg(None, "f64", 8) is None

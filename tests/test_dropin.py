@@ -67,6 +67,36 @@ def test_add_particles_and_deposit(square, oracle_square, cuda):
     assert method.depositor.deposit_counter >= 3
 
 
+@pytest.mark.parametrize("pslice", [slice(1, None, 3), slice(None, None, -2),
+                                    slice(2500, 100, -7), slice(10, 10)])
+def test_deposit_strided_slice(square, oracle_square, cuda, pslice):
+    """deposit_* over a Python slice with a step (tools.hpp:38-61 walks start,
+    start + step, ...): the selected particles' rho and j, as the oracle
+    deposits that subset."""
+    from pyrticle_amd import _internal
+    from pyrticle_amd.device import element_numpy
+    wl = square
+    method = _method(wl.discr, 2)
+    state = _load(method, wl)
+    sf = method.get_shape_function_class()(wl.radius, 2, wl.alpha)
+    method.depositor.set_shape_function(state, sf)
+    n = len(state)
+    v = method.velocities(state)
+    rho, j = _internal.deposit_densities(method.depositor.backend, state.depositor_state,
+                                         state.particle_state, len(wl.discr), v, pslice)
+    idx = np.arange(n)[pslice]
+    if idx.size == 0:
+        assert float(rho.abs().max()) == 0.0 and float(j.abs().max()) == 0.0
+        return
+    el = element_numpy(state.containing_elements)[idx]
+    x = state.positions.cpu().numpy()[idx]
+    q = state.particle_state.charges[:n].cpu().numpy()[idx]
+    rho_o, j_o, _ = O.deposit(oracle_square, wl.radius, wl.alpha, x, q,
+                              v.cpu().numpy()[idx], el)
+    assert rel_l2(rho.cpu().numpy(), rho_o) < 1e-12
+    assert rel_l2(j.cpu().numpy(), j_o) < 1e-12
+
+
 def test_normalized_depositor_dropin(square, oracle_square, cuda):
     """NormalizedShapeFunctionDepositor through PicMethod: rho and j as the
     oracle's normalized deposit, the state's normalization_stats gather q / I,
