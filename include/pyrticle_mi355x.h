@@ -132,6 +132,19 @@ int pyr_compact_plan(int64_t n, const uint8_t *dead, uint32_t *move_from,
 int pyr_apply_moves(void *buffer, int64_t elem_bytes, int64_t n_moves,
                     const uint32_t *move_from, const uint32_t *move_to);
 
+/* The same plan as pyr_compact_plan for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  It allocates nothing:
+ * its scratch (the scan of the dead flags, the holes, the scan's tile
+ * partials) is a static array of about 8 MB in the library image on each
+ * device, held until the process ends.  n is limited to PYR_STATIC_PLAN_MAX;
+ * larger n returns PYR_ERR_INVALID_ARG.  Calls on one device share that
+ * array, so calls that overlap on one device race (serialize them, as one
+ * stream does). */
+#define PYR_STATIC_PLAN_MAX 1048576
+int pyr_compact_plan_static(int64_t n, const uint8_t *dead, uint32_t *move_from,
+                            uint32_t *move_to, int64_t *n_new_out, int64_t *n_moves_out);
+
 /* Frees the library-level scratch that pyr_compact_plan grows on demand (the
  * per-mesh scratch goes with pyr_mesh_destroy).  No counterpart in the
  * reference, whose kill loop allocates nothing on a device; call it when no

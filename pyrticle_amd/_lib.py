@@ -54,6 +54,8 @@ _SIGNATURES = {
     "pyr_compact_plan": (C.c_int, [_I64, _P, _P, _P, C.POINTER(_I64), C.POINTER(_I64)]),
     "pyr_apply_moves": (C.c_int, [_P, _I64, _I64, _P, _P]),
     "pyr_release_scratch": (C.c_int, []),
+    "pyr_compact_plan_static": (C.c_int, [_I64, _P, _P, _P, C.POINTER(_I64),
+                                          C.POINTER(_I64)]),
     "pyr_forces": (C.c_int, [_P, _I64, C.c_int, _P * 6, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "pyr_set_static_fields": (C.c_int, [_P, _P * 6]),
     "pyr_lsrk4_steps": (C.c_int, [_P, C.c_int, C.c_int, _D, _D, C.POINTER(_I64), _P, _P, _P,

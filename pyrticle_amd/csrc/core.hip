@@ -361,6 +361,9 @@ pyr_mesh::~pyr_mesh() {
 
 void pyr_deposit_cache_free(pyr_mesh *mesh);   // deposit.hip
 
+// the current device's copy of pyr_compact_plan_static's scratch (below)
+static uint32_t *static_scratch();
+
 extern "C" {
 
 int pyr_version(void) { return 1; }
@@ -374,6 +377,7 @@ int pyr_init(int device) {
     PYR_REQUIRE(count > 0, "no HIP device visible");
     PYR_REQUIRE(device >= 0 && device < count, "device index out of range");
     PYR_HIP(hipSetDevice(device));
+    static_scratch();   // this device's copy of the static plan scratch
   });
 }
 
@@ -593,6 +597,70 @@ int pyr_release_scratch(void) {
   });
 }
 
+}  // extern "C"
+
+// The swap-remove plan of particle_state.hpp:283-293/361-377 from dead flags,
+// on caller-provided scratch: excl (n words, the exclusive scan of the flags),
+// holes (n words), tmp (scan_tmp_words(n) words) and total (1 word).  Every
+// scratch word that is read was written earlier in the same call (the scan
+// writes excl and total; k_holes writes holes[0 .. holes below n_new), the
+// entries k_donors reads), so nothing needs clearing between calls.
+static void compact_plan_on(int64_t n, const uint8_t *dead, uint32_t *move_from,
+                            uint32_t *move_to, int64_t *n_new_out, int64_t *n_moves_out,
+                            uint32_t *excl, uint32_t *holes, uint32_t *tmp, uint32_t *total) {
+  scan_exclusive_u8(dead, excl, n, total, tmp, stream());
+  uint32_t n_dead = 0;
+  PYR_HIP(hipMemcpyAsync(&n_dead, total, sizeof(uint32_t), hipMemcpyDeviceToHost, stream()));
+  PYR_HIP(hipStreamSynchronize(stream()));
+  const int64_t n_new = n - (int64_t)n_dead;
+  int64_t n_moves = 0;
+  if (n_dead && n_new > 0) {
+    uint32_t holes_count = 0;   // dead positions below n_new
+    PYR_HIP(hipMemcpyAsync(&holes_count, excl + n_new, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           stream()));
+    hipLaunchKernelGGL(k_holes, dim3(blocks_for(n_new)), dim3(256), 0, stream(), n_new, dead,
+                       excl, holes);
+    PYR_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_donors, dim3(blocks_for(n - n_new)), dim3(256), 0, stream(), n, n_new,
+                       n_dead, dead, excl, holes, move_from, move_to);
+    PYR_CHECK_LAUNCH();
+    PYR_HIP(hipStreamSynchronize(stream()));
+    n_moves = holes_count;
+  }
+  *n_new_out = n_new;
+  *n_moves_out = n_moves;
+}
+
+// Scratch of pyr_compact_plan_static, in the library image of every device:
+// 64 words (the dead count at word 0), then excl and holes (PYR_STATIC_PLAN_MAX
+// words each), then the scan's tile partials.  Host code uses the per-device
+// address from hipGetSymbolAddress, never the symbol itself.
+constexpr int64_t STATIC_TMP_WORDS = 2048;
+constexpr int64_t STATIC_WORDS = 64 + 2 * (int64_t)PYR_STATIC_PLAN_MAX + STATIC_TMP_WORDS;
+static __device__ __align__(256) uint32_t scratch[STATIC_WORDS];
+static_assert(2 * ((PYR_STATIC_PLAN_MAX + 2047) / 2048) + 2 <= STATIC_TMP_WORDS,
+              "the scan partials of the largest plan must fit");
+
+static std::mutex g_static_mu;
+static uint32_t *g_static_ptr[64] = {};   // per device
+
+// Resolves the current device's copy of the scratch array (once per device).
+static uint32_t *static_scratch() {
+  int dev = 0;
+  PYR_HIP(hipGetDevice(&dev));
+  PYR_REQUIRE(dev >= 0 && dev < 64, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_static_mu);
+  if (!g_static_ptr[dev]) {
+    void *p = nullptr;
+    PYR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    PYR_REQUIRE(p, "static scratch has no device address");
+    g_static_ptr[dev] = static_cast<uint32_t *>(p);
+  }
+  return g_static_ptr[dev];
+}
+
+extern "C" {
+
 int pyr_compact_plan(int64_t n, const uint8_t *dead, uint32_t *move_from, uint32_t *move_to,
                      int64_t *n_new_out, int64_t *n_moves_out) {
   return guarded([&] {
@@ -605,35 +673,35 @@ int pyr_compact_plan(int64_t n, const uint8_t *dead, uint32_t *move_from, uint32
     }
     PYR_REQUIRE(dead && move_from && move_to, "null pointer");
     std::lock_guard<std::mutex> lock(g_plan.mu);
-    DevBuf &excl = g_plan.excl, &tmp = g_plan.tmp, &holes = g_plan.holes, &small = g_plan.small;
-    excl.reserve(sizeof(uint32_t) * (size_t)n);
-    tmp.reserve(sizeof(uint32_t) * scan_tmp_words(n));
-    holes.reserve(sizeof(uint32_t) * (size_t)n);
-    small.reserve(2 * sizeof(uint32_t));
-    uint32_t *total = small.as<uint32_t>();
-    scan_exclusive_u8(dead, excl.as<uint32_t>(), n, total, tmp.as<uint32_t>(), stream());
-    uint32_t n_dead = 0;
-    PYR_HIP(hipMemcpyAsync(&n_dead, total, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                           stream()));
-    PYR_HIP(hipStreamSynchronize(stream()));
-    const int64_t n_new = n - (int64_t)n_dead;
-    int64_t n_moves = 0;
-    if (n_dead && n_new > 0) {
-      uint32_t holes_count = 0;   // dead positions below n_new
-      PYR_HIP(hipMemcpyAsync(&holes_count, excl.as<uint32_t>() + n_new, sizeof(uint32_t),
-                             hipMemcpyDeviceToHost, stream()));
-      hipLaunchKernelGGL(k_holes, dim3(blocks_for(n_new)), dim3(256), 0, stream(), n_new,
-                         dead, excl.as<uint32_t>(), holes.as<uint32_t>());
-      PYR_CHECK_LAUNCH();
-      hipLaunchKernelGGL(k_donors, dim3(blocks_for(n - n_new)), dim3(256), 0, stream(), n,
-                         n_new, n_dead, dead, excl.as<uint32_t>(), holes.as<uint32_t>(),
-                         move_from, move_to);
-      PYR_CHECK_LAUNCH();
-      PYR_HIP(hipStreamSynchronize(stream()));
-      n_moves = holes_count;
+    g_plan.excl.reserve(sizeof(uint32_t) * (size_t)n);
+    g_plan.tmp.reserve(sizeof(uint32_t) * scan_tmp_words(n));
+    g_plan.holes.reserve(sizeof(uint32_t) * (size_t)n);
+    g_plan.small.reserve(2 * sizeof(uint32_t));
+    compact_plan_on(n, dead, move_from, move_to, n_new_out, n_moves_out,
+                    g_plan.excl.as<uint32_t>(), g_plan.holes.as<uint32_t>(),
+                    g_plan.tmp.as<uint32_t>(), g_plan.small.as<uint32_t>());
+  });
+}
+
+int pyr_compact_plan_static(int64_t n, const uint8_t *dead, uint32_t *move_from,
+                            uint32_t *move_to, int64_t *n_new_out, int64_t *n_moves_out) {
+  return guarded([&] {
+    PYR_REQUIRE(n >= 0, "invalid particle count");
+    PYR_REQUIRE(n <= (int64_t)PYR_STATIC_PLAN_MAX,
+                "pyr_compact_plan_static handles at most PYR_STATIC_PLAN_MAX (1048576) "
+                "particles; use pyr_compact_plan for larger counts");
+    PYR_REQUIRE(n_new_out && n_moves_out, "null pointer");
+    uint32_t *base = static_scratch();
+    if (n == 0) {
+      *n_new_out = 0;
+      *n_moves_out = 0;
+      return;
     }
-    *n_new_out = n_new;
-    *n_moves_out = n_moves;
+    PYR_REQUIRE(dead && move_from && move_to, "null pointer");
+    PYR_REQUIRE((int64_t)scan_tmp_words(n) <= STATIC_TMP_WORDS, "scan partials exceed scratch");
+    compact_plan_on(n, dead, move_from, move_to, n_new_out, n_moves_out, base + 64,
+                    base + 64 + PYR_STATIC_PLAN_MAX, base + 64 + 2 * PYR_STATIC_PLAN_MAX,
+                    base);
   });
 }
 
