@@ -20,6 +20,8 @@
 // nodes within R of a particle are exactly the nodes of elements its ball
 // meets, so the gather and the reference's face walk sum the same terms.
 #include <algorithm>
+#include <array>
+#include <tuple>
 #include <atomic>
 #include <cstdlib>
 #include <cmath>
@@ -221,6 +223,173 @@ __global__ void k_node_table(int D, int K, int np, const double *__restrict__ no
   out[(size_t)t * 4 + 3] = (T)(s * invR);
 }
 
+
+// Grouped FP32 node table of the cluster gather: slot 4 g + i of element e
+// holds node node_slot[4 g + i] as ((n - c_e) / P, -|n - c_e|^2 / (R P))
+// (padding slots (0, 0, 0, -1): t = c_p - 1 <= 0), and ngsph[e NG + g] the
+// bounding sphere of group g (centre relative to c_e, radius rounded up).
+__global__ void k_node_table_g(int D, int K, int np, int ng, const double *__restrict__ nodes,
+                               const double *__restrict__ geom, double invR, double inv_p,
+                               const int *__restrict__ node_slot, float4 *__restrict__ tab,
+                               float4 *__restrict__ sph) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)K * ng) return;
+  const int e = (int)(t / ng), grp = (int)(t % ng);
+  double d[4][3], ctr[3] = {0, 0, 0};
+  int cnt = 0;
+  for (int i = 0; i < 4; ++i) {
+    const int node = node_slot[4 * grp + i];
+    const size_t slot = (size_t)e * 4 * ng + 4 * grp + i;
+    if (node < 0) {
+      tab[slot] = make_float4(0.f, 0.f, 0.f, -1.f);
+      continue;
+    }
+    double s2 = 0;
+    for (int k = 0; k < 3; ++k) {
+      d[cnt][k] = k < D ? nodes[((size_t)e * np + node) * D + k] - geom[(size_t)e * 8 + k] : 0.0;
+      s2 += d[cnt][k] * d[cnt][k];
+      ctr[k] += d[cnt][k];
+    }
+    tab[slot] = make_float4((float)(d[cnt][0] * inv_p), (float)(d[cnt][1] * inv_p),
+                            (float)(d[cnt][2] * inv_p), (float)(-(s2 * invR) * inv_p));
+    ++cnt;
+  }
+  double rad = 0;
+  for (int k = 0; k < 3; ++k) ctr[k] = cnt ? ctr[k] / cnt : 0.0;
+  for (int i = 0; i < cnt; ++i) {
+    double s2 = 0;
+    for (int k = 0; k < 3; ++k) s2 += (d[i][k] - ctr[k]) * (d[i][k] - ctr[k]);
+    rad = fmax(rad, sqrt(s2));
+  }
+  // an empty group (never: the padding is in the last group only) reaches nothing
+  sph[(size_t)e * ng + grp] = cnt ? make_float4((float)ctr[0], (float)ctr[1], (float)ctr[2],
+                                                (float)(rad * (1.0 + 1e-6) + 1e-12))
+                                  : make_float4(0.f, 0.f, 0.f, -1e30f);
+}
+
+// Chunk staging of the cluster gather: chunk c (one wave) = the particles
+// [first, first + count) of the element-sorted order, all in source element
+// s, in slots 64 c .. 64 c + 63:
+//   pos[64 c + l]  = (x - c_s, 0) in FP32 (formed in FP64, rounded once),
+//   wgt[64 c + 4 (l / 4) + k][l % 4] = w_l[k], w = (a, a v) in FP32, a = q *
+//                    scale formed in FP64 (the weights of each quad of slots,
+//                    transposed),
+// padding slots (l >= count) at the source centroid with zero weights; and
+// meta[c] = a sphere around the FP32 offsets of the chunk (centre of their
+// bounding box, radius to the farthest, rounded up).
+__global__ __launch_bounds__(256) void k_stage32c(int D, int64_t nch,
+                                                  const uint2 *__restrict__ chunks,
+                                                  const uint32_t *__restrict__ seg,
+                                                  const uint32_t *__restrict__ perm,
+                                                  const double *__restrict__ x,
+                                                  const double *__restrict__ q,
+                                                  const double *__restrict__ v, double scale,
+                                                  const double *__restrict__ geom,
+                                                  float4 *__restrict__ pos,
+                                                  float *__restrict__ wgt_t,
+                                                  float4 *__restrict__ meta) {
+  const int lane = threadIdx.x & 63;
+  const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= nch) return;   // uniform per wave
+  const uint2 ch = chunks[c];
+  const uint32_t s = ch.x, first = ch.y;
+  const uint32_t count = min(64u, seg[s + 1] - first);
+  const bool valid = (uint32_t)lane < count;
+  float xs[3] = {0.f, 0.f, 0.f}, w[4] = {0.f, 0.f, 0.f, 0.f};
+  if (valid) {
+    const uint32_t i = perm[first + lane];
+    const double a = q[i] * scale;
+    w[0] = (float)a;
+    for (int k = 0; k < D; ++k) {
+      xs[k] = (float)(x[(size_t)i * D + k] - geom[(size_t)s * 8 + k]);
+      if (v) w[1 + k] = (float)(a * v[(size_t)i * D + k]);
+    }
+  }
+  const int64_t slot = 64 * c + lane;
+  pos[slot] = make_float4(xs[0], xs[1], xs[2], 0.f);
+  const int64_t quad = slot & ~(int64_t)3;
+  for (int k = 0; k < 4; ++k) wgt_t[(quad + k) * 4 + (slot & 3)] = w[k];
+  float mn[3], mx[3];
+  for (int k = 0; k < 3; ++k) {
+    mn[k] = valid ? xs[k] : INFINITY;
+    mx[k] = valid ? xs[k] : -INFINITY;
+  }
+  for (int o = 32; o > 0; o >>= 1)
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = fminf(mn[k], __shfl_xor(mn[k], o, 64));
+      mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], o, 64));
+    }
+  float ctr[3];
+  for (int k = 0; k < 3; ++k) ctr[k] = 0.5f * (mn[k] + mx[k]);
+  float r2 = 0.f;
+  if (valid)
+    for (int k = 0; k < 3; ++k) r2 = fmaf(xs[k] - ctr[k], xs[k] - ctr[k], r2);
+  for (int o = 32; o > 0; o >>= 1) r2 = fmaxf(r2, __shfl_xor(r2, o, 64));
+  if (lane == 0) meta[c] = make_float4(ctr[0], ctr[1], ctr[2], sqrtf(r2) * (1.f + 1e-5f) + 1e-30f);
+}
+
+// rho and j of every target from its work items' partial sums, added in item
+// order (FP64): one thread per (target, node slot).
+struct CombineArgs {
+  int K, NS;                   // targets, node slots (4 NG)
+  int Np, D;
+  const uint32_t *order;       // K: targets in Morton order (item order)
+  const uint32_t *item_start;  // K + 1 by Morton rank
+  const double *partial;
+  const int *node_of_slot;
+  double *rho, *j;
+};
+__global__ void k_combine_items(CombineArgs g) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)g.K * g.NS) return;
+  const int r = (int)(t / g.NS), slot = (int)(t % g.NS);
+  const int node = g.node_of_slot[slot];
+  if (node < 0) return;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (uint32_t i = g.item_start[r]; i < g.item_start[r + 1]; ++i) {
+    const double *p = g.partial + ((size_t)i * g.NS + slot) * 4;
+    for (int k = 0; k < 4; ++k) v[k] += p[k];
+  }
+  const size_t gn = (size_t)g.order[r] * g.Np + node;
+  if (g.rho) g.rho[gn] = v[0];
+  if (g.j)
+    for (int k = 0; k < g.D; ++k) g.j[gn * g.D + k] = v[1 + k];
+}
+
+// Work items of the list gather: target rank r (Morton order) gets
+// ceil(n_live / 64) items of up to 64 list entries each.
+__global__ void k_item_counts(int K, const uint32_t *__restrict__ order,
+                              const uint32_t *__restrict__ n_live, uint32_t *__restrict__ cnt) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < K) cnt[r] = (n_live[order[r]] + 63u) / 64u;
+}
+__global__ void k_item_fill(int K, const uint32_t *__restrict__ order,
+                            const uint32_t *__restrict__ item_start,
+                            uint2 *__restrict__ items) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= K) return;
+  const uint32_t e = order[r];
+  for (uint32_t i = item_start[r], k = 0; i < item_start[r + 1]; ++i, ++k)
+    items[i] = make_uint2(e, 64u * k);
+}
+
+// Candidate chunks of every target (the capacity of its chunk list): one wave
+// per target sums the chunk counts of its candidate source elements.
+__global__ __launch_bounds__(256) void k_cand_chunks(int K, const uint32_t *__restrict__ cand_start,
+                                                     const uint32_t *__restrict__ cand,
+                                                     const uint32_t *__restrict__ ch_start,
+                                                     uint32_t *__restrict__ cap) {
+  const int lane = threadIdx.x & 63;
+  const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (e >= K) return;
+  uint32_t sum = 0;
+  for (uint32_t k = cand_start[e] + lane; k < cand_start[e + 1]; k += 64) {
+    const uint32_t s = cand[k];
+    sum += ch_start[s + 1] - ch_start[s];
+  }
+  for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+  if (lane == 0) cap[e] = sum;
+}
 }  // namespace pyr
 
 using namespace pyr;
@@ -239,6 +408,11 @@ struct DepositCache {
   // sort / staging scratch
   DevBuf keys_a, keys_b, vals_a, vals_b, hist, hist_scan, seg, tmp, pos, wgt, stats;
   DevBuf pos32, wgt32;   // FP32 staging
+  // matrix-core gather over culled chunk lists: targets in Morton order,
+  // grouped node tables, per-deposition chunk staging and the lists
+  DevBuf t_order, tab32g, ngsph, node_slot, ch_meta, pos32c, wgt32c;
+  DevBuf cap, ent, ent_off, n_live, items, partial;
+  int64_t max_indeg = 0;   // most targets whose candidate list holds one source
   DevBuf nrm;            // normalized deposition: integrals, scaled charges, counter
   DevBuf skeys;   // Morton sort keys
   DevBuf chunks;         // per-particle gathers: 64-particle chunks (element, start),
@@ -256,7 +430,9 @@ void pyr_deposit_cache_free(pyr_mesh *mesh) {
                     &c->tab_R, &c->tr_start, &c->tr, &c->tr_shift, &c->keys_a, &c->keys_b,
                     &c->vals_a, &c->vals_b, &c->skeys, &c->hist,
                     &c->hist_scan, &c->seg, &c->tmp, &c->pos, &c->wgt, &c->stats,
-                    &c->pos32, &c->wgt32, &c->nrm, &c->chunks})
+                    &c->pos32, &c->wgt32, &c->nrm, &c->chunks, &c->t_order, &c->tab32g,
+                    &c->ngsph, &c->node_slot, &c->ch_meta, &c->pos32c, &c->wgt32c, &c->cap,
+                    &c->ent, &c->ent_off, &c->n_live, &c->items, &c->partial})
     b->release();
   delete c;
   mesh->dep = nullptr;
@@ -303,6 +479,120 @@ static HostGeom download_geom(pyr_mesh *h) {
     g.per_max[i] = m.per_max[i];
   }
   return g;
+}
+
+// Spatial grouping of an element's nodes into groups of four (the node groups
+// of the matrix-core gather): in barycentric coordinates of element 0 mapped
+// onto a regular simplex, repeatedly take the remaining node farthest from the
+// centre together with its three nearest remaining nodes.  Every element
+// holds the same unit nodes, so one grouping serves all.  Returns the node of
+// each slot 4 g + i (-1: padding of the last group).
+static std::vector<int> group_nodes(const HostGeom &g) {
+  const int D = g.D, F = D + 1, np = g.Np;
+  const int ng = (np + 3) / 4;
+  // regular simplex vertices
+  const double reg2[3][3] = {{0, 0, 0}, {1, 0, 0}, {0.5, 0.8660254037844386, 0}};
+  const double reg3[4][3] = {{1, 1, 1}, {1, -1, -1}, {-1, 1, -1}, {-1, -1, 1}};
+  const double *v0 = &g.verts[0];
+  // barycentric coordinates from the edge matrix T (columns v_k - v_0)
+  double T[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (int k = 1; k < F; ++k)
+    for (int i = 0; i < D; ++i) T[i][k - 1] = g.verts[(size_t)k * D + i] - v0[i];
+  // inverse of T (D x D) by Gauss-Jordan
+  double a[3][6];
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < 2 * D; ++j) a[i][j] = j < D ? T[i][j] : (j - D == i ? 1.0 : 0.0);
+  for (int c = 0; c < D; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < D; ++r)
+      if (std::fabs(a[r][c]) > std::fabs(a[piv][c])) piv = r;
+    for (int j = 0; j < 2 * D; ++j) std::swap(a[c][j], a[piv][j]);
+    const double s = 1.0 / a[c][c];
+    for (int j = 0; j < 2 * D; ++j) a[c][j] *= s;
+    for (int r = 0; r < D; ++r)
+      if (r != c) {
+        const double f = a[r][c];
+        for (int j = 0; j < 2 * D; ++j) a[r][j] -= f * a[c][j];
+      }
+  }
+  std::vector<std::array<double, 3>> y(np);
+  for (int n = 0; n < np; ++n) {
+    double lam[4] = {1, 0, 0, 0};
+    for (int i = 0; i < D; ++i) {
+      double l = 0;
+      for (int j = 0; j < D; ++j) l += a[i][D + j] * (g.nodes[(size_t)n * D + j] - v0[j]);
+      lam[i + 1] = l;
+      lam[0] -= l;
+    }
+    for (int k = 0; k < 3; ++k) {
+      double s = 0;
+      for (int v = 0; v < F; ++v) s += lam[v] * (D == 2 ? reg2[v][k] : reg3[v][k]);
+      y[n][k] = s;
+    }
+  }
+  double ctr[3] = {0, 0, 0};
+  for (int n = 0; n < np; ++n)
+    for (int k = 0; k < 3; ++k) ctr[k] += y[n][k] / np;
+  auto dist2 = [&](const double *p, const double *q) {
+    double s = 0;
+    for (int k = 0; k < 3; ++k) s += (p[k] - q[k]) * (p[k] - q[k]);
+    return s;
+  };
+  std::vector<int> slots(4 * (size_t)ng, -1);
+  std::vector<char> used(np, 0);
+  for (int grp = 0; grp < ng; ++grp) {
+    int far = -1;
+    double best = -1;
+    for (int n = 0; n < np; ++n)
+      if (!used[n] && dist2(y[n].data(), ctr) > best + 1e-12) {
+        best = dist2(y[n].data(), ctr);
+        far = n;
+      }
+    if (far < 0) break;
+    used[far] = 1;
+    slots[4 * grp] = far;
+    for (int i = 1; i < 4; ++i) {
+      int near = -1;
+      double bd = 0;
+      for (int n = 0; n < np; ++n)
+        if (!used[n] && (near < 0 || dist2(y[n].data(), y[far].data()) < bd - 1e-12)) {
+          near = n;
+          bd = dist2(y[n].data(), y[far].data());
+        }
+      if (near < 0) break;
+      used[near] = 1;
+      slots[4 * grp + i] = near;
+    }
+  }
+  return slots;
+}
+
+// Targets in Morton order of their centroids (10 bits per axis over the
+// bounding box of the centroids): the list gather's processing order, so that
+// the waves in flight on one XCD work on nearby targets and share chunks.
+static std::vector<uint32_t> morton_order(const HostGeom &g, const std::vector<double> &cen) {
+  const int D = g.D, K = g.K;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int e = 0; e < K; ++e)
+    for (int i = 0; i < D; ++i) {
+      lo[i] = std::min(lo[i], cen[(size_t)e * 3 + i]);
+      hi[i] = std::max(hi[i], cen[(size_t)e * 3 + i]);
+    }
+  std::vector<std::pair<uint64_t, uint32_t>> key(K);
+  for (int e = 0; e < K; ++e) {
+    uint64_t code = 0;
+    for (int i = 0; i < D; ++i) {
+      const double span = hi[i] > lo[i] ? hi[i] - lo[i] : 1.0;
+      const uint64_t q = (uint64_t)std::min(
+          1023.0, std::max(0.0, (cen[(size_t)e * 3 + i] - lo[i]) / span * 1024.0));
+      for (int b = 0; b < 10; ++b) code |= ((q >> b) & 1u) << (b * D + i);
+    }
+    key[e] = {code, (uint32_t)e};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<uint32_t> order(K);
+  for (int e = 0; e < K; ++e) order[e] = key[e].second;
+  return order;
 }
 
 // Builds, for every target element, the list of (source element, periodic
@@ -464,6 +754,13 @@ static void build_candidates(pyr_mesh *h, DepositCache *c, double R) {
   upload(c->cand_shift, flat_s.data(), std::max<size_t>(1, flat_s.size()));
   upload(c->geom, c->h_geom.data(), c->h_geom.size());
   c->n_cand = starts[K];
+  const std::vector<uint32_t> order = morton_order(g, cen);
+  upload(c->t_order, order.data(), order.size());
+  std::vector<uint32_t> indeg(K, 0);
+  for (uint32_t s : flat) indeg[s]++;
+  c->max_indeg = K ? *std::max_element(indeg.begin(), indeg.end()) : 0;
+  const std::vector<int> slots = group_nodes(g);
+  upload(c->node_slot, slots.data(), slots.size());
   c->radius = R;
 }
 
@@ -600,6 +897,13 @@ static const uint32_t *sort_and_stage(pyr_mesh *mesh, DepositCache *c, int64_t n
   return perm;
 }
 
+// P = 2^ceil(log2 R): the FP32 gathers evaluate t / P <= 1 (exact scaling)
+static double p_scale(double radius) {
+  int p_exp = 0;
+  const double p_mant = std::frexp(radius, &p_exp);
+  return std::ldexp(1.0, p_mant == 0.5 ? p_exp - 1 : p_exp);
+}
+
 // candidate lists and node tables for radius R (cached per radius)
 static void prepare_tables(pyr_mesh *mesh, DepositCache *c, double radius) {
   const int K = mesh->K, Np = mesh->Np, D = mesh->dim;
@@ -614,6 +918,14 @@ static void prepare_tables(pyr_mesh *mesh, DepositCache *c, double radius) {
     hipLaunchKernelGGL(k_node_table<float>, dim3(blocks_for((int64_t)K * Np)), dim3(256), 0, s,
                        D, K, Np, mesh->dev.nodes, c->geom.as<double>(), 1.0 / radius,
                        c->tab32.as<float>());
+    PYR_CHECK_LAUNCH();
+    const int ng = (Np + 3) / 4;
+    c->tab32g.reserve(sizeof(float4) * (size_t)K * 4 * ng);
+    c->ngsph.reserve(sizeof(float4) * (size_t)K * ng);
+    hipLaunchKernelGGL(k_node_table_g, dim3(blocks_for((int64_t)K * ng)), dim3(256), 0, s, D, K,
+                       Np, ng, mesh->dev.nodes, c->geom.as<double>(), 1.0 / radius,
+                       1.0 / p_scale(radius), c->node_slot.as<int>(), c->tab32g.as<float4>(),
+                       c->ngsph.as<float4>());
     PYR_CHECK_LAUNCH();
     c->tab_radius = radius;
   }
@@ -695,6 +1007,21 @@ static unsigned build_chunks(pyr_mesh *mesh, DepositCache *c, int64_t n, ChunkLi
   return gather_grid((int)bound);
 }
 
+// The chunk list of the current sort (c->seg) for the cluster gather, with
+// its per-element chunk ranges (K + 1 starts); returns the number of chunks
+// (read back to the host: the staging is sized by it).
+static int64_t chunk_table(pyr_mesh *mesh, DepositCache *c, int64_t n, ChunkList &cl,
+                           const uint32_t **ch_start) {
+  build_chunks(mesh, c, n, cl);
+  // build_chunks lays out chunks[bound], counts[K], offsets[K], total: the
+  // offsets followed by the total are the K + 1 starts
+  *ch_start = cl.n_chunks - mesh->K;
+  uint32_t total = 0;
+  PYR_HIP(hipMemcpyAsync(&total, cl.n_chunks, sizeof(uint32_t), hipMemcpyDeviceToHost, stream()));
+  PYR_HIP(hipStreamSynchronize(stream()));
+  return (int64_t)total;
+}
+
 static int g_deposit_count_mode = 0;
 
 // q / I per particle for the normalized deposition; located particles whose
@@ -733,6 +1060,7 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
     const int K = mesh->K, Np = mesh->Np, D = mesh->dim;
     hipStream_t s = stream();
     prepare_tables(mesh, c, radius);
+    const uint32_t *perm = nullptr;
     for (DevBuf *b : {&c->pos, &c->wgt}) b->reserve(sizeof(double4) * (size_t)std::max<int64_t>(1, n));
     for (DevBuf *b : {&c->pos32, &c->wgt32})
       b->reserve(sizeof(float4) * (size_t)std::max<int64_t>(4, (n + 3) & ~(int64_t)3));
@@ -740,9 +1068,9 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
     {
       ProfScope prof(PROF_SORT);
       if (n > 0) {
-        const bool mf = precision == PYR_FP32C;
-        sort_and_stage(mesh, c, n, x, q, j ? v : nullptr, elem, normalizer, !mf, mf, radius,
-                       alpha, sorted);
+        const bool fp64 = precision == PYR_FP64;
+        perm = sort_and_stage(mesh, c, n, x, q, j ? v : nullptr, elem, normalizer, fp64, false,
+                              radius, alpha, sorted);
       } else {
         c->seg.reserve(sizeof(uint32_t) * ((size_t)K + 1));
         PYR_HIP(hipMemsetAsync(c->seg.ptr, 0, sizeof(uint32_t) * ((size_t)K + 1), s));
@@ -775,10 +1103,118 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
     ga.stats = c->stats.as<unsigned long long>();
     const bool count = g_deposit_count_mode != 0;
     if (count) PYR_HIP(hipMemsetAsync(ga.stats, 0, 4 * sizeof(unsigned long long), s));
+    if (precision == PYR_FP32C) {
+      if (count) throw Error(PYR_ERR_INVALID_ARG, "deposit counting needs the FP64 precision");
+      // chunks of up to 64 sorted particles per source element, staged in
+      // FP32 with their bounding spheres, then the cluster gather
+      ChunkList cl{};
+      const uint32_t *ch_start = nullptr;
+      int64_t nch = 0;
+      {
+        ProfScope prof(PROF_SORT);
+        nch = chunk_table(mesh, c, n, cl, &ch_start);
+        c->pos32c.reserve(sizeof(float4) * 64 * (size_t)std::max<int64_t>(1, nch));
+        c->wgt32c.reserve(sizeof(float4) * 64 * (size_t)std::max<int64_t>(1, nch));
+        c->ch_meta.reserve(sizeof(float4) * (size_t)std::max<int64_t>(1, nch));
+        if (nch > 0) {
+          const double scale = normalizer * std::pow(p_scale(radius), alpha);
+          hipLaunchKernelGGL(k_stage32c, dim3((unsigned)((nch + 3) / 4)), dim3(256), 0, s, D,
+                             nch, cl.chunks, c->seg.as<uint32_t>(), perm, x, q, j ? v : nullptr,
+                             scale, c->geom.as<double>(), c->pos32c.as<float4>(),
+                             c->wgt32c.as<float>(), c->ch_meta.as<float4>());
+          PYR_CHECK_LAUNCH();
+        }
+      }
+      // per-target lists of the chunks that reach a node group: capacity =
+      // candidate chunks (at most nch x max_indeg in total), then the cull
+      const int64_t bound = std::max<int64_t>(1, nch * std::max<int64_t>(1, c->max_indeg));
+      PYR_REQUIRE(bound < ((int64_t)1 << 32), "deposition chunk lists exceed 2^32 entries");
+      // one scratch: capacities (later item counts) K, list offsets K + 1, scan
+      // partials, item starts K + 1
+      c->cap.reserve(sizeof(uint32_t) * ((size_t)3 * K + 3 + scan_tmp_words(K)));
+      uint32_t *cap = c->cap.as<uint32_t>(), *cap_off = cap + K + 1;
+      uint32_t *scan_tmp = cap_off + K + 1;
+      uint32_t *istart = scan_tmp + scan_tmp_words(K);
+      c->ent.reserve(sizeof(uint2) * (size_t)bound);
+      c->ent_off.reserve(sizeof(float4) * (size_t)bound);
+      c->n_live.reserve(sizeof(uint32_t) * (size_t)std::max(1, K));
+      {
+        ProfScope prof(PROF_SORT);
+        hipLaunchKernelGGL(k_cand_chunks, dim3((unsigned)((K + 3) / 4)), dim3(256), 0, s, K,
+                           c->cand_start.as<uint32_t>(), c->cand.as<uint32_t>(), ch_start, cap);
+        PYR_CHECK_LAUNCH();
+        scan_exclusive_u32(cap, cap_off, K, cap_off + K, scan_tmp, s);
+        CullArgs cu{};
+        cu.K = K;
+        cu.R = radius;
+        for (int a = 0; a < 3; ++a) cu.shift_len[a] = ga.shift_len[a];
+        cu.cand_start = c->cand_start.as<uint32_t>();
+        cu.cand = c->cand.as<uint32_t>();
+        cu.cand_shift = c->cand_shift.as<uint8_t>();
+        cu.ch_start = ch_start;
+        cu.ch_meta = c->ch_meta.as<float4>();
+        cu.geom = c->geom.as<double>();
+        cu.ngsph = c->ngsph.as<float4>();
+        cu.cap_off = cap_off;
+        cu.ent = c->ent.as<uint2>();
+        cu.ent_off = c->ent_off.as<float4>();
+        cu.n_live = c->n_live.as<uint32_t>();
+        launch_cull_chunks(D, Np, cu);
+      }
+      // work items of up to 64 list entries, in Morton order of the targets
+      const int ns = 4 * ((Np + 3) / 4);
+      uint32_t *icnt = cap;   // the capacities are no longer needed
+      uint32_t n_items = 0;
+      {
+        ProfScope prof(PROF_SORT);
+        hipLaunchKernelGGL(k_item_counts, dim3(blocks_for(K)), dim3(256), 0, s, K,
+                           c->t_order.as<uint32_t>(), c->n_live.as<uint32_t>(), icnt);
+        PYR_CHECK_LAUNCH();
+        scan_exclusive_u32(icnt, istart, K, istart + K, scan_tmp, s);
+        PYR_HIP(hipMemcpyAsync(&n_items, istart + K, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        PYR_HIP(hipStreamSynchronize(s));
+        c->items.reserve(sizeof(uint2) * (size_t)std::max<uint32_t>(1, n_items));
+        c->partial.reserve(sizeof(double) * 4 * ns * (size_t)std::max<uint32_t>(1, n_items));
+        hipLaunchKernelGGL(k_item_fill, dim3(blocks_for(K)), dim3(256), 0, s, K,
+                           c->t_order.as<uint32_t>(), istart, c->items.as<uint2>());
+        PYR_CHECK_LAUNCH();
+      }
+      ListArgs la{};
+      la.K = K;
+      la.R = radius;
+      la.invR = 1.0 / radius;
+      la.inv_p = (float)(1.0 / p_scale(radius));
+      la.alpha = (float)alpha;
+      la.items = c->items.as<uint2>();
+      la.n_items = istart + K;
+      la.cap_off = cap_off;
+      la.n_live = c->n_live.as<uint32_t>();
+      la.ent = c->ent.as<uint2>();
+      la.ent_off = c->ent_off.as<float4>();
+      la.pos = c->pos32c.as<float4>();
+      la.wgt = c->wgt32c.as<float4>();
+      la.ntab = c->tab32g.as<float4>();
+      la.partial = c->partial.as<double>();
+      CombineArgs cb{};
+      cb.K = K;
+      cb.NS = ns;
+      cb.Np = Np;
+      cb.D = D;
+      cb.order = c->t_order.as<uint32_t>();
+      cb.item_start = istart;
+      cb.partial = c->partial.as<double>();
+      cb.node_of_slot = c->node_slot.as<int>();
+      cb.rho = rho;
+      cb.j = j;
+      ProfScope prof(PROF_GATHER);
+      if (n_items) launch_gather_cl(D, Np, la, n_items, alpha == 2.0);
+      hipLaunchKernelGGL(k_combine_items, dim3(blocks_for((int64_t)K * ns)), dim3(256), 0, s,
+                         cb);
+      PYR_CHECK_LAUNCH();
+      return;
+    }
     ProfScope prof(PROF_GATHER);
-    if (precision == PYR_FP64)
-      launch_gather_fp64(D, Np, ga, K, shape_unit_square(alpha, radius), count);
-    else launch_gather_fp32(D, Np, ga, K, alpha == 2.0, count);
+    launch_gather_fp64(D, Np, ga, K, shape_unit_square(alpha, radius), count);
   }
 }
 

@@ -661,6 +661,315 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
 }
 
 // ---------------------------------------------------------------------------
+// FP32-compensated gather over culled chunk lists (matrix cores)
+// ---------------------------------------------------------------------------
+// The particles of every source element are staged per deposition in chunks
+// of up to 64 consecutive element-sorted particles, 64 slots each (k_stage32c:
+// offsets x - c_src, the quad-transposed weights, and a bounding sphere of
+// the chunk).  The nodes of every target element are grouped spatially in
+// groups of four, each with a bounding sphere.
+//
+// k_cull_chunks (one wave per target): for every chunk of every candidate
+// source element, the groups whose sphere lies within R of the chunk's sphere;
+// the chunks with at least one such group are written, in candidate order, to
+// the target's list (chunk id and group mask; offset of the source centroid,
+// + periodic shift, from the target centroid, formed in FP64 and rounded
+// once).  A group none of whose nodes is within R of a chunk's particles adds
+// exactly zero for that chunk, so the culling leaves every sum unchanged.
+//
+// k_gather_cl (one wave per target, targets in Morton order of their
+// centroids so that the waves in flight on an XCD share chunks in its L2):
+// streams the target's list, two chunks of loads in flight, and forms the
+// pairs of every listed group as t = (R - |n - x|^2 / R) / P on
+// v_mfma_f32_4x4x1_16b_f32: C + sum_k A B over A = (2 d / R, c_p), B =
+// ((n - c_e) / P, 1), c_p = (R - |d|^2 / R) / P entering last; S =
+// max(t, 0)^alpha; the four components (q N, q N v) accumulate on four more
+// products.  Every FOLD chunks the per-lane FP32 partials are summed over the
+// lanes l, l^32, l^16 (a fixed butterfly) into Kahan pairs (s, c) in LDS for
+// lanes 0..15; at the end the four remaining blocks are combined in FP64 in a
+// fixed order, so results are bitwise reproducible.
+struct CullArgs {
+  int K;
+  double R;
+  double shift_len[3];
+  const uint32_t *cand_start;  // K + 1
+  const uint32_t *cand;        // candidate source elements (ascending)
+  const uint8_t *cand_shift;   // periodic shift codes
+  const uint32_t *ch_start;    // K + 1: chunk range of each source element
+  const float4 *ch_meta;       // per chunk: sphere centre rel. source centroid (3), radius
+  const double *geom;          // K x 8: centroid (3), ...
+  const float4 *ngsph;         // K x NG: group centre rel. c_e (3), radius
+  const uint32_t *cap_off;     // K + 1: list region of each target
+  uint2 *ent;                  // list entries: chunk id, group mask
+  float4 *ent_off;             //               source offset from the target (3), 0
+  uint32_t *n_live;            // K: list length
+};
+
+struct ListArgs {
+  int K;
+  double R, invR;
+  float inv_p, alpha;          // 1 / P (P = 2^ceil(log2 R)), shape exponent
+  const uint2 *items;          // work items: target, first list entry
+  const uint32_t *n_items;     // device count of work items
+  const uint32_t *cap_off;     // K + 1
+  const uint32_t *n_live;      // K
+  const uint2 *ent;
+  const float4 *ent_off;
+  const float4 *pos;           // 64 slots per chunk: x - c_src (3), 0
+  const float4 *wgt;           // 64 slots per chunk: weights of each quad, transposed
+  const float4 *ntab;          // K x 4 NG slots: (n - c_e) / P (3), -|n - c_e|^2 / (R P)
+  double *partial;             // per item: 4 NG slots x 4 components
+};
+
+// relative margin of the FP32 sphere tests (they may only keep extra chunks)
+constexpr float CL_MARGIN = 1e-4f;
+// targets (waves) per workgroup of the list gather
+constexpr int CL_W = 4;
+
+template <int D, int NG>
+__global__ __launch_bounds__(256) void k_cull_chunks(CullArgs g) {
+  const int lane = threadIdx.x & 63;
+  const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (e >= g.K) return;   // uniform per wave
+  const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
+  auto uni = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
+  const float rf = (float)g.R;
+  float gx[NG], gy[NG], gz[NG], gr[NG];
+#pragma unroll
+  for (int ng = 0; ng < NG; ++ng) {
+    const float4 sp = g.ngsph[(size_t)e * NG + ng];
+    gx[ng] = uni(sp.x);
+    gy[ng] = uni(sp.y);
+    gz[ng] = uni(sp.z);
+    gr[ng] = uni((rf + sp.w) * (1.f + CL_MARGIN));
+  }
+  const double cx = g.geom[(size_t)e * 8 + 0], cy = g.geom[(size_t)e * 8 + 1],
+               cz = D == 3 ? g.geom[(size_t)e * 8 + 2] : 0.0;
+  const uint32_t base = g.cap_off[e];
+  uint32_t n_out = 0;
+  const uint32_t c_begin = g.cand_start[e], c_end = g.cand_start[e + 1];
+  for (uint32_t cw = c_begin; cw < c_end; cw += 64) {
+    uint32_t c0 = 0, cnt = 0;
+    float ox = 0.f, oy = 0.f, oz = 0.f;
+    if (cw + lane < c_end) {
+      const uint32_t src = g.cand[cw + lane];
+      c0 = g.ch_start[src];
+      cnt = g.ch_start[src + 1] - c0;
+      if (cnt) {
+        const uint32_t code = g.cand_shift[cw + lane];
+        double sh[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          const uint32_t c2 = (code >> (2 * a)) & 3u;
+          sh[a] = c2 == 1 ? g.shift_len[a] : (c2 == 2 ? -g.shift_len[a] : 0.0);
+        }
+        ox = (float)(g.geom[(size_t)src * 8 + 0] + sh[0] - cx);
+        oy = (float)(g.geom[(size_t)src * 8 + 1] + sh[1] - cy);
+        oz = D == 3 ? (float)(g.geom[(size_t)src * 8 + 2] + sh[2] - cz) : 0.f;
+      }
+    }
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += y;
+    }
+    const uint32_t excl = incl - cnt;
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    for (uint32_t fb = 0; fb < total; fb += 64) {
+      const uint32_t f = fb + (uint32_t)lane;
+      int k = 0;   // the candidate lane holding flat chunk f
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1) {
+        const uint32_t v = __shfl(incl, k + step - 1, 64);
+        if (v <= f) k += step;
+      }
+      k = min(k, 63);
+      const uint32_t ch = __shfl(c0, k, 64) + (f - __shfl(excl, k, 64));
+      const float px = __shfl(ox, k, 64), py = __shfl(oy, k, 64), pz = __shfl(oz, k, 64);
+      uint32_t ngm = 0;
+      if (f < total) {
+        const float4 meta = g.ch_meta[ch];
+        const float mw = meta.w * (1.f + CL_MARGIN);
+#pragma unroll
+        for (int ng = 0; ng < NG; ++ng) {
+          const float dx = meta.x + px - gx[ng], dy = meta.y + py - gy[ng],
+                      dz = D == 3 ? meta.z + pz - gz[ng] : 0.f;
+          const float lim = gr[ng] + mw;
+          if (fmaf(dz, dz, fmaf(dy, dy, dx * dx)) <= lim * lim) ngm |= 1u << ng;
+        }
+      }
+      const uint64_t live = __ballot(ngm != 0);
+      if (ngm) {
+        const uint32_t at = base + n_out + (uint32_t)__popcll(live & lt_mask);
+        g.ent[at] = make_uint2(ch, ngm);
+        g.ent_off[at] = make_float4(px, py, pz, 0.f);
+      }
+      n_out += (uint32_t)__popcll(live);
+    }
+  }
+  if (lane == 0) g.n_live[e] = n_out;
+}
+
+template <int D, int NP, int W, bool A2>
+__global__ __launch_bounds__(64 * W) void k_gather_cl(ListArgs g) {
+  constexpr int NG = (NP + 3) / 4;
+  constexpr int FOLD = FOLD_STEPS;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int64_t item = (int64_t)xcd_element(blockIdx.x) * W + wave;
+  if (item >= (int64_t)*g.n_items) return;   // wave-uniform; no block-level synchronization
+  const uint2 it = g.items[item];
+  const uint32_t e = it.x;
+
+  __shared__ float2 s_kahan[W][NG * 4][16];
+
+  const float rf = (float)g.R, inv_rf = (float)g.invR, two_inv_r = (float)(2.0 * g.invR);
+  const float inv_p = g.inv_p;
+  // this item's list entries: [first, first + cnt), at most one entry per lane
+  const uint32_t base = g.cap_off[e] + it.y;
+  const uint32_t cnt = min(64u, g.n_live[e] - it.y);
+
+  pyr_v4f acc[NG];
+#pragma unroll
+  for (int ng = 0; ng < NG; ++ng) acc[ng] = pyr_v4f{0.f, 0.f, 0.f, 0.f};
+  int n_steps = 0;
+  bool folded = false;   // wave-uniform
+
+  auto fold = [&]() {
+#pragma unroll
+    for (int ng = 0; ng < NG; ++ng)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        float a = acc[ng][v];
+        a += __shfl_xor(a, 32, 64);
+        a += __shfl_xor(a, 16, 64);
+        if (lane < 16) {
+          float2 k = folded ? s_kahan[wave][ng * 4 + v][lane] : make_float2(0.f, 0.f);
+          const float y = a - k.y;
+          const float t = k.x + y;
+          k.y = (t - k.x) - y;
+          k.x = t;
+          s_kahan[wave][ng * 4 + v][lane] = k;
+        }
+      }
+#pragma unroll
+    for (int ng = 0; ng < NG; ++ng) acc[ng] = pyr_v4f{0.f, 0.f, 0.f, 0.f};
+    folded = true;
+  };
+
+  if (cnt) {
+    // node constants of this lane's node in every group
+    float bx[NG], by[NG], bz[NG];
+    pyr_v4f cinit[NG];
+#pragma unroll
+    for (int ng = 0; ng < NG; ++ng) {
+      const float4 nt = g.ntab[(size_t)e * (4 * NG) + 4 * ng + (lane & 3)];
+      bx[ng] = nt.x;
+      by[ng] = nt.y;
+      bz[ng] = nt.z;
+      cinit[ng] = pyr_v4f{nt.w, nt.w, nt.w, nt.w};
+    }
+    // the item's list entries, one per lane
+    uint2 ea = make_uint2(0u, 0u);
+    float4 oa = make_float4(0.f, 0.f, 0.f, 0.f);
+    if ((uint32_t)lane < cnt) {
+      ea = g.ent[base + lane];
+      oa = g.ent_off[base + lane];
+    }
+    struct Next {
+      float4 p, w;
+      float ox, oy, oz;
+      uint32_t ngm;
+    };
+    auto rdf = [](float v, int l) {
+      return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+    };
+    auto fetch = [&](uint32_t i) {
+      const int l = (int)i;
+      Next n;
+      const uint32_t ch = (uint32_t)__builtin_amdgcn_readlane((int)ea.x, l);
+      n.ngm = (uint32_t)__builtin_amdgcn_readlane((int)ea.y, l);
+      n.ox = rdf(oa.x, l);
+      n.oy = rdf(oa.y, l);
+      n.oz = rdf(oa.z, l);
+      n.p = g.pos[(size_t)ch * 64 + lane];
+      n.w = g.wgt[(size_t)ch * 64 + lane];
+      return n;
+    };
+    auto evaluate = [&](const Next &n) {
+      const float dx = n.p.x + n.ox, dy = n.p.y + n.oy, dz = D == 3 ? n.p.z + n.oz : 0.f;
+      const float dd = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+      const float mx = dx * two_inv_r, my = dy * two_inv_r, mz = dz * two_inv_r;
+      const float cp = fmaf(-dd, inv_rf, rf) * inv_p;
+#pragma unroll
+      for (int ng = 0; ng < NG; ++ng) {
+        if (!((n.ngm >> ng) & 1u)) continue;
+        pyr_v4f t = mfma4x4(mx, bx[ng], cinit[ng]);
+        t = mfma4x4(my, by[ng], t);
+        if (D == 3) t = mfma4x4(mz, bz[ng], t);
+        t = mfma4x4(cp, 1.f, t);
+        float sv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if constexpr (A2) {
+            // max(t, 0)^2 as clamp(t |t|) to [0, 1] (t <= 1 up to rounding)
+            sv[i] = __builtin_amdgcn_fmed3f(t[i] * fabsf(t[i]), 0.f, 1.f);
+          } else {
+            const float tw = __builtin_amdgcn_fmed3f(t[i], 0.f, 1.f);
+            sv[i] = tw > 0.f ? __powf(tw, g.alpha) : 0.f;
+          }
+        }
+        acc[ng] = mfma4x4(n.w.x, sv[0], acc[ng]);
+        acc[ng] = mfma4x4(n.w.y, sv[1], acc[ng]);
+        acc[ng] = mfma4x4(n.w.z, sv[2], acc[ng]);
+        acc[ng] = mfma4x4(n.w.w, sv[3], acc[ng]);
+      }
+      if (++n_steps == FOLD) {
+        n_steps = 0;
+        fold();
+      }
+    };
+    // two chunks in flight while one is evaluated
+    Next a = fetch(0);
+    Next b = a;
+    if (cnt > 1) b = fetch(1);
+#pragma unroll 1
+    for (uint32_t i = 0; i < cnt; ++i) {
+      Next c = b;
+      if (i + 2 < cnt) c = fetch(i + 2);
+      evaluate(a);
+      a = b;
+      b = c;
+    }
+    if (n_steps > 0) fold();
+  }
+  // lanes 0..15: Kahan pairs of blocks 0..3; combine the blocks in FP64
+  double out[NG][4];
+#pragma unroll
+  for (int ng = 0; ng < NG; ++ng)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      double val = 0.0;
+      if (folded && lane < 16) {
+        const float2 k = s_kahan[wave][ng * 4 + v][lane];
+        val = (double)k.x - (double)k.y;
+      }
+      val += __shfl_xor(val, 4, 64);
+      val += __shfl_xor(val, 8, 64);
+      out[ng][v] = val;
+    }
+  if (lane < 4) {
+    double *dst = g.partial + (size_t)item * (16 * NG);
+#pragma unroll
+    for (int ng = 0; ng < NG; ++ng)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) dst[(4 * ng + lane) * 4 + v] = out[ng][v];
+  }
+}
+
+
+// ---------------------------------------------------------------------------
 // adjoint of the deposition: per-particle gather of grad S . (rho_bar, j_bar)
 // ---------------------------------------------------------------------------
 // For particle p with a = q N and the nodes i within R:
@@ -1037,6 +1346,8 @@ __global__ __launch_bounds__(64) void k_shape_average(AverageArgs g) {
 // Launchers, one per translation unit.  precision: PYR_FP64 or PYR_FP32C.
 void launch_gather_fp64(int D, int Np, const GatherArgs &ga, int K, bool a2, bool count);
 void launch_gather_fp32(int D, int Np, const GatherArgs &ga, int K, bool a2, bool count);
+void launch_cull_chunks(int D, int Np, const CullArgs &ca);
+void launch_gather_cl(int D, int Np, const ListArgs &la, uint32_t n_items, bool a2);
 void launch_deposit_adj(int D, int Np, const AdjArgs &ga, unsigned grid, int precision,
                         bool a2);
 void launch_shape_integral(int D, int Np, const IntegralArgs &ga, unsigned grid, bool a2);
