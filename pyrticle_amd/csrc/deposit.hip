@@ -373,6 +373,18 @@ __global__ void k_item_fill(int K, const uint32_t *__restrict__ order,
     items[i] = make_uint2(e, 64u * k);
 }
 
+// Sort key of every work item: the chunk of its first list entry.  The list
+// gather runs the items in this order, so that the items reading a chunk
+// (from ~200 targets) run at about the same time and share it in the L2.
+__global__ void k_item_keys(uint32_t n_items, const uint2 *__restrict__ items,
+                            const uint32_t *__restrict__ cap_off, const uint2 *__restrict__ ent,
+                            uint32_t *__restrict__ key) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_items) return;
+  const uint2 it = items[i];
+  key[i] = ent[cap_off[it.x] + it.y].x;
+}
+
 // Candidate chunks of every target (the capacity of its chunk list): one wave
 // per target sums the chunk counts of its candidate source elements.
 __global__ __launch_bounds__(256) void k_cand_chunks(int K, const uint32_t *__restrict__ cand_start,
@@ -412,6 +424,7 @@ struct DepositCache {
   // grouped node tables, per-deposition chunk staging and the lists
   DevBuf t_order, tab32g, ngsph, node_slot, ch_meta, pos32c, wgt32c;
   DevBuf cap, ent, ent_off, n_live, items, partial;
+  DevBuf ikey, ik_a, ik_b, iv_a, iv_b;   // work items sorted by their first chunk
   int64_t max_indeg = 0;   // most targets whose candidate list holds one source
   DevBuf nrm;            // normalized deposition: integrals, scaled charges, counter
   DevBuf skeys;   // Morton sort keys
@@ -432,7 +445,8 @@ void pyr_deposit_cache_free(pyr_mesh *mesh) {
                     &c->hist_scan, &c->seg, &c->tmp, &c->pos, &c->wgt, &c->stats,
                     &c->pos32, &c->wgt32, &c->nrm, &c->chunks, &c->t_order, &c->tab32g,
                     &c->ngsph, &c->node_slot, &c->ch_meta, &c->pos32c, &c->wgt32c, &c->cap,
-                    &c->ent, &c->ent_off, &c->n_live, &c->items, &c->partial})
+                    &c->ent, &c->ent_off, &c->n_live, &c->items, &c->partial, &c->ikey,
+                    &c->ik_a, &c->ik_b, &c->iv_a, &c->iv_b})
     b->release();
   delete c;
   mesh->dep = nullptr;
@@ -1165,6 +1179,7 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
       const int ns = 4 * ((Np + 3) / 4);
       uint32_t *icnt = cap;   // the capacities are no longer needed
       uint32_t n_items = 0;
+      const uint32_t *item_order = nullptr;
       {
         ProfScope prof(PROF_SORT);
         hipLaunchKernelGGL(k_item_counts, dim3(blocks_for(K)), dim3(256), 0, s, K,
@@ -1178,6 +1193,16 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
         hipLaunchKernelGGL(k_item_fill, dim3(blocks_for(K)), dim3(256), 0, s, K,
                            c->t_order.as<uint32_t>(), istart, c->items.as<uint2>());
         PYR_CHECK_LAUNCH();
+        if (n_items) {
+          c->ikey.reserve(sizeof(uint32_t) * (size_t)n_items);
+          hipLaunchKernelGGL(k_item_keys, dim3(blocks_for(n_items)), dim3(256), 0, s, n_items,
+                             c->items.as<uint2>(), cap_off, c->ent.as<uint2>(),
+                             c->ikey.as<uint32_t>());
+          PYR_CHECK_LAUNCH();
+          item_order = radix_sort(c, n_items, c->ikey.as<uint32_t>(),
+                                  (uint32_t)std::max<int64_t>(1, nch), c->ik_a, c->ik_b, c->iv_a,
+                                  c->iv_b);
+        }
       }
       ListArgs la{};
       la.K = K;
@@ -1186,6 +1211,7 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
       la.inv_p = (float)(1.0 / p_scale(radius));
       la.alpha = (float)alpha;
       la.items = c->items.as<uint2>();
+      la.item_order = item_order;
       la.n_items = istart + K;
       la.cap_off = cap_off;
       la.n_live = c->n_live.as<uint32_t>();

@@ -710,6 +710,7 @@ struct ListArgs {
   double R, invR;
   float inv_p, alpha;          // 1 / P (P = 2^ceil(log2 R)), shape exponent
   const uint2 *items;          // work items: target, first list entry
+  const uint32_t *item_order;  // processing order of the items
   const uint32_t *n_items;     // device count of work items
   const uint32_t *cap_off;     // K + 1
   const uint32_t *n_live;      // K
@@ -814,25 +815,31 @@ __global__ __launch_bounds__(256) void k_cull_chunks(CullArgs g) {
 template <int D, int NP, int W, bool A2>
 __global__ __launch_bounds__(64 * W) void k_gather_cl(ListArgs g) {
   constexpr int NG = (NP + 3) / 4;
-  constexpr int FOLD = FOLD_STEPS;
+  constexpr int FOLD = FOLD_STEPS / 2;   // chunk pairs per fold
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int64_t item = (int64_t)xcd_element(blockIdx.x) * W + wave;
-  if (item >= (int64_t)*g.n_items) return;   // wave-uniform; no block-level synchronization
+  const int64_t k_item = (int64_t)xcd_element(blockIdx.x) * W + wave;
+  if (k_item >= (int64_t)*g.n_items) return;   // wave-uniform; no block-level synchronization
+  const uint32_t item = g.item_order[k_item];
   const uint2 it = g.items[item];
   const uint32_t e = it.x;
 
-  __shared__ float2 s_kahan[W][NG * 4][16];
+  // per-lane Kahan pairs (s, c) of every accumulator entry, each lane
+  // touching only its own slots
+  __shared__ float2 s_kahan[W][NG * 4][64];
+  float2 *kl = &s_kahan[wave][0][lane];
 
   const float rf = (float)g.R, inv_rf = (float)g.invR, two_inv_r = (float)(2.0 * g.invR);
   const float inv_p = g.inv_p;
-  // this item's list entries: [first, first + cnt), at most one entry per lane
+  // this item's list entries: [first, first + cnt), one per lane
   const uint32_t base = g.cap_off[e] + it.y;
   const uint32_t cnt = min(64u, g.n_live[e] - it.y);
 
-  pyr_v4f acc[NG];
+  // two accumulator sets: the chunks of a pair accumulate independently, so
+  // their product chains interleave
+  pyr_v4f acc0[NG], acc1[NG];
 #pragma unroll
-  for (int ng = 0; ng < NG; ++ng) acc[ng] = pyr_v4f{0.f, 0.f, 0.f, 0.f};
+  for (int ng = 0; ng < NG; ++ng) acc0[ng] = acc1[ng] = pyr_v4f{0.f, 0.f, 0.f, 0.f};
   int n_steps = 0;
   bool folded = false;   // wave-uniform
 
@@ -841,122 +848,143 @@ __global__ __launch_bounds__(64 * W) void k_gather_cl(ListArgs g) {
     for (int ng = 0; ng < NG; ++ng)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        float a = acc[ng][v];
-        a += __shfl_xor(a, 32, 64);
-        a += __shfl_xor(a, 16, 64);
-        if (lane < 16) {
-          float2 k = folded ? s_kahan[wave][ng * 4 + v][lane] : make_float2(0.f, 0.f);
-          const float y = a - k.y;
-          const float t = k.x + y;
-          k.y = (t - k.x) - y;
-          k.x = t;
-          s_kahan[wave][ng * 4 + v][lane] = k;
-        }
+        float2 k = folded ? kl[(ng * 4 + v) * 64] : make_float2(0.f, 0.f);
+        const float y = (acc0[ng][v] + acc1[ng][v]) - k.y;
+        const float t = k.x + y;
+        k.y = (t - k.x) - y;
+        k.x = t;
+        kl[(ng * 4 + v) * 64] = k;
       }
 #pragma unroll
-    for (int ng = 0; ng < NG; ++ng) acc[ng] = pyr_v4f{0.f, 0.f, 0.f, 0.f};
+    for (int ng = 0; ng < NG; ++ng) acc0[ng] = acc1[ng] = pyr_v4f{0.f, 0.f, 0.f, 0.f};
     folded = true;
   };
 
-  if (cnt) {
-    // node constants of this lane's node in every group
-    float bx[NG], by[NG], bz[NG];
-    pyr_v4f cinit[NG];
+  // node constants of this lane's node in every group
+  float bx[NG], by[NG], bz[NG], cc[NG];
+#pragma unroll
+  for (int ng = 0; ng < NG; ++ng) {
+    const float4 nt = g.ntab[(size_t)e * (4 * NG) + 4 * ng + (lane & 3)];
+    bx[ng] = nt.x;
+    by[ng] = nt.y;
+    bz[ng] = nt.z;
+    cc[ng] = nt.w;
+  }
+  // the item's list entries, one per lane
+  uint2 ea = make_uint2(0u, 0u);
+  float4 oa = make_float4(0.f, 0.f, 0.f, 0.f);
+  if ((uint32_t)lane < cnt) {
+    ea = g.ent[base + lane];
+    oa = g.ent_off[base + lane];
+  }
+  struct Next {
+    float4 p, w;
+    float ox, oy, oz;
+    uint32_t ngm;
+  };
+  auto rdf = [](float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+  };
+  // the loads of list entry i (i < cnt: a chunk; else nothing: weights zero)
+  auto fetch = [&](Next &n, uint32_t i) {
+    if (i >= cnt) {
+      n.ngm = 0;
+      n.p = n.w = make_float4(0.f, 0.f, 0.f, 0.f);
+      n.ox = n.oy = n.oz = 0.f;
+      return;
+    }
+    const int l = (int)i;
+    const uint32_t ch = (uint32_t)__builtin_amdgcn_readlane((int)ea.x, l);
+    n.ngm = (uint32_t)__builtin_amdgcn_readlane((int)ea.y, l);
+    n.ox = rdf(oa.x, l);
+    n.oy = rdf(oa.y, l);
+    n.oz = rdf(oa.z, l);
+    n.p = g.pos[(size_t)ch * 64 + lane];
+    n.w = g.wgt[(size_t)ch * 64 + lane];
+  };
+  // t = (R - |n - x|^2 / R) / P: the chain starts from 1 * cc = cc exactly,
+  // so it rounds as C = cc followed by the four products
+  auto pair_t = [&](float mx, float my, float mz, float cp, int ng) {
+    pyr_v4f t = mfma4x4(1.f, cc[ng], pyr_v4f{0.f, 0.f, 0.f, 0.f});
+    t = mfma4x4(mx, bx[ng], t);
+    t = mfma4x4(my, by[ng], t);
+    if (D == 3) t = mfma4x4(mz, bz[ng], t);
+    return mfma4x4(cp, 1.f, t);
+  };
+  auto shape = [&](float t) {
+    if constexpr (A2) {
+      // max(t, 0)^2 as clamp(t |t|) to [0, 1] (t <= 1 up to rounding)
+      return __builtin_amdgcn_fmed3f(t * fabsf(t), 0.f, 1.f);
+    } else {
+      const float tw = __builtin_amdgcn_fmed3f(t, 0.f, 1.f);
+      return tw > 0.f ? __powf(tw, g.alpha) : 0.f;
+    }
+  };
+  // Two chunks against the groups either of them reaches.  A group the cull
+  // rejected for one of them has t < 0 at every pair of that chunk (all its
+  // nodes are farther than R from all its particles, by more than the FP32
+  // rounding of t), so S = 0 there and its sums are unchanged.
+  auto evaluate2 = [&](const Next &a, const Next &b) {
+    const uint32_t mask = a.ngm | b.ngm;
+    if (!mask) return;
+    const float ax = a.p.x + a.ox, ay = a.p.y + a.oy, az = D == 3 ? a.p.z + a.oz : 0.f;
+    const float add = fmaf(az, az, fmaf(ay, ay, ax * ax));
+    const float amx = ax * two_inv_r, amy = ay * two_inv_r, amz = az * two_inv_r;
+    const float acp = fmaf(-add, inv_rf, rf) * inv_p;
+    const float bxx = b.p.x + b.ox, byy = b.p.y + b.oy, bzz = D == 3 ? b.p.z + b.oz : 0.f;
+    const float bdd = fmaf(bzz, bzz, fmaf(byy, byy, bxx * bxx));
+    const float bmx = bxx * two_inv_r, bmy = byy * two_inv_r, bmz = bzz * two_inv_r;
+    const float bcp = fmaf(-bdd, inv_rf, rf) * inv_p;
 #pragma unroll
     for (int ng = 0; ng < NG; ++ng) {
-      const float4 nt = g.ntab[(size_t)e * (4 * NG) + 4 * ng + (lane & 3)];
-      bx[ng] = nt.x;
-      by[ng] = nt.y;
-      bz[ng] = nt.z;
-      cinit[ng] = pyr_v4f{nt.w, nt.w, nt.w, nt.w};
+      if (!((mask >> ng) & 1u)) continue;
+      const pyr_v4f ta = pair_t(amx, amy, amz, acp, ng);
+      const pyr_v4f tb = pair_t(bmx, bmy, bmz, bcp, ng);
+      acc0[ng] = mfma4x4(a.w.x, shape(ta[0]), acc0[ng]);
+      acc1[ng] = mfma4x4(b.w.x, shape(tb[0]), acc1[ng]);
+      acc0[ng] = mfma4x4(a.w.y, shape(ta[1]), acc0[ng]);
+      acc1[ng] = mfma4x4(b.w.y, shape(tb[1]), acc1[ng]);
+      acc0[ng] = mfma4x4(a.w.z, shape(ta[2]), acc0[ng]);
+      acc1[ng] = mfma4x4(b.w.z, shape(tb[2]), acc1[ng]);
+      acc0[ng] = mfma4x4(a.w.w, shape(ta[3]), acc0[ng]);
+      acc1[ng] = mfma4x4(b.w.w, shape(tb[3]), acc1[ng]);
     }
-    // the item's list entries, one per lane
-    uint2 ea = make_uint2(0u, 0u);
-    float4 oa = make_float4(0.f, 0.f, 0.f, 0.f);
-    if ((uint32_t)lane < cnt) {
-      ea = g.ent[base + lane];
-      oa = g.ent_off[base + lane];
+    if (++n_steps == FOLD) {
+      n_steps = 0;
+      fold();
     }
-    struct Next {
-      float4 p, w;
-      float ox, oy, oz;
-      uint32_t ngm;
-    };
-    auto rdf = [](float v, int l) {
-      return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-    };
-    auto fetch = [&](uint32_t i) {
-      const int l = (int)i;
-      Next n;
-      const uint32_t ch = (uint32_t)__builtin_amdgcn_readlane((int)ea.x, l);
-      n.ngm = (uint32_t)__builtin_amdgcn_readlane((int)ea.y, l);
-      n.ox = rdf(oa.x, l);
-      n.oy = rdf(oa.y, l);
-      n.oz = rdf(oa.z, l);
-      n.p = g.pos[(size_t)ch * 64 + lane];
-      n.w = g.wgt[(size_t)ch * 64 + lane];
-      return n;
-    };
-    auto evaluate = [&](const Next &n) {
-      const float dx = n.p.x + n.ox, dy = n.p.y + n.oy, dz = D == 3 ? n.p.z + n.oz : 0.f;
-      const float dd = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-      const float mx = dx * two_inv_r, my = dy * two_inv_r, mz = dz * two_inv_r;
-      const float cp = fmaf(-dd, inv_rf, rf) * inv_p;
-#pragma unroll
-      for (int ng = 0; ng < NG; ++ng) {
-        if (!((n.ngm >> ng) & 1u)) continue;
-        pyr_v4f t = mfma4x4(mx, bx[ng], cinit[ng]);
-        t = mfma4x4(my, by[ng], t);
-        if (D == 3) t = mfma4x4(mz, bz[ng], t);
-        t = mfma4x4(cp, 1.f, t);
-        float sv[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          if constexpr (A2) {
-            // max(t, 0)^2 as clamp(t |t|) to [0, 1] (t <= 1 up to rounding)
-            sv[i] = __builtin_amdgcn_fmed3f(t[i] * fabsf(t[i]), 0.f, 1.f);
-          } else {
-            const float tw = __builtin_amdgcn_fmed3f(t[i], 0.f, 1.f);
-            sv[i] = tw > 0.f ? __powf(tw, g.alpha) : 0.f;
-          }
-        }
-        acc[ng] = mfma4x4(n.w.x, sv[0], acc[ng]);
-        acc[ng] = mfma4x4(n.w.y, sv[1], acc[ng]);
-        acc[ng] = mfma4x4(n.w.z, sv[2], acc[ng]);
-        acc[ng] = mfma4x4(n.w.w, sv[3], acc[ng]);
-      }
-      if (++n_steps == FOLD) {
-        n_steps = 0;
-        fold();
-      }
-    };
-    // two chunks in flight while one is evaluated
-    Next a = fetch(0);
-    Next b = a;
-    if (cnt > 1) b = fetch(1);
+  };
+  // four named buffers: while one pair is evaluated the loads of the next
+  // pair are in flight, and no buffer is copied
+  Next a, b, c, d;
+  fetch(a, 0);
+  fetch(b, 1);
+  fetch(c, 2);
+  fetch(d, 3);
 #pragma unroll 1
-    for (uint32_t i = 0; i < cnt; ++i) {
-      Next c = b;
-      if (i + 2 < cnt) c = fetch(i + 2);
-      evaluate(a);
-      a = b;
-      b = c;
-    }
-    if (n_steps > 0) fold();
+  for (uint32_t i = 0; i < cnt; i += 4) {
+    evaluate2(a, b);
+    fetch(a, i + 4);
+    fetch(b, i + 5);
+    evaluate2(c, d);
+    fetch(c, i + 6);
+    fetch(d, i + 7);
   }
-  // lanes 0..15: Kahan pairs of blocks 0..3; combine the blocks in FP64
+  if (n_steps > 0) fold();
+  // combine the sixteen blocks (lanes l, l^4, ..., l^60) in FP64 in a fixed
+  // butterfly; lanes 0..3 then hold the item's sums of node slot 4 ng + lane
   double out[NG][4];
 #pragma unroll
   for (int ng = 0; ng < NG; ++ng)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       double val = 0.0;
-      if (folded && lane < 16) {
-        const float2 k = s_kahan[wave][ng * 4 + v][lane];
+      if (folded) {
+        const float2 k = kl[(ng * 4 + v) * 64];
         val = (double)k.x - (double)k.y;
       }
-      val += __shfl_xor(val, 4, 64);
-      val += __shfl_xor(val, 8, 64);
+#pragma unroll
+      for (int o = 4; o < 64; o <<= 1) val += __shfl_xor(val, o, 64);
       out[ng][v] = val;
     }
   if (lane < 4) {
@@ -967,7 +995,6 @@ __global__ __launch_bounds__(64 * W) void k_gather_cl(ListArgs g) {
       for (int v = 0; v < 4; ++v) dst[(4 * ng + lane) * 4 + v] = out[ng][v];
   }
 }
-
 
 // ---------------------------------------------------------------------------
 // adjoint of the deposition: per-particle gather of grad S . (rho_bar, j_bar)
