@@ -385,15 +385,33 @@ __global__ void k_item_keys(uint32_t n_items, const uint2 *__restrict__ items,
   key[i] = ent[cap_off[it.x] + it.y].x;
 }
 
+// Targets that may receive charge: those whose candidate list holds a source
+// element with particles, found through the transposed lists (one wave per
+// source element; concurrent writes store the same value).
+__global__ __launch_bounds__(256) void k_mark_active(int K, const uint32_t *__restrict__ ch_start,
+                                                     const uint32_t *__restrict__ tr_start,
+                                                     const uint32_t *__restrict__ tr,
+                                                     uint8_t *__restrict__ active) {
+  const int lane = threadIdx.x & 63;
+  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (s >= K || ch_start[s + 1] == ch_start[s]) return;
+  for (uint32_t k = tr_start[s] + lane; k < tr_start[s + 1]; k += 64) active[tr[k]] = 1;
+}
+
 // Candidate chunks of every target (the capacity of its chunk list): one wave
 // per target sums the chunk counts of its candidate source elements.
 __global__ __launch_bounds__(256) void k_cand_chunks(int K, const uint32_t *__restrict__ cand_start,
                                                      const uint32_t *__restrict__ cand,
                                                      const uint32_t *__restrict__ ch_start,
+                                                     const uint8_t *__restrict__ active,
                                                      uint32_t *__restrict__ cap) {
   const int lane = threadIdx.x & 63;
   const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (e >= K) return;
+  if (!active[e]) {   // no candidate source element holds a particle
+    if (lane == 0) cap[e] = 0;
+    return;
+  }
   uint32_t sum = 0;
   for (uint32_t k = cand_start[e] + lane; k < cand_start[e + 1]; k += 64) {
     const uint32_t s = cand[k];
@@ -425,6 +443,7 @@ struct DepositCache {
   DevBuf t_order, tab32g, ngsph, node_slot, ch_meta, pos32c, wgt32c;
   DevBuf cap, ent, ent_off, n_live, items, partial;
   DevBuf ikey, ik_a, ik_b, iv_a, iv_b;   // work items sorted by their first chunk
+  DevBuf active;                         // targets that may receive charge
   int64_t max_indeg = 0;   // most targets whose candidate list holds one source
   DevBuf nrm;            // normalized deposition: integrals, scaled charges, counter
   DevBuf skeys;   // Morton sort keys
@@ -446,7 +465,7 @@ void pyr_deposit_cache_free(pyr_mesh *mesh) {
                     &c->pos32, &c->wgt32, &c->nrm, &c->chunks, &c->t_order, &c->tab32g,
                     &c->ngsph, &c->node_slot, &c->ch_meta, &c->pos32c, &c->wgt32c, &c->cap,
                     &c->ent, &c->ent_off, &c->n_live, &c->items, &c->partial, &c->ikey,
-                    &c->ik_a, &c->ik_b, &c->iv_a, &c->iv_b})
+                    &c->ik_a, &c->ik_b, &c->iv_a, &c->iv_b, &c->active})
     b->release();
   delete c;
   mesh->dep = nullptr;
@@ -1154,8 +1173,16 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
       c->n_live.reserve(sizeof(uint32_t) * (size_t)std::max(1, K));
       {
         ProfScope prof(PROF_SORT);
+        if (c->tr_radius != radius) build_transpose(mesh, c);
+        c->active.reserve((size_t)std::max(1, K));
+        PYR_HIP(hipMemsetAsync(c->active.ptr, 0, (size_t)K, s));
+        hipLaunchKernelGGL(k_mark_active, dim3((unsigned)((K + 3) / 4)), dim3(256), 0, s, K,
+                           ch_start, c->tr_start.as<uint32_t>(), c->tr.as<uint32_t>(),
+                           c->active.as<uint8_t>());
+        PYR_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_cand_chunks, dim3((unsigned)((K + 3) / 4)), dim3(256), 0, s, K,
-                           c->cand_start.as<uint32_t>(), c->cand.as<uint32_t>(), ch_start, cap);
+                           c->cand_start.as<uint32_t>(), c->cand.as<uint32_t>(), ch_start,
+                           c->active.as<uint8_t>(), cap);
         PYR_CHECK_LAUNCH();
         scan_exclusive_u32(cap, cap_off, K, cap_off + K, scan_tmp, s);
         CullArgs cu{};
@@ -1173,6 +1200,7 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
         cu.ent = c->ent.as<uint2>();
         cu.ent_off = c->ent_off.as<float4>();
         cu.n_live = c->n_live.as<uint32_t>();
+        cu.active = c->active.as<uint8_t>();
         launch_cull_chunks(D, Np, cu);
       }
       // work items of up to 64 list entries, in Morton order of the targets

@@ -703,6 +703,7 @@ struct CullArgs {
   uint2 *ent;                  // list entries: chunk id, group mask
   float4 *ent_off;             //               source offset from the target (3), 0
   uint32_t *n_live;            // K: list length
+  const uint8_t *active;       // K: 0 when no candidate source element holds a particle
 };
 
 struct ListArgs {
@@ -732,6 +733,10 @@ __global__ __launch_bounds__(256) void k_cull_chunks(CullArgs g) {
   const int lane = threadIdx.x & 63;
   const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (e >= g.K) return;   // uniform per wave
+  if (!g.active[e]) {
+    if (lane == 0) g.n_live[e] = 0;
+    return;
+  }
   const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
   auto uni = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
   const float rf = (float)g.R;
