@@ -268,20 +268,16 @@ __constant__ double c_rk4b[5] = {
     1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
     2277821191437.0 / 14882151754819.0};
 
-// One stage of the fused step for one particle: k = [v(p), F(x, p)],
-// res = a_s res + dt k, y += b_s res, then relocalization.  Returns false if
-// the particle left the mesh.  Every forward variant (per-step, checkpointed,
-// and the adjoint's recompute) runs this one function, so they agree bitwise.
+// The update half of a stage for one particle: k = [v(p), F(x, p)] at unit
+// coordinates r in element e, res = a_s res + dt k, y += b_s res.  Returns
+// the wave-uniform element (INVALID if the wave's particles are in several).
 // MF (FP32 only): a full wave interpolates on the matrix cores, one pass per
-// element of its most mixed block of four particles.  Which waves qualify
-// depends on how particles share waves, so only the per-step push, which is
-// never recomputed, turns it on.
+// element of its most mixed block of four particles.
 template <int D, int N, class T, bool MF = false>
-__device__ __forceinline__ bool lsrk4_stage(const DevMesh &m, const T *__restrict__ coef, int s,
-                                            double dt, double c, double qi, double mi,
-                                            double *xi, double *pi, double *rx, double *rp,
-                                            double *r, uint32_t &e, FindCounters &cnt,
-                                            int &e_flag, unsigned &n_mf) {
+__device__ __forceinline__ uint32_t stage_update(const T *__restrict__ coef, int s, double dt,
+                                                 double c, double qi, double mi, double *xi,
+                                                 double *pi, double *rx, double *rp,
+                                                 const double *r, uint32_t e, unsigned &n_mf) {
   double v[D], F[D];
   velocity<D>(pi, mi, c, v);
   const uint32_t e_uni = wave_uniform_element(e);
@@ -316,8 +312,15 @@ __device__ __forceinline__ bool lsrk4_stage(const DevMesh &m, const T *__restric
     xi[k] += b * rx[k];
     pi[k] += b * rp[k];
   }
-  // fast path: still inside the (wave-uniform) element, same arithmetic
-  // as the first branch of find_new_containing_element
+  return e_uni;
+}
+
+// Unit coordinates of x in element e (the first test of
+// find_new_containing_element): through scalar loads when e is wave-uniform,
+// the same arithmetic either way.  True if x is still inside e.
+template <int D>
+__device__ __forceinline__ bool still_inside(const DevMesh &m, uint32_t e, uint32_t e_uni,
+                                             const double *xi, double *r) {
   if (e_uni != INVALID) {
     const uint32_t eu = (uint32_t)__builtin_amdgcn_readfirstlane((int)e_uni);
     const_ptr<double> am = as_constant(m.inv_map) + (size_t)eu * (D * D + D);
@@ -328,10 +331,29 @@ __device__ __forceinline__ bool lsrk4_stage(const DevMesh &m, const T *__restric
       for (int jj = 0; jj < D; ++jj) t += am[ii * D + jj] * xi[jj];
       r[ii] = t + am[D * D + ii];
     }
-    if (in_unit_simplex<D>(r)) {
-      cnt.same++;
-      return true;
-    }
+    return in_unit_simplex<D>(r);
+  }
+  return in_element<D>(m, e, xi, r);
+}
+
+// One stage of the fused step for one particle: the update, then
+// relocalization.  Returns false if the particle left the mesh.  Every forward
+// variant (the fast kernel, the per-step kernel, the checkpointed one and the
+// adjoint's recompute) runs stage_update and the same first containment test,
+// so they agree bitwise.
+template <int D, int N, class T, bool MF = false>
+__device__ __forceinline__ bool lsrk4_stage(const DevMesh &m, const T *__restrict__ coef, int s,
+                                            double dt, double c, double qi, double mi,
+                                            double *xi, double *pi, double *rx, double *rp,
+                                            double *r, uint32_t &e, FindCounters &cnt,
+                                            int &e_flag, unsigned &n_mf) {
+  const uint32_t e_uni =
+      stage_update<D, N, T, MF>(coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, n_mf);
+  // fast path: still inside the (wave-uniform) element, same arithmetic as
+  // the first branch of find_new_containing_element
+  if (e_uni != INVALID && still_inside<D>(m, e, e_uni, xi, r)) {
+    cnt.same++;
+    return true;
   }
   return relocate<D>(m, xi, pi, e, cnt, e_flag, r);
 }
@@ -352,10 +374,13 @@ __global__ __launch_bounds__(256, (push_min_waves<N, T>())) void k_lsrk4_step(
     double *__restrict__ x, double *__restrict__ p, const double *__restrict__ q,
     const double *__restrict__ mass, uint32_t *__restrict__ elem,
     uint32_t *__restrict__ death, unsigned long long *counters, int *err,
-    unsigned int *n_dead, unsigned long long *mf_waves, const uint32_t *__restrict__ order) {
-  // order (a permutation of [0, n), or null): thread t steps particle order[t]
+    unsigned int *n_dead, unsigned long long *mf_waves, const uint32_t *__restrict__ order,
+    const uint32_t *n_dev) {
+  // order (particle indices, or null): thread t steps particle order[t], for
+  // t < n (n_dev: a device count at most n)
   const int64_t t =
       (order ? xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x) * blockDim.x + threadIdx.x;
+  if (n_dev) n = min(n, (int64_t)*n_dev);
   FindCounters cnt;
   unsigned n_mf = 0;   // stages this particle was interpolated on the matrix cores
   if (t < n) {
@@ -935,17 +960,17 @@ int pyr_lsrk4_steps(pyr_mesh *mesh, int precision, int nsteps, double dt, double
             hipLaunchKernelGGL((k_lsrk4_step<D_, N_, double, false>), dim3(blocks_for(n)),
                                dim3(256), 0, stream(), mesh->dev, mesh->sf64.as<double>(), n, dt,
                                vacuum_c, x, p, q, mass, elem, death, counters, err, n_dead, mf_waves,
-                               order);
+                               order, (const uint32_t *)nullptr);
           else if (g_push_mfma)
             hipLaunchKernelGGL((k_lsrk4_step<D_, N_, float, true>), dim3(blocks_for(n)),
                                dim3(256), 0, stream(), mesh->dev, mesh->sf32.as<float>(), n, dt,
                                vacuum_c, x, p, q, mass, elem, death, counters, err, n_dead, mf_waves,
-                               order);
+                               order, (const uint32_t *)nullptr);
           else
             hipLaunchKernelGGL((k_lsrk4_step<D_, N_, float, false>), dim3(blocks_for(n)),
                                dim3(256), 0, stream(), mesh->dev, mesh->sf32.as<float>(), n, dt,
                                vacuum_c, x, p, q, mass, elem, death, counters, err, n_dead, mf_waves,
-                               order);
+                               order, (const uint32_t *)nullptr);
           PYR_CHECK_LAUNCH();
         });
       }

@@ -195,6 +195,44 @@ __device__ uint32_t find_global(const DevMesh &m, const double *x, double *r) {
   return INVALID;
 }
 
+// The tail of the cascade (particle_state.hpp:198-261): the elements around
+// the closest vertex of prev, then the global search.  (Out of line it saved
+// three registers and cost the push 2x in spills around the call.)
+template <int D>
+__device__ __forceinline__ uint32_t find_by_vertex_or_global(const DevMesh &m,
+                                                             const double *x, uint32_t prev,
+                                                             FindCounters &cnt, double *r) {
+  if (prev != INVALID && prev < (uint32_t)m.K) {
+    uint32_t closest_vertex = INVALID;
+    double min_dist = __builtin_inf();
+#pragma unroll
+    for (int k = 0; k <= D; ++k) {
+      uint32_t vi = m.el_vertices[(size_t)prev * (D + 1) + k];
+      double s = 0;
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        double dv = m.vertices[(size_t)vi * D + i] - x[i];
+        s += dv * dv;
+      }
+      double dist = sqrt(s);
+      if (dist < min_dist) {
+        closest_vertex = vi;
+        min_dist = dist;
+      }
+    }
+    for (uint32_t k = m.vadj_starts[closest_vertex]; k < m.vadj_starts[closest_vertex + 1];
+         ++k) {
+      uint32_t possible2 = m.vadj_elements[k];
+      if (in_element<D>(m, possible2, x, r)) {
+        cnt.vertex++;
+        return possible2;
+      }
+    }
+  }
+  cnt.global++;
+  return find_global<D>(m, x, r);
+}
+
 // particle_state.hpp:130-261.  *err is set when the reference would throw
 // "no best normal found--weird".  r receives the unit coordinates of x in the
 // returned element (valid when the result is not INVALID).
@@ -229,34 +267,8 @@ __device__ uint32_t find_new_containing_element(const DevMesh &m, const double *
       cnt.neighbor++;
       return possible;
     }
-    uint32_t closest_vertex = INVALID;
-    double min_dist = __builtin_inf();
-#pragma unroll
-    for (int k = 0; k <= D; ++k) {
-      uint32_t vi = m.el_vertices[(size_t)prev * (D + 1) + k];
-      double s = 0;
-#pragma unroll
-      for (int i = 0; i < D; ++i) {
-        double dv = m.vertices[(size_t)vi * D + i] - x[i];
-        s += dv * dv;
-      }
-      double dist = sqrt(s);
-      if (dist < min_dist) {
-        closest_vertex = vi;
-        min_dist = dist;
-      }
-    }
-    for (uint32_t k = m.vadj_starts[closest_vertex]; k < m.vadj_starts[closest_vertex + 1];
-         ++k) {
-      uint32_t possible2 = m.vadj_elements[k];
-      if (in_element<D>(m, possible2, x, r)) {
-        cnt.vertex++;
-        return possible2;
-      }
-    }
   }
-  cnt.global++;
-  return find_global<D>(m, x, r);
+  return find_by_vertex_or_global<D>(m, x, prev, cnt, r);
 }
 
 // update_containing_elements + boundary_hit for one particle
