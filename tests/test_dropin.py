@@ -352,6 +352,167 @@ def test_static_field_kat_on_gpu(periodic_cylinder, cuda, kind, pusher):
     assert xe < 2e-12 and ve < 2e-13, (xe, ve)
 
 
+def _kv_beam(n, rng, units, radii, emittances, beta, z_pos, z_length, p_mass):
+    """A chargeless KV beam uniform over a z interval (the reference's
+    KVZIntervalBeam, distribution.py:234-366): (x/r, x' r/eps) uniform on the
+    unit 3-sphere in the 4D transverse phase space; transverse velocity x' |v_z|;
+    v_z = beta c.  Returns (positions, velocities, charges, masses)."""
+    s = rng.normal(size=(n, 4))
+    s /= np.linalg.norm(s, axis=1, keepdims=True)
+    vz = beta * units.VACUUM_LIGHT_SPEED()
+    x = s[:, :2] * np.asarray(radii)
+    xp = s[:, 2:] / np.asarray(radii) * np.asarray(emittances)
+    z = rng.uniform(z_pos - z_length / 2, z_pos + z_length / 2, size=n)
+    pos = np.column_stack([x, z])
+    vel = np.column_stack([xp * vz, np.full(n, vz)])
+    return pos, vel, np.zeros(n), np.full(n, p_mass)
+
+
+def test_kv_beam_free_streaming(cuda):
+    """test_pyrticle.py:79-183 (test_kv_with_no_charge): a chargeless KV beam
+    streams for 1 m along a 100 mm periodic cylinder (it wraps around ten
+    times) through the per-stage LSRK4 path with the reference's right-hand
+    side [v, 0, 0] (v frozen at the start); the RMS radius along x, logged
+    before every step, stays within 1% of the chargeless KV envelope
+    sqrt(a0^2 + (eps/a0)^2 s^2), a0 and eps the RMS radius and emittance
+    (distribution.py:411-417)."""
+    from pyrticle_amd import discretization as D
+    from pyrticle_amd import _internal
+    from pyrticle_amd.cloud import TimesteppablePicState
+    from pyrticle_amd.timestep import LSRK4TimeStepper
+    from pyrticle_amd.tools import NumberShiftableVector, make_obj_array
+    from pyrticle_amd.units import SIUnitsWithNaturalConstants
+    units = SIUnitsWithNaturalConstants()
+    tube_length = 100 * units.MM
+    mesh = D.make_cylinder_mesh(radius=25 * units.MM, z0=0.0, z1=tube_length, n_rings=4,
+                                n_layers=16, periodic=True, seed=0)
+    discr = D.Discretization(mesh, 3)
+    c = units.VACUUM_LIGHT_SPEED()
+    final_time = 1 * units.M / c
+    dt = 0.5 * D.min_edge_length(mesh) / c
+    nsteps = int(final_time / dt) + 1
+    dt = final_time / nsteps
+    method = _method(discr, 3)
+    nparticles = 10000
+    cloud_charge = 1e-9 * units.C
+    electrons_per_particle = cloud_charge / nparticles / units.EL_CHARGE
+    gamma = 5.2e6 * units.EV / units.EL_REST_ENERGY()
+    beta = (1 - 1 / gamma ** 2) ** 0.5
+    radius, emittance = 2.5 * units.MM, 5 * units.MM * units.MRAD
+    pos, vel, q, m = _kv_beam(nparticles, np.random.default_rng(0), units, 2 * [radius],
+                              2 * [emittance], beta, 10 * units.MM, 5 * units.MM,
+                              electrons_per_particle * units.EL_MASS)
+    state = method.make_state()
+    method.add_particles(state, zip(pos, vel, q, m))
+    assert len(state) == nparticles
+    # RMS radius and emittance are the total ones over 2 and 4
+    a0, eps = radius / 2, emittance / 4
+    v = method.velocities(state)
+
+    def rhs(t, ts):
+        return make_obj_array([
+            NumberShiftableVector(v, signaller=ts.state.particle_number_shift_signaller),
+            0, 0])
+
+    # the exact free-streaming RMS of this sample: x(t) = x0 + v_x t
+    x0 = state.positions[:, 0].cpu().numpy().copy()
+    vx0 = v[:, 0].cpu().numpy().copy()
+    stepper = LSRK4TimeStepper()
+    ts = TimesteppablePicState(method, state)
+    t, worst, worst_exact = 0.0, 0.0, 0.0
+    for step in range(nsteps + 1):
+        rx = _internal.rms_beam_size(ts.state.particle_state, 0)
+        s_path = beta * c * t
+        theory = (a0 ** 2 + (eps / a0) ** 2 * s_path ** 2) ** 0.5
+        worst = max(worst, abs(rx - theory) / theory)
+        exact = float(np.sqrt(np.mean((x0 + vx0 * t) ** 2)))
+        worst_exact = max(worst_exact, abs(rx - exact) / exact)
+        if step == nsteps:
+            break
+        ts = stepper(ts, t, dt, rhs)
+        method.upkeep(ts.state)
+        t += dt
+    z = ts.state.positions[:, 2].cpu().numpy()
+    print("kv beam: %d steps, max relative rms error %.4f (%.1e against the exact streaming "
+          "of the sample)" % (nsteps, worst, worst_exact))
+    assert len(ts.state) == nparticles
+    assert z.min() >= 0.0 and z.max() <= tube_length      # wrapped into the period
+    assert worst_exact < 1e-9
+    assert worst < 0.01
+
+
+@pytest.mark.parametrize("kind", ["screw", "epb"])
+def test_static_field_force_bound_per_stage(periodic_cylinder, cuda, kind):
+    """The force bound of test/static_field.py:415 (< 2e-4), read as the
+    reference reads it: the el_force + mag_force vis vectors that the
+    MonomialParticlePusher stores during the per-stage RHS (static_field.py:
+    339-342, push_monomial.hpp:307-322), which hold the forces of the last stage
+    evaluated (time t - 0.042 dt), against the Lorentz force q(E + v x B) on the
+    analytic trajectory at t; with the position and velocity bounds (2e-12,
+    2e-13).  The reference's dt (period / 1e4) and check cadence (every
+    nsteps/300 steps), over the first ten checks."""
+    from pyrticle_amd.cloud import (FieldToParticleRhsCalculator, ParticleRhsCalculator,
+                                    TimesteppablePicState)
+    from pyrticle_amd.fields import StaticFieldOperator
+    from pyrticle_amd.timestep import LSRK4TimeStepper
+    from pyrticle_amd.tools import NumberShiftableVector
+    from pyrticle_amd.units import SIUnitsWithNaturalConstants as U
+    discr, ma = periodic_cylinder
+    setup = _Setup(kind)
+    method = _method(discr, 3)
+    x0, v0 = setup.state(0.0)
+    c = U.VACUUM_LIGHT_SPEED()
+    gam = 1 / np.sqrt(1 - np.sum(v0 ** 2, axis=1) / c ** 2)
+    state = method.make_state()
+    n = len(x0)
+    method.add_particle_arrays(state, x0, U.EL_MASS * gam[:, None] * v0,
+                               np.full(n, setup.charge), np.full(n, U.EL_MASS))
+    op = StaticFieldOperator(eh_subset=(False, False, True, False, False, True))
+    nn = ma.nnodes
+    full = lambda v: torch.full((nn,), v, dtype=torch.float64, device=cuda)  # noqa: E731
+    fields = op.assemble_eh(e=[full(setup.ez)], h=[full(setup.bz / U.MU0)])
+    p_rhs = ParticleRhsCalculator(method, op)
+    f_rhs = FieldToParticleRhsCalculator(method, op)
+
+    def rhs(t, ts):
+        return p_rhs(t, lambda: fields, lambda: ts.state) + f_rhs(t, lambda: fields,
+                                                                   lambda: ts.state)
+
+    nsteps = 10 ** 4
+    dt = setup.T / nsteps
+    every = nsteps // 300
+    stepper = LSRK4TimeStepper()
+    ts = TimesteppablePicState(method, state)
+    rhs(0.0, ts)          # fills the vis vectors, as the reference does before its loop
+    z0, z1 = ma.periodicity[2]
+    period = z1 - z0
+    e_field = np.array([0.0, 0.0, setup.ez])
+    b_field = np.array([0.0, 0.0, setup.bz])
+    xe = ve = fe = 0.0
+    t = 0.0
+    for step in range(10 * every):
+        if step % every == 0:
+            tx, tv = setup.state(t)
+            true_f = setup.charge * (e_field[None, :] + np.cross(tv, b_field[None, :]))
+            vis = ts.state.vis_listener.particle_vis_map
+            sim_f = (NumberShiftableVector.unwrap(vis["mag_force"])
+                     + NumberShiftableVector.unwrap(vis["el_force"])).cpu().numpy()
+            x = ts.state.positions.cpu().numpy()
+            v = method.velocities(ts.state).cpu().numpy()
+            diff = x - tx
+            diff[:, 2] -= period * np.round(diff[:, 2] / period)
+            tx[:, 2] = np.mod(tx[:, 2] - z0, period) + z0
+            xe = max(xe, np.max(np.linalg.norm(diff, axis=1) / np.linalg.norm(tx, axis=1)))
+            ve = max(ve, np.max(np.linalg.norm(v - tv, axis=1) / np.linalg.norm(tv, axis=1)))
+            fe = max(fe, np.max(np.linalg.norm(sim_f[:, :3] - true_f, axis=1)
+                                / np.linalg.norm(true_f, axis=1)))
+        method.upkeep(ts.state)
+        ts = stepper(ts, t, dt, rhs)
+        t += dt
+    print("%s: pos %.1e vel %.1e force %.1e" % (kind, xe, ve, fe))
+    assert xe < 2e-12 and ve < 2e-13 and fe < 2e-4, (xe, ve, fe)
+
+
 @pytest.mark.parametrize("normalized", [False, True])
 def test_average_pusher_dropin(square, oracle_square, cuda, normalized):
     """AverageParticlePusher through PicMethod (pusher.py:167-207): forces from
