@@ -290,6 +290,173 @@ def measured_traffic(kernel):
         return None
 
 
+# ---- rocprofv3 summaries (CPU only: reads the CSV files of profiled runs) ----
+# The profiled command and the layout of its output directory:
+#   <dir>/kt/run_kernel_stats.csv                 --kernel-trace --stats
+#   <dir>/pmc_<C>/run_counter_collection.csv      --pmc C, one run per C below
+#   <dir>/copy_<C>/run_counter_collection.csv     --pmc C over --copy-probe
+PROFILE_ARGS = ("--steps 2 --warmup 1 --no-cpu-baseline --no-adjoint --no-dense --no-square "
+                "--extension-reps 2 --average-steps 1")
+PROFILE_COUNTERS = ("FETCH_SIZE", "WRITE_SIZE", "TCC_HIT_sum TCC_MISS_sum", "VALUBusy",
+                    "MfmaUtil", "OccupancyPercent")
+# traffic keys read by measured_traffic: the kernel (regular expression on the
+# dispatched name; the instantiation with the most dispatches is taken)
+TRAFFIC_KERNELS = {
+    "k_gather_cl": r"k_gather_cl<", "k_cull_chunks": r"k_cull_chunks<",
+    "k_stage32c": r"k_stage32c\b", "k_combine_items": r"k_combine_items\b",
+    "k_gather": r"k_gather<", "k_lsrk4_step": r"k_lsrk4_step<3, 3, double",
+    "k_lsrk4_step_fp32c": r"k_lsrk4_step<3, 3, float, false", "k_stage": r"k_stage\(",
+    "k_rs_scatter": r"k_rs_scatter\b", "k_shape_integral": r"k_shape_integral<",
+    "k_shape_average": r"k_shape_average<", "k_reduce": r"k_reduce\b",
+}
+
+
+def _profile_cmd(kind, counters=None, probe=False):
+    base = "cd /tmp && export TMPDIR=/tmp && rocprofv3 "
+    if kind == "kt":
+        return (base + "--kernel-trace --stats --output-format csv -d "
+                "$GRAFT_REPO_ROOT/gpurun_out/pf/kt -o run -- python3 $GRAFT_REPO_ROOT/bench.py "
+                + PROFILE_ARGS)
+    tag = ("copy_" if probe else "pmc_") + counters.split()[0].replace("_sum", "")
+    return (base + "--pmc %s --output-format csv -d $GRAFT_REPO_ROOT/gpurun_out/pf/%s -o run "
+            "-- python3 $GRAFT_REPO_ROOT/bench.py %s"
+            % (counters, tag, "--copy-probe" if probe else PROFILE_ARGS))
+
+
+def _kernel_key(name):
+    """Dispatched kernel name without its argument list."""
+    depth = 0
+    for k, ch in enumerate(name):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            return name[:k]
+    return name
+
+
+def _counter_means(path):
+    """{kernel: {"dispatches": n, counter: mean per dispatch, "grid": max grid}}."""
+    import csv
+    acc = {}
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            k = _kernel_key(row["Kernel_Name"])
+            d = acc.setdefault(k, {"_ids": set(), "grid": 0})
+            d["_ids"].add(row["Dispatch_Id"])
+            d["grid"] = max(d["grid"], int(row["Grid_Size"]))
+            c = row["Counter_Name"]
+            d.setdefault(c, []).append(float(row["Counter_Value"]))
+    out = {}
+    for k, d in acc.items():
+        o = {"dispatches": len(d.pop("_ids")), "grid": d.pop("grid")}
+        for c, vals in d.items():
+            o[c] = sum(vals) / len(vals)
+        out[k] = o
+    return out
+
+
+def _pick(table, pattern):
+    import re
+    hits = [(v["dispatches"] if isinstance(v, dict) and "dispatches" in v else v.get("calls", 0),
+             k) for k, v in table.items() if re.search(pattern, k)]
+    return max(hits)[1] if hits else None
+
+
+def summarize_profiles(root, workload, out_json):
+    """Kernel-trace table, HBM bytes per launch and counter averages from the
+    CSV output of the rocprofv3 runs named by _profile_cmd; writes out_json
+    (read by measured_traffic) and prints markdown tables."""
+    import csv
+    stats = {}
+    with open(os.path.join(root, "kt", "run_kernel_stats.csv"), newline="") as f:
+        for row in csv.DictReader(f):
+            stats[_kernel_key(row["Name"])] = {"calls": int(row["Calls"]),
+                                               "avg_us": float(row["AverageNs"]) / 1e3,
+                                               "total_us": float(row["TotalDurationNs"]) / 1e3,
+                                               "pct": float(row["Percentage"])}
+    pmc = {}
+    for c in PROFILE_COUNTERS:
+        tag = c.split()[0].replace("_sum", "")
+        path = os.path.join(root, "pmc_" + tag, "run_counter_collection.csv")
+        if os.path.exists(path):
+            for k, v in _counter_means(path).items():
+                pmc.setdefault(k, {"dispatches": 0})
+                pmc[k]["dispatches"] = max(pmc[k]["dispatches"], v.pop("dispatches"))
+                v.pop("grid")
+                pmc[k].update(v)
+    cal = {"kernel": "__amd_rocclr_copyBuffer (512 MiB device-to-device copy)",
+           "bytes_read": 512 << 20, "bytes_written": 512 << 20}
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        rows = _counter_means_rows(os.path.join(root, "copy_" + c, "run_counter_collection.csv"),
+                                   c)
+        cal[c + "_KiB"] = rows
+        cal[("fetch" if c == "FETCH_SIZE" else "write") + "_factor"] = \
+            (512 << 20) / (rows * 1024.0)
+    kernels = {}
+    for key, pat in TRAFFIC_KERNELS.items():
+        name = _pick(pmc, pat)
+        if name is None or "FETCH_SIZE" not in pmc[name] or "WRITE_SIZE" not in pmc[name]:
+            continue
+        f, w = pmc[name]["FETCH_SIZE"], pmc[name]["WRITE_SIZE"]
+        kernels[key] = {"name": name, "FETCH_SIZE_KiB": f, "WRITE_SIZE_KiB": w,
+                        "hbm_bytes": 1024.0 * (f * cal["fetch_factor"]
+                                               + w * cal["write_factor"])}
+    doc = {"workload": workload,
+           "counter_unit": "KiB per dispatch (average over the run's dispatches of the kernel)",
+           "calibration": cal,
+           "commands": [_profile_cmd("pmc", "FETCH_SIZE"), _profile_cmd("pmc", "WRITE_SIZE"),
+                        _profile_cmd("pmc", "FETCH_SIZE", probe=True),
+                        _profile_cmd("pmc", "WRITE_SIZE", probe=True)],
+           "kernels": kernels}
+    with open(out_json, "w") as f:
+        json.dump(doc, f, indent=1)
+        f.write("\n")
+    print("## kernel trace\n\n| kernel | calls | avg µs | total µs | % |\n|---|---:|---:|---:|---:|")
+    for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["total_us"])[:30]:
+        print("| `%s` | %d | %.1f | %.1f | %.1f |" % (k[:100], v["calls"], v["avg_us"],
+                                                   v["total_us"], v["pct"]))
+    print("\n## counters\n\n| kernel | avg µs (trace) | HBM GB/launch | L2 hit % | VALUBusy % "
+          "| MfmaUtil % | OccupancyPercent |\n|---|---:|---:|---:|---:|---:|---:|")
+    for key, pat in TRAFFIC_KERNELS.items():
+        name = _pick(pmc, pat)
+        if name is None:
+            continue
+        v = pmc[name]
+        hit, miss = v.get("TCC_HIT_sum"), v.get("TCC_MISS_sum")
+        l2 = 100.0 * hit / (hit + miss) if hit is not None and miss and hit + miss else None
+        tr = kernels.get(key, {}).get("hbm_bytes")
+
+        def fmt(x, spec="%.1f"):
+            return spec % x if x is not None else "-"
+        print("| `%s` | %s | %s | %s | %s | %s | %s |" % (
+            name, fmt(stats.get(name, {}).get("avg_us")), fmt(tr / 1e9 if tr else None, "%.2f"),
+            fmt(l2), fmt(v.get("VALUBusy")), fmt(v.get("MfmaUtil")),
+            fmt(v.get("OccupancyPercent"))))
+    print("\ncommands:")
+    print(_profile_cmd("kt"))
+    for c in PROFILE_COUNTERS:
+        print(_profile_cmd("pmc", c))
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        print(_profile_cmd("pmc", c, probe=True))
+
+
+def _counter_means_rows(path, counter):
+    """Mean counter value over the copy kernel's dispatches that read at least
+    half the largest value (the 512 MiB copies of --copy-probe)."""
+    import csv
+    vals = []
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            if row["Counter_Name"] == counter and "copyBuffer" in row["Kernel_Name"]:
+                vals.append(float(row["Counter_Value"]))
+    if not vals:
+        raise ValueError("%s: no copy dispatch with counter %s" % (path, counter))
+    top = [v for v in vals if v >= 0.5 * max(vals)]
+    return sum(top) / len(top)
+
+
 def adjoint_leg(wl, dm, device, torch, normalizer, n, nsteps, ckpt, fd_n):
     """Gradient of the density-matching loss w.r.t. the 6 beam parameters and
     all initial phase-space coordinates: forward = nsteps push steps + final
@@ -416,16 +583,16 @@ def dense_p4_leg(device, torch, steps, warmup, n_particles):
     try:
         for name, prec, mf in (("fp64", dv.FP64, False), ("fp32c_valu", dv.FP32C, False),
                                ("fp32c_mfma", dv.FP32C, True)):
-            dv.set_push_mfma(mf)
+            dm.set_push_mfma(mf)
             if mf:   # untimed pass: share of particle-stages on the matrix cores
-                dv.set_push_mfma(True, count=True)
-                dv.push_mfma_waves()
+                dm.set_push_mfma(True, count=True)
+                dm.push_mfma_waves()
                 st = clone_state(st0)
                 n_b = st["n"]
                 st["n"] = dm.lsrk4_steps(1, wl.dt, wl.c, st["n"], st["x"], st["p"], st["q"],
                                          st["m"], st["elem"], precision=prec)
-                res["mfma_share_of_particle_stages"] = dv.push_mfma_waves() / (5.0 * n_b)
-                dv.set_push_mfma(True)
+                res["mfma_share_of_particle_stages"] = dm.push_mfma_waves() / (5.0 * n_b)
+                dm.set_push_mfma(True)
             st = clone_state(st0)
             for _ in range(warmup):
                 st["n"] = dm.lsrk4_steps(1, wl.dt, wl.c, st["n"], st["x"], st["p"], st["q"],
@@ -450,7 +617,7 @@ def dense_p4_leg(device, torch, steps, warmup, n_particles):
             log("dense P4 %s: %.3f ms/step, push kernel %.3f ms" % (
                 name, 1e3 * wall / steps, push_ms / max(1, push_n)))
     finally:
-        dv.set_push_mfma(False)
+        dm.set_push_mfma(False)
     ref = finals["fp64"]
     for name in ("fp32c_valu", "fp32c_mfma"):
         st = finals[name]
@@ -613,7 +780,17 @@ def main(argv=None):
     ap.add_argument("--average-steps", type=int, default=3)
     ap.add_argument("--no-square", action="store_true",
                     help="skip the 2D workload leg (FP64 and FP32C against the oracle)")
+    ap.add_argument("--summarize-profiles", metavar="DIR",
+                    help="CPU only: summarize the rocprofv3 CSV output under DIR (kt/, "
+                         "pmc_<counter>/, copy_<counter>/) into profiles/"
+                         "traffic_cylinder3d_4M.json and print the tables")
     args = ap.parse_args(argv)
+    if args.summarize_profiles:
+        summarize_profiles(args.summarize_profiles,
+                           "cylinder_3d, 4M electrons, 200,910 P3 tets (FP64 and FP32C steps, "
+                           "normalized deposition, averaging pusher, diagnostics)",
+                           os.path.join(ROOT, "profiles", "traffic_cylinder3d_4M.json"))
+        return 0
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py runs on one GPU only: the particle path here is built and measured "
@@ -671,12 +848,12 @@ def main(argv=None):
 
     # work counts of the last state (counting pass, outside the timed region)
     n_last = st["n"]
-    dv.set_deposit_counting(True)
+    dm.set_deposit_counting(True)
     v = dv.velocities(st["p"][:n_last], st["m"][:n_last], wl.c)
     rho64, j64 = dm.deposit(wl.radius, wl.alpha, normalizer, st["x"][:n_last],
                             st["q"][:n_last], st["elem"][:n_last], v=v)
     n_eval, n_in, n_cand, n_pass = dm.deposit_stats()
-    dv.set_deposit_counting(False)
+    dm.set_deposit_counting(False)
     occupied = int(torch.unique(st["elem"][:n_last]).numel())
 
     # ---- FP32C ------------------------------------------------------------
@@ -685,6 +862,7 @@ def main(argv=None):
         rho32, j32 = dm.deposit(wl.radius, wl.alpha, normalizer, st["x"][:n_last],
                                 st["q"][:n_last], st["elem"][:n_last], v=v,
                                 precision=dv.FP32C)
+        list_entries, list_groups, list_instr, list_items = dm.deposit_list_stats()
         rho_err = float(torch.linalg.norm(rho32 - rho64) / torch.linalg.norm(rho64))
         j_err = float(torch.linalg.norm(j32 - j64) / torch.linalg.norm(j64))
         st32 = clone_state(st0)
@@ -707,7 +885,10 @@ def main(argv=None):
                  "speedup_vs_fp64": (alive32 / (t1 - t0)) / value,
                  "rho_rel_l2_vs_fp64": rho_err, "j_rel_l2_vs_fp64": j_err,
                  "x_rel_l2_vs_fp64": x_err, "p_rel_l2_vs_fp64": p_err,
-                 "gather_ms": g32_ms / max(1, g32_n), "push_ms": p32_ms / max(1, p32_n)}
+                 "gather_ms": g32_ms / max(1, g32_n), "push_ms": p32_ms / max(1, p32_n),
+                 "lists": {"chunk_target_entries": list_entries,
+                           "chunk_group_pairs": list_groups,
+                           "mfma_4x4x1_instructions": list_instr, "work_items": list_items}}
         log("fp32c: %.3f ms/step (x%.2f), rho err %.2e" % (fp32c["ms_per_step"],
                                                            fp32c["speedup_vs_fp64"], rho_err))
 
@@ -803,21 +984,23 @@ def main(argv=None):
     if headline32:
         # FP32C gather: the same algorithmic work against the FP32 peak (the
         # matrix cores and the VALU share it: 157.3 TFLOP/s).  The matrix-core
-        # flops the kernel issues (every candidate particle against every node:
-        # K=4 distance product plus the 4-component accumulation, 2 * Np * 8 per
-        # candidate) are reported beside it.
+        # flops the list gather issues (512 per 4x4x1 instruction, counted
+        # exactly from the last deposition's lists) are reported beside it.
+        # The region timed as "gather" is the list gather k_gather_cl alone;
+        # the fixed-order combination of its partials (k_combine_items) is
+        # timed with the sort and the other set-up work.
         g32_s = fp32c["gather_ms"] / 1e3
         tflops = flops / g32_s / 1e12 if g32_s > 0 else 0.0
-        issued = 2.0 * Np * 8 * n_cand
-        roofline = {"bound": "mfma", "kernel": "k_gather_mf (rho+j deposition, FP32 matrix "
-                    "cores, Kahan-compensated)", "achieved": tflops,
+        issued = 512.0 * fp32c["lists"]["mfma_4x4x1_instructions"]
+        roofline = {"bound": "mfma", "kernel": "k_gather_cl (rho+j deposition over culled "
+                    "chunk lists, FP32 matrix cores, Kahan-compensated)", "achieved": tflops,
                     "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": tflops / FP32_MFMA_PEAK_TFLOPS,
-                    "traffic": measured_traffic("k_gather_mf"), "traffic_source": traffic_note,
+                    "traffic": measured_traffic("k_gather_cl"), "traffic_source": traffic_note,
                     "avg_launch_ms": fp32c["gather_ms"],
                     "work": dict(work, issued_mfma_flops_per_launch=issued,
                                  issued_mfma_tflops=issued / g32_s / 1e12 if g32_s > 0 else 0.0,
-                                 candidate_particle_visits=n_cand)}
+                                 lists=fp32c["lists"])}
     else:
         roofline = roof64
 

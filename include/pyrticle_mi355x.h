@@ -182,16 +182,17 @@ int pyr_lsrk4_steps(pyr_mesh *mesh, int precision, int nsteps, double dt, double
                     double *masses, uint32_t *containing_elements, int64_t *ids,
                     unsigned long long *counters);
 
-/* PYR_FP32C pyr_lsrk4_steps: evaluate the interpolation contraction (6 fields
- * x Np monomials) of full waves on the matrix cores instead of the vector ALUs,
- * one pass per element present in a block of four consecutive particles (one
- * pass when particles are sorted by element).  mode: 0 off (default), 1 on,
- * 2 on and count the particle-stages that take the matrix-core path
- * (pyr_push_mfma_waves returns the count since the last call and resets it).
+/* PYR_FP32C pyr_lsrk4_steps on this mesh: evaluate the interpolation
+ * contraction (6 fields x Np monomials) of full waves on the matrix cores
+ * instead of the vector ALUs, one pass per element present in a block of four
+ * consecutive particles (one pass when particles are sorted by element).
+ * mode: 0 off (default), 1 on, 2 on and count the particle-stages that take
+ * the matrix-core path (pyr_push_mfma_waves returns the count since the last
+ * call and resets it).  The setting belongs to the mesh handle.
  * No reference counterpart: a switch between two evaluations of the same FP32
  * interpolant (push_monomial.hpp:99-111), kept for measurement and tests. */
-int pyr_push_set_mfma(int mode);
-int pyr_push_mfma_waves(int64_t *out);
+int pyr_push_set_mfma(pyr_mesh *mesh, int mode);
+int pyr_push_mfma_waves(pyr_mesh *mesh, int64_t *out);
 
 /* adjoint of the push ------------------------------------------------------ */
 /* pyr_lsrk4_steps with the averaging pusher's force (push_average.hpp:338-443,
@@ -353,12 +354,19 @@ int pyr_deposit_backward(pyr_mesh *mesh, double radius, double alpha, double nor
                          const uint32_t *containing_elements, const double *rho_bar,
                          const double *j_bar, double *x_bar, double *v_bar);
 
-/* Instrumentation (no reference counterpart): when on, pyr_deposit also counts
- * [node-particle pairs evaluated, pairs inside R, candidate particle visits,
- * visits passing the per-element test]; pyr_deposit_stats reads the counts of
- * the last call. */
-int pyr_deposit_set_counting(int on);
+/* Instrumentation (no reference counterpart): when on, pyr_deposit on this
+ * mesh also counts [node-particle pairs evaluated, pairs inside R, candidate
+ * particle visits, visits passing the per-element test]; pyr_deposit_stats
+ * reads the counts of the last call.  FP64 precision only. */
+int pyr_deposit_set_counting(pyr_mesh *mesh, int on);
 int pyr_deposit_stats(pyr_mesh *mesh, unsigned long long out[4]);
+
+/* Instrumentation (no reference counterpart): statistics of the culled chunk
+ * lists of the last PYR_FP32C pyr_deposit on this mesh: [list entries
+ * (chunk-target pairs), listed (chunk, node group) pairs, 4x4x1 matrix-core
+ * instructions the list gather issued, work items]; zeros when the last
+ * deposition was not PYR_FP32C.  Synchronizes the library stream. */
+int pyr_deposit_list_stats(pyr_mesh *mesh, unsigned long long out[4]);
 
 /* Number of (target element, source element) pairs in the gather candidate
  * lists for radius R (built on first use, cached per radius). */

@@ -76,10 +76,11 @@ _SIGNATURES = {
     "pyr_lsrk4_steps_average": (C.c_int, [_P, C.c_int, _D, _D, _D, _D, _D, _P,
                                           C.POINTER(_I64), _P, _P, _P, _P, _P, _P, _P]),
     "pyr_face_walk_stats": (C.c_int, [_P, _D, _I64, _P, _P, _P, _P, _P, _P, _P]),
-    "pyr_deposit_set_counting": (C.c_int, [C.c_int]),
-    "pyr_push_set_mfma": (C.c_int, [C.c_int]),
-    "pyr_push_mfma_waves": (C.c_int, [C.POINTER(_I64)]),
+    "pyr_deposit_set_counting": (C.c_int, [_P, C.c_int]),
+    "pyr_push_set_mfma": (C.c_int, [_P, C.c_int]),
+    "pyr_push_mfma_waves": (C.c_int, [_P, C.POINTER(_I64)]),
     "pyr_deposit_stats": (C.c_int, [_P, C.POINTER(C.c_ulonglong * 4)]),
+    "pyr_deposit_list_stats": (C.c_int, [_P, C.POINTER(C.c_ulonglong * 4)]),
     "pyr_deposit_candidates": (C.c_int, [_P, _D, C.POINTER(_I64)]),
     "pyr_sort_by_element": (C.c_int, [_P, _I64, _P, _P, _P]),
     "pyr_lsrk4_forward": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _D, _D, _I64, _P, _P, _P,

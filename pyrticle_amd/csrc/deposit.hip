@@ -136,39 +136,6 @@ __global__ void k_stage(int D, int64_t n, const uint32_t *__restrict__ perm,
 }
 
 
-// FP32 staging of the FP32-compensated gather, in the same order as k_stage:
-//   pos32[j]             = (x_j - c_e, 0)  offset from the vertex centroid of
-//                          the particle's own element e, formed in FP64 and
-//                          rounded once;
-//   wgtT[4 (j/4) + c][j % 4] = w_j[c],  w_j = (a, a v0, a v1, a v2),
-//                          a = q * scale formed in FP64 (scale: the shape
-//                          normalizer times P^alpha, P = 2^ceil(log2 R), since
-//                          the gather evaluates S / P^alpha): the weights of
-//                          every aligned quad of particles, transposed.
-// Runs over n rounded up to a multiple of 4; the padding entries are zero.
-// Particles without a valid element sort last and are not referenced.
-__global__ void k_stage32(int D, int K, int64_t n, const uint32_t *__restrict__ perm,
-                          const uint32_t *__restrict__ elem, const double *__restrict__ x,
-                          const double *__restrict__ q, const double *__restrict__ v,
-                          double scale, const double *__restrict__ geom,
-                          float4 *__restrict__ pos, float *__restrict__ wgt_t) {
-  const int64_t j = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (j >= ((n + 3) & ~(int64_t)3)) return;
-  float xs[3] = {0.f, 0.f, 0.f}, w[4] = {0.f, 0.f, 0.f, 0.f};
-  if (j < n) {
-    const uint32_t i = perm[j];
-    const uint32_t e = elem[i];
-    const double a = q[i] * scale;
-    w[0] = (float)a;
-    for (int k = 0; k < D; ++k) {
-      if (e < (uint32_t)K) xs[k] = (float)(x[(size_t)i * D + k] - geom[(size_t)e * 8 + k]);
-      if (v) w[1 + k] = (float)(a * v[(size_t)i * D + k]);
-    }
-  }
-  pos[j] = make_float4(xs[0], xs[1], xs[2], 0.f);
-  const int64_t quad = j & ~(int64_t)3;
-  for (int c = 0; c < 4; ++c) wgt_t[(quad + c) * 4 + (j & 3)] = w[c];
-}
 
 
 
@@ -330,6 +297,29 @@ __global__ __launch_bounds__(256) void k_stage32c(int D, int64_t nch,
 
 // rho and j of every target from its work items' partial sums, added in item
 // order (FP64): one thread per (target, node slot).
+// Statistics of the chunk lists of the last FP32C deposition, per target:
+// list entries, listed (chunk, node group) pairs, and the 4x4x1 matrix-core
+// instructions the list gather issues for them (entries 2k and 2k + 1 of a
+// list form one pair, evaluated against the union of their groups with D + 6
+// products per chunk and group: D + 2 for t, four for the accumulation).
+__global__ void k_list_stats(int K, int D, const uint32_t *__restrict__ cap_off,
+                             const uint32_t *__restrict__ n_live, const uint2 *__restrict__ ent,
+                             unsigned long long *__restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= K) return;
+  const uint32_t b = cap_off[e], n = n_live[e];
+  if (!n) return;
+  unsigned long long groups = 0, instr = 0;
+  for (uint32_t i = 0; i < n; i += 2) {
+    const uint32_t ma = ent[b + i].y, mb = i + 1 < n ? ent[b + i + 1].y : 0u;
+    groups += (unsigned)(__popc(ma) + __popc(mb));
+    instr += 2ull * (unsigned)(D + 6) * (unsigned)__popc(ma | mb);
+  }
+  atomicAdd(out + 0, (unsigned long long)n);
+  atomicAdd(out + 1, groups);
+  atomicAdd(out + 2, instr);
+}
+
 struct CombineArgs {
   int K, NS;                   // targets, node slots (4 NG)
   int Np, D;
@@ -437,7 +427,6 @@ struct DepositCache {
   double tr_radius = -1;
   // sort / staging scratch
   DevBuf keys_a, keys_b, vals_a, vals_b, hist, hist_scan, seg, tmp, pos, wgt, stats;
-  DevBuf pos32, wgt32;   // FP32 staging
   // matrix-core gather over culled chunk lists: targets in Morton order,
   // grouped node tables, per-deposition chunk staging and the lists
   DevBuf t_order, tab32g, ngsph, node_slot, ch_meta, pos32c, wgt32c;
@@ -445,6 +434,10 @@ struct DepositCache {
   DevBuf ikey, ik_a, ik_b, iv_a, iv_b;   // work items sorted by their first chunk
   DevBuf active;                         // targets that may receive charge
   int64_t max_indeg = 0;   // most targets whose candidate list holds one source
+  // the lists above are those of the last deposition when list_valid
+  bool list_valid = false;
+  uint32_t list_items = 0;
+  DevBuf list_stats;
   DevBuf nrm;            // normalized deposition: integrals, scaled charges, counter
   DevBuf skeys;   // Morton sort keys
   DevBuf chunks;         // per-particle gathers: 64-particle chunks (element, start),
@@ -462,10 +455,10 @@ void pyr_deposit_cache_free(pyr_mesh *mesh) {
                     &c->tab_R, &c->tr_start, &c->tr, &c->tr_shift, &c->keys_a, &c->keys_b,
                     &c->vals_a, &c->vals_b, &c->skeys, &c->hist,
                     &c->hist_scan, &c->seg, &c->tmp, &c->pos, &c->wgt, &c->stats,
-                    &c->pos32, &c->wgt32, &c->nrm, &c->chunks, &c->t_order, &c->tab32g,
+                    &c->nrm, &c->chunks, &c->t_order, &c->tab32g,
                     &c->ngsph, &c->node_slot, &c->ch_meta, &c->pos32c, &c->wgt32c, &c->cap,
                     &c->ent, &c->ent_off, &c->n_live, &c->items, &c->partial, &c->ikey,
-                    &c->ik_a, &c->ik_b, &c->iv_a, &c->iv_b, &c->active})
+                    &c->ik_a, &c->ik_b, &c->iv_a, &c->iv_b, &c->active, &c->list_stats})
     b->release();
   delete c;
   mesh->dep = nullptr;
@@ -872,18 +865,17 @@ const uint32_t *deposit_last_order(pyr_mesh *mesh, int64_t n) {
 }
 }  // namespace pyr
 
-// Sorts the particles by (element, Morton code inside the element) and stages
-// them in that order; returns the permutation.  The Morton sub-order makes the
-// gathers' batches of consecutive particles spatially coherent (the FP64
-// gather's queued batches, the matrix-core gather's 64-particle steps, which
-// it skips when none of them can reach the target).
+// Sorts the particles by (element, Morton code inside the element) and, with
+// stage64, stages them in that order for the FP64 kernels; returns the
+// permutation.  The Morton sub-order makes batches of consecutive particles
+// spatially coherent (the FP64 gather's queued batches, the FP32C chunks,
+// whose bounding spheres it keeps small).
 // sorted: a permutation (and c->seg) left by an earlier sort of the same
 // positions and elements, which is then only restaged.
 static const uint32_t *sort_and_stage(pyr_mesh *mesh, DepositCache *c, int64_t n,
                                       const double *x, const double *q, const double *v,
                                       const uint32_t *elem, double normalizer,
-                                      bool stage64 = true, bool stage32 = false,
-                                      double radius = 1.0, double alpha = 1.0,
+                                      bool stage64 = true,
                                       const uint32_t *sorted = nullptr) {
   const int K = mesh->K, D = mesh->dim;
   hipStream_t s = stream();
@@ -910,21 +902,6 @@ static const uint32_t *sort_and_stage(pyr_mesh *mesh, DepositCache *c, int64_t n
     c->wgt.reserve(sizeof(double4) * (size_t)n);
     hipLaunchKernelGGL(k_stage, dim3(blocks_for(n)), dim3(256), 0, s, D, n, perm, x, q, v,
                        normalizer, c->pos.as<double4>(), c->wgt.as<double4>());
-    PYR_CHECK_LAUNCH();
-  }
-  if (stage32) {
-    const int64_t n4 = (n + 3) & ~(int64_t)3;
-    c->pos32.reserve(sizeof(float4) * (size_t)n4);
-    c->wgt32.reserve(sizeof(float4) * (size_t)n4);
-    // the matrix-core gather evaluates S / P^alpha with P = 2^ceil(log2 R):
-    // P^alpha goes into the weights
-    int p_exp = 0;
-    const double p_mant = std::frexp(radius, &p_exp);
-    const double pscale = std::ldexp(1.0, p_mant == 0.5 ? p_exp - 1 : p_exp);
-    const double scale = normalizer * std::pow(pscale, alpha);
-    hipLaunchKernelGGL(k_stage32, dim3(blocks_for(n4)), dim3(256), 0, s, D, K, n, perm, elem, x,
-                       q, v, scale, c->geom.as<double>(), c->pos32.as<float4>(),
-                       c->wgt32.as<float>());
     PYR_CHECK_LAUNCH();
   }
   return perm;
@@ -1055,8 +1032,6 @@ static int64_t chunk_table(pyr_mesh *mesh, DepositCache *c, int64_t n, ChunkList
   return (int64_t)total;
 }
 
-static int g_deposit_count_mode = 0;
-
 // q / I per particle for the normalized deposition; located particles whose
 // shape integral vanishes deposit nothing and are counted (the reference warns
 // for each, dep_normshape.hpp:175-183).  scale (optional) receives q / I.
@@ -1095,15 +1070,12 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
     prepare_tables(mesh, c, radius);
     const uint32_t *perm = nullptr;
     for (DevBuf *b : {&c->pos, &c->wgt}) b->reserve(sizeof(double4) * (size_t)std::max<int64_t>(1, n));
-    for (DevBuf *b : {&c->pos32, &c->wgt32})
-      b->reserve(sizeof(float4) * (size_t)std::max<int64_t>(4, (n + 3) & ~(int64_t)3));
     if (n > 0) PYR_REQUIRE(x && q && elem, "null pointer");
     {
       ProfScope prof(PROF_SORT);
       if (n > 0) {
         const bool fp64 = precision == PYR_FP64;
-        perm = sort_and_stage(mesh, c, n, x, q, j ? v : nullptr, elem, normalizer, fp64, false,
-                              radius, alpha, sorted);
+        perm = sort_and_stage(mesh, c, n, x, q, j ? v : nullptr, elem, normalizer, fp64, sorted);
       } else {
         c->seg.reserve(sizeof(uint32_t) * ((size_t)K + 1));
         PYR_HIP(hipMemsetAsync(c->seg.ptr, 0, sizeof(uint32_t) * ((size_t)K + 1), s));
@@ -1124,8 +1096,6 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
     ga.seg = c->seg.as<uint32_t>();
     ga.pos = c->pos.as<double4>();
     ga.wgt = c->wgt.as<double4>();
-    ga.pos32 = c->pos32.as<float4>();
-    ga.wgt32 = c->wgt32.as<float4>();
     ga.geom = c->geom.as<double>();
     ga.face_normal = mesh->dev.face_normal;
     ga.face_rhs = mesh->dev.face_rhs;
@@ -1134,7 +1104,7 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
     ga.j = j;
     c->stats.reserve(4 * sizeof(unsigned long long));
     ga.stats = c->stats.as<unsigned long long>();
-    const bool count = g_deposit_count_mode != 0;
+    const bool count = mesh->deposit_count != 0;
     if (count) PYR_HIP(hipMemsetAsync(ga.stats, 0, 4 * sizeof(unsigned long long), s));
     if (precision == PYR_FP32C) {
       if (count) throw Error(PYR_ERR_INVALID_ARG, "deposit counting needs the FP64 precision");
@@ -1203,7 +1173,8 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
         cu.active = c->active.as<uint8_t>();
         launch_cull_chunks(D, Np, cu);
       }
-      // work items of up to 64 list entries, in Morton order of the targets
+      // work items of up to 64 list entries, filled in Morton order of the
+      // targets, then ordered by their first chunk
       const int ns = 4 * ((Np + 3) / 4);
       uint32_t *icnt = cap;   // the capacities are no longer needed
       uint32_t n_items = 0;
@@ -1260,13 +1231,19 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
       cb.node_of_slot = c->node_slot.as<int>();
       cb.rho = rho;
       cb.j = j;
-      ProfScope prof(PROF_GATHER);
-      if (n_items) launch_gather_cl(D, Np, la, n_items, alpha == 2.0);
+      {
+        ProfScope prof(PROF_GATHER);   // the list gather alone
+        if (n_items) launch_gather_cl(D, Np, la, n_items, alpha == 2.0);
+      }
+      ProfScope prof(PROF_SORT);       // with the other per-deposition set-up work
       hipLaunchKernelGGL(k_combine_items, dim3(blocks_for((int64_t)K * ns)), dim3(256), 0, s,
                          cb);
       PYR_CHECK_LAUNCH();
+      c->list_valid = true;
+      c->list_items = n_items;
       return;
     }
+    c->list_valid = false;
     ProfScope prof(PROF_GATHER);
     launch_gather_fp64(D, Np, ga, K, shape_unit_square(alpha, radius), count);
   }
@@ -1522,9 +1499,11 @@ int pyr_deposit_backward(pyr_mesh *mesh, double radius, double alpha, double nor
   });
 }
 
-int pyr_deposit_set_counting(int on) {
-  g_deposit_count_mode = on;
-  return PYR_OK;
+int pyr_deposit_set_counting(pyr_mesh *mesh, int on) {
+  return guarded([&] {
+    PYR_REQUIRE(mesh, "null mesh");
+    mesh->deposit_count = on;
+  });
 }
 
 int pyr_deposit_stats(pyr_mesh *mesh, unsigned long long out[4]) {
@@ -1535,6 +1514,27 @@ int pyr_deposit_stats(pyr_mesh *mesh, unsigned long long out[4]) {
     PYR_HIP(hipStreamSynchronize(stream()));
     PYR_HIP(hipMemcpy(out, mesh->dep->stats.ptr, 4 * sizeof(unsigned long long),
                       hipMemcpyDeviceToHost));
+  });
+}
+
+int pyr_deposit_list_stats(pyr_mesh *mesh, unsigned long long out[4]) {
+  return guarded([&] {
+    PYR_REQUIRE(mesh && out, "null argument");
+    for (int k = 0; k < 4; ++k) out[k] = 0;
+    DepositCache *c = mesh->dep;
+    if (!c || !c->list_valid) return;
+    const int K = mesh->K;
+    hipStream_t s = stream();
+    c->list_stats.reserve(3 * sizeof(unsigned long long));
+    PYR_HIP(hipMemsetAsync(c->list_stats.ptr, 0, 3 * sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_list_stats, dim3(blocks_for(K)), dim3(256), 0, s, K, mesh->dim,
+                       c->cap.as<uint32_t>() + K + 1, c->n_live.as<uint32_t>(),
+                       c->ent.as<uint2>(), c->list_stats.as<unsigned long long>());
+    PYR_CHECK_LAUNCH();
+    PYR_HIP(hipMemcpyAsync(out, c->list_stats.ptr, 3 * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, s));
+    PYR_HIP(hipStreamSynchronize(s));
+    out[3] = c->list_items;
   });
 }
 

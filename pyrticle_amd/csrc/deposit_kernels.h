@@ -24,8 +24,6 @@ struct GatherArgs {
   const uint32_t *seg;              // K + 1 segment starts into pos/wgt
   const double4 *pos;               // staged positions
   const double4 *wgt;               // staged weights
-  const float4 *pos32;              // FP32 staging: x - c_src (k_stage32)
-  const float4 *wgt32;              //   (q N, q N v) in FP32, quads transposed
   const double *geom;               // K x 8: centroid (3), (R + node radius)^2, vertex radius
   const double *face_normal;        // mesh arrays
   const double *face_rhs;
@@ -304,370 +302,16 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
 }
 
 // ---------------------------------------------------------------------------
-// FP32-compensated gather on the matrix cores, one particle per lane
-// ---------------------------------------------------------------------------
-// Staging (k_stage32): particle j of the element-sorted order is stored as
-//   pos32[j]            = (x_j - c_src, 0)  in FP32 (offset from the centroid
-//                         of its own element, formed in FP64),
-//   wgtT[4 (j/4) + c]   = (w_{4(j/4)}c, w_{4(j/4)+1}c, w_{4(j/4)+2}c, w_{4(j/4)+3}c)
-// with w_j = (q N, q N v) in FP32: the weights of each aligned quad of
-// particles are stored transposed.  The offset of a candidate source element
-// from the target centroid, c_src + shift - c_e, is formed in FP64 once per
-// source element, so every FP32 offset d = x - c_e stays small.
-//
-// Streaming: the particle segments of the target's candidate source elements
-// (64 at a time, one per lane) are widened to aligned quads and concatenated
-// into one flat index range, walked in chunks of 64 (one particle per lane)
-// dealt round-robin to the block's waves.  A wave-uniform cursor finds the
-// segment of a chunk's first index; segment data reach the lanes through
-// readlane (scalar), with a short uniform loop over the segments that start
-// inside the chunk.  The loads of the next chunk are in flight while the
-// current one is evaluated.  Lanes outside their segment (quad padding) or
-// past the end get a particle that no node can reach.  There is no
-// prefilter: every candidate particle is evaluated against every node on the
-// matrix cores, and pairs farther apart than R give S = 0; a step is skipped
-// when none of its 64 particles passes a conservative reach test (node sphere
-// and face planes of the target moved out by R), which the Morton sub-order of
-// the staging makes common.
-//
-// Pairs, per group of four nodes n = 4 ng + (l % 4), with
-// v_mfma_f32_4x4x1_16b_f32 (16 independent 4x4 outer products, block b =
-// lanes 4b..4b+3):
-//   t[p][n]  = (R - |n - x_p|^2 / R) / P,   P = 2^ceil(log2 R) >= R (exact scaling)
-//            = -|n - c_e|^2 / (R P) + m_p . (n - c_e) / P + c_p,
-//              m_p = 2 d_p / R,  c_p = (R - |d_p|^2 / R) / P  (c_p enters as
-//              the last term: it carries the large cancelling part),
-//              as C + sum_k A[p][k] B[k][n] over A = (m, c_p), B = ((n - c_e)/P, 1);
-//              t <= R / P <= 1, so max(t, 0) is one clamp to [0, 1] (v_med3);
-//              A of block b, row i is particle 4b + i = the lane's own particle,
-//              so one instruction covers 64 particles x 4 nodes;
-//              t lands in VGPR i of lane 4b + j: particle 4b + i, node j;
-//   S        = max(t, 0)^alpha  (P^alpha is folded into the staged weights);
-//   acc[v][n] += sum_i wgtT_i S_i   (four K = 1 products, i = 0..3): the A
-//              operand of block b is w_{4b+i} spread over its four rows, which
-//              is component i of the lane's own wgtT load.
-// Each block accumulates its own particles; FP32 partial sums over FOLD steps
-// are folded into Kahan-compensated sums (kept in LDS, one pair per lane and
-// entry), and the sixteen blocks and the waves are combined in FP64 in a fixed
-// order, so the result is bitwise reproducible.
-typedef float pyr_v4f __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ pyr_v4f mfma4x4(float a, float b, pyr_v4f c) {
-  return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 0);
-}
-
-// relative margin of the FP32 reach test (it may only keep extra particles)
-constexpr float MF_MARGIN = 1e-4f;
-
-// 16 steps (64 particles per 4x4 block) per fold: against 8, the same error on
-// the coarse test mesh (5.55e-7, dominated by the FP32 distance evaluation)
-// and 1.4% less gather time on the cylinder workload
-#ifndef PYR_MF_FOLD
-#define PYR_MF_FOLD 16
-#endif
-constexpr int FOLD_STEPS = PYR_MF_FOLD;
-
-template <int D, int NP, int W, bool A2>
-__global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
-  constexpr int NG = (NP + 3) / 4;   // node groups of four
-  constexpr int FOLD = FOLD_STEPS;   // steps (four particles per block each) per fold
-  constexpr float FAR = 1e30f;       // -c of a particle no node can reach
-
-  const int e = xcd_element(blockIdx.x);
-  if (e >= g.K) return;
-  const uint32_t c_begin = g.cand_start[e], c_end = g.cand_start[e + 1];
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-
-  // per-lane compensated sums (s, c); at the end each wave's per-node results
-  // overwrite its own slots (20 KB per two-wave block: 8 blocks per CU)
-  __shared__ float2 kahan[W][NG * 4][64];
-
-  const double cx = g.geom[(size_t)e * 8 + 0], cy = g.geom[(size_t)e * 8 + 1],
-               cz = g.geom[(size_t)e * 8 + 2];
-  const float two_inv_r = (float)(2.0 * g.invR), inv_rf = (float)g.invR, rf = (float)g.R;
-  // P = 2^ceil(log2 R): scaling by 1/P is exact
-  int p_exp = 0;
-  const double p_mant = frexp(g.R, &p_exp);   // R = p_mant 2^p_exp, p_mant in [0.5, 1)
-  const float inv_p = (float)ldexp(1.0, -(p_mant == 0.5 ? p_exp - 1 : p_exp));
-
-  // node constants of this lane's node in every group
-  // (padded nodes: B = (0, 1), C = -1, so t = c_p - 1 <= 0 and S = 0)
-  float bx[NG], by[NG], bz[NG], cc[NG];
-#pragma unroll
-  for (int ng = 0; ng < NG; ++ng) {
-    const int n = 4 * ng + (lane & 3);
-    bx[ng] = by[ng] = bz[ng] = 0.f;
-    cc[ng] = -1.f;
-    if (n < NP) {
-      const float *nd = static_cast<const float *>(g.node_tab) + ((size_t)e * NP + n) * 4;
-      bx[ng] = nd[0] * inv_p;
-      by[ng] = nd[1] * inv_p;
-      bz[ng] = D == 3 ? nd[2] * inv_p : 0.f;
-      cc[ng] = -nd[3] * inv_p;
-    }
-  }
-
-  // conservative reach test of a particle (FP32, with a relative margin): its
-  // ball of radius R meets the node sphere of the target and lies within R of
-  // every face plane.  A particle failing it has t <= 0 at every node.
-  auto uni = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
-  const float bound2 = uni((float)g.geom[(size_t)e * 8 + 3] * (1.f + 2.f * MF_MARGIN));
-  float fnl[D + 1][D], hf[D + 1];
-#pragma unroll
-  for (int f = 0; f <= D; ++f) {
-    double nc = 0;
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      const double nk = g.face_normal[((size_t)e * (D + 1) + f) * D + k];
-      fnl[f][k] = uni((float)nk);
-      nc = fma(nk, k == 0 ? cx : (k == 1 ? cy : cz), nc);
-    }
-    hf[f] = uni((float)(g.face_rhs[(size_t)e * (D + 1) + f] + g.R - nc + MF_MARGIN * g.R));
-  }
-
-  pyr_v4f cinit[NG];   // C operand of the first product: -|n - c_e|^2 / (R P) in every row
-#pragma unroll
-  for (int ng = 0; ng < NG; ++ng) cinit[ng] = pyr_v4f{cc[ng], cc[ng], cc[ng], cc[ng]};
-  pyr_v4f acc[NG];
-  // Kahan sums (s, c) of every lane's accumulator entries live in LDS, each
-  // lane touching only its own entries (no barriers needed).  They start at
-  // zero: the first fold reads (0, 0) instead of LDS, so a wave that never
-  // evaluates a step (most targets hold no candidate particle) touches no
-  // Kahan slot.
-  float2 *kl = &kahan[wave][0][lane];
-#pragma unroll
-  for (int ng = 0; ng < NG; ++ng) acc[ng] = pyr_v4f{0.f, 0.f, 0.f, 0.f};
-  int n_steps = 0;
-  bool evaluated = false, folded = false;   // wave-uniform
-
-  auto fold = [&]() {
-#pragma unroll
-    for (int ng = 0; ng < NG; ++ng)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        float2 k = folded ? kl[(ng * 4 + v) * 64] : make_float2(0.f, 0.f);
-        const float y = acc[ng][v] - k.y;
-        const float t = k.x + y;
-        k.y = (t - k.x) - y;
-        k.x = t;
-        kl[(ng * 4 + v) * 64] = k;
-      }
-#pragma unroll
-    for (int ng = 0; ng < NG; ++ng) acc[ng] = pyr_v4f{0.f, 0.f, 0.f, 0.f};
-    folded = true;
-  };
-
-  // particle data of one chunk: m = 2 d / R (3), c = (R - |d|^2 / R) / P, and
-  // the quad weights
-  struct Part {
-    float mx, my, mz, cp;
-    float4 w;
-    bool live;   // may reach a node of the target
-  };
-  // loads in flight for one chunk, before the offsets are applied
-  struct Raw {
-    float4 dp, w;
-    float ox, oy, oz;
-    bool in_seg;
-  };
-  auto derive = [&](const Raw &r) {
-    Part P;
-    const float dx = r.ox + r.dp.x, dy = r.oy + r.dp.y, dz = D == 3 ? r.oz + r.dp.z : 0.f;
-    const float dd = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-    P.mx = dx * two_inv_r;
-    P.my = dy * two_inv_r;
-    P.mz = dz * two_inv_r;
-    // padding lanes keep finite m and get a c that no node can overcome
-    P.cp = r.in_seg ? fmaf(-dd, inv_rf, rf) * inv_p : -FAR;
-    P.w = r.w;
-    float viol = -INFINITY;
-#pragma unroll
-    for (int f = 0; f <= D; ++f) {
-      float ip = fnl[f][0] * dx;
-      ip = fmaf(fnl[f][1], dy, ip);
-      if constexpr (D == 3) ip = fmaf(fnl[f][D - 1], dz, ip);
-      viol = fmaxf(viol, ip - hf[f]);
-    }
-    P.live = r.in_seg && dd <= bound2 && viol <= 0.f;
-    return P;
-  };
-  auto evaluate = [&](const Part &P) {
-    // a step none of whose particles can reach the target adds nothing
-    if (__ballot(P.live) == 0) return;
-    evaluated = true;
-#pragma unroll
-    for (int ng = 0; ng < NG; ++ng) {
-      pyr_v4f t = mfma4x4(P.mx, bx[ng], cinit[ng]);
-      t = mfma4x4(P.my, by[ng], t);
-      if (D == 3) t = mfma4x4(P.mz, bz[ng], t);
-      t = mfma4x4(P.cp, 1.f, t);
-      float s[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        if constexpr (A2) {
-          // max(t, 0)^2 as clamp(t |t|) to [0, 1]: one v_mul_f32 with an abs
-          // source and the clamp bit, the same bits as squaring the clamped t
-          // (t <= 1 up to rounding, and t > 1 gives 1 either way)
-          s[i] = __builtin_amdgcn_fmed3f(t[i] * fabsf(t[i]), 0.f, 1.f);
-        } else {
-          const float tw = __builtin_amdgcn_fmed3f(t[i], 0.f, 1.f);
-          s[i] = tw > 0.f ? __powf(tw, (float)g.alpha) : 0.f;
-        }
-      }
-      acc[ng] = mfma4x4(P.w.x, s[0], acc[ng]);
-      acc[ng] = mfma4x4(P.w.y, s[1], acc[ng]);
-      acc[ng] = mfma4x4(P.w.z, s[2], acc[ng]);
-      acc[ng] = mfma4x4(P.w.w, s[3], acc[ng]);
-    }
-    if (++n_steps == FOLD) {
-      n_steps = 0;
-      fold();
-    }
-  };
-
-  for (uint32_t cw = c_begin; cw < c_end; cw += 64) {
-    // window of up to 64 candidate source elements, one per lane: aligned
-    // segment start a0, the segment [lo, hi), and the offset of the source
-    // centroid from the target centroid (every wave holds the same window)
-    uint32_t alen = 0, a0 = 0, lo = 0, hi = 0;
-    float ofx = 0.f, ofy = 0.f, ofz = 0.f;
-    if (cw + lane < c_end) {
-      const uint32_t src = g.cand[cw + lane];
-      lo = g.seg[src];
-      hi = g.seg[src + 1];
-      if (hi > lo) {
-        a0 = lo & ~3u;
-        alen = ((hi + 3u) & ~3u) - a0;
-        double sh[3] = {0, 0, 0};
-        const uint32_t code = g.cand_shift[cw + lane];
-        if (code) shift_vec<D>(g, code, sh);
-        ofx = (float)(g.geom[(size_t)src * 8 + 0] + sh[0] - cx);
-        ofy = (float)(g.geom[(size_t)src * 8 + 1] + sh[1] - cy);
-        ofz = D == 3 ? (float)(g.geom[(size_t)src * 8 + 2] + sh[2] - cz) : 0.f;
-      }
-    }
-    // inclusive scan of the aligned lengths; the flat range is their concatenation
-    uint32_t incl = alen;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(incl, o, 64);
-      if (lane >= o) incl += y;
-    }
-    const uint32_t excl = incl - alen;
-    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-    auto rl = [](uint32_t v, int k) { return (uint32_t)__builtin_amdgcn_readlane((int)v, k); };
-    auto rlf = [](float v, int k) {
-      return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
-    };
-
-    // chunks of 64 flat indices, dealt round-robin to the waves; kc is the
-    // segment holding the chunk's first index (wave-uniform, monotone), its
-    // data cached in scalar registers and re-read only when kc moves
-    int kc = 0;
-    uint32_t c_end = rl(incl, 0), c_ex = rl(excl, 0), c_a0 = rl(a0, 0), c_lo = rl(lo, 0),
-             c_hi = rl(hi, 0);
-    float c_ox = rlf(ofx, 0), c_oy = rlf(ofy, 0), c_oz = rlf(ofz, 0);
-    auto issue = [&](uint32_t base) {
-      if (c_end <= base) {
-        do ++kc;
-        while (rl(incl, kc) <= base);
-        c_end = rl(incl, kc);
-        c_ex = rl(excl, kc);
-        c_a0 = rl(a0, kc);
-        c_lo = rl(lo, kc);
-        c_hi = rl(hi, kc);
-        c_ox = rlf(ofx, kc);
-        c_oy = rlf(ofy, kc);
-        c_oz = rlf(ofz, kc);
-      }
-      const uint32_t last = min(base + 63u, total - 1u);
-      const uint32_t f = base + (uint32_t)lane;
-      uint32_t sa = c_a0, se = c_ex, sl = c_lo, sh2 = c_hi;
-      Raw r;
-      r.ox = c_ox;
-      r.oy = c_oy;
-      r.oz = c_oz;
-      if (last >= c_end) {
-        // segments that start inside the chunk (rare): per-lane selection
-        for (int kk = kc + 1; kk < 64 && rl(excl, kk) <= last; ++kk) {
-          const bool in = f >= rl(excl, kk);
-          sa = in ? rl(a0, kk) : sa;
-          se = in ? rl(excl, kk) : se;
-          sl = in ? rl(lo, kk) : sl;
-          sh2 = in ? rl(hi, kk) : sh2;
-          r.ox = in ? rlf(ofx, kk) : r.ox;
-          r.oy = in ? rlf(ofy, kk) : r.oy;
-          r.oz = in ? rlf(ofz, kk) : r.oz;
-        }
-      }
-      const bool valid = f < total;
-      const uint32_t j = valid ? sa + (f - se) : (uint32_t)(lane & 3);
-      r.in_seg = valid && j >= sl && j < sh2;
-      r.dp = g.pos32[j];
-      r.w = g.wgt32[j];
-      return r;
-    };
-
-    uint32_t base = (uint32_t)wave * 64u;
-    if (base < total) {
-      Raw raw = issue(base);
-      for (;;) {
-        const Part cur = derive(raw);
-        const uint32_t nb = base + (uint32_t)(W * 64);
-        const bool more = nb < total;   // wave-uniform
-        if (more) raw = issue(nb);
-        evaluate(cur);
-        if (!more) break;
-        base = nb;
-      }
-    }
-  }
-  // combine: sixteen blocks by a fixed butterfly, then the waves in order.
-  // The result of node 4 ng + l, component v, goes (as the bits of a double)
-  // into slot [ng * 4 + v][l] of the wave's own Kahan array, which every lane
-  // of this wave has just read in the same instruction stream.  A wave that
-  // evaluated nothing has zero sums and writes +0.0 there directly.
-  if (evaluated) {
-    fold();
-#pragma unroll
-    for (int ng = 0; ng < NG; ++ng)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const float2 k = kl[(ng * 4 + v) * 64];
-        double val = (double)k.x - (double)k.y;
-#pragma unroll
-        for (int o = 4; o < 64; o <<= 1) val += __shfl_xor(val, o, 64);
-        if (lane < 4) kl[(ng * 4 + v) * 64] = __builtin_bit_cast(float2, val);
-      }
-  } else if (lane < 4) {
-#pragma unroll
-    for (int r = 0; r < NG * 4; ++r) kl[r * 64] = __builtin_bit_cast(float2, 0.0);
-  }
-  __syncthreads();
-  if (tid < NP) {
-    double out[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int w = 0; w < W; ++w)
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
-        out[v] += __builtin_bit_cast(double, kahan[w][(tid >> 2) * 4 + v][tid & 3]);
-    const size_t gn = (size_t)e * NP + tid;
-    if (g.rho) g.rho[gn] = out[0];
-    if (g.j)
-#pragma unroll
-      for (int k = 0; k < D; ++k) g.j[gn * D + k] = out[1 + k];
-  }
-}
-
-// ---------------------------------------------------------------------------
 // FP32-compensated gather over culled chunk lists (matrix cores)
 // ---------------------------------------------------------------------------
-// The particles of every source element are staged per deposition in chunks
-// of up to 64 consecutive element-sorted particles, 64 slots each (k_stage32c:
-// offsets x - c_src, the quad-transposed weights, and a bounding sphere of
-// the chunk).  The nodes of every target element are grouped spatially in
-// groups of four, each with a bounding sphere.
+// Staging (k_stage32c, per deposition): the element-sorted particles of every
+// source element are cut into chunks of up to 64 consecutive particles, 64
+// slots each: pos = (x - c_src, 0) in FP32 (offset from the centroid of the
+// particle's own element, formed in FP64 and rounded once), the weights
+// w = (q N, q N v) of every aligned quad of particles stored transposed, and
+// a bounding sphere of the chunk.  Empty slots carry zero weights.  The nodes
+// of every target element are grouped spatially in groups of four, each with a
+// bounding sphere (k_node_table_g).
 //
 // k_cull_chunks (one wave per target): for every chunk of every candidate
 // source element, the groups whose sphere lies within R of the chunk's sphere;
@@ -677,17 +321,43 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf(GatherArgs g) {
 // once).  A group none of whose nodes is within R of a chunk's particles adds
 // exactly zero for that chunk, so the culling leaves every sum unchanged.
 //
-// k_gather_cl (one wave per target, targets in Morton order of their
-// centroids so that the waves in flight on an XCD share chunks in its L2):
-// streams the target's list, two chunks of loads in flight, and forms the
-// pairs of every listed group as t = (R - |n - x|^2 / R) / P on
-// v_mfma_f32_4x4x1_16b_f32: C + sum_k A B over A = (2 d / R, c_p), B =
-// ((n - c_e) / P, 1), c_p = (R - |d|^2 / R) / P entering last; S =
-// max(t, 0)^alpha; the four components (q N, q N v) accumulate on four more
-// products.  Every FOLD chunks the per-lane FP32 partials are summed over the
-// lanes l, l^32, l^16 (a fixed butterfly) into Kahan pairs (s, c) in LDS for
-// lanes 0..15; at the end the four remaining blocks are combined in FP64 in a
-// fixed order, so results are bitwise reproducible.
+// k_gather_cl (one wave per work item of at most 64 consecutive entries of
+// one target's list; items run in the order of their first chunk, so the
+// waves in flight on an XCD share chunks in its L2): two pairs of chunks of
+// loads in flight, each pair evaluated against the groups either chunk
+// reaches, per group of four nodes n = 4 ng + (l % 4), on
+// v_mfma_f32_4x4x1_16b_f32 (16 independent 4x4 outer products, block b =
+// lanes 4b..4b+3):
+//   t[p][n]  = (R - |n - x_p|^2 / R) / P,   P = 2^ceil(log2 R) >= R (exact scaling)
+//            = -|n - c_e|^2 / (R P) + m_p . (n - c_e) / P + c_p,
+//              m_p = 2 d_p / R, d_p = x_p - c_e, c_p = (R - |d_p|^2 / R) / P
+//              (c_p enters last: it carries the large cancelling part), as
+//              C + sum_k A[p][k] B[k][n] over A = (m, c_p), B = ((n - c_e)/P, 1);
+//              A of block b, row i is particle 4b + i = the lane's own
+//              particle, so one instruction covers 64 particles x 4 nodes;
+//              t lands in VGPR i of lane 4b + j: particle 4b + i, node j;
+//   S        = max(t, 0)^alpha, one clamp to [0, 1] (t <= R / P <= 1; P^alpha
+//              is folded into the staged weights);
+//   acc[v][n] += sum_i wgtT_i S_i  (four K = 1 products, i = 0..3): the A
+//              operand of block b is w_{4b+i} spread over its four rows, which
+//              is component i of the lane's own transposed weight load.
+// The two chunks of a pair accumulate on separate sets so their product chains
+// interleave.  Every FOLD_STEPS chunks the per-lane FP32 partials are folded
+// into per-lane Kahan pairs (s, c) in LDS; at the end the sixteen blocks are
+// combined in FP64 in a fixed butterfly and the item's sums written as FP64
+// partials, which k_combine_items adds per target in item order: results are
+// bitwise reproducible.
+typedef float pyr_v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ pyr_v4f mfma4x4(float a, float b, pyr_v4f c) {
+  return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 0);
+}
+
+// chunks per Kahan fold: 16 (64 particles per 4x4 block) measured the same
+// error as 8 on the coarse test mesh and 1.4% less gather time on the
+// 4M-particle cylinder
+constexpr int FOLD_STEPS = 16;
+
 struct CullArgs {
   int K;
   double R;
@@ -1377,7 +1047,6 @@ __global__ __launch_bounds__(64) void k_shape_average(AverageArgs g) {
 
 // Launchers, one per translation unit.  precision: PYR_FP64 or PYR_FP32C.
 void launch_gather_fp64(int D, int Np, const GatherArgs &ga, int K, bool a2, bool count);
-void launch_gather_fp32(int D, int Np, const GatherArgs &ga, int K, bool a2, bool count);
 void launch_cull_chunks(int D, int Np, const CullArgs &ca);
 void launch_gather_cl(int D, int Np, const ListArgs &la, uint32_t n_items, bool a2);
 void launch_deposit_adj(int D, int Np, const AdjArgs &ga, unsigned grid, int precision,

@@ -200,14 +200,16 @@ struct Fields6 {
   const double *f[6];
 };
 
-// (rows of coef_stride<T>(np) entries; the padding is zero)
+// (rows of coef_stride<T>(np) entries; the padding is zero).  With occ, only
+// the rows of elements e with occ[e] == stamp are written.
 template <class T>
 __global__ void k_modal(int K, int np, Fields6 fl, const double *__restrict__ minv,
-                        T *__restrict__ out) {
+                        T *__restrict__ out, const uint32_t *__restrict__ occ, uint32_t stamp) {
   const int ns = coef_stride<T>(np);
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)K * ns) return;
   int e = (int)(t / ns), k = (int)(t % ns);
+  if (occ && occ[e] != stamp) return;
   for (int f = 0; f < 6; ++f) {
     double acc = 0;
     if (fl.f[f] && k < np) {
@@ -842,29 +844,32 @@ static void dispatch_order(int D, int N, F &&f) {
   throw Error(PYR_ERR_INVALID_ARG, "unsupported dimension/order");
 }
 
-// FP32C per-step push: interpolate full waves on the matrix cores
-// (pyr_push_set_mfma: 0 off (default: the VALU contraction measured faster
-// inside the register-bound push), 1 on, 2 on and count the particle-stages
-// that took the path in g_mf_count)
-static int g_push_mfma = 0;
-static DevBuf g_mf_count;
-
+// occ/stamp: restrict the FP64 transform to the marked elements (null: all)
 static void modal_transform(pyr_mesh *h, const double *const fields[6], double *out64,
-                            float *out32) {
+                            float *out32, const uint32_t *occ = nullptr, uint32_t stamp = 0) {
   Fields6 fl;
   for (int f = 0; f < 6; ++f) fl.f[f] = fields ? fields[f] : nullptr;
   const int64_t work = (int64_t)h->K * h->Np;
   if (out64) {
     hipLaunchKernelGGL(k_modal<double>, dim3(blocks_for(work)), dim3(256), 0, stream(), h->K,
-                       h->Np, fl, h->minv.as<double>(), out64);
+                       h->Np, fl, h->minv.as<double>(), out64, occ, stamp);
     PYR_CHECK_LAUNCH();
   }
   if (out32) {
     const int64_t work32 = (int64_t)h->K * coef_stride<float>(h->Np);
     hipLaunchKernelGGL(k_modal<float>, dim3(blocks_for(work32)), dim3(256), 0, stream(), h->K,
-                       h->Np, fl, h->minv.as<double>(), out32);
+                       h->Np, fl, h->minv.as<double>(), out32, (const uint32_t *)nullptr, 0u);
     PYR_CHECK_LAUNCH();
   }
+}
+
+// occ[elem[i]] = stamp for the located particles
+__global__ void k_mark_occupied(int64_t n, const uint32_t *__restrict__ elem, uint32_t K,
+                                uint32_t *__restrict__ occ, uint32_t stamp) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t e = elem[i];
+  if (e < K) occ[e] = stamp;
 }
 
 }  // namespace pyr
@@ -885,7 +890,21 @@ int pyr_forces(pyr_mesh *mesh, int64_t n, int vdim, const double *const fields[6
     PYR_REQUIRE(!vis || (vis_b && vis_el && vis_mag), "all four vis arrays or none");
     const int N = order_of(mesh);
     mesh->s_modal.reserve(sizeof(double) * (size_t)mesh->K * 6 * mesh->Np);
-    modal_transform(mesh, fields, mesh->s_modal.as<double>(), nullptr);
+    // modal coefficients of the occupied elements only: the forces read no
+    // other row, and a beam occupies a small share of the mesh.  A stamp that
+    // changes per call marks them, so the marks need no clearing.
+    const size_t occ_bytes = sizeof(uint32_t) * (size_t)mesh->K;
+    if (mesh->s_occ.bytes < occ_bytes || mesh->occ_stamp == UINT32_MAX) {
+      mesh->s_occ.reserve(occ_bytes);
+      PYR_HIP(hipMemsetAsync(mesh->s_occ.ptr, 0, occ_bytes, stream()));
+      mesh->occ_stamp = 0;
+    }
+    const uint32_t stamp = ++mesh->occ_stamp;
+    hipLaunchKernelGGL(k_mark_occupied, dim3(blocks_for(n)), dim3(256), 0, stream(), n, elem,
+                       (uint32_t)mesh->K, mesh->s_occ.as<uint32_t>(), stamp);
+    PYR_CHECK_LAUNCH();
+    modal_transform(mesh, fields, mesh->s_modal.as<double>(), nullptr,
+                    mesh->s_occ.as<uint32_t>(), stamp);
     dispatch_order(mesh->dim, N, [&](auto DD, auto NN) {
       constexpr int D_ = decltype(DD)::value, N_ = decltype(NN)::value;
       hipLaunchKernelGGL((k_forces<D_, N_>), dim3(blocks_for(n)), dim3(256), 0, stream(),
@@ -939,9 +958,9 @@ int pyr_lsrk4_steps(pyr_mesh *mesh, int precision, int nsteps, double dt, double
     int *err = mesh->s_misc.as<int>();
     unsigned int *n_dead = reinterpret_cast<unsigned int *>(err + 1);
     unsigned long long *mf_waves = nullptr;
-    if (g_push_mfma == 2) {
-      g_mf_count.reserve(sizeof(unsigned long long));
-      mf_waves = g_mf_count.as<unsigned long long>();
+    if (mesh->push_mfma == 2) {
+      mesh->mf_count.reserve(sizeof(unsigned long long));
+      mf_waves = mesh->mf_count.as<unsigned long long>();
     }
     for (int step = 0; step < nsteps && n > 0; ++step) {
       PYR_HIP(hipMemsetAsync(err, 0, 2 * sizeof(int), stream()));
@@ -961,7 +980,7 @@ int pyr_lsrk4_steps(pyr_mesh *mesh, int precision, int nsteps, double dt, double
                                dim3(256), 0, stream(), mesh->dev, mesh->sf64.as<double>(), n, dt,
                                vacuum_c, x, p, q, mass, elem, death, counters, err, n_dead, mf_waves,
                                order, (const uint32_t *)nullptr);
-          else if (g_push_mfma)
+          else if (mesh->push_mfma)
             hipLaunchKernelGGL((k_lsrk4_step<D_, N_, float, true>), dim3(blocks_for(n)),
                                dim3(256), 0, stream(), mesh->dev, mesh->sf32.as<float>(), n, dt,
                                vacuum_c, x, p, q, mass, elem, death, counters, err, n_dead, mf_waves,
@@ -1187,25 +1206,29 @@ int pyr_lsrk4_adjoint(pyr_mesh *mesh, int precision, int nsteps, int ckpt_every,
   });
 }
 
-int pyr_push_set_mfma(int mode) {
+// FP32C push interpolation on the matrix cores, per mesh: 0 off (default: the
+// VALU contraction measured faster inside the register-bound push), 1 on, 2 on
+// and count the particle-stages that take the path
+int pyr_push_set_mfma(pyr_mesh *mesh, int mode) {
   return guarded([&] {
+    PYR_REQUIRE(mesh, "null mesh");
     PYR_REQUIRE(mode >= 0 && mode <= 2, "mode: 0 off, 1 on, 2 on and counting");
-    if (mode == 2 && g_push_mfma != 2) {
-      g_mf_count.reserve(sizeof(unsigned long long));
-      PYR_HIP(hipMemsetAsync(g_mf_count.ptr, 0, sizeof(unsigned long long), stream()));
+    if (mode == 2 && mesh->push_mfma != 2) {
+      mesh->mf_count.reserve(sizeof(unsigned long long));
+      PYR_HIP(hipMemsetAsync(mesh->mf_count.ptr, 0, sizeof(unsigned long long), stream()));
     }
-    g_push_mfma = mode;
+    mesh->push_mfma = mode;
   });
 }
 
-int pyr_push_mfma_waves(int64_t *out) {
+int pyr_push_mfma_waves(pyr_mesh *mesh, int64_t *out) {
   return guarded([&] {
-    PYR_REQUIRE(out, "null argument");
+    PYR_REQUIRE(mesh && out, "null argument");
     *out = 0;
-    if (!g_mf_count.ptr) return;
+    if (!mesh->mf_count.ptr) return;
     unsigned long long v = 0;
-    PYR_HIP(hipMemcpyAsync(&v, g_mf_count.ptr, sizeof(v), hipMemcpyDeviceToHost, stream()));
-    PYR_HIP(hipMemsetAsync(g_mf_count.ptr, 0, sizeof(v), stream()));
+    PYR_HIP(hipMemcpyAsync(&v, mesh->mf_count.ptr, sizeof(v), hipMemcpyDeviceToHost, stream()));
+    PYR_HIP(hipMemsetAsync(mesh->mf_count.ptr, 0, sizeof(v), stream()));
     PYR_HIP(hipStreamSynchronize(stream()));
     *out = (int64_t)v;
   });

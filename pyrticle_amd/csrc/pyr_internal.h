@@ -413,5 +413,15 @@ struct pyr_mesh {
   pyr::DevBuf sf_nodal;
   bool sf_has[6] = {false, false, false, false, false, false};
   pyr::DevBuf s_avg;                // averaging step scratch (push.hip)
+  // pyr_forces: occ[e] == occ_stamp marks element e occupied in this call
+  pyr::DevBuf s_occ;
+  uint32_t occ_stamp = 0;
+  // measurement switches of this mesh (no reference counterpart):
+  // push_mfma: FP32C push interpolation on the matrix cores (0 off, 1 on,
+  // 2 on and count the particle-stages that took it in mf_count);
+  // deposit_count: pyr_deposit also counts pair and visit statistics
+  int push_mfma = 0;
+  pyr::DevBuf mf_count;
+  int deposit_count = 0;
   ~pyr_mesh();
 };

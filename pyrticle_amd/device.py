@@ -296,6 +296,14 @@ class DeviceMesh:
         _lib.check(self._lib.pyr_deposit_stats(self._h, C.byref(arr)))
         return tuple(int(v) for v in arr)
 
+    def deposit_list_stats(self):
+        """(list entries, listed chunk-group pairs, matrix-core instructions
+        issued, work items) of the last FP32C deposition; zeros otherwise."""
+        _lib.sync_stream()
+        arr = (C.c_ulonglong * 4)()
+        _lib.check(self._lib.pyr_deposit_list_stats(self._h, C.byref(arr)))
+        return tuple(int(v) for v in arr)
+
     def candidate_pairs(self, radius) -> int:
         out = C.c_int64()
         _lib.check(self._lib.pyr_deposit_candidates(self._h, radius, C.byref(out)))
@@ -311,22 +319,22 @@ class DeviceMesh:
                                                  _lib.ptr(seg)))
         return perm, seg
 
+    def set_deposit_counting(self, on: bool):
+        """FP64 deposition on this mesh also counts pair and visit statistics
+        (read by deposit_stats)."""
+        _lib.check(self._lib.pyr_deposit_set_counting(self._h, 1 if on else 0))
 
-def set_deposit_counting(on: bool):
-    _lib.check(_lib.lib().pyr_deposit_set_counting(1 if on else 0))
+    def set_push_mfma(self, on: bool, count: bool = False):
+        """FP32C push on this mesh: interpolate on the matrix cores for waves
+        inside one element; with count, the particle-stages that take that path
+        are counted."""
+        _lib.check(self._lib.pyr_push_set_mfma(self._h, (2 if count else 1) if on else 0))
 
-
-def set_push_mfma(on: bool, count: bool = False):
-    """FP32C push: interpolate on the matrix cores for waves inside one element;
-    with count, the particle-stages that take that path are counted."""
-    _lib.check(_lib.lib().pyr_push_set_mfma((2 if count else 1) if on else 0))
-
-
-def push_mfma_waves() -> int:
-    """Particle-stages interpolated on the matrix cores since the last call."""
-    out = C.c_int64(0)
-    _lib.check(_lib.lib().pyr_push_mfma_waves(C.byref(out)))
-    return int(out.value)
+    def push_mfma_waves(self) -> int:
+        """Particle-stages interpolated on the matrix cores since the last call."""
+        out = C.c_int64(0)
+        _lib.check(self._lib.pyr_push_mfma_waves(self._h, C.byref(out)))
+        return int(out.value)
 
 
 def velocities(p: torch.Tensor, m: torch.Tensor, c: float) -> torch.Tensor:

@@ -704,17 +704,17 @@ def test_lsrk4_fp32c_mfma_interpolation(dense_p4, cuda):
     out = {}
     try:
         for name, on in (("mfma", True), ("valu", False), ("mfma2", True)):
-            dv.set_push_mfma(on, count=True)
-            dv.push_mfma_waves()
+            dm.set_push_mfma(on, count=True)
+            dm.push_mfma_waves()
             tx, tp, tq, tm = (_dev(a, cuda) for a in (x0, p0, q0, m0))
             tel = dv.element_tensor(el0, cuda)
             n = dm.lsrk4_steps(steps, wl.dt, wl.c, x0.shape[0], tx, tp, tq, tm, tel,
                                precision=dv.FP32C)
             assert n == n_o
             assert np.array_equal(dv.element_numpy(tel[:n]), oel[:n_o])
-            out[name] = (tx[:n].cpu().numpy(), tp[:n].cpu().numpy(), dv.push_mfma_waves())
+            out[name] = (tx[:n].cpu().numpy(), tp[:n].cpu().numpy(), dm.push_mfma_waves())
     finally:
-        dv.set_push_mfma(False)
+        dm.set_push_mfma(False)
     # displacement and momentum change over the run against the FP64 oracle:
     # FP32 fields (relative rounding ~1e-7) enter through the momentum change
     dx_ref, dp_ref = ox[:n_o] - x0[:n_o], op[:n_o] - p0[:n_o]
