@@ -217,8 +217,9 @@ __global__ void k_node_table_g(int D, int K, int np, int ng, const double *__res
       s2 += d[cnt][k] * d[cnt][k];
       ctr[k] += d[cnt][k];
     }
-    tab[slot] = make_float4((float)(d[cnt][0] * inv_p), (float)(d[cnt][1] * inv_p),
-                            (float)(d[cnt][2] * inv_p), (float)(-(s2 * invR) * inv_p));
+    const double bs = 2.0 * invR * inv_p;   // B = 2 (n - c_e) / (R P)
+    tab[slot] = make_float4((float)(d[cnt][0] * bs), (float)(d[cnt][1] * bs),
+                            (float)(d[cnt][2] * bs), (float)(-(s2 * invR) * inv_p));
     ++cnt;
   }
   double rad = 0;
@@ -1116,8 +1117,13 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
       {
         ProfScope prof(PROF_SORT);
         nch = chunk_table(mesh, c, n, cl, &ch_start);
-        c->pos32c.reserve(sizeof(float4) * 64 * (size_t)std::max<int64_t>(1, nch));
-        c->wgt32c.reserve(sizeof(float4) * 64 * (size_t)std::max<int64_t>(1, nch));
+        // chunk nch: all zero, read by the list gather past the end of a list
+        c->pos32c.reserve(sizeof(float4) * 64 * (size_t)(nch + 1));
+        c->wgt32c.reserve(sizeof(float4) * 64 * (size_t)(nch + 1));
+        PYR_HIP(hipMemsetAsync(c->pos32c.as<float4>() + 64 * (size_t)nch, 0,
+                               sizeof(float4) * 64, s));
+        PYR_HIP(hipMemsetAsync(c->wgt32c.as<float4>() + 64 * (size_t)nch, 0,
+                               sizeof(float4) * 64, s));
         c->ch_meta.reserve(sizeof(float4) * (size_t)std::max<int64_t>(1, nch));
         if (nch > 0) {
           const double scale = normalizer * std::pow(p_scale(radius), alpha);
@@ -1220,6 +1226,7 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
       la.wgt = c->wgt32c.as<float4>();
       la.ntab = c->tab32g.as<float4>();
       la.partial = c->partial.as<double>();
+      la.zero_chunk = (uint32_t)nch;
       CombineArgs cb{};
       cb.K = K;
       cb.NS = ns;

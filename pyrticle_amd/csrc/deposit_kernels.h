@@ -329,10 +329,10 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
 // v_mfma_f32_4x4x1_16b_f32 (16 independent 4x4 outer products, block b =
 // lanes 4b..4b+3):
 //   t[p][n]  = (R - |n - x_p|^2 / R) / P,   P = 2^ceil(log2 R) >= R (exact scaling)
-//            = -|n - c_e|^2 / (R P) + m_p . (n - c_e) / P + c_p,
-//              m_p = 2 d_p / R, d_p = x_p - c_e, c_p = (R - |d_p|^2 / R) / P
-//              (c_p enters last: it carries the large cancelling part), as
-//              C + sum_k A[p][k] B[k][n] over A = (m, c_p), B = ((n - c_e)/P, 1);
+//            = -|n - c_e|^2 / (R P) + d_p . 2 (n - c_e) / (R P) + c_p,
+//              d_p = x_p - c_e, c_p = (R - |d_p|^2 / R) / P (c_p enters
+//              last: it carries the large cancelling part), as C + sum_k
+//              A[p][k] B[k][n] over A = (d, c_p), B = (2 (n - c_e) / (R P), 1);
 //              A of block b, row i is particle 4b + i = the lane's own
 //              particle, so one instruction covers 64 particles x 4 nodes;
 //              t lands in VGPR i of lane 4b + j: particle 4b + i, node j;
@@ -342,21 +342,20 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
 //              operand of block b is w_{4b+i} spread over its four rows, which
 //              is component i of the lane's own transposed weight load.
 // The two chunks of a pair accumulate on separate sets so their product chains
-// interleave.  Every FOLD_STEPS chunks the per-lane FP32 partials are folded
-// into per-lane Kahan pairs (s, c) in LDS; at the end the sixteen blocks are
-// combined in FP64 in a fixed butterfly and the item's sums written as FP64
-// partials, which k_combine_items adds per target in item order: results are
-// bitwise reproducible.
+// interleave.  Every FOLD_STEPS chunks the per-lane FP32 partials are added
+// to per-lane FP64 sums in LDS; at the end the sixteen blocks are combined in
+// FP64 in a fixed butterfly and the item's sums written as FP64 partials,
+// which k_combine_items adds per target in item order: results are bitwise
+// reproducible.
 typedef float pyr_v4f __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ pyr_v4f mfma4x4(float a, float b, pyr_v4f c) {
   return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 0);
 }
 
-// chunks per Kahan fold: 16 (64 particles per 4x4 block) measured the same
-// error as 8 on the coarse test mesh and 1.4% less gather time on the
-// 4M-particle cylinder
-constexpr int FOLD_STEPS = 16;
+// chunks per fold into the FP64 sums (each FP32 partial then sums 4 x
+// FOLD_STEPS particles per 4x4 block)
+constexpr int FOLD_STEPS = 32;
 
 struct CullArgs {
   int K;
@@ -389,8 +388,9 @@ struct ListArgs {
   const float4 *ent_off;
   const float4 *pos;           // 64 slots per chunk: x - c_src (3), 0
   const float4 *wgt;           // 64 slots per chunk: weights of each quad, transposed
-  const float4 *ntab;          // K x 4 NG slots: (n - c_e) / P (3), -|n - c_e|^2 / (R P)
+  const float4 *ntab;          // K x 4 NG slots: 2 (n - c_e) / (R P) (3), -|n - c_e|^2 / (R P)
   double *partial;             // per item: 4 NG slots x 4 components
+  uint32_t zero_chunk;         // a staged chunk whose 64 slots are all zero
 };
 
 // relative margin of the FP32 sphere tests (they may only keep extra chunks)
@@ -499,40 +499,48 @@ __global__ __launch_bounds__(64 * W) void k_gather_cl(ListArgs g) {
   const uint2 it = g.items[item];
   const uint32_t e = it.x;
 
-  // per-lane Kahan pairs (s, c) of every accumulator entry, each lane
-  // touching only its own slots
-  __shared__ float2 s_kahan[W][NG * 4][64];
-  float2 *kl = &s_kahan[wave][0][lane];
+  // FP64 sums of every accumulator entry per lane pair (l, l + 32); both
+  // lanes of a pair hold the same value and write the same slot
+  __shared__ double s_sum[W][NG * 4][32];
+  double *kl = &s_sum[wave][0][lane & 31];
+#pragma unroll
+  for (int r = 0; r < NG * 4; ++r) kl[r * 32] = 0.0;
 
-  const float rf = (float)g.R, inv_rf = (float)g.invR, two_inv_r = (float)(2.0 * g.invR);
-  const float inv_p = g.inv_p;
+  // c_p = (R - |d|^2 / R) / P as one fma (P is a power of two, so the scaled
+  // constants are the unscaled ones, exactly)
+  const float r_p = (float)(g.R * (double)g.inv_p), inv_rp = (float)(g.invR * (double)g.inv_p);
   // this item's list entries: [first, first + cnt), one per lane
   const uint32_t base = g.cap_off[e] + it.y;
   const uint32_t cnt = min(64u, g.n_live[e] - it.y);
 
-  // two accumulator sets: the chunks of a pair accumulate independently, so
-  // their product chains interleave
-  pyr_v4f acc0[NG], acc1[NG];
+  // one accumulator set: the products of a pair's two chunks alternate on it
+  // (a dependent 4x4x1 product takes its accumulator straight from the one
+  // before)
+  pyr_v4f acc[NG];
 #pragma unroll
-  for (int ng = 0; ng < NG; ++ng) acc0[ng] = acc1[ng] = pyr_v4f{0.f, 0.f, 0.f, 0.f};
+  for (int ng = 0; ng < NG; ++ng) acc[ng] = pyr_v4f{0.f, 0.f, 0.f, 0.f};
   int n_steps = 0;
-  bool folded = false;   // wave-uniform
 
+  // lanes l and l + 32 are added in FP32 (v_permlane32_swap gives every lane
+  // both, in the same order), then added to the pair's FP64 slot.  One group
+  // at a time: the scheduler would otherwise hoist all the LDS reads of the
+  // fold and hold them in registers.
   auto fold = [&]() {
 #pragma unroll
-    for (int ng = 0; ng < NG; ++ng)
+    for (int ng = 0; ng < NG; ++ng) {
+      double old[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) old[v] = kl[(ng * 4 + v) * 32];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        float2 k = folded ? kl[(ng * 4 + v) * 64] : make_float2(0.f, 0.f);
-        const float y = (acc0[ng][v] + acc1[ng][v]) - k.y;
-        const float t = k.x + y;
-        k.y = (t - k.x) - y;
-        k.x = t;
-        kl[(ng * 4 + v) * 64] = k;
+        const uint32_t u = __float_as_uint(acc[ng][v]);
+        const auto sw = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+        kl[(ng * 4 + v) * 32] = old[v] + (double)(__uint_as_float(sw[0]) +
+                                                  __uint_as_float(sw[1]));
       }
-#pragma unroll
-    for (int ng = 0; ng < NG; ++ng) acc0[ng] = acc1[ng] = pyr_v4f{0.f, 0.f, 0.f, 0.f};
-    folded = true;
+      acc[ng] = pyr_v4f{0.f, 0.f, 0.f, 0.f};
+      __builtin_amdgcn_sched_barrier(0);
+    }
   };
 
   // node constants of this lane's node in every group
@@ -545,32 +553,27 @@ __global__ __launch_bounds__(64 * W) void k_gather_cl(ListArgs g) {
     bz[ng] = nt.z;
     cc[ng] = nt.w;
   }
-  // the item's list entries, one per lane
   uint2 ea = make_uint2(0u, 0u);
   float4 oa = make_float4(0.f, 0.f, 0.f, 0.f);
   if ((uint32_t)lane < cnt) {
     ea = g.ent[base + lane];
     oa = g.ent_off[base + lane];
   }
+  auto rdf = [](float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+  };
   struct Next {
     float4 p, w;
     float ox, oy, oz;
     uint32_t ngm;
   };
-  auto rdf = [](float v, int l) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-  };
-  // the loads of list entry i (i < cnt: a chunk; else nothing: weights zero)
   auto fetch = [&](Next &n, uint32_t i) {
-    if (i >= cnt) {
-      n.ngm = 0;
-      n.p = n.w = make_float4(0.f, 0.f, 0.f, 0.f);
-      n.ox = n.oy = n.oz = 0.f;
-      return;
-    }
-    const int l = (int)i;
-    const uint32_t ch = (uint32_t)__builtin_amdgcn_readlane((int)ea.x, l);
-    n.ngm = (uint32_t)__builtin_amdgcn_readlane((int)ea.y, l);
+    const bool live = i < cnt;
+    const int l = (int)min(i, 63u);
+    const uint32_t ch_l = (uint32_t)__builtin_amdgcn_readlane((int)ea.x, l);
+    const uint32_t m_l = (uint32_t)__builtin_amdgcn_readlane((int)ea.y, l);
+    const uint32_t ch = live ? ch_l : g.zero_chunk;
+    n.ngm = live ? m_l : 0u;
     n.ox = rdf(oa.x, l);
     n.oy = rdf(oa.y, l);
     n.oz = rdf(oa.z, l);
@@ -579,11 +582,11 @@ __global__ __launch_bounds__(64 * W) void k_gather_cl(ListArgs g) {
   };
   // t = (R - |n - x|^2 / R) / P: the chain starts from 1 * cc = cc exactly,
   // so it rounds as C = cc followed by the four products
-  auto pair_t = [&](float mx, float my, float mz, float cp, int ng) {
+  auto pair_t = [&](float dx, float dy, float dz, float cp, int ng) {
     pyr_v4f t = mfma4x4(1.f, cc[ng], pyr_v4f{0.f, 0.f, 0.f, 0.f});
-    t = mfma4x4(mx, bx[ng], t);
-    t = mfma4x4(my, by[ng], t);
-    if (D == 3) t = mfma4x4(mz, bz[ng], t);
+    t = mfma4x4(dx, bx[ng], t);
+    t = mfma4x4(dy, by[ng], t);
+    if (D == 3) t = mfma4x4(dz, bz[ng], t);
     return mfma4x4(cp, 1.f, t);
   };
   auto shape = [&](float t) {
@@ -603,26 +606,22 @@ __global__ __launch_bounds__(64 * W) void k_gather_cl(ListArgs g) {
     const uint32_t mask = a.ngm | b.ngm;
     if (!mask) return;
     const float ax = a.p.x + a.ox, ay = a.p.y + a.oy, az = D == 3 ? a.p.z + a.oz : 0.f;
-    const float add = fmaf(az, az, fmaf(ay, ay, ax * ax));
-    const float amx = ax * two_inv_r, amy = ay * two_inv_r, amz = az * two_inv_r;
-    const float acp = fmaf(-add, inv_rf, rf) * inv_p;
+    const float acp = fmaf(-fmaf(az, az, fmaf(ay, ay, ax * ax)), inv_rp, r_p);
     const float bxx = b.p.x + b.ox, byy = b.p.y + b.oy, bzz = D == 3 ? b.p.z + b.oz : 0.f;
-    const float bdd = fmaf(bzz, bzz, fmaf(byy, byy, bxx * bxx));
-    const float bmx = bxx * two_inv_r, bmy = byy * two_inv_r, bmz = bzz * two_inv_r;
-    const float bcp = fmaf(-bdd, inv_rf, rf) * inv_p;
+    const float bcp = fmaf(-fmaf(bzz, bzz, fmaf(byy, byy, bxx * bxx)), inv_rp, r_p);
 #pragma unroll
     for (int ng = 0; ng < NG; ++ng) {
       if (!((mask >> ng) & 1u)) continue;
-      const pyr_v4f ta = pair_t(amx, amy, amz, acp, ng);
-      const pyr_v4f tb = pair_t(bmx, bmy, bmz, bcp, ng);
-      acc0[ng] = mfma4x4(a.w.x, shape(ta[0]), acc0[ng]);
-      acc1[ng] = mfma4x4(b.w.x, shape(tb[0]), acc1[ng]);
-      acc0[ng] = mfma4x4(a.w.y, shape(ta[1]), acc0[ng]);
-      acc1[ng] = mfma4x4(b.w.y, shape(tb[1]), acc1[ng]);
-      acc0[ng] = mfma4x4(a.w.z, shape(ta[2]), acc0[ng]);
-      acc1[ng] = mfma4x4(b.w.z, shape(tb[2]), acc1[ng]);
-      acc0[ng] = mfma4x4(a.w.w, shape(ta[3]), acc0[ng]);
-      acc1[ng] = mfma4x4(b.w.w, shape(tb[3]), acc1[ng]);
+      const pyr_v4f ta = pair_t(ax, ay, az, acp, ng);
+      const pyr_v4f tb = pair_t(bxx, byy, bzz, bcp, ng);
+      acc[ng] = mfma4x4(a.w.x, shape(ta[0]), acc[ng]);
+      acc[ng] = mfma4x4(b.w.x, shape(tb[0]), acc[ng]);
+      acc[ng] = mfma4x4(a.w.y, shape(ta[1]), acc[ng]);
+      acc[ng] = mfma4x4(b.w.y, shape(tb[1]), acc[ng]);
+      acc[ng] = mfma4x4(a.w.z, shape(ta[2]), acc[ng]);
+      acc[ng] = mfma4x4(b.w.z, shape(tb[2]), acc[ng]);
+      acc[ng] = mfma4x4(a.w.w, shape(ta[3]), acc[ng]);
+      acc[ng] = mfma4x4(b.w.w, shape(tb[3]), acc[ng]);
     }
     if (++n_steps == FOLD) {
       n_steps = 0;
@@ -639,36 +638,31 @@ __global__ __launch_bounds__(64 * W) void k_gather_cl(ListArgs g) {
 #pragma unroll 1
   for (uint32_t i = 0; i < cnt; i += 4) {
     evaluate2(a, b);
+    __builtin_amdgcn_sched_barrier(0);
     fetch(a, i + 4);
     fetch(b, i + 5);
+    __builtin_amdgcn_sched_barrier(0);
     evaluate2(c, d);
+    __builtin_amdgcn_sched_barrier(0);
     fetch(c, i + 6);
     fetch(d, i + 7);
+    __builtin_amdgcn_sched_barrier(0);
   }
   if (n_steps > 0) fold();
-  // combine the sixteen blocks (lanes l, l^4, ..., l^60) in FP64 in a fixed
-  // butterfly; lanes 0..3 then hold the item's sums of node slot 4 ng + lane
-  double out[NG][4];
+  // combine the sixteen blocks (slots of lanes l, l^4, ..., l^28, each
+  // holding two blocks) in FP64 in a fixed butterfly; lanes 0..3 then hold
+  // the item's sums of node slot 4 ng + lane, stored one by one (holding all
+  // of them would set the register peak)
+  double *dst = g.partial + (size_t)item * (16 * NG);
 #pragma unroll
   for (int ng = 0; ng < NG; ++ng)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      double val = 0.0;
-      if (folded) {
-        const float2 k = kl[(ng * 4 + v) * 64];
-        val = (double)k.x - (double)k.y;
-      }
+      double val = kl[(ng * 4 + v) * 32];
 #pragma unroll
-      for (int o = 4; o < 64; o <<= 1) val += __shfl_xor(val, o, 64);
-      out[ng][v] = val;
+      for (int o = 4; o < 32; o <<= 1) val += __shfl_xor(val, o, 64);
+      if (lane < 4) dst[(4 * ng + lane) * 4 + v] = val;
     }
-  if (lane < 4) {
-    double *dst = g.partial + (size_t)item * (16 * NG);
-#pragma unroll
-    for (int ng = 0; ng < NG; ++ng)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) dst[(4 * ng + lane) * 4 + v] = out[ng][v];
-  }
 }
 
 // ---------------------------------------------------------------------------
