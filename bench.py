@@ -325,6 +325,7 @@ def _profile_cmd(kind, counters=None, probe=False):
 
 def _kernel_key(name):
     """Dispatched kernel name without its argument list."""
+    name = name.replace("(anonymous namespace)", "{anon}")
     depth = 0
     for k, ch in enumerate(name):
         if ch == "<":
