@@ -191,14 +191,16 @@ __global__ void k_node_table(int D, int K, int np, const double *__restrict__ no
 }
 
 
-// Grouped FP32 node table of the cluster gather: slot 4 g + i of element e
-// holds node node_slot[4 g + i] as ((n - c_e) / P, -|n - c_e|^2 / (R P))
-// (padding slots (0, 0, 0, -1): t = c_p - 1 <= 0), and ngsph[e NG + g] the
-// bounding sphere of group g (centre relative to c_e, radius rounded up).
+// Grouped node tables of the list gathers: slot 4 g + i of element e holds
+// node node_slot[4 g + i], in FP32 as (2 (n - c_e) / (R P), -|n - c_e|^2 /
+// (R P)) (padding slots (0, 0, 0, -1): t = c_p - 1 <= 0) and in FP64 as
+// (n - c_e, |n - c_e|^2 / R), the values of k_node_table (padding slots
+// (0, 0, 0, 2 R): t <= -R); ngsph[e NG + g] is the bounding sphere of group g
+// (centre relative to c_e, radius rounded up).
 __global__ void k_node_table_g(int D, int K, int np, int ng, const double *__restrict__ nodes,
                                const double *__restrict__ geom, double invR, double inv_p,
                                const int *__restrict__ node_slot, float4 *__restrict__ tab,
-                               float4 *__restrict__ sph) {
+                               float4 *__restrict__ sph, double4 *__restrict__ tab64) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)K * ng) return;
   const int e = (int)(t / ng), grp = (int)(t % ng);
@@ -209,6 +211,8 @@ __global__ void k_node_table_g(int D, int K, int np, int ng, const double *__res
     const size_t slot = (size_t)e * 4 * ng + 4 * grp + i;
     if (node < 0) {
       tab[slot] = make_float4(0.f, 0.f, 0.f, -1.f);
+      // FP64 gather: t = (R - |d|^2 / R) - 2 R <= -R, a zero shape value
+      tab64[slot] = make_double4(0.0, 0.0, 0.0, 2.0 / invR);
       continue;
     }
     double s2 = 0;
@@ -217,6 +221,7 @@ __global__ void k_node_table_g(int D, int K, int np, int ng, const double *__res
       s2 += d[cnt][k] * d[cnt][k];
       ctr[k] += d[cnt][k];
     }
+    tab64[slot] = make_double4(d[cnt][0], d[cnt][1], d[cnt][2], s2 * invR);
     const double bs = 2.0 * invR * inv_p;   // B = 2 (n - c_e) / (R P)
     tab[slot] = make_float4((float)(d[cnt][0] * bs), (float)(d[cnt][1] * bs),
                             (float)(d[cnt][2] * bs), (float)(-(s2 * invR) * inv_p));
@@ -312,7 +317,8 @@ __global__ void k_list_stats(int K, int D, const uint32_t *__restrict__ cap_off,
   if (!n) return;
   unsigned long long groups = 0, instr = 0;
   for (uint32_t i = 0; i < n; i += 2) {
-    const uint32_t ma = ent[b + i].y, mb = i + 1 < n ? ent[b + i + 1].y : 0u;
+    const uint32_t ma = ent[b + i].y & ENT_MASK,
+                   mb = i + 1 < n ? ent[b + i + 1].y & ENT_MASK : 0u;
     groups += (unsigned)(__popc(ma) + __popc(mb));
     instr += 2ull * (unsigned)(D + 6) * (unsigned)__popc(ma | mb);
   }
@@ -430,7 +436,7 @@ struct DepositCache {
   DevBuf keys_a, keys_b, vals_a, vals_b, hist, hist_scan, seg, tmp, pos, wgt, stats;
   // matrix-core gather over culled chunk lists: targets in Morton order,
   // grouped node tables, per-deposition chunk staging and the lists
-  DevBuf t_order, tab32g, ngsph, node_slot, ch_meta, pos32c, wgt32c;
+  DevBuf t_order, tab32g, tab64g, ngsph, node_slot, ch_meta, pos32c, wgt32c;
   DevBuf cap, ent, ent_off, n_live, items, partial;
   DevBuf ikey, ik_a, ik_b, iv_a, iv_b;   // work items sorted by their first chunk
   DevBuf active;                         // targets that may receive charge
@@ -456,7 +462,7 @@ void pyr_deposit_cache_free(pyr_mesh *mesh) {
                     &c->tab_R, &c->tr_start, &c->tr, &c->tr_shift, &c->keys_a, &c->keys_b,
                     &c->vals_a, &c->vals_b, &c->skeys, &c->hist,
                     &c->hist_scan, &c->seg, &c->tmp, &c->pos, &c->wgt, &c->stats,
-                    &c->nrm, &c->chunks, &c->t_order, &c->tab32g,
+                    &c->nrm, &c->chunks, &c->t_order, &c->tab32g, &c->tab64g,
                     &c->ngsph, &c->node_slot, &c->ch_meta, &c->pos32c, &c->wgt32c, &c->cap,
                     &c->ent, &c->ent_off, &c->n_live, &c->items, &c->partial, &c->ikey,
                     &c->ik_a, &c->ik_b, &c->iv_a, &c->iv_b, &c->active, &c->list_stats})
@@ -932,11 +938,12 @@ static void prepare_tables(pyr_mesh *mesh, DepositCache *c, double radius) {
     PYR_CHECK_LAUNCH();
     const int ng = (Np + 3) / 4;
     c->tab32g.reserve(sizeof(float4) * (size_t)K * 4 * ng);
+    c->tab64g.reserve(sizeof(double4) * (size_t)K * 4 * ng);
     c->ngsph.reserve(sizeof(float4) * (size_t)K * ng);
     hipLaunchKernelGGL(k_node_table_g, dim3(blocks_for((int64_t)K * ng)), dim3(256), 0, s, D, K,
                        Np, ng, mesh->dev.nodes, c->geom.as<double>(), 1.0 / radius,
                        1.0 / p_scale(radius), c->node_slot.as<int>(), c->tab32g.as<float4>(),
-                       c->ngsph.as<float4>());
+                       c->ngsph.as<float4>(), c->tab64g.as<double4>());
     PYR_CHECK_LAUNCH();
     c->tab_radius = radius;
   }
@@ -1107,10 +1114,11 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
     ga.stats = c->stats.as<unsigned long long>();
     const bool count = mesh->deposit_count != 0;
     if (count) PYR_HIP(hipMemsetAsync(ga.stats, 0, 4 * sizeof(unsigned long long), s));
-    if (precision == PYR_FP32C) {
+    if (precision == PYR_FP32C || !count) {
       if (count) throw Error(PYR_ERR_INVALID_ARG, "deposit counting needs the FP64 precision");
       // chunks of up to 64 sorted particles per source element, staged in
-      // FP32 with their bounding spheres, then the cluster gather
+      // FP32 with their bounding spheres, per-target culled chunk lists, then
+      // the list gather (FP32C: matrix cores; FP64: the FP64 staging)
       ChunkList cl{};
       const uint32_t *ch_start = nullptr;
       int64_t nch = 0;
@@ -1240,13 +1248,39 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
       cb.j = j;
       {
         ProfScope prof(PROF_GATHER);   // the list gather alone
-        if (n_items) launch_gather_cl(D, Np, la, n_items, alpha == 2.0);
+        if (n_items && precision == PYR_FP32C) {
+          launch_gather_cl(D, Np, la, n_items, alpha == 2.0);
+        } else if (n_items) {
+          ListArgs64 l64{};
+          l64.K = K;
+          l64.Np = Np;
+          l64.R = radius;
+          l64.invR = 1.0 / radius;
+          l64.alpha = alpha;
+          for (int a = 0; a < 3; ++a) l64.shift_len[a] = ga.shift_len[a];
+          l64.items = la.items;
+          l64.item_order = la.item_order;
+          l64.n_items = la.n_items;
+          l64.cap_off = cap_off;
+          l64.n_live = la.n_live;
+          l64.ent = la.ent;
+          l64.chunks = cl.chunks;
+          l64.seg = c->seg.as<uint32_t>();
+          l64.pos = c->pos.as<double4>();
+          l64.wgt = c->wgt.as<double4>();
+          l64.geom = c->geom.as<double>();
+          l64.face_normal = mesh->dev.face_normal;
+          l64.face_rhs = mesh->dev.face_rhs;
+          l64.node_tab = c->tab64g.as<double4>();
+          l64.partial = c->partial.as<double>();
+          launch_gather_cl64(D, Np, l64, n_items, shape_unit_square(alpha, radius));
+        }
       }
       ProfScope prof(PROF_SORT);       // with the other per-deposition set-up work
       hipLaunchKernelGGL(k_combine_items, dim3(blocks_for((int64_t)K * ns)), dim3(256), 0, s,
                          cb);
       PYR_CHECK_LAUNCH();
-      c->list_valid = true;
+      c->list_valid = precision == PYR_FP32C;
       c->list_items = n_items;
       return;
     }

@@ -369,7 +369,7 @@ struct CullArgs {
   const double *geom;          // K x 8: centroid (3), ...
   const float4 *ngsph;         // K x NG: group centre rel. c_e (3), radius
   const uint32_t *cap_off;     // K + 1: list region of each target
-  uint2 *ent;                  // list entries: chunk id, group mask
+  uint2 *ent;                  // list entries: chunk id, group mask | shift code << 16
   float4 *ent_off;             //               source offset from the target (3), 0
   uint32_t *n_live;            // K: list length
   const uint8_t *active;       // K: 0 when no candidate source element holds a particle
@@ -395,6 +395,10 @@ struct ListArgs {
 
 // relative margin of the FP32 sphere tests (they may only keep extra chunks)
 constexpr float CL_MARGIN = 1e-4f;
+// list entry word y: the group mask in the low bits (at most 9 groups), the
+// periodic shift code of the chunk's source element from bit 16
+constexpr int ENT_SHIFT_BIT = 16;
+constexpr uint32_t ENT_MASK = (1u << ENT_SHIFT_BIT) - 1;
 // targets (waves) per workgroup of the list gather
 constexpr int CL_W = 4;
 
@@ -425,14 +429,14 @@ __global__ __launch_bounds__(256) void k_cull_chunks(CullArgs g) {
   uint32_t n_out = 0;
   const uint32_t c_begin = g.cand_start[e], c_end = g.cand_start[e + 1];
   for (uint32_t cw = c_begin; cw < c_end; cw += 64) {
-    uint32_t c0 = 0, cnt = 0;
+    uint32_t c0 = 0, cnt = 0, code = 0;
     float ox = 0.f, oy = 0.f, oz = 0.f;
     if (cw + lane < c_end) {
       const uint32_t src = g.cand[cw + lane];
       c0 = g.ch_start[src];
       cnt = g.ch_start[src + 1] - c0;
       if (cnt) {
-        const uint32_t code = g.cand_shift[cw + lane];
+        code = g.cand_shift[cw + lane];
         double sh[3] = {0.0, 0.0, 0.0};
 #pragma unroll
         for (int a = 0; a < D; ++a) {
@@ -463,6 +467,9 @@ __global__ __launch_bounds__(256) void k_cull_chunks(CullArgs g) {
       k = min(k, 63);
       const uint32_t ch = __shfl(c0, k, 64) + (f - __shfl(excl, k, 64));
       const float px = __shfl(ox, k, 64), py = __shfl(oy, k, 64), pz = __shfl(oz, k, 64);
+      // (in uniform control flow: a shuffle from a lane inactive in a branch
+      // reads zero)
+      const uint32_t pcode = (uint32_t)__shfl((int)code, k, 64);
       uint32_t ngm = 0;
       if (f < total) {
         const float4 meta = g.ch_meta[ch];
@@ -478,7 +485,7 @@ __global__ __launch_bounds__(256) void k_cull_chunks(CullArgs g) {
       const uint64_t live = __ballot(ngm != 0);
       if (ngm) {
         const uint32_t at = base + n_out + (uint32_t)__popcll(live & lt_mask);
-        g.ent[at] = make_uint2(ch, ngm);
+        g.ent[at] = make_uint2(ch, ngm | (pcode << ENT_SHIFT_BIT));
         g.ent_off[at] = make_float4(px, py, pz, 0.f);
       }
       n_out += (uint32_t)__popcll(live);
@@ -577,7 +584,7 @@ __global__ __launch_bounds__(64 * W) void k_gather_cl(ListArgs g) {
     const uint32_t ch_l = (uint32_t)__builtin_amdgcn_readlane((int)ea.x, l);
     const uint32_t m_l = (uint32_t)__builtin_amdgcn_readlane((int)ea.y, l);
     const uint32_t ch = live ? ch_l : g.zero_chunk;
-    n.ngm = live ? m_l : 0u;
+    n.ngm = live ? (m_l & ENT_MASK) : 0u;
     n.ox = rdf(oa.x, l);
     n.oy = rdf(oa.y, l);
     n.oz = rdf(oa.z, l);
@@ -670,6 +677,244 @@ __global__ __launch_bounds__(64 * W) void k_gather_cl(ListArgs g) {
       for (int o = 4; o < 32; o <<= 1) val += __shfl_xor(val, o, 64);
       if (lane < 4) dst[(4 * ng + lane) * 4 + v] = val;
     }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 gather over the culled chunk lists
+// ---------------------------------------------------------------------------
+// The FP32C lists are conservative for FP64 too: a chunk is left out of a
+// target's list only when every node of the target lies farther than R from
+// all of its particles by more than FP32 rounding, so the FP64 shape there is
+// exactly zero.  One block per work item (at most 64 entries of one target's
+// list); each wave owns SPW node slots (the grouped FP64 node table).  Per iteration every wave takes one list entry, one chunk
+// particle per lane (FP64 staging of the element-sorted order, the entry's
+// periodic shift added); the particles whose ball can reach the target
+// (bounding sphere of the nodes inflated by R, face planes moved out by R)
+// are queued in LDS and evaluated in batches of 64, one particle per lane,
+// against each wave's slots (node data through scalar loads), in the
+// arithmetic of k_gather.  The item's sums are reduced across lanes in a fixed
+// tree and written as FP64 partials in slot order; k_combine_items adds them
+// per target in item order, so results are bitwise reproducible.
+struct ListArgs64 {
+  int K, Np;
+  double R, invR, alpha;
+  double shift_len[3];
+  const uint2 *items;          // work items: target, first list entry
+  const uint32_t *item_order;  // processing order of the items
+  const uint32_t *n_items;     // device count of work items
+  const uint32_t *cap_off;     // K + 1
+  const uint32_t *n_live;      // K
+  const uint2 *ent;            // list entries: chunk id, group mask | shift code << 16
+  const uint2 *chunks;         // per chunk: source element, sorted index of its first particle
+  const uint32_t *seg;         // K + 1: sorted range of each source element
+  const double4 *pos;          // FP64 staging: x (3), 0
+  const double4 *wgt;          //               (q N, q N v)
+  const double *geom;          // K x 8: centroid (3), (R + node radius)^2, ...
+  const double *face_normal;
+  const double *face_rhs;
+  const double4 *node_tab;     // K x 4 NG slots: n - c, |n - c|^2 / R (padding: t <= -R)
+  double *partial;             // per item: 4 NG slots x 4 components
+};
+
+template <int D, int NP, int SPW, bool A2>
+__global__ __launch_bounds__(64 * ((4 * ((NP + 3) / 4)) / SPW)) void k_gather_cl64(ListArgs64 g) {
+  using T = double;
+  constexpr int NS = 4 * ((NP + 3) / 4);
+  constexpr int G = NS / SPW;               // waves per block
+  static_assert(G * SPW == NS, "slots per wave must divide the slot count");
+  constexpr int BS = 64 * G;
+  constexpr int QCAP = 64 + BS;             // ring buffer of queued particles
+  constexpr int NV = D + 1;                 // rho + j components
+  constexpr int REC = 8;                    // record: m (3), c, a, b (3)
+
+  const int64_t k_item = xcd_element(blockIdx.x);
+  if (k_item >= (int64_t)*g.n_items) return;   // block-uniform
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int slot0 = wave * SPW;
+  const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const uint32_t item = g.item_order[k_item];
+  const uint2 it = g.items[item];
+  const uint32_t e = it.x;
+  const uint32_t base = g.cap_off[e] + it.y;
+  const uint32_t cnt = min(64u, g.n_live[e] - it.y);
+
+  __shared__ T q_rec[QCAP][REC];
+  __shared__ int wave_pass[G];
+
+  const double cx = g.geom[(size_t)e * 8 + 0], cy = g.geom[(size_t)e * 8 + 1],
+               cz = g.geom[(size_t)e * 8 + 2], bound2 = g.geom[(size_t)e * 8 + 3];
+  double fnl[D + 1][D], frl[D + 1];
+#pragma unroll
+  for (int f = 0; f <= D; ++f) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) fnl[f][k] = g.face_normal[((size_t)e * (D + 1) + f) * D + k];
+    frl[f] = g.face_rhs[(size_t)e * (D + 1) + f] + g.R * (1.0 + 1e-12);
+  }
+  using CT = const T __attribute__((address_space(4))) *;
+  const CT ntab = (CT)g.node_tab + ((size_t)e * NS + slot0) * 4;
+
+  T acc[SPW][NV];
+#pragma unroll
+  for (int i = 0; i < SPW; ++i)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[i][v] = T(0);
+  int q_head = 0, q_count = 0;   // identical in every thread of the block
+
+  auto process_batch = [&](int count) {
+    if (lane < count) {
+      int slot = q_head + lane;
+      if (slot >= QCAP) slot -= QCAP;
+      T rec[REC];
+#pragma unroll
+      for (int k = 0; k < REC; ++k) rec[k] = q_rec[slot][k];
+#pragma unroll
+      for (int i = 0; i < SPW; ++i) {
+        const CT nd = ntab + i * 4;
+        T t = rec[3] - nd[3];
+#pragma unroll
+        for (int k = D - 1; k >= 0; --k) t = fma(nd[k], rec[k], t);
+        T s;
+        if constexpr (A2) {
+          s = sq_pos_unit(t);
+        } else {
+          const T w = t > T(0) ? t : T(0);
+          s = g.alpha == 2.0 ? w * w : (T)pow((double)w, g.alpha);
+        }
+        acc[i][0] = fma(rec[4], s, acc[i][0]);
+#pragma unroll
+        for (int k = 0; k < D; ++k) acc[i][1 + k] = fma(rec[5 + k], s, acc[i][1 + k]);
+      }
+    }
+  };
+
+  // the item's list entries, one per lane, resolved once: the sorted index
+  // of the chunk's first particle, its particle count, the shift code
+  uint32_t e_first = 0, e_count = 0, e_code = 0;
+  if ((uint32_t)lane < cnt) {
+    const uint2 en = g.ent[base + lane];
+    const uint2 ck = g.chunks[en.x];   // source element, first sorted index
+    e_first = ck.y;
+    e_count = min(64u, g.seg[ck.x + 1] - ck.y);
+    e_code = en.y >> ENT_SHIFT_BIT;
+  }
+
+  // the loads of list entry ei for this wave, one chunk particle per lane.
+  // Issued on every call with an in-bounds index (lanes past the chunk's
+  // count, and entries past the item's end, re-read a staged particle and are
+  // dropped by count = 0 or lane >= count), so the same number of loads is in
+  // flight on every path and the wait before an entry's use does not cover
+  // the entry fetched after it.
+  struct Pre {
+    double4 pp, ww;
+    uint32_t count, code;
+  };
+  auto fetch = [&](Pre &d, uint32_t ei) {
+    const int l = (int)min(ei, 63u);
+    const uint32_t s0 = (uint32_t)__builtin_amdgcn_readlane((int)e_first, l);
+    const uint32_t ce = (uint32_t)__builtin_amdgcn_readlane((int)e_count, l);
+    d.code = (uint32_t)__builtin_amdgcn_readlane((int)e_code, l);
+    d.count = ei < cnt ? ce : 0u;
+    const uint32_t idx = s0 + min((uint32_t)lane, ce > 0u ? ce - 1u : 0u);
+    d.pp = g.pos[idx];
+    d.ww = g.wgt[idx];
+  };
+  // prefilter and queue the fetched entry (all waves of the block take part)
+  auto step = [&](const Pre &d) {
+    bool pass = false;
+    T rec[REC];
+    if ((uint32_t)lane < d.count) {
+      double sh[3] = {0, 0, 0};
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        const uint32_t c2 = (d.code >> (2 * a)) & 3u;
+        sh[a] = c2 == 1 ? g.shift_len[a] : (c2 == 2 ? -g.shift_len[a] : 0.0);
+      }
+      const double x[3] = {d.pp.x + sh[0], d.pp.y + sh[1], D == 3 ? d.pp.z + sh[2] : 0.0};
+      const double dl[3] = {x[0] - cx, x[1] - cy, x[2] - cz};
+      const double dd = fma(dl[0], dl[0], fma(dl[1], dl[1], D == 3 ? dl[2] * dl[2] : 0.0));
+      double viol = -INFINITY;
+#pragma unroll
+      for (int f = 0; f <= D; ++f) {
+        double ip = fnl[f][0] * x[0];
+#pragma unroll
+        for (int kk = 1; kk < D; ++kk) ip = fma(fnl[f][kk], x[kk], ip);
+        viol = fmax(viol, ip - frl[f]);
+      }
+      pass = (dd <= bound2) & (viol <= 0.0);
+      if (pass) {
+        const double two_inv_r = 2.0 * g.invR;
+        rec[0] = (T)(dl[0] * two_inv_r);
+        rec[1] = (T)(dl[1] * two_inv_r);
+        rec[2] = (T)(dl[2] * two_inv_r);
+        rec[3] = (T)fma(-dd, g.invR, g.R);
+        rec[4] = (T)d.ww.x;
+        rec[5] = (T)d.ww.y;
+        rec[6] = (T)d.ww.z;
+        rec[7] = (T)d.ww.w;
+      }
+    }
+    const uint64_t bal = __ballot(pass);
+    if (G > 1) {
+      if (lane == 0) wave_pass[wave] = __popcll(bal);
+      __syncthreads();
+    }
+    int off = q_count, added = 0;
+    if (G > 1) {
+#pragma unroll
+      for (int w = 0; w < G; ++w) {
+        if (w < wave) off += wave_pass[w];
+        added += wave_pass[w];
+      }
+    } else {
+      added = __popcll(bal);
+    }
+    if (pass) {
+      int slot = q_head + off + __popcll(bal & lt_mask);
+      if (slot >= QCAP) slot -= QCAP;
+#pragma unroll
+      for (int kk = 0; kk < REC; ++kk) q_rec[slot][kk] = rec[kk];
+    }
+    q_count += added;
+    __syncthreads();
+    while (q_count >= 64) {
+      process_batch(64);
+      q_head += 64;
+      if (q_head >= QCAP) q_head -= QCAP;
+      q_count -= 64;
+      __syncthreads();   // the freed slots may be refilled next
+    }
+  };
+
+  // wave w takes entries w, w + G, w + 2 G, ...; two named buffers, the loads
+  // of the next entry in flight while one is queued and evaluated
+  Pre pa, pb;
+  fetch(pa, (uint32_t)wave);
+#pragma unroll 1
+  for (uint32_t i0 = 0; i0 < cnt; i0 += 2 * G) {
+    fetch(pb, i0 + G + (uint32_t)wave);
+    __builtin_amdgcn_sched_barrier(0);
+    step(pa);
+    __builtin_amdgcn_sched_barrier(0);
+    fetch(pa, i0 + 2 * G + (uint32_t)wave);
+    __builtin_amdgcn_sched_barrier(0);
+    step(pb);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  if (q_count > 0) process_batch(q_count);
+
+  // reduce across lanes in a fixed tree; lane 0 writes this wave's slots
+  double *dst = g.partial + (size_t)item * NS * 4;
+#pragma unroll
+  for (int i = 0; i < SPW; ++i) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      double val = v < NV ? acc[i][v < NV ? v : 0] : 0.0;
+      if (v < NV)
+        for (int o = 32; o > 0; o >>= 1) val += __shfl_xor(val, o, 64);
+      if (lane == 0) dst[(slot0 + i) * 4 + v] = val;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1050,6 +1295,7 @@ __global__ __launch_bounds__(64) void k_shape_average(AverageArgs g) {
 void launch_gather_fp64(int D, int Np, const GatherArgs &ga, int K, bool a2, bool count);
 void launch_cull_chunks(int D, int Np, const CullArgs &ca);
 void launch_gather_cl(int D, int Np, const ListArgs &la, uint32_t n_items, bool a2);
+void launch_gather_cl64(int D, int Np, const ListArgs64 &la, uint32_t n_items, bool a2);
 void launch_deposit_adj(int D, int Np, const AdjArgs &ga, unsigned grid, int precision,
                         bool a2);
 void launch_shape_integral(int D, int Np, const IntegralArgs &ga, unsigned grid, bool a2);

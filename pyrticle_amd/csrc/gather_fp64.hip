@@ -40,4 +40,35 @@ void launch_gather_fp64(int D, int Np, const GatherArgs &ga, int K, bool a2, boo
   throw Error(PYR_ERR_INVALID_ARG, "deposition supports P1..P4 (2D) and P1..P4 (3D) elements");
 }
 
+// the list gather: node slots per wave (4 NG slots in all): the whole element
+// up to 16 slots, 10 for P3 and 12 for P4 tetrahedra
+template <int D, int NP, int SPW>
+static void launch_cl64(const ListArgs64 &la, uint32_t n_items, bool a2) {
+  constexpr int G = (4 * ((NP + 3) / 4)) / SPW;
+  dim3 grid(gather_grid((int)n_items)), block(64 * G);
+  hipStream_t s = stream();
+  if (a2) hipLaunchKernelGGL((k_gather_cl64<D, NP, SPW, true>), grid, block, 0, s, la);
+  else hipLaunchKernelGGL((k_gather_cl64<D, NP, SPW, false>), grid, block, 0, s, la);
+  PYR_CHECK_LAUNCH();
+}
+
+void launch_gather_cl64(int D, int Np, const ListArgs64 &la, uint32_t n_items, bool a2) {
+  if (D == 2) {
+    switch (Np) {
+      case 3: return launch_cl64<2, 3, 4>(la, n_items, a2);
+      case 6: return launch_cl64<2, 6, 8>(la, n_items, a2);
+      case 10: return launch_cl64<2, 10, 12>(la, n_items, a2);
+      case 15: return launch_cl64<2, 15, 16>(la, n_items, a2);
+    }
+  } else {
+    switch (Np) {
+      case 4: return launch_cl64<3, 4, 4>(la, n_items, a2);
+      case 10: return launch_cl64<3, 10, 12>(la, n_items, a2);
+      case 20: return launch_cl64<3, 20, 10>(la, n_items, a2);
+      case 35: return launch_cl64<3, 35, 12>(la, n_items, a2);
+    }
+  }
+  throw Error(PYR_ERR_INVALID_ARG, "deposition supports P1..P4 (2D) and P1..P4 (3D) elements");
+}
+
 }  // namespace pyr
