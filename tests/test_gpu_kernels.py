@@ -220,6 +220,43 @@ def test_deposit_3d_matches_oracle(small_cylinder, dmesh3, cuda, precision):
     assert torch.equal(rho, rho2) and torch.equal(j, j2)     # deterministic
 
 
+def test_deposit_counting_mode(small_cylinder, dmesh3, cuda):
+    """The pair-counting mode runs the candidate gather (every candidate
+    particle of every target, the target test, all nodes of the target): its
+    rho and j equal the list gather's to rounding, it counts exactly the
+    node-particle pairs closer than R (a k-d tree brute force) and evaluates
+    every node of every particle that passes the target test."""
+    from scipy.spatial import cKDTree
+    from pyrticle_amd import device as dv
+    wl = small_cylinder
+    n = 5000
+    x, q = wl.x[:n], wl.q[:n]
+    el = dmesh3.find_containing_element(_dev(x, cuda))
+    v = O.velocities(wl.p[:n], wl.m[:n], wl.c)
+    nrm = O.shape_normalizer(wl.radius, 3, wl.alpha)
+    args = (wl.radius, wl.alpha, nrm, _dev(x, cuda), _dev(q, cuda), el)
+    rho_l, j_l = dmesh3.deposit(*args, v=_dev(v, cuda))
+    dmesh3.set_deposit_counting(True)
+    try:
+        rho_c, j_c = dmesh3.deposit(*args, v=_dev(v, cuda))
+        n_eval, n_in, n_cand, n_pass = dmesh3.deposit_stats()
+    finally:
+        dmesh3.set_deposit_counting(False)
+    assert rel_l2(rho_c.cpu().numpy(), rho_l.cpu().numpy()) < 1e-13
+    assert rel_l2(j_c.cpu().numpy(), j_l.cpu().numpy()) < 1e-13
+    tree = cKDTree(np.asarray(wl.ma.nodes).reshape(-1, 3))
+    inside = sum(len(h) for h in tree.query_ball_point(x, wl.radius * (1 - 1e-12)))
+    touching = sum(len(h) for h in tree.query_ball_point(x, wl.radius * (1 + 1e-12)))
+    assert inside <= n_in <= touching
+    assert n_pass <= n_cand and n_eval == n_pass * wl.ma.Np
+    with pytest.raises(ValueError):   # FP32C has no counting pass
+        dmesh3.set_deposit_counting(True)
+        try:
+            dmesh3.deposit(*args, v=_dev(v, cuda), precision=dv.FP32C)
+        finally:
+            dmesh3.set_deposit_counting(False)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32c"])
 def test_deposit_dense_beam_matches_oracle(small_cylinder, dmesh3, cuda, precision):
     """Thousands of particles per element: the gather's LDS queue runs deep."""
