@@ -43,8 +43,9 @@ enum {
 /* precision forms */
 enum {
   PYR_FP64 = 0,   /* plain FP64 throughout: the drop-in */
-  PYR_FP32C = 1   /* FP32 bulk arithmetic, Kahan-compensated per-node sums,
-                     FP64 positions, face tests and charge normalization */
+  PYR_FP32C = 1   /* FP32 bulk arithmetic, FP32 partial sums folded into FP64
+                     per-node sums, FP64 positions, face tests and charge
+                     normalization */
 };
 
 typedef struct pyr_mesh pyr_mesh;
@@ -247,9 +248,10 @@ int pyr_velocities_backward(int64_t n, int vdim, double vacuum_c, const double *
  * normalizer as tools.cpp:43-58 computes it).  rho (K*Np) and j (K*Np x
  * vdim) are overwritten; either may be NULL; velocities may be NULL when j is.
  * Particles whose containing element is invalid contribute nothing.
- * precision PYR_FP32C evaluates the shape and sums in FP32 with Kahan-
- * compensated per-node sums; positions relative to the element, q*normalizer
- * and the outputs stay FP64. */
+ * precision PYR_FP32C evaluates the shape and its products in FP32 (on the
+ * matrix cores) and folds short FP32 partial sums into FP64 per-node sums;
+ * positions relative to the element, q*normalizer and the outputs stay FP64.
+ * Both precisions give bitwise reproducible results. */
 int pyr_deposit(pyr_mesh *mesh, double radius, double alpha, double normalizer,
                 int precision, int64_t n, int vdim, const double *positions,
                 const double *charges, const double *velocities,
