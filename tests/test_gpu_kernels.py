@@ -604,6 +604,44 @@ def test_new_entry_points_edge_cases(square, oracle_square, dmesh2, cuda):
         dmesh2.face_walk_stats(3.0, _dev(x[:1], cuda), dv.element_tensor(el[:1], cuda))
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_deposit_list_edge_cases(small_cylinder, dmesh3, cuda, precision):
+    """Inputs that shape the chunk lists unusually, against the oracle: no
+    particles, only particles without an element, a single particle, many
+    particles at one point (full chunks of identical positions), and one
+    particle in each of many elements (chunks of one)."""
+    from pyrticle_amd import device as dv
+    wl = small_cylinder
+    om = O.OracleMesh(wl.ma)
+    nrm = O.shape_normalizer(wl.radius, 3, wl.alpha)
+    prec = dv.FP64 if precision == "fp64" else dv.FP32C
+    tol = FP64_TOL if precision == "fp64" else FP32C_TOL
+
+    def run(x, el, v, q):
+        rho, j = dmesh3.deposit(wl.radius, wl.alpha, nrm, _dev(x, cuda), _dev(q, cuda),
+                                dv.element_tensor(el, cuda), v=_dev(v, cuda), precision=prec)
+        return rho.cpu().numpy(), j.cpu().numpy()
+
+    # no particles, and particles none of which has an element: zeros
+    for n in (0, 5):
+        x = np.tile(wl.x[:1], (n, 1))
+        rho, j = run(x, np.full(n, 0xFFFFFFFF, dtype=np.uint32), np.ones((n, 3)),
+                     np.full(n, wl.q[0]))
+        assert not np.abs(rho).max(initial=0) and not np.abs(j).max(initial=0)
+    el_all = dv.element_numpy(dmesh3.find_containing_element(_dev(wl.x, cuda)))
+    v_all = O.velocities(wl.p, wl.m, wl.c)
+    cases = {
+        "single": np.array([0]),
+        "coincident": np.zeros(300, dtype=np.int64),
+        "one_per_element": np.unique(el_all, return_index=True)[1][:2000],
+    }
+    for name, idx in cases.items():
+        x, el, v, q = wl.x[idx], el_all[idx], v_all[idx], wl.q[idx]
+        rho_o, j_o, _ = O.deposit(om, wl.radius, wl.alpha, x, q, v, el)
+        rho, j = run(x, el, v, q)
+        assert rel_l2(rho, rho_o) < tol and rel_l2(j, j_o) < tol, name
+
+
 def test_diagnostics_match_oracle(cuda):
     """The seven diagnostics on 1M particles against the oracle's sequential
     sums (summation order differs: sums within 1e-12, the cancelling emittance
