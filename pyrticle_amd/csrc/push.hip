@@ -353,9 +353,13 @@ __device__ __forceinline__ bool lsrk4_stage(const DevMesh &m, const T *__restric
       stage_update<D, N, T, MF>(coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, n_mf);
   // fast path: still inside the (wave-uniform) element, same arithmetic as
   // the first branch of find_new_containing_element
-  if (e_uni != INVALID && still_inside<D>(m, e, e_uni, xi, r)) {
-    cnt.same++;
-    return true;
+  if (e_uni != INVALID) {
+    if (still_inside<D>(m, e, e_uni, xi, r)) {
+      cnt.same++;
+      return true;
+    }
+    // outside, by the same arithmetic as the cascade's first test
+    return relocate<D, true>(m, xi, pi, e, cnt, e_flag, r);
   }
   return relocate<D>(m, xi, pi, e, cnt, e_flag, r);
 }
@@ -420,6 +424,482 @@ __global__ __launch_bounds__(256, (push_min_waves<N, T>())) void k_lsrk4_step(
     unsigned long long v = n_mf;
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     if ((threadIdx.x & 63) == 0 && v) atomicAdd(mf_waves, v);
+  }
+  flush_counters(cnt, counters);
+}
+
+// ---------------------------------------------------------------------------
+// fused LSRK4 step with a per-wave LDS element cache
+// ---------------------------------------------------------------------------
+// A wave's particles start a step in one or two elements (the deposition's
+// sort order) and spread into their face neighbours as they cross faces; after
+// the first stage three quarters of the waves hold several elements (2.1 on
+// average on the 4M-particle cylinder).  Per-lane loads of the field
+// coefficients, inverse maps and face data then cost the texture data path a
+// full wave of data per load instruction, which saturated it (TD busy 87-91%
+// of the push).  Here each wave copies, once per step and with coalesced
+// loads, the data of its start elements and of their face neighbours into its
+// own LDS slots, and the five stages read them from LDS: interpolation
+// (6 x Np coefficients), the containment test (inverse map) and the first
+// steps of relocation (face normals, the neighbour across the chosen face and
+// its inverse map).  Element-uniform waves keep the scalar loads of the
+// uncached kernel.  A lane in an element without a slot (two faces crossed in
+// one step, the vertex fan, the global search, a periodic trip) sends its wave
+// through the uncached stage until the lane is back in a cached element.  The
+// arithmetic is that of k_lsrk4_step on the same values: the same bits.
+template <int D, int N, class T>
+struct StepCache {
+  static constexpr int NP = Basis<D, N>::NP, NS = coef_stride<T>(NP);
+  static constexpr int INV = 6 * NS * (int)sizeof(T);    // byte offsets in a slot
+  static constexpr int NRM = INV + (D * D + D) * 8;
+  static constexpr int NBR = NRM + (D + 1) * D * 8;      // neighbour ids (u32)
+  static constexpr int NBS = NBR + (D + 1) * 4;          // neighbour slots (int)
+  static constexpr int BYTES = (NBS + (D + 1) * 4 + 15) / 16 * 16;
+  static constexpr int Q = NBR / 16;                     // 16-byte chunks copied per slot
+  // about 10 KB per wave: 4 waves per SIMD
+  static constexpr int SLOTS = 10240 / BYTES < 16 ? 10240 / BYTES : 16;
+  static_assert(INV % 16 == 0 && NRM % 16 == 0 && NBR % 16 == 0, "slot layout");
+  static_assert(SLOTS >= D + 2, "a slot set must hold one element and its neighbours");
+};
+
+template <int D, int N, class T>
+__device__ __forceinline__ int cache_lookup(const uint32_t *cel, int nsl, uint32_t e) {
+  int s = -1;
+  for (int j = 0; j < nsl; ++j)
+    if (cel[j] == e) {
+      s = j;
+      break;
+    }
+  return s;
+}
+
+// the k-th set bit of mask (k < popcount(mask))
+__device__ __forceinline__ int nth_bit(uint32_t mask, int k) {
+  for (int j = 0; j < k; ++j) mask &= mask - 1;
+  return __builtin_ctz(mask);
+}
+
+// Chunk c (16 bytes) of the copy of the slots in mask: its source (the slot's
+// element's coefficients, inverse map and face normals lie in a slot as in
+// their tables) and its byte offset in the wave's slots.
+template <int D, int N, class T>
+__device__ __forceinline__ uint4 cache_chunk(const DevMesh &m, const T *__restrict__ coef,
+                                             const uint32_t *cel, uint32_t mask, int c,
+                                             int &dst) {
+  using C = StepCache<D, N, T>;
+  const int s = nth_bit(mask, c / C::Q), qq = c % C::Q;
+  const uint32_t el = cel[s];
+  const char *src;
+  if (qq * 16 < C::INV)
+    src = (const char *)coef + (size_t)el * C::INV + qq * 16;
+  else if (qq * 16 < C::NRM)
+    src = (const char *)m.inv_map + (size_t)el * (D * D + D) * 8 + (qq * 16 - C::INV);
+  else
+    src = (const char *)m.face_normal + (size_t)el * (D + 1) * D * 8 + (qq * 16 - C::NRM);
+  dst = s * C::BYTES + qq * 16;
+  return *(const uint4 *)src;
+}
+
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Gives slots to the elements of the active lanes in `need` (lowest lane's
+// element first), then to the face neighbours of those elements, in slots no
+// lane holds (used), as many as fit; copies their data with the active lanes
+// (four 16-byte chunks in flight per lane) and rewrites every slot's
+// neighbour slots.  Lanes whose element finds no slot keep slot = -1.
+template <int D, int N, class T>
+__device__ __forceinline__ void cache_update(const DevMesh &m, const T *__restrict__ coef,
+                                             char *base, uint32_t *cel, int &nsl, uint32_t used,
+                                             uint64_t need, uint32_t e, int &slot) {
+  using C = StepCache<D, N, T>;
+  constexpr uint32_t ALL = C::SLOTS >= 32 ? ~0u : (1u << C::SLOTS) - 1;
+  uint32_t fresh = 0;
+  while (need && (used & ALL) != ALL) {
+    const int j = __builtin_ctz(~used & ALL);
+    const uint32_t eu = (uint32_t)__builtin_amdgcn_readlane((int)e, (int)__builtin_ctzll(need));
+    cel[j] = eu;
+    used |= 1u << j;
+    fresh |= 1u << j;
+    nsl = j + 1 > nsl ? j + 1 : nsl;
+    const bool mine = e == eu;
+    if (mine) slot = j;
+    need &= ~__ballot(mine);
+  }
+  wave_lds_fence();
+  uint32_t fr = fresh;
+  while (fr && (used & ALL) != ALL) {
+    const int j0 = __builtin_ctz(fr);
+    fr &= fr - 1;
+    const uint32_t ej = (uint32_t)__builtin_amdgcn_readfirstlane((int)cel[j0]);
+#pragma unroll
+    for (int f = 0; f <= D; ++f) {
+      const uint32_t nb = as_constant(m.face_neighbor)[(size_t)ej * (D + 1) + f];
+      if (nb == INVALID || (used & ALL) == ALL) continue;
+      const int have = __builtin_amdgcn_readfirstlane(cache_lookup<D, N, T>(cel, nsl, nb));
+      if (have >= 0) {
+        used |= 1u << have;   // keep it
+        continue;
+      }
+      const int j = __builtin_ctz(~used & ALL);
+      cel[j] = nb;
+      used |= 1u << j;
+      fresh |= 1u << j;
+      nsl = j + 1 > nsl ? j + 1 : nsl;
+      wave_lds_fence();
+    }
+  }
+  wave_lds_fence();
+  const uint64_t act = __ballot(1);
+  const int nact = __popcll(act);
+  const int rank = __popcll(act & ((1ull << (threadIdx.x & 63)) - 1));
+  const int total = __builtin_popcount(fresh) * C::Q;
+  for (int c0 = rank; c0 < total; c0 += 4 * nact) {
+    const int cb = c0 + nact, ccc = c0 + 2 * nact, cd = c0 + 3 * nact;
+    int da = 0, db = 0, dc = 0, dd = 0;
+    uint4 va = {}, vb = {}, vc = {}, vd = {};
+    va = cache_chunk<D, N, T>(m, coef, cel, fresh, c0, da);
+    if (cb < total) vb = cache_chunk<D, N, T>(m, coef, cel, fresh, cb, db);
+    if (ccc < total) vc = cache_chunk<D, N, T>(m, coef, cel, fresh, ccc, dc);
+    if (cd < total) vd = cache_chunk<D, N, T>(m, coef, cel, fresh, cd, dd);
+    *(uint4 *)(base + da) = va;
+    if (cb < total) *(uint4 *)(base + db) = vb;
+    if (ccc < total) *(uint4 *)(base + dc) = vc;
+    if (cd < total) *(uint4 *)(base + dd) = vd;
+  }
+  for (int w = rank; w < __builtin_popcount(fresh) * (D + 1); w += nact) {
+    const int s = nth_bit(fresh, w / (D + 1)), f = w % (D + 1);
+    ((uint32_t *)(base + s * C::BYTES + C::NBR))[f] =
+        m.face_neighbor[(size_t)cel[s] * (D + 1) + f];
+  }
+  wave_lds_fence();
+  for (int w = rank; w < nsl * (D + 1); w += nact) {
+    const int s = w / (D + 1), f = w % (D + 1);
+    const uint32_t nb = ((const uint32_t *)(base + s * C::BYTES + C::NBR))[f];
+    ((int *)(base + s * C::BYTES + C::NBS))[f] =
+        nb == INVALID ? -1 : cache_lookup<D, N, T>(cel, nsl, nb);
+  }
+  wave_lds_fence();
+}
+
+// inverse_map / in_unit_simplex on a cached inverse map
+template <int D>
+__device__ __forceinline__ bool inside_cached(const char *slot_inv, const double *x, double *r) {
+  const double *a = (const double *)slot_inv;
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    double t = 0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) t += a[i * D + j] * x[j];
+    r[i] = t + a[D * D + i];
+  }
+  return in_unit_simplex<D>(r);
+}
+
+// The head of relocate<D, true> (particle_state.hpp:160-190: the face whose
+// normal best matches the momentum, the neighbour across it) on the cached
+// data of the lane's slot (slot < 0: from the mesh arrays).  Returns 1 when
+// the neighbour contains x (elem, slot and r updated), 0 on the reference's
+// "no best normal" error (err set), 2 when the cascade continues with the
+// closest-vertex fan.
+template <int D, int N, class T>
+__device__ __forceinline__ int relocate_head(const DevMesh &m, const char *base,
+                                             const uint32_t *cel, int nsl, const double *x,
+                                             const double *p, uint32_t &elem, int &slot,
+                                             FindCounters &cnt, int &err, double *r) {
+  using C = StepCache<D, N, T>;
+  double nrm[(D + 1) * D];
+  const char *sb = base + (slot < 0 ? 0 : slot) * C::BYTES;
+  if (slot >= 0) {
+#pragma unroll
+    for (int k = 0; k < (D + 1) * D; ++k) nrm[k] = ((const double *)(sb + C::NRM))[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < (D + 1) * D; ++k) nrm[k] = m.face_normal[(size_t)elem * (D + 1) * D + k];
+  }
+  int closest = -1;
+  double max_ip = 0;
+#pragma unroll
+  for (int f = 0; f <= D; ++f) {
+    double ip = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) ip += nrm[f * D + i] * p[i];
+    if (ip > max_ip) {
+      closest = f;
+      max_ip = ip;
+    }
+  }
+  if (closest == -1) {
+    err = 1;
+    return 0;
+  }
+  uint32_t possible;
+  int ps = -1;
+  if (slot >= 0) {
+    possible = ((const uint32_t *)(sb + C::NBR))[closest];
+    ps = ((const int *)(sb + C::NBS))[closest];
+  } else {
+    possible = m.face_neighbor[(size_t)elem * (D + 1) + closest];
+  }
+  if (possible != INVALID) {
+    const bool in = ps >= 0 ? inside_cached<D>(base + ps * C::BYTES + C::INV, x, r)
+                            : in_element<D>(m, possible, x, r);
+    if (in) {
+      cnt.neighbor++;
+      elem = possible;
+      slot = ps >= 0 ? ps : cache_lookup<D, N, T>(cel, nsl, possible);
+      return 1;
+    }
+  }
+  return 2;
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// First element of list[k0, k1) (in list order) that contains x, tested by
+// the wave's active lanes 64 at a time; INVALID if none.  rl receives the
+// unit coordinates in the element found (from the lane that tested it).
+template <int D>
+__device__ __forceinline__ uint32_t wave_first_containing(const DevMesh &m,
+                                                          const uint32_t *list, uint32_t k0,
+                                                          uint32_t k1, const double *x,
+                                                          double *rl) {
+  const uint64_t act = __ballot(1);
+  const int nact = __popcll(act);
+  const int rank = __popcll(act & ((1ull << (threadIdx.x & 63)) - 1));
+  for (uint32_t b = k0; b < k1; b += (uint32_t)nact) {
+    const uint32_t k = b + (uint32_t)rank;
+    uint32_t cand = INVALID;
+    double rr[D];
+    bool hit = false;
+    if (k < k1) {
+      cand = list[k];
+      hit = in_element<D>(m, cand, x, rr);
+    }
+    const uint64_t hb = __ballot(hit);
+    if (hb) {
+      const int j = (int)__builtin_ctzll(hb);   // lowest hit lane = lowest k
+#pragma unroll
+      for (int i = 0; i < D; ++i) rl[i] = readlane_f64(rr[i], j);
+      return (uint32_t)__builtin_amdgcn_readlane((int)cand, j);
+    }
+  }
+  return INVALID;
+}
+
+// find_by_vertex_or_global (particle_state.hpp:198-261, meshdata.hpp:208-215)
+// for the particle of lane L (position x, previous element prev: uniform
+// values), run by the whole wave: the elements around prev's closest vertex,
+// then the global grid cell, each list tested 64 elements at a time and the
+// first containing one (in list order) taken, as the sequential scan would.
+// kind: 1 vertex fan, 2 global search (the counter the lane bumps).
+template <int D>
+__device__ __forceinline__ uint32_t wave_vertex_or_global(const DevMesh &m, const double *x,
+                                                          uint32_t prev, double *rl,
+                                                          int &kind) {
+  uint32_t closest_vertex = INVALID;
+  double min_dist = __builtin_inf();
+#pragma unroll
+  for (int k = 0; k <= D; ++k) {
+    const uint32_t vi = as_constant(m.el_vertices)[(size_t)prev * (D + 1) + k];
+    double s = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const double dv = as_constant(m.vertices)[(size_t)vi * D + i] - x[i];
+      s += dv * dv;
+    }
+    const double dist = sqrt(s);
+    if (dist < min_dist) {
+      closest_vertex = vi;
+      min_dist = dist;
+    }
+  }
+  closest_vertex = (uint32_t)__builtin_amdgcn_readfirstlane((int)closest_vertex);
+  uint32_t res = wave_first_containing<D>(m, m.vadj_elements,
+                                          as_constant(m.vadj_starts)[closest_vertex],
+                                          as_constant(m.vadj_starts)[closest_vertex + 1], x, rl);
+  if (res != INVALID) {
+    kind = 1;
+    return res;
+  }
+  kind = 2;
+  int cell = 0;
+#pragma unroll
+  for (int i = D - 1; i >= 0; --i) {
+    const double c = (x[i] - m.g_lo[i]) * m.g_inv_h[i];
+    if (!(c >= 0.0) || c >= (double)m.g_n[i]) return INVALID;
+    cell = cell * m.g_n[i] + (int)c;
+  }
+  cell = __builtin_amdgcn_readfirstlane(cell);
+  return wave_first_containing<D>(m, m.g_elems, as_constant(m.g_starts)[cell],
+                                  as_constant(m.g_starts)[cell + 1], x, rl);
+}
+
+template <int D, int N, class T>
+__global__ __launch_bounds__(256, (push_min_waves<N, T>())) void k_lsrk4_step_c(
+    DevMesh m, const T *__restrict__ coef, int64_t n, double dt, double c,
+    double *__restrict__ x, double *__restrict__ p, const double *__restrict__ q,
+    const double *__restrict__ mass, uint32_t *__restrict__ elem,
+    uint32_t *__restrict__ death, unsigned long long *counters, int *err,
+    unsigned int *n_dead, const uint32_t *__restrict__ order) {
+  using C = StepCache<D, N, T>;
+  __shared__ uint4 lds[4][C::SLOTS * C::BYTES / 16];
+  __shared__ uint32_t lds_el[4][C::SLOTS];
+  const int wv = threadIdx.x >> 6;
+  char *base = (char *)&lds[wv][0];
+  uint32_t *cel = lds_el[wv];
+  // thread t steps particle order[t] (t with a null order)
+  const int64_t t =
+      (order ? xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x) * blockDim.x + threadIdx.x;
+  FindCounters cnt;
+  const bool live = t < n;
+  const int64_t i = live ? (order ? (int64_t)order[t] : t) : 0;
+  uint32_t e = live ? elem[i] : INVALID;
+  const bool located = e < (uint32_t)m.K;
+  // slots: the wave's start elements (lowest lane's first), then their face
+  // neighbours, as many as fit
+  int slot = -1, nsl = 0;
+  cache_update<D, N, T>(m, coef, base, cel, nsl, 0u, __ballot(located), e, slot);
+  if (live) {
+    double xi[D], pi[D], r[D], rx[D], rp[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      xi[k] = x[i * D + k];
+      pi[k] = p[i * D + k];
+      rx[k] = 0;
+      rp[k] = 0;
+    }
+    const double qi = q[i], mi = mass[i];
+    uint32_t died = 0;
+    int e_flag = 0;
+    if (!located) died = 1;   // not located: removed at the first stage
+    else if (slot >= 0) inside_cached<D>(base + slot * C::BYTES + C::INV, xi, r);
+    else inverse_map<D>(m, e, xi, r);
+#pragma unroll 1
+    for (int s = 0; s < 5 && !died; ++s) {
+      // coefficients: scalar loads for an element-uniform wave, LDS when every
+      // active lane's element is cached, else per-lane loads
+      const bool cached = __ballot(slot < 0) == 0;
+      double v[D], F[D];
+      velocity<D>(pi, mi, c, v);
+      const uint32_t e_uni = wave_uniform_element(e);
+      const T *cc = (const T *)(base + (slot < 0 ? 0 : slot) * C::BYTES);
+      if constexpr (sizeof(T) == 8) {
+        double f6[6], el[3], mag[3];
+        if (e_uni != INVALID)
+          interp6<D, N, double>(as_constant(coef),
+                                (uint32_t)__builtin_amdgcn_readfirstlane((int)e_uni), r, f6);
+        else if (cached)
+          interp6<D, N, double>(cc, 0u, r, f6);
+        else
+          interp6<D, N, double>(coef, e, r, f6);
+        lorentz<D, double>(qi, f6, v, F, el, mag);
+      } else {
+        float rf[D], f6[6], vf[D], Ff[D], el[3], mag[3];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          rf[k] = (float)r[k];
+          vf[k] = (float)v[k];
+        }
+        if (e_uni != INVALID)
+          interp6<D, N, float>(as_constant(coef),
+                               (uint32_t)__builtin_amdgcn_readfirstlane((int)e_uni), rf, f6);
+        else if (cached)
+          interp6<D, N, float>(cc, 0u, rf, f6);
+        else
+          interp6<D, N, float>(coef, e, rf, f6);
+        lorentz<D, float>((float)qi, f6, vf, Ff, el, mag);
+#pragma unroll
+        for (int k = 0; k < D; ++k) F[k] = (double)Ff[k];
+      }
+      const double a = c_rk4a[s], b = c_rk4b[s];
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        rx[k] = a * rx[k] + dt * v[k];
+        rp[k] = a * rp[k] + dt * F[k];
+      }
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        xi[k] += b * rx[k];
+        pi[k] += b * rp[k];
+      }
+      // the first test of find_new_containing_element, then the rest of the
+      // cascade
+      bool in;
+      if (e_uni != INVALID)
+        in = still_inside<D>(m, e, e_uni, xi, r);
+      else if (slot >= 0)
+        in = inside_cached<D>(base + slot * C::BYTES + C::INV, xi, r);
+      else
+        in = in_element<D>(m, e, xi, r);
+      bool ok = true;
+      int head = 1;
+      if (in)
+        cnt.same++;
+      else
+        head = relocate_head<D, N, T>(m, base, cel, nsl, xi, pi, e, slot, cnt, e_flag, r);
+      // the rest of the cascade, one particle at a time with the whole wave
+      // (a sequential fan costs two dependent loads per candidate element)
+      uint64_t pend = __ballot(head == 2);
+      while (pend) {
+        const int L = (int)__builtin_ctzll(pend);
+        pend &= pend - 1;
+        double xl[D], rl[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) xl[k] = readlane_f64(xi[k], L);
+        const uint32_t prev = (uint32_t)__builtin_amdgcn_readlane((int)e, L);
+        int kind = 0;
+        const uint32_t ne = wave_vertex_or_global<D>(m, xl, prev, rl, kind);
+        if ((int)(threadIdx.x & 63) == L) {
+          if (kind == 1) cnt.vertex++;
+          else cnt.global++;
+          if (ne != INVALID) {
+            e = ne;
+#pragma unroll
+            for (int k = 0; k < D; ++k) r[k] = rl[k];
+            head = 3;
+          }
+        }
+      }
+      if (head == 0) {
+        ok = false;
+      } else if (head == 2) {
+        // no element contains x: periodic trip or death (boundary_hit)
+        ok = boundary_hit<D>(m, xi, pi, e, cnt, e_flag, r);
+        slot = ok ? cache_lookup<D, N, T>(cel, nsl, e) : -1;
+      } else if (head == 3) {
+        slot = cache_lookup<D, N, T>(cel, nsl, e);   // found by the fan or the grid
+      }
+      if (!ok) {
+        died = (uint32_t)s + 1;
+        slot = -1;
+      }
+      // slots for the elements the wave's lanes entered, in slots no lane holds
+      const uint64_t need = __ballot(!died && slot < 0);
+      if (need && s < 4) {
+        uint32_t used = 0;
+        for (int j = 0; j < nsl; ++j)
+          if (__ballot(slot == j)) used |= 1u << j;
+        cache_update<D, N, T>(m, coef, base, cel, nsl, used, need, e, slot);
+      }
+    }
+    if (e_flag) atomicOr(err, 1);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      x[i * D + k] = xi[k];
+      p[i * D + k] = pi[k];
+    }
+    elem[i] = e;
+    death[i] = died;
+    if (died) atomicAdd(n_dead, 1u);
   }
   flush_counters(cnt, counters);
 }
@@ -976,20 +1456,18 @@ int pyr_lsrk4_steps(pyr_mesh *mesh, int precision, int nsteps, double dt, double
         dispatch_order(D, N, [&](auto DD, auto NN) {
           constexpr int D_ = decltype(DD)::value, N_ = decltype(NN)::value;
           if (precision == PYR_FP64)
-            hipLaunchKernelGGL((k_lsrk4_step<D_, N_, double, false>), dim3(blocks_for(n)),
+            hipLaunchKernelGGL((k_lsrk4_step_c<D_, N_, double>), dim3(blocks_for(n)),
                                dim3(256), 0, stream(), mesh->dev, mesh->sf64.as<double>(), n, dt,
-                               vacuum_c, x, p, q, mass, elem, death, counters, err, n_dead, mf_waves,
-                               order, (const uint32_t *)nullptr);
+                               vacuum_c, x, p, q, mass, elem, death, counters, err, n_dead, order);
           else if (mesh->push_mfma)
             hipLaunchKernelGGL((k_lsrk4_step<D_, N_, float, true>), dim3(blocks_for(n)),
                                dim3(256), 0, stream(), mesh->dev, mesh->sf32.as<float>(), n, dt,
                                vacuum_c, x, p, q, mass, elem, death, counters, err, n_dead, mf_waves,
                                order, (const uint32_t *)nullptr);
           else
-            hipLaunchKernelGGL((k_lsrk4_step<D_, N_, float, false>), dim3(blocks_for(n)),
+            hipLaunchKernelGGL((k_lsrk4_step_c<D_, N_, float>), dim3(blocks_for(n)),
                                dim3(256), 0, stream(), mesh->dev, mesh->sf32.as<float>(), n, dt,
-                               vacuum_c, x, p, q, mass, elem, death, counters, err, n_dead, mf_waves,
-                               order, (const uint32_t *)nullptr);
+                               vacuum_c, x, p, q, mass, elem, death, counters, err, n_dead, order);
           PYR_CHECK_LAUNCH();
         });
       }

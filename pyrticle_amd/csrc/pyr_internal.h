@@ -233,18 +233,13 @@ __device__ __forceinline__ uint32_t find_by_vertex_or_global(const DevMesh &m,
   return find_global<D>(m, x, r);
 }
 
-// particle_state.hpp:130-261.  *err is set when the reference would throw
-// "no best normal found--weird".  r receives the unit coordinates of x in the
-// returned element (valid when the result is not INVALID).
+// particle_state.hpp:160-261: the cascade once x is known to lie outside prev
+// (the neighbour across the face whose normal best matches the momentum, then
+// the closest-vertex fan, then the global search).
 template <int D>
-__device__ uint32_t find_new_containing_element(const DevMesh &m, const double *x,
-                                                const double *p, uint32_t prev,
-                                                FindCounters &cnt, int &err, double *r) {
+__device__ uint32_t find_outside_prev(const DevMesh &m, const double *x, const double *p,
+                                      uint32_t prev, FindCounters &cnt, int &err, double *r) {
   if (prev != INVALID && prev < (uint32_t)m.K) {
-    if (in_element<D>(m, prev, x, r)) {
-      cnt.same++;
-      return prev;
-    }
     int closest = -1;
     double max_ip = 0;
 #pragma unroll
@@ -271,19 +266,27 @@ __device__ uint32_t find_new_containing_element(const DevMesh &m, const double *
   return find_by_vertex_or_global<D>(m, x, prev, cnt, r);
 }
 
-// update_containing_elements + boundary_hit for one particle
-// (particle_state.hpp:266-357).  Returns false if the particle dies.  x is
-// wrapped in place on a periodic trip, as the reference writes it back (:341).
+// particle_state.hpp:130-261.  *err is set when the reference would throw
+// "no best normal found--weird".  r receives the unit coordinates of x in the
+// returned element (valid when the result is not INVALID).
 template <int D>
-__device__ __forceinline__ bool relocate(const DevMesh &m, double *x, const double *p,
-                                         uint32_t &elem, FindCounters &cnt, int &err,
-                                         double *r) {
-  uint32_t ne = find_new_containing_element<D>(m, x, p, elem, cnt, err, r);
-  if (err) return false;
-  if (ne != INVALID) {
-    elem = ne;
-    return true;
+__device__ uint32_t find_new_containing_element(const DevMesh &m, const double *x,
+                                                const double *p, uint32_t prev,
+                                                FindCounters &cnt, int &err, double *r) {
+  if (prev != INVALID && prev < (uint32_t)m.K && in_element<D>(m, prev, x, r)) {
+    cnt.same++;
+    return prev;
   }
+  return find_outside_prev<D>(m, x, p, prev, cnt, err, r);
+}
+
+// boundary_hit for a particle no element contains (particle_state.hpp:
+// 301-357): on a periodic trip x is wrapped in place, as the reference writes
+// it back (:341), and located again.  Returns false if the particle dies.
+template <int D>
+__device__ __forceinline__ bool boundary_hit(const DevMesh &m, double *x, const double *p,
+                                             uint32_t &elem, FindCounters &cnt, int &err,
+                                             double *r) {
   bool trip = false;
 #pragma unroll
   for (int a = 0; a < D; ++a) {
@@ -311,6 +314,24 @@ __device__ __forceinline__ bool relocate(const DevMesh &m, double *x, const doub
     }
   }
   return false;
+}
+
+// update_containing_elements + boundary_hit for one particle
+// (particle_state.hpp:266-357).  Returns false if the particle dies.
+// OUTSIDE: the caller has already found x outside elem with the same
+// arithmetic as the cascade's first test, which is then not repeated.
+template <int D, bool OUTSIDE = false>
+__device__ __forceinline__ bool relocate(const DevMesh &m, double *x, const double *p,
+                                         uint32_t &elem, FindCounters &cnt, int &err,
+                                         double *r) {
+  uint32_t ne = OUTSIDE ? find_outside_prev<D>(m, x, p, elem, cnt, err, r)
+                        : find_new_containing_element<D>(m, x, p, elem, cnt, err, r);
+  if (err) return false;
+  if (ne != INVALID) {
+    elem = ne;
+    return true;
+  }
+  return boundary_hit<D>(m, x, p, elem, cnt, err, r);
 }
 
 // particle_state.hpp:98-125 get_velocities, same operation order.
