@@ -178,16 +178,21 @@ __global__ void k_locate(DevMesh m, int64_t n, double *__restrict__ x,
                          uint8_t *__restrict__ dead, unsigned long long *counters, int *err) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   FindCounters cnt;
-  if (i < n) {
-    double xi[D], pi[D], r[D];
+  // every lane takes part in the wave's fan and grid searches
+  const bool live = i < n;
+  double xi[D] = {}, pi[D] = {}, r[D];
+  uint32_t e = INVALID;
+  if (live) {
 #pragma unroll
     for (int k = 0; k < D; ++k) {
       xi[k] = x[i * D + k];
       pi[k] = p[i * D + k];
     }
-    uint32_t e = elem[i];
-    int e_flag = 0;
-    bool alive = relocate<D>(m, xi, pi, e, cnt, e_flag, r);
+    e = elem[i];
+  }
+  int e_flag = 0;
+  const bool alive = relocate_wave<D>(m, live, false, xi, pi, e, cnt, e_flag, r);
+  if (live) {
     if (e_flag) atomicOr(err, 1);
 #pragma unroll
     for (int k = 0; k < D; ++k) x[i * D + k] = xi[k];

@@ -62,15 +62,6 @@ __device__ __forceinline__ void monomials(const T *r, T *phi) {
   }
 }
 
-// Read-only data seen through the constant address space: loads with a
-// wave-uniform address become scalar loads into SGPRs.
-template <class T>
-using const_ptr = const T __attribute__((address_space(4))) *;
-template <class T>
-__device__ __forceinline__ const_ptr<T> as_constant(const T *p) {
-  return (const_ptr<T>)p;
-}
-
 // Row stride of the modal coefficient tables [element][field][monomial]: the
 // FP32 table pads every row to a multiple of four monomials (16-byte aligned
 // rows for the matrix-core interpolation), the FP64 table is dense.
@@ -351,17 +342,15 @@ __device__ __forceinline__ bool lsrk4_stage(const DevMesh &m, const T *__restric
                                             int &e_flag, unsigned &n_mf) {
   const uint32_t e_uni =
       stage_update<D, N, T, MF>(coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, n_mf);
-  // fast path: still inside the (wave-uniform) element, same arithmetic as
-  // the first branch of find_new_containing_element
+  // still inside the (wave-uniform) element: the first branch of
+  // find_new_containing_element through scalar loads; otherwise the cascade,
+  // with every lane of the wave helping the fan and grid searches
+  bool in = false;
   if (e_uni != INVALID) {
-    if (still_inside<D>(m, e, e_uni, xi, r)) {
-      cnt.same++;
-      return true;
-    }
-    // outside, by the same arithmetic as the cascade's first test
-    return relocate<D, true>(m, xi, pi, e, cnt, e_flag, r);
+    in = still_inside<D>(m, e, e_uni, xi, r);
+    if (in) cnt.same++;
   }
-  return relocate<D>(m, xi, pi, e, cnt, e_flag, r);
+  return relocate_wave<D>(m, !in, e_uni != INVALID, xi, pi, e, cnt, e_flag, r);
 }
 
 // Minimum waves per SIMD the push is compiled for (a register budget of
@@ -599,7 +588,7 @@ __device__ __forceinline__ bool inside_cached(const char *slot_inv, const double
   return in_unit_simplex<D>(r);
 }
 
-// The head of relocate<D, true> (particle_state.hpp:160-190: the face whose
+// The head of relocate_wave with x known outside (particle_state.hpp:160-190: the face whose
 // normal best matches the momentum, the neighbour across it) on the cached
 // data of the lane's slot (slot < 0: from the mesh arrays).  Returns 1 when
 // the neighbour contains x (elem, slot and r updated), 0 on the reference's
@@ -655,92 +644,6 @@ __device__ __forceinline__ int relocate_head(const DevMesh &m, const char *base,
     }
   }
   return 2;
-}
-
-__device__ __forceinline__ double readlane_f64(double v, int lane) {
-  const uint64_t u = __builtin_bit_cast(uint64_t, v);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
-  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
-
-// First element of list[k0, k1) (in list order) that contains x, tested by
-// the wave's active lanes 64 at a time; INVALID if none.  rl receives the
-// unit coordinates in the element found (from the lane that tested it).
-template <int D>
-__device__ __forceinline__ uint32_t wave_first_containing(const DevMesh &m,
-                                                          const uint32_t *list, uint32_t k0,
-                                                          uint32_t k1, const double *x,
-                                                          double *rl) {
-  const uint64_t act = __ballot(1);
-  const int nact = __popcll(act);
-  const int rank = __popcll(act & ((1ull << (threadIdx.x & 63)) - 1));
-  for (uint32_t b = k0; b < k1; b += (uint32_t)nact) {
-    const uint32_t k = b + (uint32_t)rank;
-    uint32_t cand = INVALID;
-    double rr[D];
-    bool hit = false;
-    if (k < k1) {
-      cand = list[k];
-      hit = in_element<D>(m, cand, x, rr);
-    }
-    const uint64_t hb = __ballot(hit);
-    if (hb) {
-      const int j = (int)__builtin_ctzll(hb);   // lowest hit lane = lowest k
-#pragma unroll
-      for (int i = 0; i < D; ++i) rl[i] = readlane_f64(rr[i], j);
-      return (uint32_t)__builtin_amdgcn_readlane((int)cand, j);
-    }
-  }
-  return INVALID;
-}
-
-// find_by_vertex_or_global (particle_state.hpp:198-261, meshdata.hpp:208-215)
-// for the particle of lane L (position x, previous element prev: uniform
-// values), run by the whole wave: the elements around prev's closest vertex,
-// then the global grid cell, each list tested 64 elements at a time and the
-// first containing one (in list order) taken, as the sequential scan would.
-// kind: 1 vertex fan, 2 global search (the counter the lane bumps).
-template <int D>
-__device__ __forceinline__ uint32_t wave_vertex_or_global(const DevMesh &m, const double *x,
-                                                          uint32_t prev, double *rl,
-                                                          int &kind) {
-  uint32_t closest_vertex = INVALID;
-  double min_dist = __builtin_inf();
-#pragma unroll
-  for (int k = 0; k <= D; ++k) {
-    const uint32_t vi = as_constant(m.el_vertices)[(size_t)prev * (D + 1) + k];
-    double s = 0;
-#pragma unroll
-    for (int i = 0; i < D; ++i) {
-      const double dv = as_constant(m.vertices)[(size_t)vi * D + i] - x[i];
-      s += dv * dv;
-    }
-    const double dist = sqrt(s);
-    if (dist < min_dist) {
-      closest_vertex = vi;
-      min_dist = dist;
-    }
-  }
-  closest_vertex = (uint32_t)__builtin_amdgcn_readfirstlane((int)closest_vertex);
-  uint32_t res = wave_first_containing<D>(m, m.vadj_elements,
-                                          as_constant(m.vadj_starts)[closest_vertex],
-                                          as_constant(m.vadj_starts)[closest_vertex + 1], x, rl);
-  if (res != INVALID) {
-    kind = 1;
-    return res;
-  }
-  kind = 2;
-  int cell = 0;
-#pragma unroll
-  for (int i = D - 1; i >= 0; --i) {
-    const double c = (x[i] - m.g_lo[i]) * m.g_inv_h[i];
-    if (!(c >= 0.0) || c >= (double)m.g_n[i]) return INVALID;
-    cell = cell * m.g_n[i] + (int)c;
-  }
-  cell = __builtin_amdgcn_readfirstlane(cell);
-  return wave_first_containing<D>(m, m.g_elems, as_constant(m.g_starts)[cell],
-                                  as_constant(m.g_starts)[cell + 1], x, rl);
 }
 
 template <int D, int N, class T>
@@ -847,28 +750,7 @@ __global__ __launch_bounds__(256, (push_min_waves<N, T>())) void k_lsrk4_step_c(
       else
         head = relocate_head<D, N, T>(m, base, cel, nsl, xi, pi, e, slot, cnt, e_flag, r);
       // the rest of the cascade, one particle at a time with the whole wave
-      // (a sequential fan costs two dependent loads per candidate element)
-      uint64_t pend = __ballot(head == 2);
-      while (pend) {
-        const int L = (int)__builtin_ctzll(pend);
-        pend &= pend - 1;
-        double xl[D], rl[D];
-#pragma unroll
-        for (int k = 0; k < D; ++k) xl[k] = readlane_f64(xi[k], L);
-        const uint32_t prev = (uint32_t)__builtin_amdgcn_readlane((int)e, L);
-        int kind = 0;
-        const uint32_t ne = wave_vertex_or_global<D>(m, xl, prev, rl, kind);
-        if ((int)(threadIdx.x & 63) == L) {
-          if (kind == 1) cnt.vertex++;
-          else cnt.global++;
-          if (ne != INVALID) {
-            e = ne;
-#pragma unroll
-            for (int k = 0; k < D; ++k) r[k] = rl[k];
-            head = 3;
-          }
-        }
-      }
+      wave_cascade_tail<D>(m, xi, e, cnt, r, head);
       if (head == 0) {
         ok = false;
       } else if (head == 2) {
@@ -1252,9 +1134,12 @@ __global__ __launch_bounds__(256) void k_avg_stage(DevMesh m, int64_t n, int s, 
                                                    unsigned int *n_dead) {
   FindCounters cnt{};
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && death[i] == 0) {
+  // every lane takes part in the wave's fan and grid searches
+  const bool live = i < n && death[i] == 0;
+  double xi[D] = {}, pi[D] = {}, r[D];
+  uint32_t e = INVALID;
+  if (live) {
     const double a = c_rk4a[s], b = c_rk4b[s];
-    double xi[D], pi[D], r[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) {
       const double rxk = a * rx[i * D + k] + dt * v[i * D + k];
@@ -1264,9 +1149,11 @@ __global__ __launch_bounds__(256) void k_avg_stage(DevMesh m, int64_t n, int s, 
       xi[k] = x[i * D + k] + b * rxk;
       pi[k] = p[i * D + k] + b * rpk;
     }
-    uint32_t e = elem[i];
-    int e_flag = 0;
-    const bool alive = relocate<D>(m, xi, pi, e, cnt, e_flag, r);
+    e = elem[i];
+  }
+  int e_flag = 0;
+  const bool alive = relocate_wave<D>(m, live, false, xi, pi, e, cnt, e_flag, r);
+  if (live) {
     if (e_flag) atomicOr(err, 1);
 #pragma unroll
     for (int k = 0; k < D; ++k) {

@@ -316,22 +316,184 @@ __device__ __forceinline__ bool boundary_hit(const DevMesh &m, double *x, const 
   return false;
 }
 
-// update_containing_elements + boundary_hit for one particle
-// (particle_state.hpp:266-357).  Returns false if the particle dies.
-// OUTSIDE: the caller has already found x outside elem with the same
-// arithmetic as the cascade's first test, which is then not repeated.
-template <int D, bool OUTSIDE = false>
-__device__ __forceinline__ bool relocate(const DevMesh &m, double *x, const double *p,
-                                         uint32_t &elem, FindCounters &cnt, int &err,
-                                         double *r) {
-  uint32_t ne = OUTSIDE ? find_outside_prev<D>(m, x, p, elem, cnt, err, r)
-                        : find_new_containing_element<D>(m, x, p, elem, cnt, err, r);
-  if (err) return false;
-  if (ne != INVALID) {
-    elem = ne;
-    return true;
+// Read-only data seen through the constant address space: loads with a
+// wave-uniform address become scalar loads into SGPRs.
+template <class T>
+using const_ptr = const T __attribute__((address_space(4))) *;
+template <class T>
+__device__ __forceinline__ const_ptr<T> as_constant(const T *p) {
+  return (const_ptr<T>)p;
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// First element of list[k0, k1) (in list order) that contains x, tested by
+// the wave's active lanes 64 at a time; INVALID if none.  rl receives the
+// unit coordinates in the element found (from the lane that tested it).
+template <int D>
+__device__ __forceinline__ uint32_t wave_first_containing(const DevMesh &m,
+                                                          const uint32_t *list, uint32_t k0,
+                                                          uint32_t k1, const double *x,
+                                                          double *rl) {
+  const uint64_t act = __ballot(1);
+  const int nact = __popcll(act);
+  const int rank = __popcll(act & ((1ull << (threadIdx.x & 63)) - 1));
+  for (uint32_t b = k0; b < k1; b += (uint32_t)nact) {
+    const uint32_t k = b + (uint32_t)rank;
+    uint32_t cand = INVALID;
+    double rr[D];
+    bool hit = false;
+    if (k < k1) {
+      cand = list[k];
+      hit = in_element<D>(m, cand, x, rr);
+    }
+    const uint64_t hb = __ballot(hit);
+    if (hb) {
+      const int j = (int)__builtin_ctzll(hb);   // lowest hit lane = lowest k
+#pragma unroll
+      for (int i = 0; i < D; ++i) rl[i] = readlane_f64(rr[i], j);
+      return (uint32_t)__builtin_amdgcn_readlane((int)cand, j);
+    }
   }
-  return boundary_hit<D>(m, x, p, elem, cnt, err, r);
+  return INVALID;
+}
+
+// find_by_vertex_or_global (particle_state.hpp:198-261, meshdata.hpp:208-215)
+// for the particle of lane L (position x, previous element prev: uniform
+// values), run by the whole wave: the elements around prev's closest vertex
+// (when prev is an element),
+// then the global grid cell, each list tested 64 elements at a time and the
+// first containing one (in list order) taken, as the sequential scan would.
+// kind: 1 vertex fan, 2 global search (the counter the lane bumps).
+template <int D>
+__device__ __forceinline__ uint32_t wave_vertex_or_global(const DevMesh &m, const double *x,
+                                                          uint32_t prev, double *rl,
+                                                          int &kind) {
+  if (prev != INVALID && prev < (uint32_t)m.K) {
+  uint32_t closest_vertex = INVALID;
+  double min_dist = __builtin_inf();
+#pragma unroll
+  for (int k = 0; k <= D; ++k) {
+    const uint32_t vi = as_constant(m.el_vertices)[(size_t)prev * (D + 1) + k];
+    double s = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const double dv = as_constant(m.vertices)[(size_t)vi * D + i] - x[i];
+      s += dv * dv;
+    }
+    const double dist = sqrt(s);
+    if (dist < min_dist) {
+      closest_vertex = vi;
+      min_dist = dist;
+    }
+  }
+  closest_vertex = (uint32_t)__builtin_amdgcn_readfirstlane((int)closest_vertex);
+  uint32_t res = wave_first_containing<D>(m, m.vadj_elements,
+                                          as_constant(m.vadj_starts)[closest_vertex],
+                                          as_constant(m.vadj_starts)[closest_vertex + 1], x, rl);
+  if (res != INVALID) {
+    kind = 1;
+    return res;
+  }
+  }
+  kind = 2;
+  int cell = 0;
+#pragma unroll
+  for (int i = D - 1; i >= 0; --i) {
+    const double c = (x[i] - m.g_lo[i]) * m.g_inv_h[i];
+    if (!(c >= 0.0) || c >= (double)m.g_n[i]) return INVALID;
+    cell = cell * m.g_n[i] + (int)c;
+  }
+  cell = __builtin_amdgcn_readfirstlane(cell);
+  return wave_first_containing<D>(m, m.g_elems, as_constant(m.g_starts)[cell],
+                                  as_constant(m.g_starts)[cell + 1], x, rl);
+}
+
+// The tail of the cascade for the active lanes with head == 2, one particle
+// at a time with the whole wave (a lane's own sequential fan costs its wave two
+// dependent loads per candidate element).  A lane whose particle some element
+// contains gets head = 3 with elem and r set; head stays 2 when none does.
+template <int D>
+__device__ __forceinline__ void wave_cascade_tail(const DevMesh &m, const double *x,
+                                                  uint32_t &elem, FindCounters &cnt, double *r,
+                                                  int &head) {
+  uint64_t pend = __ballot(head == 2);
+  while (pend) {
+    const int L = (int)__builtin_ctzll(pend);
+    pend &= pend - 1;
+    double xl[D], rl[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) xl[k] = readlane_f64(x[k], L);
+    const uint32_t prev = (uint32_t)__builtin_amdgcn_readlane((int)elem, L);
+    int kind = 0;
+    const uint32_t ne = wave_vertex_or_global<D>(m, xl, prev, rl, kind);
+    if ((int)(threadIdx.x & 63) == L) {
+      if (kind == 1) cnt.vertex++;
+      else cnt.global++;
+      if (ne != INVALID) {
+        elem = ne;
+#pragma unroll
+        for (int k = 0; k < D; ++k) r[k] = rl[k];
+        head = 3;
+      }
+    }
+  }
+}
+
+// update_containing_elements + boundary_hit (particle_state.hpp:266-357) for
+// the active lanes with need: the cascade of find_new_containing_element from
+// the element held (skipping its first test when outside: the caller found x
+// outside that element with the same arithmetic), then the periodic trip or
+// death.  Returns false if the particle dies; the other lanes keep their
+// element and return true.  The fan and grid searches are shared by the wave,
+// so call it with every lane that can help active, not from a branch on need.
+template <int D>
+__device__ __forceinline__ bool relocate_wave(const DevMesh &m, bool need, bool outside,
+                                              double *x, const double *p, uint32_t &elem,
+                                              FindCounters &cnt, int &err, double *r) {
+  int head = 1;
+  if (need) {
+    head = 2;
+    const bool valid = elem != INVALID && elem < (uint32_t)m.K;
+    if (valid && !outside && in_element<D>(m, elem, x, r)) {
+      cnt.same++;
+      head = 1;
+    } else if (valid) {
+      int closest = -1;
+      double max_ip = 0;
+#pragma unroll
+      for (int f = 0; f <= D; ++f) {
+        const double *n = m.face_normal + ((size_t)elem * (D + 1) + f) * D;
+        double ip = 0;
+#pragma unroll
+        for (int i = 0; i < D; ++i) ip += n[i] * p[i];
+        if (ip > max_ip) {
+          closest = f;
+          max_ip = ip;
+        }
+      }
+      if (closest == -1) {
+        err = 1;
+        head = 0;
+      } else {
+        const uint32_t possible = m.face_neighbor[(size_t)elem * (D + 1) + closest];
+        if (possible != INVALID && in_element<D>(m, possible, x, r)) {
+          cnt.neighbor++;
+          elem = possible;
+          head = 1;
+        }
+      }
+    }
+  }
+  wave_cascade_tail<D>(m, x, elem, cnt, r, head);
+  if (head == 0) return false;
+  if (head == 2) return boundary_hit<D>(m, x, p, elem, cnt, err, r);
+  return true;
 }
 
 // particle_state.hpp:98-125 get_velocities, same operation order.
