@@ -161,6 +161,7 @@ def ptr(t) -> int | None:
 # holding the same bits
 _KINDS = {"f64": ("float64",), "u32": ("int32", "uint32"), "i64": ("int64",),
           "u8": ("uint8", "bool")}
+_DTYPE_NAMES = {}   # torch dtype -> its name
 
 
 def arg(t, kind: str, rows: int | None = None, cols: int | None = None,
@@ -175,7 +176,9 @@ def arg(t, kind: str, rows: int | None = None, cols: int | None = None,
     if t is None:
         return None
     want = _KINDS[kind]
-    got = str(t.dtype).replace("torch.", "")
+    got = _DTYPE_NAMES.get(t.dtype)
+    if got is None:
+        got = _DTYPE_NAMES[t.dtype] = str(t.dtype).replace("torch.", "")
     if got not in want:
         raise ValueError("%s: expected dtype %s, got %s" % (name, " or ".join(want), got))
     if cols is not None:
@@ -190,8 +193,15 @@ def arg(t, kind: str, rows: int | None = None, cols: int | None = None,
     return ptr(t)
 
 
+_stream_set = None
+
+
 def sync_stream():
-    """Run library work on torch's current stream."""
+    """Run library work on torch's current stream (set once per stream change)."""
+    global _stream_set
     import torch
-    s = torch.cuda.current_stream()
-    check(lib().pyr_set_stream(C.c_void_p(s.cuda_stream)))
+    h = torch.cuda.current_stream().cuda_stream
+    L = lib()
+    if h != _stream_set:
+        check(L.pyr_set_stream(C.c_void_p(h)))
+        _stream_set = h

@@ -453,7 +453,25 @@ struct DepositCache {
   const uint32_t *last_perm = nullptr;
   int64_t last_perm_n = -1;
   std::vector<double> h_geom;   // K x 8
+  // counts read back during a deposition (pinned), and the event after each
+  // copy: the host waits on the event while later independent work runs
+  uint32_t *h_cnt = nullptr;
+  hipEvent_t ev_cnt[2] = {nullptr, nullptr};
 };
+
+// Reads a device count through pinned memory: the copy and an event go on the
+// stream now; count_wait returns the value once the copy has landed, without
+// waiting for work queued after it.
+static void count_post(DepositCache *c, int k, const uint32_t *dev_count, hipStream_t s) {
+  if (!c->h_cnt) PYR_HIP(hipHostMalloc((void **)&c->h_cnt, 4 * sizeof(uint32_t)));
+  if (!c->ev_cnt[k]) PYR_HIP(hipEventCreateWithFlags(&c->ev_cnt[k], hipEventDisableTiming));
+  PYR_HIP(hipMemcpyAsync(c->h_cnt + k, dev_count, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  PYR_HIP(hipEventRecord(c->ev_cnt[k], s));
+}
+static uint32_t count_wait(DepositCache *c, int k) {
+  PYR_HIP(hipEventSynchronize(c->ev_cnt[k]));
+  return c->h_cnt[k];
+}
 
 void pyr_deposit_cache_free(pyr_mesh *mesh) {
   if (!mesh || !mesh->dep) return;
@@ -467,6 +485,9 @@ void pyr_deposit_cache_free(pyr_mesh *mesh) {
                     &c->ent, &c->ent_off, &c->n_live, &c->items, &c->partial, &c->ikey,
                     &c->ik_a, &c->ik_b, &c->iv_a, &c->iv_b, &c->active, &c->list_stats})
     b->release();
+  if (c->h_cnt) (void)hipHostFree(c->h_cnt);
+  for (hipEvent_t ev : c->ev_cnt)
+    if (ev) (void)hipEventDestroy(ev);
   delete c;
   mesh->dep = nullptr;
 }
@@ -1025,21 +1046,6 @@ static unsigned build_chunks(pyr_mesh *mesh, DepositCache *c, int64_t n, ChunkLi
   return gather_grid((int)bound);
 }
 
-// The chunk list of the current sort (c->seg) for the cluster gather, with
-// its per-element chunk ranges (K + 1 starts); returns the number of chunks
-// (read back to the host: the staging is sized by it).
-static int64_t chunk_table(pyr_mesh *mesh, DepositCache *c, int64_t n, ChunkList &cl,
-                           const uint32_t **ch_start) {
-  build_chunks(mesh, c, n, cl);
-  // build_chunks lays out chunks[bound], counts[K], offsets[K], total: the
-  // offsets followed by the total are the K + 1 starts
-  *ch_start = cl.n_chunks - mesh->K;
-  uint32_t total = 0;
-  PYR_HIP(hipMemcpyAsync(&total, cl.n_chunks, sizeof(uint32_t), hipMemcpyDeviceToHost, stream()));
-  PYR_HIP(hipStreamSynchronize(stream()));
-  return (int64_t)total;
-}
-
 // q / I per particle for the normalized deposition; located particles whose
 // shape integral vanishes deposit nothing and are counted (the reference warns
 // for each, dep_normshape.hpp:175-183).  scale (optional) receives q / I.
@@ -1121,10 +1127,41 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
       // the list gather (FP32C: matrix cores; FP64: the FP64 staging)
       ChunkList cl{};
       const uint32_t *ch_start = nullptr;
+      {
+        ProfScope prof(PROF_SORT);
+        build_chunks(mesh, c, n, cl);
+        // build_chunks lays out chunks[bound], counts[K], offsets[K], total:
+        // the offsets followed by the total are the K + 1 starts
+        ch_start = cl.n_chunks - K;
+        count_post(c, 0, cl.n_chunks, s);
+      }
+      // one scratch: capacities (later item counts) K, list offsets K + 1, scan
+      // partials, item starts K + 1
+      c->cap.reserve(sizeof(uint32_t) * ((size_t)3 * K + 3 + scan_tmp_words(K)));
+      uint32_t *cap = c->cap.as<uint32_t>(), *cap_off = cap + K + 1;
+      uint32_t *scan_tmp = cap_off + K + 1;
+      uint32_t *istart = scan_tmp + scan_tmp_words(K);
+      {
+        // the list capacities need no chunk count: queued before the host
+        // waits for it
+        ProfScope prof(PROF_SORT);
+        if (c->tr_radius != radius) build_transpose(mesh, c);
+        c->active.reserve((size_t)std::max(1, K));
+        PYR_HIP(hipMemsetAsync(c->active.ptr, 0, (size_t)K, s));
+        hipLaunchKernelGGL(k_mark_active, dim3((unsigned)((K + 3) / 4)), dim3(256), 0, s, K,
+                           ch_start, c->tr_start.as<uint32_t>(), c->tr.as<uint32_t>(),
+                           c->active.as<uint8_t>());
+        PYR_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_cand_chunks, dim3((unsigned)((K + 3) / 4)), dim3(256), 0, s, K,
+                           c->cand_start.as<uint32_t>(), c->cand.as<uint32_t>(), ch_start,
+                           c->active.as<uint8_t>(), cap);
+        PYR_CHECK_LAUNCH();
+        scan_exclusive_u32(cap, cap_off, K, cap_off + K, scan_tmp, s);
+      }
       int64_t nch = 0;
       {
         ProfScope prof(PROF_SORT);
-        nch = chunk_table(mesh, c, n, cl, &ch_start);
+        nch = count_wait(c, 0);
         // chunk nch: all zero, read by the list gather past the end of a list
         c->pos32c.reserve(sizeof(float4) * 64 * (size_t)(nch + 1));
         c->wgt32c.reserve(sizeof(float4) * 64 * (size_t)(nch + 1));
@@ -1146,29 +1183,11 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
       // candidate chunks (at most nch x max_indeg in total), then the cull
       const int64_t bound = std::max<int64_t>(1, nch * std::max<int64_t>(1, c->max_indeg));
       PYR_REQUIRE(bound < ((int64_t)1 << 32), "deposition chunk lists exceed 2^32 entries");
-      // one scratch: capacities (later item counts) K, list offsets K + 1, scan
-      // partials, item starts K + 1
-      c->cap.reserve(sizeof(uint32_t) * ((size_t)3 * K + 3 + scan_tmp_words(K)));
-      uint32_t *cap = c->cap.as<uint32_t>(), *cap_off = cap + K + 1;
-      uint32_t *scan_tmp = cap_off + K + 1;
-      uint32_t *istart = scan_tmp + scan_tmp_words(K);
       c->ent.reserve(sizeof(uint2) * (size_t)bound);
       c->ent_off.reserve(sizeof(float4) * (size_t)bound);
       c->n_live.reserve(sizeof(uint32_t) * (size_t)std::max(1, K));
       {
         ProfScope prof(PROF_SORT);
-        if (c->tr_radius != radius) build_transpose(mesh, c);
-        c->active.reserve((size_t)std::max(1, K));
-        PYR_HIP(hipMemsetAsync(c->active.ptr, 0, (size_t)K, s));
-        hipLaunchKernelGGL(k_mark_active, dim3((unsigned)((K + 3) / 4)), dim3(256), 0, s, K,
-                           ch_start, c->tr_start.as<uint32_t>(), c->tr.as<uint32_t>(),
-                           c->active.as<uint8_t>());
-        PYR_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_cand_chunks, dim3((unsigned)((K + 3) / 4)), dim3(256), 0, s, K,
-                           c->cand_start.as<uint32_t>(), c->cand.as<uint32_t>(), ch_start,
-                           c->active.as<uint8_t>(), cap);
-        PYR_CHECK_LAUNCH();
-        scan_exclusive_u32(cap, cap_off, K, cap_off + K, scan_tmp, s);
         CullArgs cu{};
         cu.K = K;
         cu.R = radius;
@@ -1199,13 +1218,16 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
                            c->t_order.as<uint32_t>(), c->n_live.as<uint32_t>(), icnt);
         PYR_CHECK_LAUNCH();
         scan_exclusive_u32(icnt, istart, K, istart + K, scan_tmp, s);
-        PYR_HIP(hipMemcpyAsync(&n_items, istart + K, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        PYR_HIP(hipStreamSynchronize(s));
-        c->items.reserve(sizeof(uint2) * (size_t)std::max<uint32_t>(1, n_items));
-        c->partial.reserve(sizeof(double) * 4 * ns * (size_t)std::max<uint32_t>(1, n_items));
+        count_post(c, 1, istart + K, s);
+        // the items fit in (list entries / 64 + targets): filled before the
+        // host waits for their count
+        const int64_t item_bound = (bound + 63) / 64 + K;
+        c->items.reserve(sizeof(uint2) * (size_t)item_bound);
         hipLaunchKernelGGL(k_item_fill, dim3(blocks_for(K)), dim3(256), 0, s, K,
                            c->t_order.as<uint32_t>(), istart, c->items.as<uint2>());
         PYR_CHECK_LAUNCH();
+        n_items = count_wait(c, 1);
+        c->partial.reserve(sizeof(double) * 4 * ns * (size_t)std::max<uint32_t>(1, n_items));
         if (n_items) {
           c->ikey.reserve(sizeof(uint32_t) * (size_t)n_items);
           hipLaunchKernelGGL(k_item_keys, dim3(blocks_for(n_items)), dim3(256), 0, s, n_items,
