@@ -100,53 +100,6 @@ __device__ __forceinline__ void interp6(P coef, uint32_t en, const T *r, T *f6) 
 // the same bits.
 typedef float pyr_push_v4f __attribute__((ext_vector_type(4)));
 
-template <int D, int N>
-__device__ __forceinline__ void interp6_mfma(const float *__restrict__ coef, uint32_t e,
-                                             const float *r, float *f6) {
-  constexpr int NP = Basis<D, N>::NP, NS = coef_stride<float>(NP);
-  float phi[NS];
-  monomials<D, N, float>(r, phi);
-#pragma unroll
-  for (int k = NP; k < NS; ++k) phi[k] = 0.f;
-  const int row = threadIdx.x & 3;
-  const bool rb = row < 2;
-  bool done = false;
-  for (;;) {
-    // element served in this pass: the smallest pending element of the block
-    uint32_t ce = done ? INVALID : e;
-    ce = min(ce, (uint32_t)__shfl_xor((int)ce, 1, 64));
-    ce = min(ce, (uint32_t)__shfl_xor((int)ce, 2, 64));
-    if (__ballot(ce != INVALID) == 0) break;
-    const uint32_t el = ce == INVALID ? 0u : ce;
-    const float4 *ca = reinterpret_cast<const float4 *>(coef + ((size_t)el * 6 + row) * NS);
-    const float4 *cb =
-        reinterpret_cast<const float4 *>(coef + ((size_t)el * 6 + 4 + (row & 1)) * NS);
-    pyr_push_v4f a = {0.f, 0.f, 0.f, 0.f}, b = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int k4 = 0; k4 < NS / 4; ++k4) {
-      const float4 x = ca[k4];
-      const float4 y = rb ? cb[k4] : make_float4(0.f, 0.f, 0.f, 0.f);
-      a = __builtin_amdgcn_mfma_f32_4x4x1f32(x.x, phi[4 * k4 + 0], a, 0, 0, 0);
-      b = __builtin_amdgcn_mfma_f32_4x4x1f32(y.x, phi[4 * k4 + 0], b, 0, 0, 0);
-      a = __builtin_amdgcn_mfma_f32_4x4x1f32(x.y, phi[4 * k4 + 1], a, 0, 0, 0);
-      b = __builtin_amdgcn_mfma_f32_4x4x1f32(y.y, phi[4 * k4 + 1], b, 0, 0, 0);
-      a = __builtin_amdgcn_mfma_f32_4x4x1f32(x.z, phi[4 * k4 + 2], a, 0, 0, 0);
-      b = __builtin_amdgcn_mfma_f32_4x4x1f32(y.z, phi[4 * k4 + 2], b, 0, 0, 0);
-      a = __builtin_amdgcn_mfma_f32_4x4x1f32(x.w, phi[4 * k4 + 3], a, 0, 0, 0);
-      b = __builtin_amdgcn_mfma_f32_4x4x1f32(y.w, phi[4 * k4 + 3], b, 0, 0, 0);
-    }
-    if (!done && ce == e) {
-      f6[0] = a[0];
-      f6[1] = a[1];
-      f6[2] = a[2];
-      f6[3] = a[3];
-      f6[4] = b[0];
-      f6[5] = b[1];
-      done = true;
-    }
-  }
-}
-
 // The element of every active lane, if the wave sits in one element (the
 // common case for element-sorted particles), else INVALID.  A uniform element
 // lets per-element data come through scalar loads instead of per-lane ones.
@@ -264,13 +217,11 @@ __constant__ double c_rk4b[5] = {
 // The update half of a stage for one particle: k = [v(p), F(x, p)] at unit
 // coordinates r in element e, res = a_s res + dt k, y += b_s res.  Returns
 // the wave-uniform element (INVALID if the wave's particles are in several).
-// MF (FP32 only): a full wave interpolates on the matrix cores, one pass per
-// element of its most mixed block of four particles.
-template <int D, int N, class T, bool MF = false>
+template <int D, int N, class T>
 __device__ __forceinline__ uint32_t stage_update(const T *__restrict__ coef, int s, double dt,
                                                  double c, double qi, double mi, double *xi,
                                                  double *pi, double *rx, double *rp,
-                                                 const double *r, uint32_t e, unsigned &n_mf) {
+                                                 const double *r, uint32_t e) {
   double v[D], F[D];
   velocity<D>(pi, mi, c, v);
   const uint32_t e_uni = wave_uniform_element(e);
@@ -285,11 +236,7 @@ __device__ __forceinline__ uint32_t stage_update(const T *__restrict__ coef, int
       rf[k] = (float)r[k];
       vf[k] = (float)v[k];
     }
-    if (MF && __ballot(1) == ~0ull) {
-      interp6_mfma<D, N>(coef, e, rf, f6);
-      ++n_mf;
-    } else
-      interp6_any<D, N, float>(coef, e, e_uni, rf, f6);
+    interp6_any<D, N, float>(coef, e, e_uni, rf, f6);
     lorentz<D, float>((float)qi, f6, vf, Ff, el, mag);
 #pragma unroll
     for (int k = 0; k < D; ++k) F[k] = (double)Ff[k];
@@ -334,14 +281,13 @@ __device__ __forceinline__ bool still_inside(const DevMesh &m, uint32_t e, uint3
 // variant (the fast kernel, the per-step kernel, the checkpointed one and the
 // adjoint's recompute) runs stage_update and the same first containment test,
 // so they agree bitwise.
-template <int D, int N, class T, bool MF = false>
+template <int D, int N, class T>
 __device__ __forceinline__ bool lsrk4_stage(const DevMesh &m, const T *__restrict__ coef, int s,
                                             double dt, double c, double qi, double mi,
                                             double *xi, double *pi, double *rx, double *rp,
                                             double *r, uint32_t &e, FindCounters &cnt,
-                                            int &e_flag, unsigned &n_mf) {
-  const uint32_t e_uni =
-      stage_update<D, N, T, MF>(coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, n_mf);
+                                            int &e_flag) {
+  const uint32_t e_uni = stage_update<D, N, T>(coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e);
   // still inside the (wave-uniform) element: the first branch of
   // find_new_containing_element through scalar loads; otherwise the cascade,
   // with every lane of the wave helping the fan and grid searches
@@ -363,60 +309,6 @@ constexpr int push_min_waves() {
   return (sizeof(T) == 8 && N >= 4) ? 3 : 1;
 }
 
-template <int D, int N, class T, bool MF>
-__global__ __launch_bounds__(256, (push_min_waves<N, T>())) void k_lsrk4_step(
-    DevMesh m, const T *__restrict__ coef, int64_t n, double dt, double c,
-    double *__restrict__ x, double *__restrict__ p, const double *__restrict__ q,
-    const double *__restrict__ mass, uint32_t *__restrict__ elem,
-    uint32_t *__restrict__ death, unsigned long long *counters, int *err,
-    unsigned int *n_dead, unsigned long long *mf_waves, const uint32_t *__restrict__ order,
-    const uint32_t *n_dev) {
-  // order (particle indices, or null): thread t steps particle order[t], for
-  // t < n (n_dev: a device count at most n)
-  const int64_t t =
-      (order ? xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x) * blockDim.x + threadIdx.x;
-  if (n_dev) n = min(n, (int64_t)*n_dev);
-  FindCounters cnt;
-  unsigned n_mf = 0;   // stages this particle was interpolated on the matrix cores
-  if (t < n) {
-    const int64_t i = order ? (int64_t)order[t] : t;
-    double xi[D], pi[D], r[D], rx[D], rp[D];
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      xi[k] = x[i * D + k];
-      pi[k] = p[i * D + k];
-      rx[k] = 0;
-      rp[k] = 0;
-    }
-    const double qi = q[i], mi = mass[i];
-    uint32_t e = elem[i];
-    uint32_t died = 0;
-    int e_flag = 0;
-    if (e >= (uint32_t)m.K) died = 1;   // not located: removed at the first stage
-    else inverse_map<D>(m, e, xi, r);
-#pragma unroll 1
-    for (int s = 0; s < 5 && !died; ++s)
-      if (!lsrk4_stage<D, N, T, MF>(m, coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, cnt, e_flag, n_mf))
-        died = (uint32_t)s + 1;
-    if (e_flag) atomicOr(err, 1);
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      x[i * D + k] = xi[k];
-      p[i * D + k] = pi[k];
-    }
-    elem[i] = e;
-    death[i] = died;
-    if (died) atomicAdd(n_dead, 1u);
-  }
-  // particle-stages interpolated on the matrix cores, summed over the wave
-  if (MF && mf_waves) {
-    unsigned long long v = n_mf;
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(mf_waves, v);
-  }
-  flush_counters(cnt, counters);
-}
-
 // ---------------------------------------------------------------------------
 // fused LSRK4 step with a per-wave LDS element cache
 // ---------------------------------------------------------------------------
@@ -435,7 +327,7 @@ __global__ __launch_bounds__(256, (push_min_waves<N, T>())) void k_lsrk4_step(
 // uncached kernel.  A lane in an element without a slot (two faces crossed in
 // one step, the vertex fan, the global search, a periodic trip) sends its wave
 // through the uncached stage until the lane is back in a cached element.  The
-// arithmetic is that of k_lsrk4_step on the same values: the same bits.
+// arithmetic is that of lsrk4_stage on the same values: the same bits.
 template <int D, int N, class T>
 struct StepCache {
   static constexpr int NP = Basis<D, N>::NP, NS = coef_stride<T>(NP);
@@ -450,6 +342,58 @@ struct StepCache {
   static_assert(INV % 16 == 0 && NRM % 16 == 0 && NBR % 16 == 0, "slot layout");
   static_assert(SLOTS >= D + 2, "a slot set must hold one element and its neighbours");
 };
+
+// The six values on the matrix cores from the cached coefficients, for a
+// full wave: the rank-1 form above, block b's rows taken from the slot of the
+// pass (the smallest pending slot of its four lanes), every lane keeping the
+// pass of its own slot.
+template <int D, int N>
+__device__ __forceinline__ void interp6_mfma_slots(const char *base, int slot, const float *r,
+                                                   float *f6) {
+  using C = StepCache<D, N, float>;
+  constexpr int NP = Basis<D, N>::NP, NS = coef_stride<float>(NP);
+  float phi[NS];
+  monomials<D, N, float>(r, phi);
+#pragma unroll
+  for (int k = NP; k < NS; ++k) phi[k] = 0.f;
+  const int row = threadIdx.x & 3;
+  const bool rb = row < 2;
+  constexpr int NONE = 1 << 30;
+  bool done = false;
+  for (;;) {
+    int cs = done ? NONE : slot;
+    cs = min(cs, __shfl_xor(cs, 1, 64));
+    cs = min(cs, __shfl_xor(cs, 2, 64));
+    if (__ballot(cs != NONE) == 0) break;
+    const int sl = cs == NONE ? 0 : cs;
+    const float4 *ca = reinterpret_cast<const float4 *>(base + sl * C::BYTES) + row * (NS / 4);
+    const float4 *cb =
+        reinterpret_cast<const float4 *>(base + sl * C::BYTES) + (4 + (row & 1)) * (NS / 4);
+    pyr_push_v4f a = {0.f, 0.f, 0.f, 0.f}, b = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k4 = 0; k4 < NS / 4; ++k4) {
+      const float4 x = ca[k4];
+      const float4 y = rb ? cb[k4] : make_float4(0.f, 0.f, 0.f, 0.f);
+      a = __builtin_amdgcn_mfma_f32_4x4x1f32(x.x, phi[4 * k4 + 0], a, 0, 0, 0);
+      b = __builtin_amdgcn_mfma_f32_4x4x1f32(y.x, phi[4 * k4 + 0], b, 0, 0, 0);
+      a = __builtin_amdgcn_mfma_f32_4x4x1f32(x.y, phi[4 * k4 + 1], a, 0, 0, 0);
+      b = __builtin_amdgcn_mfma_f32_4x4x1f32(y.y, phi[4 * k4 + 1], b, 0, 0, 0);
+      a = __builtin_amdgcn_mfma_f32_4x4x1f32(x.z, phi[4 * k4 + 2], a, 0, 0, 0);
+      b = __builtin_amdgcn_mfma_f32_4x4x1f32(y.z, phi[4 * k4 + 2], b, 0, 0, 0);
+      a = __builtin_amdgcn_mfma_f32_4x4x1f32(x.w, phi[4 * k4 + 3], a, 0, 0, 0);
+      b = __builtin_amdgcn_mfma_f32_4x4x1f32(y.w, phi[4 * k4 + 3], b, 0, 0, 0);
+    }
+    if (!done && cs == slot) {
+      f6[0] = a[0];
+      f6[1] = a[1];
+      f6[2] = a[2];
+      f6[3] = a[3];
+      f6[4] = b[0];
+      f6[5] = b[1];
+      done = true;
+    }
+  }
+}
 
 template <int D, int N, class T>
 __device__ __forceinline__ int cache_lookup(const uint32_t *cel, int nsl, uint32_t e) {
@@ -646,13 +590,13 @@ __device__ __forceinline__ int relocate_head(const DevMesh &m, const char *base,
   return 2;
 }
 
-template <int D, int N, class T>
+template <int D, int N, class T, bool MF>
 __global__ __launch_bounds__(256, (push_min_waves<N, T>())) void k_lsrk4_step_c(
     DevMesh m, const T *__restrict__ coef, int64_t n, double dt, double c,
     double *__restrict__ x, double *__restrict__ p, const double *__restrict__ q,
     const double *__restrict__ mass, uint32_t *__restrict__ elem,
     uint32_t *__restrict__ death, unsigned long long *counters, int *err,
-    unsigned int *n_dead, const uint32_t *__restrict__ order) {
+    unsigned int *n_dead, const uint32_t *__restrict__ order, unsigned long long *mf_waves) {
   using C = StepCache<D, N, T>;
   __shared__ uint4 lds[4][C::SLOTS * C::BYTES / 16];
   __shared__ uint32_t lds_el[4][C::SLOTS];
@@ -663,6 +607,7 @@ __global__ __launch_bounds__(256, (push_min_waves<N, T>())) void k_lsrk4_step_c(
   const int64_t t =
       (order ? xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x) * blockDim.x + threadIdx.x;
   FindCounters cnt;
+  unsigned n_mf = 0;   // stages this particle was interpolated on the matrix cores
   const bool live = t < n;
   const int64_t i = live ? (order ? (int64_t)order[t] : t) : 0;
   uint32_t e = live ? elem[i] : INVALID;
@@ -712,7 +657,11 @@ __global__ __launch_bounds__(256, (push_min_waves<N, T>())) void k_lsrk4_step_c(
           rf[k] = (float)r[k];
           vf[k] = (float)v[k];
         }
-        if (e_uni != INVALID)
+        if (MF && cached && __ballot(1) == ~0ull) {
+          // MF: full waves with every element cached contract on the matrix cores
+          interp6_mfma_slots<D, N>(base, slot, rf, f6);
+          ++n_mf;
+        } else if (e_uni != INVALID)
           interp6<D, N, float>(as_constant(coef),
                                (uint32_t)__builtin_amdgcn_readfirstlane((int)e_uni), rf, f6);
         else if (cached)
@@ -783,6 +732,11 @@ __global__ __launch_bounds__(256, (push_min_waves<N, T>())) void k_lsrk4_step_c(
     death[i] = died;
     if (died) atomicAdd(n_dead, 1u);
   }
+  if (MF && mf_waves) {
+    unsigned long long v = n_mf;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(mf_waves, v);
+  }
   flush_counters(cnt, counters);
 }
 
@@ -803,7 +757,6 @@ __global__ __launch_bounds__(256) void k_lsrk4_ckpt(
     int *err) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   FindCounters cnt;
-  unsigned n_mf = 0;
   if (i < n) {
     double xi[D], pi[D], r[D], rx[D], rp[D];
 #pragma unroll
@@ -837,7 +790,7 @@ __global__ __launch_bounds__(256) void k_lsrk4_ckpt(
 #pragma unroll 1
       for (int s = 0; s < 5; ++s)
         if (!lsrk4_stage<D, N, T>(m, coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, cnt,
-                                  e_flag, n_mf)) {
+                                  e_flag)) {
           died = (uint32_t)(5 * step + s + 1);
           break;
         }
@@ -990,7 +943,6 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) void k_lsrk4_adjoint
   }
   const double qi = q[i], mi = mass[i];
   FindCounters cnt;
-  unsigned n_mf = 0;
   int e_flag = 0, lost = 0;
   const size_t nn = (size_t)n, ii = (size_t)i;
   const int nseg = (nsteps + ckpt_every - 1) / ckpt_every;
@@ -1023,7 +975,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) void k_lsrk4_adjoint
       }
 #pragma unroll 1
       for (int s = 0; s < 5; ++s)
-        if (!lsrk4_stage<D, N, T>(m, coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, cnt, e_flag, n_mf))
+        if (!lsrk4_stage<D, N, T>(m, coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, cnt, e_flag))
           lost = 1;
     }
 #pragma unroll 1
@@ -1048,7 +1000,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) void k_lsrk4_adjoint
         }
         sebuf[(size_t)s * nn + ii] = e;
         if (s == 4) break;   // the last stage's output is not needed
-        if (!lsrk4_stage<D, N, T>(m, coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, cnt, e_flag, n_mf))
+        if (!lsrk4_stage<D, N, T>(m, coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, cnt, e_flag))
           lost = 1;
       }
       double rbx[D], rbp[D];
@@ -1343,18 +1295,20 @@ int pyr_lsrk4_steps(pyr_mesh *mesh, int precision, int nsteps, double dt, double
         dispatch_order(D, N, [&](auto DD, auto NN) {
           constexpr int D_ = decltype(DD)::value, N_ = decltype(NN)::value;
           if (precision == PYR_FP64)
-            hipLaunchKernelGGL((k_lsrk4_step_c<D_, N_, double>), dim3(blocks_for(n)),
+            hipLaunchKernelGGL((k_lsrk4_step_c<D_, N_, double, false>), dim3(blocks_for(n)),
                                dim3(256), 0, stream(), mesh->dev, mesh->sf64.as<double>(), n, dt,
-                               vacuum_c, x, p, q, mass, elem, death, counters, err, n_dead, order);
+                               vacuum_c, x, p, q, mass, elem, death, counters, err, n_dead, order,
+                               nullptr);
           else if (mesh->push_mfma)
-            hipLaunchKernelGGL((k_lsrk4_step<D_, N_, float, true>), dim3(blocks_for(n)),
+            hipLaunchKernelGGL((k_lsrk4_step_c<D_, N_, float, true>), dim3(blocks_for(n)),
                                dim3(256), 0, stream(), mesh->dev, mesh->sf32.as<float>(), n, dt,
-                               vacuum_c, x, p, q, mass, elem, death, counters, err, n_dead, mf_waves,
-                               order, (const uint32_t *)nullptr);
+                               vacuum_c, x, p, q, mass, elem, death, counters, err, n_dead, order,
+                               mf_waves);
           else
-            hipLaunchKernelGGL((k_lsrk4_step_c<D_, N_, float>), dim3(blocks_for(n)),
+            hipLaunchKernelGGL((k_lsrk4_step_c<D_, N_, float, false>), dim3(blocks_for(n)),
                                dim3(256), 0, stream(), mesh->dev, mesh->sf32.as<float>(), n, dt,
-                               vacuum_c, x, p, q, mass, elem, death, counters, err, n_dead, order);
+                               vacuum_c, x, p, q, mass, elem, death, counters, err, n_dead, order,
+                               nullptr);
           PYR_CHECK_LAUNCH();
         });
       }
