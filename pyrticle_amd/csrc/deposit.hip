@@ -887,6 +887,11 @@ static const uint32_t *sort_by_element(pyr_mesh *h, DepositCache *c, int64_t n,
 }
 
 namespace pyr {
+const uint32_t *element_order(pyr_mesh *mesh, int64_t n, const uint32_t *elem) {
+  if (!mesh->dep) mesh->dep = new DepositCache();
+  return sort_by_element(mesh, mesh->dep, n, elem);
+}
+
 const uint32_t *deposit_last_order(pyr_mesh *mesh, int64_t n) {
   if (!mesh || !mesh->dep || n <= 0 || mesh->dep->last_perm_n != n) return nullptr;
   return mesh->dep->last_perm;

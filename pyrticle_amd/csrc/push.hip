@@ -590,6 +590,110 @@ __device__ __forceinline__ int relocate_head(const DevMesh &m, const char *base,
   return 2;
 }
 
+// One stage of the fused step on the wave's cached elements (k_lsrk4_step_c,
+// k_lsrk4_ckpt): the update, the containment test and the cascade.  Returns
+// false if the particle left the mesh (slot is then -1).
+template <int D, int N, class T, bool MF>
+__device__ __forceinline__ bool cached_stage(const DevMesh &m, const T *__restrict__ coef,
+                                             char *base, const uint32_t *cel, int nsl, int s,
+                                             double dt, double c, double qi, double mi,
+                                             double *xi, double *pi, double *rx, double *rp,
+                                             double *r, uint32_t &e, int &slot,
+                                             FindCounters &cnt, int &e_flag, unsigned &n_mf) {
+  using C = StepCache<D, N, T>;
+  // coefficients: scalar loads for an element-uniform wave, LDS when every
+  // active lane's element is cached, else per-lane loads
+  const bool cached = __ballot(slot < 0) == 0;
+  double v[D], F[D];
+  velocity<D>(pi, mi, c, v);
+  const uint32_t e_uni = wave_uniform_element(e);
+  const T *cc = (const T *)(base + (slot < 0 ? 0 : slot) * C::BYTES);
+  if constexpr (sizeof(T) == 8) {
+    double f6[6], el[3], mag[3];
+    if (e_uni != INVALID)
+      interp6<D, N, double>(as_constant(coef),
+                            (uint32_t)__builtin_amdgcn_readfirstlane((int)e_uni), r, f6);
+    else if (cached)
+      interp6<D, N, double>(cc, 0u, r, f6);
+    else
+      interp6<D, N, double>(coef, e, r, f6);
+    lorentz<D, double>(qi, f6, v, F, el, mag);
+  } else {
+    float rf[D], f6[6], vf[D], Ff[D], el[3], mag[3];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      rf[k] = (float)r[k];
+      vf[k] = (float)v[k];
+    }
+    if (MF && cached && __ballot(1) == ~0ull) {
+      // MF: full waves with every element cached contract on the matrix cores
+      interp6_mfma_slots<D, N>(base, slot, rf, f6);
+      ++n_mf;
+    } else if (e_uni != INVALID)
+      interp6<D, N, float>(as_constant(coef),
+                           (uint32_t)__builtin_amdgcn_readfirstlane((int)e_uni), rf, f6);
+    else if (cached)
+      interp6<D, N, float>(cc, 0u, rf, f6);
+    else
+      interp6<D, N, float>(coef, e, rf, f6);
+    lorentz<D, float>((float)qi, f6, vf, Ff, el, mag);
+#pragma unroll
+    for (int k = 0; k < D; ++k) F[k] = (double)Ff[k];
+  }
+  const double a = c_rk4a[s], b = c_rk4b[s];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    rx[k] = a * rx[k] + dt * v[k];
+    rp[k] = a * rp[k] + dt * F[k];
+  }
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    xi[k] += b * rx[k];
+    pi[k] += b * rp[k];
+  }
+  // the first test of find_new_containing_element, then the rest of the
+  // cascade
+  bool in;
+  if (e_uni != INVALID)
+    in = still_inside<D>(m, e, e_uni, xi, r);
+  else if (slot >= 0)
+    in = inside_cached<D>(base + slot * C::BYTES + C::INV, xi, r);
+  else
+    in = in_element<D>(m, e, xi, r);
+  bool ok = true;
+  int head = 1;
+  if (in)
+    cnt.same++;
+  else
+    head = relocate_head<D, N, T>(m, base, cel, nsl, xi, pi, e, slot, cnt, e_flag, r);
+  // the rest of the cascade, one particle at a time with the whole wave
+  wave_cascade_tail<D>(m, xi, e, cnt, r, head);
+  if (head == 0) {
+    ok = false;
+  } else if (head == 2) {
+    // no element contains x: periodic trip or death (boundary_hit)
+    ok = boundary_hit<D>(m, xi, pi, e, cnt, e_flag, r);
+    slot = ok ? cache_lookup<D, N, T>(cel, nsl, e) : -1;
+  } else if (head == 3) {
+    slot = cache_lookup<D, N, T>(cel, nsl, e);   // found by the fan or the grid
+  }
+  if (!ok) slot = -1;
+  return ok;
+}
+
+// Slots for the elements the wave's live lanes entered, in slots no lane holds.
+template <int D, int N, class T>
+__device__ __forceinline__ void cache_refresh(const DevMesh &m, const T *__restrict__ coef,
+                                              char *base, uint32_t *cel, int &nsl, bool live,
+                                              uint32_t e, int &slot) {
+  const uint64_t need = __ballot(live && slot < 0);
+  if (!need) return;
+  uint32_t used = 0;
+  for (int j = 0; j < nsl; ++j)
+    if (__ballot(slot == j)) used |= 1u << j;
+  cache_update<D, N, T>(m, coef, base, cel, nsl, used, need, e, slot);
+}
+
 template <int D, int N, class T, bool MF>
 __global__ __launch_bounds__(256, (push_min_waves<N, T>())) void k_lsrk4_step_c(
     DevMesh m, const T *__restrict__ coef, int64_t n, double dt, double c,
@@ -633,94 +737,10 @@ __global__ __launch_bounds__(256, (push_min_waves<N, T>())) void k_lsrk4_step_c(
     else inverse_map<D>(m, e, xi, r);
 #pragma unroll 1
     for (int s = 0; s < 5 && !died; ++s) {
-      // coefficients: scalar loads for an element-uniform wave, LDS when every
-      // active lane's element is cached, else per-lane loads
-      const bool cached = __ballot(slot < 0) == 0;
-      double v[D], F[D];
-      velocity<D>(pi, mi, c, v);
-      const uint32_t e_uni = wave_uniform_element(e);
-      const T *cc = (const T *)(base + (slot < 0 ? 0 : slot) * C::BYTES);
-      if constexpr (sizeof(T) == 8) {
-        double f6[6], el[3], mag[3];
-        if (e_uni != INVALID)
-          interp6<D, N, double>(as_constant(coef),
-                                (uint32_t)__builtin_amdgcn_readfirstlane((int)e_uni), r, f6);
-        else if (cached)
-          interp6<D, N, double>(cc, 0u, r, f6);
-        else
-          interp6<D, N, double>(coef, e, r, f6);
-        lorentz<D, double>(qi, f6, v, F, el, mag);
-      } else {
-        float rf[D], f6[6], vf[D], Ff[D], el[3], mag[3];
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-          rf[k] = (float)r[k];
-          vf[k] = (float)v[k];
-        }
-        if (MF && cached && __ballot(1) == ~0ull) {
-          // MF: full waves with every element cached contract on the matrix cores
-          interp6_mfma_slots<D, N>(base, slot, rf, f6);
-          ++n_mf;
-        } else if (e_uni != INVALID)
-          interp6<D, N, float>(as_constant(coef),
-                               (uint32_t)__builtin_amdgcn_readfirstlane((int)e_uni), rf, f6);
-        else if (cached)
-          interp6<D, N, float>(cc, 0u, rf, f6);
-        else
-          interp6<D, N, float>(coef, e, rf, f6);
-        lorentz<D, float>((float)qi, f6, vf, Ff, el, mag);
-#pragma unroll
-        for (int k = 0; k < D; ++k) F[k] = (double)Ff[k];
-      }
-      const double a = c_rk4a[s], b = c_rk4b[s];
-#pragma unroll
-      for (int k = 0; k < D; ++k) {
-        rx[k] = a * rx[k] + dt * v[k];
-        rp[k] = a * rp[k] + dt * F[k];
-      }
-#pragma unroll
-      for (int k = 0; k < D; ++k) {
-        xi[k] += b * rx[k];
-        pi[k] += b * rp[k];
-      }
-      // the first test of find_new_containing_element, then the rest of the
-      // cascade
-      bool in;
-      if (e_uni != INVALID)
-        in = still_inside<D>(m, e, e_uni, xi, r);
-      else if (slot >= 0)
-        in = inside_cached<D>(base + slot * C::BYTES + C::INV, xi, r);
-      else
-        in = in_element<D>(m, e, xi, r);
-      bool ok = true;
-      int head = 1;
-      if (in)
-        cnt.same++;
-      else
-        head = relocate_head<D, N, T>(m, base, cel, nsl, xi, pi, e, slot, cnt, e_flag, r);
-      // the rest of the cascade, one particle at a time with the whole wave
-      wave_cascade_tail<D>(m, xi, e, cnt, r, head);
-      if (head == 0) {
-        ok = false;
-      } else if (head == 2) {
-        // no element contains x: periodic trip or death (boundary_hit)
-        ok = boundary_hit<D>(m, xi, pi, e, cnt, e_flag, r);
-        slot = ok ? cache_lookup<D, N, T>(cel, nsl, e) : -1;
-      } else if (head == 3) {
-        slot = cache_lookup<D, N, T>(cel, nsl, e);   // found by the fan or the grid
-      }
-      if (!ok) {
+      if (!cached_stage<D, N, T, MF>(m, coef, base, cel, nsl, s, dt, c, qi, mi, xi, pi, rx, rp,
+                                     r, e, slot, cnt, e_flag, n_mf))
         died = (uint32_t)s + 1;
-        slot = -1;
-      }
-      // slots for the elements the wave's lanes entered, in slots no lane holds
-      const uint64_t need = __ballot(!died && slot < 0);
-      if (need && s < 4) {
-        uint32_t used = 0;
-        for (int j = 0; j < nsl; ++j)
-          if (__ballot(slot == j)) used |= 1u << j;
-        cache_update<D, N, T>(m, coef, base, cel, nsl, used, need, e, slot);
-      }
+      if (s < 4) cache_refresh<D, N, T>(m, coef, base, cel, nsl, !died, e, slot);
     }
     if (e_flag) atomicOr(err, 1);
 #pragma unroll
@@ -754,10 +774,26 @@ __global__ __launch_bounds__(256) void k_lsrk4_ckpt(
     double c, double *__restrict__ x, double *__restrict__ p, const double *__restrict__ q,
     const double *__restrict__ mass, uint32_t *__restrict__ elem, uint32_t *__restrict__ death,
     double *__restrict__ ckpt_xp, uint32_t *__restrict__ ckpt_e, unsigned long long *counters,
-    int *err) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int *err, const uint32_t *__restrict__ order) {
+  // the stages run on the wave's LDS element cache (k_lsrk4_step_c), kept
+  // across the steps: thread t steps particle order[t] (the element order at
+  // the start), and the wave's particles travel together
+  using C = StepCache<D, N, T>;
+  __shared__ uint4 lds[4][C::SLOTS * C::BYTES / 16];
+  __shared__ uint32_t lds_el[4][C::SLOTS];
+  const int wv = threadIdx.x >> 6;
+  char *base = (char *)&lds[wv][0];
+  uint32_t *cel = lds_el[wv];
+  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   FindCounters cnt;
-  if (i < n) {
+  unsigned n_mf = 0;
+  const bool live = t < n;
+  const int64_t i = live ? (int64_t)order[t] : 0;
+  uint32_t e = live ? elem[i] : INVALID;
+  const bool located = e < (uint32_t)m.K;
+  int slot = -1, nsl = 0;
+  cache_update<D, N, T>(m, coef, base, cel, nsl, 0u, __ballot(located), e, slot);
+  if (live) {
     double xi[D], pi[D], r[D], rx[D], rp[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) {
@@ -765,10 +801,10 @@ __global__ __launch_bounds__(256) void k_lsrk4_ckpt(
       pi[k] = p[i * D + k];
     }
     const double qi = q[i], mi = mass[i];
-    uint32_t e = elem[i];
     uint32_t died = 0;
     int e_flag = 0;
-    if (e >= (uint32_t)m.K) died = 1;
+    if (!located) died = 1;
+    else if (slot >= 0) inside_cached<D>(base + slot * C::BYTES + C::INV, xi, r);
     else inverse_map<D>(m, e, xi, r);
 #pragma unroll 1
     for (int step = 0; step < nsteps && !died; ++step) {
@@ -788,12 +824,12 @@ __global__ __launch_bounds__(256) void k_lsrk4_ckpt(
         rp[k] = 0;
       }
 #pragma unroll 1
-      for (int s = 0; s < 5; ++s)
-        if (!lsrk4_stage<D, N, T>(m, coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, cnt,
-                                  e_flag)) {
+      for (int s = 0; s < 5 && !died; ++s) {
+        if (!cached_stage<D, N, T, false>(m, coef, base, cel, nsl, s, dt, c, qi, mi, xi, pi, rx,
+                                          rp, r, e, slot, cnt, e_flag, n_mf))
           died = (uint32_t)(5 * step + s + 1);
-          break;
-        }
+        cache_refresh<D, N, T>(m, coef, base, cel, nsl, !died, e, slot);
+      }
     }
     if (e_flag) atomicOr(err, 1);
 #pragma unroll
@@ -924,9 +960,12 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) void k_lsrk4_adjoint
     const uint32_t *__restrict__ death, const double *__restrict__ ckpt_xp,
     const uint32_t *__restrict__ ckpt_e, double *__restrict__ ybuf, uint32_t *__restrict__ ebuf,
     double *__restrict__ sbuf, uint32_t *__restrict__ sebuf, double *__restrict__ x_bar,
-    double *__restrict__ p_bar, int *err) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+    double *__restrict__ p_bar, int *err, const uint32_t *__restrict__ order) {
+  // thread t takes particle order[t] (the element order at the start), so a
+  // wave's particles share elements
+  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t i = (int64_t)order[t];
   if (death[i]) {
 #pragma unroll
     for (int k = 0; k < D; ++k) {
@@ -944,7 +983,8 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) void k_lsrk4_adjoint
   const double qi = q[i], mi = mass[i];
   FindCounters cnt;
   int e_flag = 0, lost = 0;
-  const size_t nn = (size_t)n, ii = (size_t)i;
+  // checkpoints are per particle (i); the recompute buffers per thread (t)
+  const size_t nn = (size_t)n, ii = (size_t)i, tt = (size_t)t;
   const int nseg = (nsteps + ckpt_every - 1) / ckpt_every;
 #pragma unroll 1
   for (int seg = nseg - 1; seg >= 0; --seg) {
@@ -960,13 +1000,13 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) void k_lsrk4_adjoint
     inverse_map<D>(m, e, xi, r);
 #pragma unroll 1
     for (int k = 0; k < steps; ++k) {
-      double *dst = ybuf + ((size_t)k * nn + ii) * 2 * D;
+      double *dst = ybuf + ((size_t)k * nn + tt) * 2 * D;
 #pragma unroll
       for (int kk = 0; kk < D; ++kk) {
         dst[kk] = xi[kk];
         dst[D + kk] = pi[kk];
       }
-      ebuf[(size_t)k * nn + ii] = e;
+      ebuf[(size_t)k * nn + tt] = e;
       if (k + 1 == steps) break;
 #pragma unroll
       for (int kk = 0; kk < D; ++kk) {
@@ -980,7 +1020,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) void k_lsrk4_adjoint
     }
 #pragma unroll 1
     for (int k = steps - 1; k >= 0; --k) {
-      const double *y = ybuf + ((size_t)k * nn + ii) * 2 * D;
+      const double *y = ybuf + ((size_t)k * nn + tt) * 2 * D;
 #pragma unroll
       for (int kk = 0; kk < D; ++kk) {
         xi[kk] = y[kk];
@@ -988,17 +1028,17 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) void k_lsrk4_adjoint
         rx[kk] = 0;
         rp[kk] = 0;
       }
-      e = ebuf[(size_t)k * nn + ii];
+      e = ebuf[(size_t)k * nn + tt];
       inverse_map<D>(m, e, xi, r);
 #pragma unroll 1
       for (int s = 0; s < 5; ++s) {
-        double *st = sbuf + ((size_t)s * nn + ii) * 2 * D;
+        double *st = sbuf + ((size_t)s * nn + tt) * 2 * D;
 #pragma unroll
         for (int kk = 0; kk < D; ++kk) {
           st[kk] = xi[kk];
           st[D + kk] = pi[kk];
         }
-        sebuf[(size_t)s * nn + ii] = e;
+        sebuf[(size_t)s * nn + tt] = e;
         if (s == 4) break;   // the last stage's output is not needed
         if (!lsrk4_stage<D, N, T>(m, coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, cnt, e_flag))
           lost = 1;
@@ -1022,14 +1062,14 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) void k_lsrk4_adjoint
           rbx[kk] *= a;
           rbp[kk] *= a;
         }
-        const double *st = sbuf + ((size_t)s * nn + ii) * 2 * D;
+        const double *st = sbuf + ((size_t)s * nn + tt) * 2 * D;
         double xs[D], ps[D];
 #pragma unroll
         for (int kk = 0; kk < D; ++kk) {
           xs[kk] = st[kk];
           ps[kk] = st[D + kk];
         }
-        rhs_adjoint<D, N, T>(m, coef, c, qi, mi, xs, ps, sebuf[(size_t)s * nn + ii], kbx, kbp,
+        rhs_adjoint<D, N, T>(m, coef, c, qi, mi, xs, ps, sebuf[(size_t)s * nn + tt], kbx, kbp,
                              ax, ap);
       }
     }
@@ -1447,18 +1487,19 @@ int pyr_lsrk4_forward(pyr_mesh *mesh, int precision, int nsteps, int ckpt_every,
     PYR_HIP(hipMemsetAsync(err, 0, sizeof(int), stream()));
     {
       ProfScope prof(PROF_PUSH);
+      const uint32_t *order = element_order(mesh, n, elem);
       dispatch_order(D, N, [&](auto DD, auto NN) {
         constexpr int D_ = decltype(DD)::value, N_ = decltype(NN)::value;
         if (precision == PYR_FP64)
           hipLaunchKernelGGL((k_lsrk4_ckpt<D_, N_, double>), dim3(blocks_for(n)), dim3(256), 0,
                              stream(), mesh->dev, mesh->sf64.as<double>(), n, nsteps, ckpt_every,
                              dt, vacuum_c, x, p, q, mass, elem, death, ckpt_xp, ckpt_elem,
-                             counters, err);
+                             counters, err, order);
         else
           hipLaunchKernelGGL((k_lsrk4_ckpt<D_, N_, float>), dim3(blocks_for(n)), dim3(256), 0,
                              stream(), mesh->dev, mesh->sf32.as<float>(), n, nsteps, ckpt_every,
                              dt, vacuum_c, x, p, q, mass, elem, death, ckpt_xp, ckpt_elem,
-                             counters, err);
+                             counters, err, order);
         PYR_CHECK_LAUNCH();
       });
     }
@@ -1500,18 +1541,21 @@ int pyr_lsrk4_adjoint(pyr_mesh *mesh, int precision, int nsteps, int ckpt_every,
     PYR_HIP(hipMemsetAsync(err, 0, sizeof(int), stream()));
     {
       ProfScope prof(PROF_ADJOINT);
+      // the element order at the start (first checkpoint): a wave's particles
+      // share elements through the segments
+      const uint32_t *order = element_order(mesh, n, ckpt_elem);
       dispatch_order(D, N, [&](auto DD, auto NN) {
         constexpr int D_ = decltype(DD)::value, N_ = decltype(NN)::value;
         if (precision == PYR_FP64)
           hipLaunchKernelGGL((k_lsrk4_adjoint<D_, N_, double>), dim3(blocks_for(n)), dim3(256),
                              0, stream(), mesh->dev, mesh->sf64.as<double>(), n, nsteps,
                              ckpt_every, dt, vacuum_c, q, mass, death, ckpt_xp, ckpt_elem, ybuf,
-                             ebuf, sbuf, sebuf, x_bar, p_bar, err);
+                             ebuf, sbuf, sebuf, x_bar, p_bar, err, order);
         else
           hipLaunchKernelGGL((k_lsrk4_adjoint<D_, N_, float>), dim3(blocks_for(n)), dim3(256),
                              0, stream(), mesh->dev, mesh->sf32.as<float>(), n, nsteps,
                              ckpt_every, dt, vacuum_c, q, mass, death, ckpt_xp, ckpt_elem, ybuf,
-                             ebuf, sbuf, sebuf, x_bar, p_bar, err);
+                             ebuf, sbuf, sebuf, x_bar, p_bar, err, order);
         PYR_CHECK_LAUNCH();
       });
     }

@@ -563,6 +563,10 @@ namespace pyr {
 // the last deposition's sort groups the particles of each element into
 // consecutive lanes, so a wave mostly sees one element.
 const uint32_t *deposit_last_order(pyr_mesh *mesh, int64_t n);
+// deposit.hip: a stable sort of n particles by element (elem, device): the
+// permutation sorted position -> particle index, valid until the next sort on
+// this mesh (stream-ordered).
+const uint32_t *element_order(pyr_mesh *mesh, int64_t n, const uint32_t *elem);
 // deposit.hip: the averaging pusher's shape-weighted forces, stream-ordered
 // (returns the device count of particles with a zero shape integral, or null
 // when n = 0).  vis: six arrays (pt_e, pt_b, pt_e_stddev, pt_b_stddev,
