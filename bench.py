@@ -305,8 +305,8 @@ TRAFFIC_KERNELS = {
     "k_gather_cl": r"k_gather_cl<", "k_cull_chunks": r"k_cull_chunks<",
     "k_stage32c": r"k_stage32c\b", "k_combine_items": r"k_combine_items\b",
     "k_gather_cl64": r"k_gather_cl64<", "k_gather": r"k_gather<",
-    "k_lsrk4_step": r"k_lsrk4_step<3, 3, double",
-    "k_lsrk4_step_fp32c": r"k_lsrk4_step<3, 3, float, false", "k_stage": r"k_stage\(",
+    "k_lsrk4_step": r"k_lsrk4_step_c<3, 3, double",
+    "k_lsrk4_step_fp32c": r"k_lsrk4_step_c<3, 3, float, false", "k_stage": r"k_stage\(",
     "k_rs_scatter": r"k_rs_scatter\b", "k_shape_integral": r"k_shape_integral<",
     "k_shape_average": r"k_shape_average<", "k_reduce": r"k_reduce\b",
 }
@@ -972,7 +972,8 @@ def main(argv=None):
     field_bytes = occupied * (6 * Np * 8 + 12 * 8 + 4 * (3 + 1) * 8 + 4 * 4)
     push_bytes = n_last * (68 + 52) + field_bytes
     push_gbs = push_bytes / push_avg_s / 1e9 if push_avg_s > 0 else 0.0
-    roof_push = {"bound": "hbm", "kernel": "k_lsrk4_step (fused 5-stage push, FP64)",
+    roof_push = {"bound": "hbm",
+                 "kernel": "k_lsrk4_step_c (fused 5-stage push on per-wave LDS element slots, FP64)",
                  "achieved": push_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": push_gbs / HBM_PEAK_GBS, "traffic": measured_traffic("k_lsrk4_step"),
                  "traffic_source": traffic_note,
