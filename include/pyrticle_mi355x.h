@@ -218,7 +218,9 @@ int pyr_lsrk4_steps_average(pyr_mesh *mesh, int nsteps, double dt, double vacuum
  * death[i] = 5*step + stage (1-based; 0 = alive) and containing_elements[i] =
  * INVALID.  If ckpt_xp/ckpt_elem are given, the state at the start of every
  * ckpt_every-th step is stored there: ckpt_xp[(seg*n + i)*2*dim + k] holds x
- * (k < dim) then p, for seg = 0 .. ceil(nsteps/ckpt_every)-1. */
+ * (k < dim) then p, for seg = 0 .. ceil(nsteps/ckpt_every)-1.  The particles
+ * are stepped in element order (a sort on the mesh's sort scratch, as
+ * pyr_sort_by_element). */
 int pyr_lsrk4_forward(pyr_mesh *mesh, int precision, int nsteps, int ckpt_every, double dt,
                       double vacuum_c, int64_t n, double *positions, double *momenta,
                       const double *charges, const double *masses,
@@ -230,7 +232,8 @@ int pyr_lsrk4_forward(pyr_mesh *mesh, int precision, int nsteps, int ckpt_every,
  * state.  Segments are recomputed from the checkpoints (library scratch of
  * ckpt_every*n*(16*dim+4) + 5*n*(16*dim+4) bytes).  Particles with death != 0
  * get zero gradients.  precision PYR_FP32C evaluates fields and field
- * gradients in FP32 and accumulates the adjoint in FP64. */
+ * gradients in FP32 and accumulates the adjoint in FP64.  The particles are
+ * processed in the element order of the first checkpoint. */
 int pyr_lsrk4_adjoint(pyr_mesh *mesh, int precision, int nsteps, int ckpt_every, double dt,
                       double vacuum_c, int64_t n, const double *charges, const double *masses,
                       const uint32_t *death, const double *ckpt_xp, const uint32_t *ckpt_elem,
