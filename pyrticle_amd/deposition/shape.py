@@ -50,7 +50,10 @@ class NormalizedShapeFunctionDepositor(Depositor):
     ``stats.normalization_stats`` gathers q / integral of every deposited
     particle; ``centroid_distance_stats`` and ``el_per_particle_stats`` follow
     the reference's face walk, run as a pass of its own when ``walk_stats`` is
-    set (and from add_instrumentation on)."""
+    set (and from add_instrumentation on).  That pass is instrumentation and
+    costs far more than the deposition: a depth-first walk per particle with
+    its visited set in private memory, about 216 ms for 4M particles on the
+    200,910-tet cylinder against 17 ms for the normalized deposition."""
 
     name = "NormShape"
 
