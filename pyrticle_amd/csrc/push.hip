@@ -858,15 +858,23 @@ __global__ __launch_bounds__(256) void k_lsrk4_ckpt(
 //   pb += (c/s) (vb - p (p . vb) / s^2),       s = sqrt(m^2 c^2 + |p|^2).
 // With T = float the field values and gradients are evaluated in FP32 on a
 // power-of-two rescaled weight vector; the accumulation stays FP64.
-template <int D, int N, class T>
-__device__ __forceinline__ void rhs_adjoint(const DevMesh &m, const T *__restrict__ coef,
-                                            double c, double qi, double mi, const double *xi,
-                                            const double *pi, uint32_t e, const double *kbx,
-                                            const double *kbp, double *xb, double *pb) {
+// cf: the element's 6 x coef_stride field coefficients, am: its inverse map
+// (D x D + D), both in global memory or in the wave's LDS slots.
+template <int D, int N, class T, class CP, class AP>
+__device__ __forceinline__ void rhs_adjoint(CP cf, AP am, double c, double qi, double mi,
+                                            const double *xi, const double *pi,
+                                            const double *kbx, const double *kbp, double *xb,
+                                            double *pb) {
   constexpr Basis<D, N> B{};
   constexpr int NP = Basis<D, N>::NP;
   double r[D], v[D];
-  inverse_map<D>(m, e, xi, r);
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    double t = 0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) t += am[i * D + j] * xi[j];
+    r[i] = t + am[D * D + i];
+  }
   velocity<D>(pi, mi, c, v);
   double v3[3], k3[3];
 #pragma unroll
@@ -898,20 +906,31 @@ __device__ __forceinline__ void rhs_adjoint(const DevMesh &m, const T *__restric
     for (int k = 1; k <= N; ++k) pw[ii][k] = pw[ii][k - 1] * rt[ii];
   }
   constexpr int NS = coef_stride<T>(NP);
-  const T *cf = coef + (size_t)e * 6 * NS;
   T bf[3] = {T(0), T(0), T(0)}, g[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) g[k] = T(0);
+  // the coefficients V monomials at a time, one 16-byte load per field row
+  // when the rows allow it (same products and order as one at a time)
+  constexpr int V = (NS * sizeof(T)) % 16 == 0 ? 16 / (int)sizeof(T) : 1;
+  struct alignas(V * sizeof(T)) Vec {
+    T v[V];
+  };
 #pragma unroll
-  for (int k = 0; k < NP; ++k) {
+  for (int k0 = 0; k0 < NP; k0 += V) {
+    Vec cv[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) cv[f] = *reinterpret_cast<const Vec *>(&cf[f * NS + k0]);
+#pragma unroll
+  for (int kk = 0; kk < V && k0 + kk < NP; ++kk) {
+    const int k = k0 + kk;
     T phi = pw[0][B.e[k][0]];
 #pragma unroll
     for (int ii = 1; ii < D; ++ii) phi = phi * pw[ii][B.e[k][ii]];
 #pragma unroll
-    for (int f = 0; f < 3; ++f) bf[f] = fma(cf[(3 + f) * NS + k], phi, bf[f]);
+    for (int f = 0; f < 3; ++f) bf[f] = fma(cv[3 + f].v[kk], phi, bf[f]);
     T wk = T(0);
 #pragma unroll
-    for (int f = 0; f < 6; ++f) wk = fma(ws[f], cf[f * NS + k], wk);
+    for (int f = 0; f < 6; ++f) wk = fma(ws[f], cv[f].v[kk], wk);
 #pragma unroll
     for (int jj = 0; jj < D; ++jj) {
       const int ej = B.e[k][jj];
@@ -924,8 +943,8 @@ __device__ __forceinline__ void rhs_adjoint(const DevMesh &m, const T *__restric
       }
     }
   }
+  }
   // x: A^T grad_r, A row-major in the inverse map
-  const double *am = m.inv_map + (size_t)e * (D * D + D);
 #pragma unroll
   for (int jj = 0; jj < D; ++jj) {
     double acc = 0;
@@ -969,6 +988,15 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) void k_lsrk4_adjoint
     const uint32_t *__restrict__ ckpt_e, double *__restrict__ ybuf, uint32_t *__restrict__ ebuf,
     double *__restrict__ sbuf, uint32_t *__restrict__ sebuf, double *__restrict__ x_bar,
     double *__restrict__ p_bar, int *err, const uint32_t *__restrict__ order) {
+  // the recomputed stages and the adjoint right-hand sides read the elements
+  // from the wave's LDS slots (k_lsrk4_step_c), kept across the segments
+  using C = StepCache<D, N, T>;
+  __shared__ uint4 lds[4][C::SLOTS * C::BYTES / 16];
+  __shared__ uint32_t lds_el[4][C::SLOTS];
+  const int wv = threadIdx.x >> 6;
+  char *base = (char *)&lds[wv][0];
+  uint32_t *cel = lds_el[wv];
+  int nsl = 0;
   // thread t takes particle order[t] (the element order at the start), so a
   // wave's particles share elements
   const int64_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -990,7 +1018,20 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) void k_lsrk4_adjoint
   }
   const double qi = q[i], mi = mass[i];
   FindCounters cnt;
+  unsigned n_mf = 0;
   int e_flag = 0, lost = 0;
+  // the lane's slot of element e (wave-collective: the cached copy, else one
+  // inserted now; -1 when the slots are full)
+  auto slot_of = [&](uint32_t e) {
+    int sl = cache_lookup<D, N, T>(cel, nsl, e);
+    cache_refresh<D, N, T>(m, coef, base, cel, nsl, true, e, sl);
+    return sl;
+  };
+  // unit coordinates of x in element e through its slot
+  auto coords = [&](uint32_t e, int sl, const double *xi, double *r) {
+    if (sl >= 0) inside_cached<D>(base + sl * C::BYTES + C::INV, xi, r);
+    else inverse_map<D>(m, e, xi, r);
+  };
   // checkpoints are per particle (i); the recompute buffers per thread (t)
   const size_t nn = (size_t)n, ii = (size_t)i, tt = (size_t)t;
   const int nseg = (nsteps + ckpt_every - 1) / ckpt_every;
@@ -1005,7 +1046,8 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) void k_lsrk4_adjoint
       pi[k] = src[D + k];
     }
     uint32_t e = ckpt_e[(size_t)seg * nn + ii];
-    inverse_map<D>(m, e, xi, r);
+    int slot = slot_of(e);
+    coords(e, slot, xi, r);
 #pragma unroll 1
     for (int k = 0; k < steps; ++k) {
       double *dst = ybuf + ((size_t)k * nn + tt) * 2 * D;
@@ -1022,9 +1064,12 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) void k_lsrk4_adjoint
         rp[kk] = 0;
       }
 #pragma unroll 1
-      for (int s = 0; s < 5; ++s)
-        if (!lsrk4_stage<D, N, T>(m, coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, cnt, e_flag))
+      for (int s = 0; s < 5; ++s) {
+        if (!cached_stage<D, N, T, false>(m, coef, base, cel, nsl, s, dt, c, qi, mi, xi, pi, rx,
+                                          rp, r, e, slot, cnt, e_flag, n_mf))
           lost = 1;
+        cache_refresh<D, N, T>(m, coef, base, cel, nsl, true, e, slot);
+      }
     }
 #pragma unroll 1
     for (int k = steps - 1; k >= 0; --k) {
@@ -1037,7 +1082,8 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) void k_lsrk4_adjoint
         rp[kk] = 0;
       }
       e = ebuf[(size_t)k * nn + tt];
-      inverse_map<D>(m, e, xi, r);
+      slot = slot_of(e);
+      coords(e, slot, xi, r);
 #pragma unroll 1
       for (int s = 0; s < 5; ++s) {
         double *st = sbuf + ((size_t)s * nn + tt) * 2 * D;
@@ -1048,8 +1094,10 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) void k_lsrk4_adjoint
         }
         sebuf[(size_t)s * nn + tt] = e;
         if (s == 4) break;   // the last stage's output is not needed
-        if (!lsrk4_stage<D, N, T>(m, coef, s, dt, c, qi, mi, xi, pi, rx, rp, r, e, cnt, e_flag))
+        if (!cached_stage<D, N, T, false>(m, coef, base, cel, nsl, s, dt, c, qi, mi, xi, pi, rx,
+                                          rp, r, e, slot, cnt, e_flag, n_mf))
           lost = 1;
+        cache_refresh<D, N, T>(m, coef, base, cel, nsl, true, e, slot);
       }
       double rbx[D], rbp[D];
 #pragma unroll
@@ -1077,8 +1125,29 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) void k_lsrk4_adjoint
           xs[kk] = st[kk];
           ps[kk] = st[D + kk];
         }
-        rhs_adjoint<D, N, T>(m, coef, c, qi, mi, xs, ps, sebuf[(size_t)s * nn + tt], kbx, kbp,
-                             ax, ap);
+        // FP64: from the slots, in rounds when the wave's elements outnumber
+        // them (each round frees the slots of the lanes it served); FP32C:
+        // from the mesh arrays (its FP32 rows read from LDS measured slower,
+        // 129 -> 148 ms for the 4M-particle 200-step adjoint, against
+        // 239 -> 179 ms for FP64)
+        const uint32_t es = sebuf[(size_t)s * nn + tt];
+        if constexpr (sizeof(T) == 4) {
+          rhs_adjoint<D, N, T>(coef + (size_t)es * 6 * coef_stride<T>(Basis<D, N>::NP),
+                               m.inv_map + (size_t)es * (D * D + D), c, qi, mi, xs, ps, kbx,
+                               kbp, ax, ap);
+          continue;
+        }
+        bool done = false;
+        do {
+          int ss = done ? -1 : cache_lookup<D, N, T>(cel, nsl, es);
+          cache_refresh<D, N, T>(m, coef, base, cel, nsl, !done, es, ss);
+          if (!done && ss >= 0) {
+            const char *sb = base + ss * C::BYTES;
+            rhs_adjoint<D, N, T>((const T *)sb, (const double *)(sb + C::INV), c, qi, mi, xs,
+                                 ps, kbx, kbp, ax, ap);
+            done = true;
+          }
+        } while (__ballot(!done));
       }
     }
   }
