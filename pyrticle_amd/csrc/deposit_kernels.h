@@ -401,6 +401,9 @@ constexpr int ENT_SHIFT_BIT = 16;
 constexpr uint32_t ENT_MASK = (1u << ENT_SHIFT_BIT) - 1;
 // targets (waves) per workgroup of the list gather
 constexpr int CL_W = 4;
+// word 3 of a raw buffer resource on gfx950 (32-bit data format; the untyped
+// buffer loads ignore the format otherwise)
+constexpr int BUF_WORD3 = 0x00020000;
 
 template <int D, int NG>
 __global__ __launch_bounds__(256) void k_cull_chunks(CullArgs g) {
@@ -560,14 +563,27 @@ __global__ __launch_bounds__(64 * W) void k_gather_cl(ListArgs g) {
     bz[ng] = nt.z;
     cc[ng] = nt.w;
   }
-  uint2 ea = make_uint2(0u, 0u);
-  float4 oa = make_float4(0.f, 0.f, 0.f, 0.f);
-  if ((uint32_t)lane < cnt) {
-    ea = g.ent[base + lane];
-    oa = g.ent_off[base + lane];
-  }
-  auto rdf = [](float v, int l) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+  // The item's list entries (chunk, group mask, source offset) are read with
+  // scalar loads, four at a time one loop iteration before their chunks are
+  // fetched; the chunk slots are read with buffer loads whose 64-bit base is
+  // the chunk's (scalar) address and whose per-lane offset is fixed, so a
+  // fetch costs no vector instruction besides its two loads.
+  const uint32_t ubase = __builtin_amdgcn_readfirstlane(base);
+  const auto s_ent = as_constant(reinterpret_cast<const uint32_t *>(g.ent + ubase));
+  const auto s_off = as_constant(reinterpret_cast<const float *>(g.ent_off + ubase));
+  const int lane_off = lane * (int)sizeof(float4);
+  struct Ent {
+    uint32_t ch, y;
+    float ox, oy, oz, ow;
+  };
+  auto load_ent = [&](Ent &en, uint32_t i) {
+    const uint32_t l = min(i, cnt - 1);
+    en.ch = s_ent[2 * l];
+    en.y = s_ent[2 * l + 1];
+    en.ox = s_off[4 * l];
+    en.oy = s_off[4 * l + 1];
+    en.oz = s_off[4 * l + 2];
+    en.ow = s_off[4 * l + 3];
   };
   struct Next {
     float4 p, w;
@@ -578,18 +594,31 @@ __global__ __launch_bounds__(64 * W) void k_gather_cl(ListArgs g) {
   // all-zero chunk with no groups.  The loads are issued on every call, so
   // the same number is in flight on every path and the wait before a pair's
   // evaluation does not cover the pair fetched after it.
-  auto fetch = [&](Next &n, uint32_t i) {
+  auto fetch = [&](Next &n, uint32_t i, const Ent &en) {
     const bool live = i < cnt;   // wave-uniform
-    const int l = (int)min(i, 63u);
-    const uint32_t ch_l = (uint32_t)__builtin_amdgcn_readlane((int)ea.x, l);
-    const uint32_t m_l = (uint32_t)__builtin_amdgcn_readlane((int)ea.y, l);
-    const uint32_t ch = live ? ch_l : g.zero_chunk;
-    n.ngm = live ? (m_l & ENT_MASK) : 0u;
-    n.ox = rdf(oa.x, l);
-    n.oy = rdf(oa.y, l);
-    n.oz = rdf(oa.z, l);
-    n.p = g.pos[(size_t)ch * 64 + lane];
-    n.w = g.wgt[(size_t)ch * 64 + lane];
+    const uint32_t ch = live ? en.ch : g.zero_chunk;
+    // (the shift code above ENT_SHIFT_BIT is never tested: only the NG <= 9
+    // group bits are, and every listed entry has one)
+    n.ngm = live ? en.y : 0u;
+    n.ox = en.ox;
+    n.oy = en.oy;
+    n.oz = en.oz;
+    // the unused fourth word stays allocated until here: a register the
+    // scalar load still writes is not handed out meanwhile (that reuse
+    // would wait for the load)
+    asm volatile("" ::"s"(en.ow));
+    const size_t off = (size_t)ch * (64 * sizeof(float4));
+    const auto rp = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(reinterpret_cast<const char *>(g.pos) + off), (short)0, 64 * (int)sizeof(float4),
+        BUF_WORD3);
+    const auto rw = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(reinterpret_cast<const char *>(g.wgt) + off), (short)0, 64 * (int)sizeof(float4),
+        BUF_WORD3);
+    const auto pv = __builtin_amdgcn_raw_buffer_load_b96(rp, lane_off, 0, 0);
+    const auto wv = __builtin_amdgcn_raw_buffer_load_b128(rw, lane_off, 0, 0);
+    n.p = make_float4(__uint_as_float(pv[0]), __uint_as_float(pv[1]), __uint_as_float(pv[2]), 0.f);
+    n.w = make_float4(__uint_as_float(wv[0]), __uint_as_float(wv[1]), __uint_as_float(wv[2]),
+                      __uint_as_float(wv[3]));
   };
   // t = (R - |n - x|^2 / R) / P: the chain starts from 1 * cc = cc exactly,
   // so it rounds as C = cc followed by the four products
@@ -645,21 +674,33 @@ __global__ __launch_bounds__(64 * W) void k_gather_cl(ListArgs g) {
   // scheduler would otherwise start them early into fresh registers: 116
   // instead of 92 VGPRs, 4 instead of 5 waves per SIMD).
   Next a, b, c, d;
-  fetch(a, 0);
-  fetch(b, 1);
-  fetch(c, 2);
-  fetch(d, 3);
+  Ent e0, e1, e2, e3;
+  load_ent(e0, 0);
+  load_ent(e1, 1);
+  load_ent(e2, 2);
+  load_ent(e3, 3);
+  fetch(a, 0, e0);
+  fetch(b, 1, e1);
+  fetch(c, 2, e2);
+  fetch(d, 3, e3);
 #pragma unroll 1
   for (uint32_t i = 0; i < cnt; i += 4) {
+    // the entries of this iteration's fetches, loaded one pair evaluation
+    // before the first of them
+    load_ent(e0, i + 4);
+    load_ent(e1, i + 5);
+    load_ent(e2, i + 6);
+    load_ent(e3, i + 7);
+    __builtin_amdgcn_sched_barrier(0);
     evaluate2(a, b);
     __builtin_amdgcn_sched_barrier(0);
-    fetch(a, i + 4);
-    fetch(b, i + 5);
+    fetch(a, i + 4, e0);
+    fetch(b, i + 5, e1);
     __builtin_amdgcn_sched_barrier(0);
     evaluate2(c, d);
     __builtin_amdgcn_sched_barrier(0);
-    fetch(c, i + 6);
-    fetch(d, i + 7);
+    fetch(c, i + 6, e2);
+    fetch(d, i + 7, e3);
     __builtin_amdgcn_sched_barrier(0);
   }
   if (n_steps > 0) fold();
