@@ -370,16 +370,18 @@ __global__ void k_item_fill(int K, const uint32_t *__restrict__ order,
     items[i] = make_uint2(e, 64u * k);
 }
 
-// Sort key of every work item: the chunk of its first list entry.  The list
-// gather runs the items in this order, so that the items reading a chunk
-// (from ~200 targets) run at about the same time and share it in the L2.
+// Sort key of every work item: the chunk of its first list entry, shifted
+// right by `shift` (at most 16 key bits: two 8-bit sort passes; items whose
+// first chunks share a key keep their target order).  The list gather runs the
+// items in this order, so that the items reading a chunk (from ~200 targets)
+// run at about the same time and share it in the L2.
 __global__ void k_item_keys(uint32_t n_items, const uint2 *__restrict__ items,
                             const uint32_t *__restrict__ cap_off, const uint2 *__restrict__ ent,
-                            uint32_t *__restrict__ key) {
+                            int shift, uint32_t *__restrict__ key) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_items) return;
   const uint2 it = items[i];
-  key[i] = ent[cap_off[it.x] + it.y].x;
+  key[i] = ent[cap_off[it.x] + it.y].x >> shift;
 }
 
 // Targets that may receive charge: those whose candidate list holds a source
@@ -432,8 +434,8 @@ struct DepositCache {
   // transposed lists (per source element) for the adjoint
   DevBuf tr_start, tr, tr_shift;
   double tr_radius = -1;
-  // sort / staging scratch
-  DevBuf keys_a, keys_b, vals_a, vals_b, hist, hist_scan, seg, tmp, pos, wgt, stats;
+  // sort / staging scratch (sort_tmp: the library sort's temporary storage)
+  DevBuf keys_a, keys_b, vals_a, vals_b, hist, hist_scan, sort_tmp, seg, tmp, pos, wgt, stats;
   // matrix-core gather over culled chunk lists: targets in Morton order,
   // grouped node tables, per-deposition chunk staging and the lists
   DevBuf t_order, tab32g, tab64g, ngsph, node_slot, ch_meta, pos32c, wgt32c;
@@ -478,8 +480,8 @@ void pyr_deposit_cache_free(pyr_mesh *mesh) {
   DepositCache *c = mesh->dep;
   for (DevBuf *b : {&c->cand_start, &c->cand, &c->cand_shift, &c->geom, &c->tab64, &c->tab32,
                     &c->tab_R, &c->tr_start, &c->tr, &c->tr_shift, &c->keys_a, &c->keys_b,
-                    &c->vals_a, &c->vals_b, &c->skeys, &c->hist,
-                    &c->hist_scan, &c->seg, &c->tmp, &c->pos, &c->wgt, &c->stats,
+                    &c->vals_a, &c->vals_b, &c->skeys, &c->hist, &c->hist_scan,
+                    &c->sort_tmp, &c->seg, &c->tmp, &c->pos, &c->wgt, &c->stats,
                     &c->nrm, &c->chunks, &c->t_order, &c->tab32g, &c->tab64g,
                     &c->ngsph, &c->node_slot, &c->ch_meta, &c->pos32c, &c->wgt32c, &c->cap,
                     &c->ent, &c->ent_off, &c->n_live, &c->items, &c->partial, &c->ikey,
@@ -822,17 +824,30 @@ static unsigned blocks_for(int64_t n, int bs = 256) {
   return (unsigned)std::max<int64_t>(1, (n + bs - 1) / bs);
 }
 
-// Stable LSD radix sort of the indices 0..n-1 by keys clamped to [0, kmax]
-// (8-bit digits); returns the sorted indices (in va or vb).
+// Stable radix sort of the indices 0..n-1 by keys clamped to [0, kmax];
+// returns the sorted indices (in va or vb) and, with sorted_keys, the sorted
+// keys.  Large sorts (the particles) run on rocPRIM's onesweep radix sort;
+// small ones (the work items) as 8-bit LSD passes of k_rs_hist / k_rs_scatter,
+// a few launches each where rocPRIM would take a block sort and a chain of
+// merge passes.  Both are stable, so the permutation is the same.
+constexpr int64_t SORT_LIBRARY_MIN = 1 << 20;
+
 static const uint32_t *radix_sort(DepositCache *c, int64_t n, const uint32_t *keys,
                                   uint32_t kmax, DevBuf &ka, DevBuf &kb, DevBuf &va,
                                   DevBuf &vb, const uint32_t **sorted_keys = nullptr) {
   hipStream_t s = stream();
+  if (n >= SORT_LIBRARY_MIN) {
+    for (DevBuf *b : {&ka, &va}) b->reserve(sizeof(uint32_t) * (size_t)n);
+    const uint32_t *perm =
+        sort_pairs(c->sort_tmp, n, keys, kmax, ka.as<uint32_t>(), va.as<uint32_t>(), s);
+    if (sorted_keys) *sorted_keys = ka.as<uint32_t>();
+    return perm;
+  }
   const int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
   int bits = 1;
   while (bits < 32 && ((uint64_t)1 << bits) <= (uint64_t)kmax) ++bits;
   const int passes = (bits + 7) / 8;
-  for (DevBuf *b : {&ka, &kb, &va, &vb}) b->reserve(sizeof(uint32_t) * n);
+  for (DevBuf *b : {&ka, &kb, &va, &vb}) b->reserve(sizeof(uint32_t) * (size_t)std::max<int64_t>(1, n));
   c->hist.reserve(sizeof(uint32_t) * 256 * ntiles);
   c->hist_scan.reserve(sizeof(uint32_t) * 256 * ntiles);
   c->tmp.reserve(sizeof(uint32_t) * scan_tmp_words(256 * ntiles));
@@ -1235,13 +1250,15 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
         c->partial.reserve(sizeof(double) * 4 * ns * (size_t)std::max<uint32_t>(1, n_items));
         if (n_items) {
           c->ikey.reserve(sizeof(uint32_t) * (size_t)n_items);
+          int ishift = 0;
+          while (((uint64_t)std::max<int64_t>(1, nch) >> ishift) >= ((uint64_t)1 << 16)) ++ishift;
           hipLaunchKernelGGL(k_item_keys, dim3(blocks_for(n_items)), dim3(256), 0, s, n_items,
-                             c->items.as<uint2>(), cap_off, c->ent.as<uint2>(),
+                             c->items.as<uint2>(), cap_off, c->ent.as<uint2>(), ishift,
                              c->ikey.as<uint32_t>());
           PYR_CHECK_LAUNCH();
           item_order = radix_sort(c, n_items, c->ikey.as<uint32_t>(),
-                                  (uint32_t)std::max<int64_t>(1, nch), c->ik_a, c->ik_b, c->iv_a,
-                                  c->iv_b);
+                                  (uint32_t)(std::max<int64_t>(1, nch) >> ishift), c->ik_a,
+                                  c->ik_b, c->iv_a, c->iv_b);
         }
       }
       ListArgs la{};

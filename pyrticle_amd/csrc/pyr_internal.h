@@ -567,6 +567,11 @@ const uint32_t *deposit_last_order(pyr_mesh *mesh, int64_t n);
 // permutation sorted position -> particle index, valid until the next sort on
 // this mesh (stream-ordered).
 const uint32_t *element_order(pyr_mesh *mesh, int64_t n, const uint32_t *elem);
+// sort.hip: stable radix sort of the indices 0..n-1 by keys clamped to
+// [0, kmax] (stream-ordered on s): sorted keys in kout, sorted indices in
+// vout (returned); scratch grows to the sort's temporary storage.
+const uint32_t *sort_pairs(DevBuf &scratch, int64_t n, const uint32_t *keys, uint32_t kmax,
+                           uint32_t *kout, uint32_t *vout, hipStream_t s);
 // deposit.hip: the averaging pusher's shape-weighted forces, stream-ordered
 // (returns the device count of particles with a zero shape integral, or null
 // when n = 0).  vis: six arrays (pt_e, pt_b, pt_e_stddev, pt_b_stddev,
