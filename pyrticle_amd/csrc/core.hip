@@ -347,7 +347,21 @@ static void build_grid(pyr_mesh *h, const pyr_mesh_desc *d) {
 
 using namespace pyr;
 
+namespace pyr {
+void read_step_flags(pyr_mesh *mesh, const int *dev, int out[2]) {
+  if (!mesh->h_step) PYR_HIP(hipHostMalloc((void **)&mesh->h_step, 2 * sizeof(int)));
+  if (!mesh->ev_step) PYR_HIP(hipEventCreateWithFlags(&mesh->ev_step, hipEventDisableTiming));
+  PYR_HIP(hipMemcpyAsync(mesh->h_step, dev, 2 * sizeof(int), hipMemcpyDeviceToHost, stream()));
+  PYR_HIP(hipEventRecord(mesh->ev_step, stream()));
+  PYR_HIP(hipEventSynchronize(mesh->ev_step));
+  out[0] = mesh->h_step[0];
+  out[1] = mesh->h_step[1];
+}
+}  // namespace pyr
+
 pyr_mesh::~pyr_mesh() {
+  if (h_step) (void)hipHostFree(h_step);
+  if (ev_step) (void)hipEventDestroy(ev_step);
   for (auto &b : owned) b.release();
   minv.release();
   minv32.release();
@@ -362,6 +376,10 @@ pyr_mesh::~pyr_mesh() {
   counters.release();
   sf64.release();
   sf32.release();
+  sf_nodal.release();
+  s_avg.release();
+  s_occ.release();
+  mf_count.release();
 }
 
 void pyr_deposit_cache_free(pyr_mesh *mesh);   // deposit.hip

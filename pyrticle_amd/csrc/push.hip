@@ -1430,8 +1430,7 @@ int pyr_lsrk4_steps(pyr_mesh *mesh, int precision, int nsteps, double dt, double
         });
       }
       int h[2] = {0, 0};
-      PYR_HIP(hipMemcpyAsync(h, err, 2 * sizeof(int), hipMemcpyDeviceToHost, stream()));
-      PYR_HIP(hipStreamSynchronize(stream()));
+      read_step_flags(mesh, err, h);
       if (h[0]) throw Error(PYR_ERR_NO_BEST_NORMAL, "no best normal found--weird");
       if (h[1] == 0) continue;
       // the reference compacts after each stage (cloud.py:587-592); replay
@@ -1514,8 +1513,7 @@ int pyr_lsrk4_steps_average(pyr_mesh *mesh, int nsteps, double dt, double vacuum
         PYR_CHECK_LAUNCH();
       }
       int h[2] = {0, 0};
-      PYR_HIP(hipMemcpyAsync(h, err, 2 * sizeof(int), hipMemcpyDeviceToHost, stream()));
-      PYR_HIP(hipStreamSynchronize(stream()));
+      read_step_flags(mesh, err, h);
       if (h[0]) throw Error(PYR_ERR_NO_BEST_NORMAL, "no best normal found--weird");
       if (h[1] == 0) continue;
       // compactions in stage order, as after each stage of the per-stage path

@@ -567,6 +567,10 @@ const uint32_t *deposit_last_order(pyr_mesh *mesh, int64_t n);
 // permutation sorted position -> particle index, valid until the next sort on
 // this mesh (stream-ordered).
 const uint32_t *element_order(pyr_mesh *mesh, int64_t n, const uint32_t *elem);
+// core.hip: copies the two ints at dev (stream-ordered) into out, through
+// the mesh's pinned words and an event, and waits for them (not for work
+// queued later).
+void read_step_flags(pyr_mesh *mesh, const int *dev, int out[2]);
 // sort.hip: stable radix sort of the indices 0..n-1 by keys clamped to
 // [0, kmax] (stream-ordered on s): sorted keys in kout, sorted indices in
 // vout (returned); scratch grows to the sort's temporary storage.
@@ -615,5 +619,8 @@ struct pyr_mesh {
   int push_mfma = 0;
   pyr::DevBuf mf_count;
   int deposit_count = 0;
+  // pinned words and event of the fused steps' per-step flag read-back
+  int *h_step = nullptr;
+  hipEvent_t ev_step = nullptr;
   ~pyr_mesh();
 };
