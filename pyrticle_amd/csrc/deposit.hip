@@ -262,7 +262,9 @@ __global__ __launch_bounds__(256) void k_stage32c(int D, int64_t nch,
                                                   float *__restrict__ wgt_t,
                                                   float4 *__restrict__ meta) {
   const int lane = threadIdx.x & 63;
-  const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  // XCD-chunked blocks: the chunks of one element (consecutive) read the same
+  // lines of x, q and v through one L2
+  const int64_t c = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   if (c >= nch) return;   // uniform per wave
   const uint2 ch = chunks[c];
   const uint32_t s = ch.x, first = ch.y;
