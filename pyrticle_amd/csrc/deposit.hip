@@ -1221,6 +1221,8 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
         cu.ch_meta = c->ch_meta.as<float4>();
         cu.geom = c->geom.as<double>();
         cu.ngsph = c->ngsph.as<float4>();
+        cu.nodes = c->tab64g.as<double4>();
+        cu.node_slot = c->node_slot.as<int>();
         cu.cap_off = cap_off;
         cu.ent = c->ent.as<uint2>();
         cu.ent_off = c->ent_off.as<float4>();

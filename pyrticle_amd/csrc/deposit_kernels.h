@@ -368,6 +368,8 @@ struct CullArgs {
   const float4 *ch_meta;       // per chunk: sphere centre rel. source centroid (3), radius
   const double *geom;          // K x 8: centroid (3), ...
   const float4 *ngsph;         // K x NG: group centre rel. c_e (3), radius
+  const double4 *nodes;        // K x 4 NG slots: node - c_e (3), ... (the FP64 grouped table)
+  const int *node_slot;        // 4 NG: node of each slot, -1 for padding
   const uint32_t *cap_off;     // K + 1: list region of each target
   uint2 *ent;                  // list entries: chunk id, group mask | shift code << 16
   float4 *ent_off;             //               source offset from the target (3), 0
@@ -426,6 +428,17 @@ __global__ __launch_bounds__(256) void k_cull_chunks(CullArgs g) {
     gz[ng] = uni(sp.z);
     gr[ng] = uni((rf + sp.w) * (1.f + CL_MARGIN));
   }
+  // the nodes of every group (FP32 offsets from c_e; padding slots far away)
+  float nx[4 * NG], ny[4 * NG], nz[4 * NG];
+#pragma unroll
+  for (int sl = 0; sl < 4 * NG; ++sl) {
+    const double4 nd = g.nodes[(size_t)e * 4 * NG + sl];
+    const bool pad = g.node_slot[sl] < 0;
+    nx[sl] = uni(pad ? 1e30f : (float)nd.x);
+    ny[sl] = uni(pad ? 1e30f : (float)nd.y);
+    nz[sl] = uni(pad ? 1e30f : (float)nd.z);
+  }
+  const float rn = rf * (1.f + CL_MARGIN);
   const double cx = g.geom[(size_t)e * 8 + 0], cy = g.geom[(size_t)e * 8 + 1],
                cz = D == 3 ? g.geom[(size_t)e * 8 + 2] : 0.0;
   const uint32_t base = g.cap_off[e];
@@ -477,12 +490,24 @@ __global__ __launch_bounds__(256) void k_cull_chunks(CullArgs g) {
       if (f < total) {
         const float4 meta = g.ch_meta[ch];
         const float mw = meta.w * (1.f + CL_MARGIN);
+        const float mx = meta.x + px, my = meta.y + py, mz = D == 3 ? meta.z + pz : 0.f;
+        const float ln = rn + mw;
 #pragma unroll
         for (int ng = 0; ng < NG; ++ng) {
-          const float dx = meta.x + px - gx[ng], dy = meta.y + py - gy[ng],
-                      dz = D == 3 ? meta.z + pz - gz[ng] : 0.f;
+          const float dx = mx - gx[ng], dy = my - gy[ng], dz = mz - gz[ng];
           const float lim = gr[ng] + mw;
-          if (fmaf(dz, dz, fmaf(dy, dy, dx * dx)) <= lim * lim) ngm |= 1u << ng;
+          if (fmaf(dz, dz, fmaf(dy, dy, dx * dx)) > lim * lim) continue;
+          // the group's sphere reaches the chunk's: keep the group when one of
+          // its nodes lies within R of the chunk's sphere (else every node is
+          // farther than R from every particle of the chunk)
+          bool near = false;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int sl = 4 * ng + i;
+            const float ex = mx - nx[sl], ey = my - ny[sl], ez = mz - nz[sl];
+            near |= fmaf(ez, ez, fmaf(ey, ey, ex * ex)) <= ln * ln;
+          }
+          if (near) ngm |= 1u << ng;
         }
       }
       const uint64_t live = __ballot(ngm != 0);
