@@ -195,17 +195,14 @@ __global__ void k_node_table(int D, int K, int np, const double *__restrict__ no
 // node node_slot[4 g + i], in FP32 as (2 (n - c_e) / (R P), -|n - c_e|^2 /
 // (R P)) (padding slots (0, 0, 0, -1): t = c_p - 1 <= 0) and in FP64 as
 // (n - c_e, |n - c_e|^2 / R), the values of k_node_table (padding slots
-// (0, 0, 0, 2 R): t <= -R); ngsph[e NG + g] is the bounding sphere of group g
-// (centre relative to c_e, radius rounded up).
+// (0, 0, 0, 2 R): t <= -R).
 __global__ void k_node_table_g(int D, int K, int np, int ng, const double *__restrict__ nodes,
                                const double *__restrict__ geom, double invR, double inv_p,
                                const int *__restrict__ node_slot, float4 *__restrict__ tab,
-                               float4 *__restrict__ sph, double4 *__restrict__ tab64) {
+                               double4 *__restrict__ tab64) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)K * ng) return;
   const int e = (int)(t / ng), grp = (int)(t % ng);
-  double d[4][3], ctr[3] = {0, 0, 0};
-  int cnt = 0;
   for (int i = 0; i < 4; ++i) {
     const int node = node_slot[4 * grp + i];
     const size_t slot = (size_t)e * 4 * ng + 4 * grp + i;
@@ -215,29 +212,16 @@ __global__ void k_node_table_g(int D, int K, int np, int ng, const double *__res
       tab64[slot] = make_double4(0.0, 0.0, 0.0, 2.0 / invR);
       continue;
     }
-    double s2 = 0;
+    double d[3], s2 = 0;
     for (int k = 0; k < 3; ++k) {
-      d[cnt][k] = k < D ? nodes[((size_t)e * np + node) * D + k] - geom[(size_t)e * 8 + k] : 0.0;
-      s2 += d[cnt][k] * d[cnt][k];
-      ctr[k] += d[cnt][k];
+      d[k] = k < D ? nodes[((size_t)e * np + node) * D + k] - geom[(size_t)e * 8 + k] : 0.0;
+      s2 += d[k] * d[k];
     }
-    tab64[slot] = make_double4(d[cnt][0], d[cnt][1], d[cnt][2], s2 * invR);
+    tab64[slot] = make_double4(d[0], d[1], d[2], s2 * invR);
     const double bs = 2.0 * invR * inv_p;   // B = 2 (n - c_e) / (R P)
-    tab[slot] = make_float4((float)(d[cnt][0] * bs), (float)(d[cnt][1] * bs),
-                            (float)(d[cnt][2] * bs), (float)(-(s2 * invR) * inv_p));
-    ++cnt;
+    tab[slot] = make_float4((float)(d[0] * bs), (float)(d[1] * bs), (float)(d[2] * bs),
+                            (float)(-(s2 * invR) * inv_p));
   }
-  double rad = 0;
-  for (int k = 0; k < 3; ++k) ctr[k] = cnt ? ctr[k] / cnt : 0.0;
-  for (int i = 0; i < cnt; ++i) {
-    double s2 = 0;
-    for (int k = 0; k < 3; ++k) s2 += (d[i][k] - ctr[k]) * (d[i][k] - ctr[k]);
-    rad = fmax(rad, sqrt(s2));
-  }
-  // an empty group (never: the padding is in the last group only) reaches nothing
-  sph[(size_t)e * ng + grp] = cnt ? make_float4((float)ctr[0], (float)ctr[1], (float)ctr[2],
-                                                (float)(rad * (1.0 + 1e-6) + 1e-12))
-                                  : make_float4(0.f, 0.f, 0.f, -1e30f);
 }
 
 // Chunk staging of the cluster gather: chunk c (one wave) = the particles
@@ -440,7 +424,7 @@ struct DepositCache {
   DevBuf keys_a, keys_b, vals_a, vals_b, hist, hist_scan, sort_tmp, seg, tmp, pos, wgt, stats;
   // matrix-core gather over culled chunk lists: targets in Morton order,
   // grouped node tables, per-deposition chunk staging and the lists
-  DevBuf t_order, tab32g, tab64g, ngsph, node_slot, ch_meta, pos32c, wgt32c;
+  DevBuf t_order, tab32g, tab64g, node_slot, ch_meta, pos32c, wgt32c;
   DevBuf cap, ent, ent_off, n_live, items, partial;
   DevBuf ikey, ik_a, ik_b, iv_a, iv_b;   // work items sorted by their first chunk
   DevBuf active;                         // targets that may receive charge
@@ -485,7 +469,7 @@ void pyr_deposit_cache_free(pyr_mesh *mesh) {
                     &c->vals_a, &c->vals_b, &c->skeys, &c->hist, &c->hist_scan,
                     &c->sort_tmp, &c->seg, &c->tmp, &c->pos, &c->wgt, &c->stats,
                     &c->nrm, &c->chunks, &c->t_order, &c->tab32g, &c->tab64g,
-                    &c->ngsph, &c->node_slot, &c->ch_meta, &c->pos32c, &c->wgt32c, &c->cap,
+                    &c->node_slot, &c->ch_meta, &c->pos32c, &c->wgt32c, &c->cap,
                     &c->ent, &c->ent_off, &c->n_live, &c->items, &c->partial, &c->ikey,
                     &c->ik_a, &c->ik_b, &c->iv_a, &c->iv_b, &c->active, &c->list_stats})
     b->release();
@@ -982,11 +966,10 @@ static void prepare_tables(pyr_mesh *mesh, DepositCache *c, double radius) {
     const int ng = (Np + 3) / 4;
     c->tab32g.reserve(sizeof(float4) * (size_t)K * 4 * ng);
     c->tab64g.reserve(sizeof(double4) * (size_t)K * 4 * ng);
-    c->ngsph.reserve(sizeof(float4) * (size_t)K * ng);
     hipLaunchKernelGGL(k_node_table_g, dim3(blocks_for((int64_t)K * ng)), dim3(256), 0, s, D, K,
                        Np, ng, mesh->dev.nodes, c->geom.as<double>(), 1.0 / radius,
                        1.0 / p_scale(radius), c->node_slot.as<int>(), c->tab32g.as<float4>(),
-                       c->ngsph.as<float4>(), c->tab64g.as<double4>());
+                       c->tab64g.as<double4>());
     PYR_CHECK_LAUNCH();
     c->tab_radius = radius;
   }
@@ -1220,7 +1203,6 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
         cu.ch_start = ch_start;
         cu.ch_meta = c->ch_meta.as<float4>();
         cu.geom = c->geom.as<double>();
-        cu.ngsph = c->ngsph.as<float4>();
         cu.nodes = c->tab64g.as<double4>();
         cu.node_slot = c->node_slot.as<int>();
         cu.cap_off = cap_off;

@@ -310,11 +310,11 @@ __global__ __launch_bounds__(64 * ((NP + NPW - 1) / NPW)) void k_gather(GatherAr
 // particle's own element, formed in FP64 and rounded once), the weights
 // w = (q N, q N v) of every aligned quad of particles stored transposed, and
 // a bounding sphere of the chunk.  Empty slots carry zero weights.  The nodes
-// of every target element are grouped spatially in groups of four, each with a
-// bounding sphere (k_node_table_g).
+// of every target element are grouped spatially in groups of four
+// (k_node_table_g).
 //
 // k_cull_chunks (one wave per target): for every chunk of every candidate
-// source element, the groups whose sphere lies within R of the chunk's sphere;
+// source element, the groups with a node within R of the chunk's sphere;
 // the chunks with at least one such group are written, in candidate order, to
 // the target's list (chunk id and group mask; offset of the source centroid,
 // + periodic shift, from the target centroid, formed in FP64 and rounded
@@ -367,7 +367,6 @@ struct CullArgs {
   const uint32_t *ch_start;    // K + 1: chunk range of each source element
   const float4 *ch_meta;       // per chunk: sphere centre rel. source centroid (3), radius
   const double *geom;          // K x 8: centroid (3), ...
-  const float4 *ngsph;         // K x NG: group centre rel. c_e (3), radius
   const double4 *nodes;        // K x 4 NG slots: node - c_e (3), ... (the FP64 grouped table)
   const int *node_slot;        // 4 NG: node of each slot, -1 for padding
   const uint32_t *cap_off;     // K + 1: list region of each target
@@ -419,15 +418,6 @@ __global__ __launch_bounds__(256) void k_cull_chunks(CullArgs g) {
   const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
   auto uni = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
   const float rf = (float)g.R;
-  float gx[NG], gy[NG], gz[NG], gr[NG];
-#pragma unroll
-  for (int ng = 0; ng < NG; ++ng) {
-    const float4 sp = g.ngsph[(size_t)e * NG + ng];
-    gx[ng] = uni(sp.x);
-    gy[ng] = uni(sp.y);
-    gz[ng] = uni(sp.z);
-    gr[ng] = uni((rf + sp.w) * (1.f + CL_MARGIN));
-  }
   // the nodes of every group (FP32 offsets from c_e; padding slots far away)
   float nx[4 * NG], ny[4 * NG], nz[4 * NG];
 #pragma unroll
@@ -492,14 +482,11 @@ __global__ __launch_bounds__(256) void k_cull_chunks(CullArgs g) {
         const float mw = meta.w * (1.f + CL_MARGIN);
         const float mx = meta.x + px, my = meta.y + py, mz = D == 3 ? meta.z + pz : 0.f;
         const float ln = rn + mw;
+        // a group is kept when one of its nodes lies within R of the chunk's
+        // sphere (else every node is farther than R from every particle of
+        // the chunk); all twenty tests run, without branches
 #pragma unroll
         for (int ng = 0; ng < NG; ++ng) {
-          const float dx = mx - gx[ng], dy = my - gy[ng], dz = mz - gz[ng];
-          const float lim = gr[ng] + mw;
-          if (fmaf(dz, dz, fmaf(dy, dy, dx * dx)) > lim * lim) continue;
-          // the group's sphere reaches the chunk's: keep the group when one of
-          // its nodes lies within R of the chunk's sphere (else every node is
-          // farther than R from every particle of the chunk)
           bool near = false;
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
