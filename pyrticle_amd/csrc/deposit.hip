@@ -1203,8 +1203,9 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
         cu.ch_start = ch_start;
         cu.ch_meta = c->ch_meta.as<float4>();
         cu.geom = c->geom.as<double>();
-        cu.nodes = c->tab64g.as<double4>();
+        cu.ntab = c->tab32g.as<float4>();
         cu.node_slot = c->node_slot.as<int>();
+        cu.inv_rp = (float)(1.0 / (radius * p_scale(radius)));
         cu.cap_off = cap_off;
         cu.ent = c->ent.as<uint2>();
         cu.ent_off = c->ent_off.as<float4>();

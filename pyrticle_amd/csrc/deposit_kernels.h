@@ -367,8 +367,9 @@ struct CullArgs {
   const uint32_t *ch_start;    // K + 1: chunk range of each source element
   const float4 *ch_meta;       // per chunk: sphere centre rel. source centroid (3), radius
   const double *geom;          // K x 8: centroid (3), ...
-  const double4 *nodes;        // K x 4 NG slots: node - c_e (3), ... (the FP64 grouped table)
+  const float4 *ntab;          // K x 4 NG slots: 2 (n - c_e) / (R P) (3), -|n - c_e|^2 / (R P)
   const int *node_slot;        // 4 NG: node of each slot, -1 for padding
+  float inv_rp;                // 1 / (R P)
   const uint32_t *cap_off;     // K + 1: list region of each target
   uint2 *ent;                  // list entries: chunk id, group mask | shift code << 16
   float4 *ent_off;             //               source offset from the target (3), 0
@@ -418,15 +419,20 @@ __global__ __launch_bounds__(256) void k_cull_chunks(CullArgs g) {
   const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
   auto uni = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
   const float rf = (float)g.R;
-  // the nodes of every group (FP32 offsets from c_e; padding slots far away)
-  float nx[4 * NG], ny[4 * NG], nz[4 * NG];
+  // the nodes of every group from the gather's table, (2 n, -|n|^2) / (R P)
+  // with n = node - c_e: a node lies within L of m when
+  // (2 m . n - |n|^2) / (R P) >= (|m|^2 - L^2) / (R P) (the expanded form
+  // rounds to ~1e-8 of the squared distances here, far inside the margin's
+  // slack 2 CL_MARGIN (R + r)^2); padding slots never pass
+  float nx[4 * NG], ny[4 * NG], nz[4 * NG], nn[4 * NG];
 #pragma unroll
   for (int sl = 0; sl < 4 * NG; ++sl) {
-    const double4 nd = g.nodes[(size_t)e * 4 * NG + sl];
+    const float4 nd = g.ntab[(size_t)e * 4 * NG + sl];
     const bool pad = g.node_slot[sl] < 0;
-    nx[sl] = uni(pad ? 1e30f : (float)nd.x);
-    ny[sl] = uni(pad ? 1e30f : (float)nd.y);
-    nz[sl] = uni(pad ? 1e30f : (float)nd.z);
+    nx[sl] = uni(nd.x);
+    ny[sl] = uni(nd.y);
+    nz[sl] = uni(nd.z);
+    nn[sl] = uni(pad ? -INFINITY : nd.w);
   }
   const float rn = rf * (1.f + CL_MARGIN);
   const double cx = g.geom[(size_t)e * 8 + 0], cy = g.geom[(size_t)e * 8 + 1],
@@ -482,6 +488,7 @@ __global__ __launch_bounds__(256) void k_cull_chunks(CullArgs g) {
         const float mw = meta.w * (1.f + CL_MARGIN);
         const float mx = meta.x + px, my = meta.y + py, mz = D == 3 ? meta.z + pz : 0.f;
         const float ln = rn + mw;
+        const float rhs = fmaf(-ln, ln, fmaf(mz, mz, fmaf(my, my, mx * mx))) * g.inv_rp;
         // a group is kept when one of its nodes lies within R of the chunk's
         // sphere (else every node is farther than R from every particle of
         // the chunk); all twenty tests run, without branches
@@ -491,8 +498,7 @@ __global__ __launch_bounds__(256) void k_cull_chunks(CullArgs g) {
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const int sl = 4 * ng + i;
-            const float ex = mx - nx[sl], ey = my - ny[sl], ez = mz - nz[sl];
-            near |= fmaf(ez, ez, fmaf(ey, ey, ex * ex)) <= ln * ln;
+            near |= fmaf(nz[sl], mz, fmaf(ny[sl], my, fmaf(nx[sl], mx, nn[sl]))) >= rhs;
           }
           if (near) ngm |= 1u << ng;
         }
