@@ -1051,7 +1051,7 @@ def main(argv=None):
                    "elements": K, "particles": st0["n"],
                    "precision": ("fp32c: FP32 interpolation, shape evaluation and deposition "
                                  "products, FP32 partial sums folded into FP64 node sums "
-                                 "every 32 chunks; FP64 positions, momenta, face tests and "
+                                 "every 64 chunks; FP64 positions, momenta, face tests and "
                                  "charge normalization" if headline32 else "fp64")},
         "roofline": roofline,
         "roofline_push": roof_push,
