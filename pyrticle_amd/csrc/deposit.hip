@@ -287,6 +287,49 @@ __global__ __launch_bounds__(256) void k_stage32c(int D, int64_t nch,
   if (lane == 0) meta[c] = make_float4(ctr[0], ctr[1], ctr[2], sqrtf(r2) * (1.f + 1e-5f) + 1e-30f);
 }
 
+// Operand staging of the FP64 matrix-core gather, chunk c (one wave) = the
+// FP64-staged particles [first, first + count) of source element s, four
+// slices of 16 (slot 16 sl + r):
+//   pos_t[256 c + 64 sl + l] = row l / 16 of the slice's particle l % 16:
+//                              x - c_s (3), then -|x - c_s|^2 / R;
+//   wgt_t[256 c + 64 sl + l] = component l % 4 of particle
+//                              4 ((l / 4) % 4) + l / 16: (a, a v);
+// zeros for padding slots (slot >= count).
+__global__ __launch_bounds__(256) void k_stage64t(int64_t nch, const uint2 *__restrict__ chunks,
+                                                  const uint32_t *__restrict__ seg,
+                                                  const double4 *__restrict__ pos,
+                                                  const double4 *__restrict__ wgt,
+                                                  const double *__restrict__ geom, double invR,
+                                                  double *__restrict__ pos_t,
+                                                  double *__restrict__ wgt_t) {
+  const int lane = threadIdx.x & 63;
+  const int64_t c = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  if (c >= nch) return;   // uniform per wave
+  const uint2 ch = chunks[c];
+  const uint32_t s = ch.x, first = ch.y;
+  const uint32_t count = min(64u, seg[s + 1] - first);
+  const double sx = geom[(size_t)s * 8 + 0], sy = geom[(size_t)s * 8 + 1],
+               sz = geom[(size_t)s * 8 + 2];
+  const int kc = lane >> 4, vc = lane & 3;
+  for (int sl = 0; sl < 4; ++sl) {
+    const uint32_t pa = 16 * sl + (lane & 15);
+    double pv = 0.0;
+    if (pa < count) {
+      const double4 x = pos[first + pa];
+      const double dx = x.x - sx, dy = x.y - sy, dz = x.z - sz;
+      pv = kc == 0 ? dx : (kc == 1 ? dy : (kc == 2 ? dz : -fma(dz, dz, fma(dy, dy, dx * dx)) * invR));
+    }
+    pos_t[(size_t)c * 256 + 64 * sl + lane] = pv;
+    const uint32_t pb = 16 * sl + 4 * ((lane >> 2) & 3) + (lane >> 4);
+    double wv = 0.0;
+    if (pb < count) {
+      const double4 w = wgt[first + pb];
+      wv = vc == 0 ? w.x : (vc == 1 ? w.y : (vc == 2 ? w.z : w.w));
+    }
+    wgt_t[(size_t)c * 256 + 64 * sl + lane] = wv;
+  }
+}
+
 // rho and j of every target from its work items' partial sums, added in item
 // order (FP64): one thread per (target, node slot).
 // Statistics of the chunk lists of the last FP32C deposition, per target:
@@ -426,6 +469,7 @@ struct DepositCache {
   // grouped node tables, per-deposition chunk staging and the lists
   DevBuf t_order, tab32g, tab64g, node_slot, ch_meta, pos32c, wgt32c;
   DevBuf cap, ent, ent_off, n_live, items, partial;
+  DevBuf pos64t, wgt64t;   // operands of the FP64 matrix-core gather
   DevBuf ikey, ik_a, ik_b, iv_a, iv_b;   // work items sorted by their first chunk
   DevBuf active;                         // targets that may receive charge
   int64_t max_indeg = 0;   // most targets whose candidate list holds one source
@@ -471,6 +515,7 @@ void pyr_deposit_cache_free(pyr_mesh *mesh) {
                     &c->nrm, &c->chunks, &c->t_order, &c->tab32g, &c->tab64g,
                     &c->node_slot, &c->ch_meta, &c->pos32c, &c->wgt32c, &c->cap,
                     &c->ent, &c->ent_off, &c->n_live, &c->items, &c->partial, &c->ikey,
+                    &c->pos64t, &c->wgt64t,
                     &c->ik_a, &c->ik_b, &c->iv_a, &c->iv_b, &c->active, &c->list_stats})
     b->release();
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
@@ -1282,29 +1327,32 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
         if (n_items && precision == PYR_FP32C) {
           launch_gather_cl(D, Np, la, n_items, alpha == 2.0);
         } else if (n_items) {
-          ListArgs64 l64{};
-          l64.K = K;
-          l64.Np = Np;
-          l64.R = radius;
-          l64.invR = 1.0 / radius;
-          l64.alpha = alpha;
-          for (int a = 0; a < 3; ++a) l64.shift_len[a] = ga.shift_len[a];
-          l64.items = la.items;
-          l64.item_order = la.item_order;
-          l64.n_items = la.n_items;
-          l64.cap_off = cap_off;
-          l64.n_live = la.n_live;
-          l64.ent = la.ent;
-          l64.chunks = cl.chunks;
-          l64.seg = c->seg.as<uint32_t>();
-          l64.pos = c->pos.as<double4>();
-          l64.wgt = c->wgt.as<double4>();
-          l64.geom = c->geom.as<double>();
-          l64.face_normal = mesh->dev.face_normal;
-          l64.face_rhs = mesh->dev.face_rhs;
-          l64.node_tab = c->tab64g.as<double4>();
-          l64.partial = c->partial.as<double>();
-          launch_gather_cl64(D, Np, l64, n_items, shape_unit_square(alpha, radius));
+          c->pos64t.reserve(sizeof(double) * 256 * (size_t)std::max<int64_t>(1, nch));
+          c->wgt64t.reserve(sizeof(double) * 256 * (size_t)std::max<int64_t>(1, nch));
+          hipLaunchKernelGGL(k_stage64t, dim3((unsigned)((nch + 3) / 4)), dim3(256), 0, s, nch,
+                             cl.chunks, c->seg.as<uint32_t>(), c->pos.as<double4>(),
+                             c->wgt.as<double4>(), c->geom.as<double>(), 1.0 / radius,
+                             c->pos64t.as<double>(), c->wgt64t.as<double>());
+          PYR_CHECK_LAUNCH();
+          ListArgsM64 lm{};
+          lm.K = K;
+          lm.R = radius;
+          lm.invR = 1.0 / radius;
+          lm.alpha = alpha;
+          for (int a = 0; a < 3; ++a) lm.shift_len[a] = ga.shift_len[a];
+          lm.items = la.items;
+          lm.item_order = la.item_order;
+          lm.n_items = la.n_items;
+          lm.cap_off = cap_off;
+          lm.n_live = la.n_live;
+          lm.ent = la.ent;
+          lm.chunks = cl.chunks;
+          lm.geom = c->geom.as<double>();
+          lm.node_tab = c->tab64g.as<double4>();
+          lm.pos_t = c->pos64t.as<double>();
+          lm.wgt_t = c->wgt64t.as<double>();
+          lm.partial = c->partial.as<double>();
+          launch_gather_mf64(D, Np, lm, n_items, shape_unit_square(alpha, radius));
         }
       }
       ProfScope prof(PROF_SORT);       // with the other per-deposition set-up work

@@ -742,23 +742,28 @@ __global__ __launch_bounds__(64 * W) void k_gather_cl(ListArgs g) {
 }
 
 // ---------------------------------------------------------------------------
-// FP64 gather over the culled chunk lists
+// FP64 gather over the culled chunk lists on the FP64 matrix cores
 // ---------------------------------------------------------------------------
-// The FP32C lists are conservative for FP64 too: a chunk is left out of a
-// target's list only when every node of the target lies farther than R from
-// all of its particles by more than FP32 rounding, so the FP64 shape there is
-// exactly zero.  One block per work item (at most 64 entries of one target's
-// list); each wave owns SPW node slots (the grouped FP64 node table).  Per iteration every wave takes one list entry, one chunk
-// particle per lane (FP64 staging of the element-sorted order, the entry's
-// periodic shift added); the particles whose ball can reach the target
-// (bounding sphere of the nodes inflated by R, face planes moved out by R)
-// are queued in LDS and evaluated in batches of 64, one particle per lane,
-// against each wave's slots (node data through scalar loads), in the
-// arithmetic of k_gather.  The item's sums are reduced across lanes in a fixed
-// tree and written as FP64 partials in slot order; k_combine_items adds them
-// per target in item order, so results are bitwise reproducible.
-struct ListArgs64 {
-  int K, Np;
+// v_mfma_f64_4x4x4_4b_f64 holds four blocks b (lane bits 2-3), each
+// D = A B + C of 4x4 matrices with A[i][k] in lane 16 k + 4 b + i, B[k][j] in
+// lane 16 k + 4 b + j and C, D[i][j] in lane 16 i + 4 b + j.  Per list entry
+// (one 64-particle chunk, four slices of 16) and node group (four nodes j):
+//   t  = C + A B with A[i][k] = (p_i (3), -|p_i|^2 / R), p = x - c_src (the
+//        slice's particle 4 b + i, staged), B[k][j] = (b_j - 2 o / R, 1),
+//        b = 2 (n_j - c_e) / R, o = c_src + shift - c_e, and
+//        C[i][j] = R - |(n_j - c_e) - o|^2 / R, so that t is
+//        R - |n_j - (x + shift)|^2 / R expanded around the source centroid
+//        (C from the node's own offset to that centroid: the same cancellation
+//        as the distance itself);
+//   S  = max(t, 0)^alpha, elementwise: D's layout is the A layout of
+//   acc[j][v] += sum_i S[i][j] w_i[v]  (A = S, B[i][v] = component v of the
+//        block's particle i, staged transposed), one product per slice.
+// Each block sums its own particles; the four blocks are added at the end in a
+// fixed butterfly and the item's FP64 sums written as partials, which
+// k_combine_items adds per target in item order.  A group an entry's mask
+// leaves out has t < 0 for all of the chunk's particles and is skipped.
+struct ListArgsM64 {
+  int K;
   double R, invR, alpha;
   double shift_len[3];
   const uint2 *items;          // work items: target, first list entry
@@ -768,214 +773,131 @@ struct ListArgs64 {
   const uint32_t *n_live;      // K
   const uint2 *ent;            // list entries: chunk id, group mask | shift code << 16
   const uint2 *chunks;         // per chunk: source element, sorted index of its first particle
-  const uint32_t *seg;         // K + 1: sorted range of each source element
-  const double4 *pos;          // FP64 staging: x (3), 0
-  const double4 *wgt;          //               (q N, q N v)
-  const double *geom;          // K x 8: centroid (3), (R + node radius)^2, ...
-  const double *face_normal;
-  const double *face_rhs;
-  const double4 *node_tab;     // K x 4 NG slots: n - c, |n - c|^2 / R (padding: t <= -R)
+  const double *geom;          // K x 8: centroid (3), ...
+  const double4 *node_tab;     // K x 4 NG slots: n - c_e (3), |n - c_e|^2 / R (padding: 2 R)
+  const double *pos_t;         // 256 per chunk: the A operands of t, slice by slice
+  const double *wgt_t;         // 256 per chunk: the B operands of the sums, slice by slice
   double *partial;             // per item: 4 NG slots x 4 components
 };
 
-template <int D, int NP, int SPW, bool A2>
-__global__ __launch_bounds__(64 * ((4 * ((NP + 3) / 4)) / SPW)) void k_gather_cl64(ListArgs64 g) {
-  using T = double;
-  constexpr int NS = 4 * ((NP + 3) / 4);
-  constexpr int G = NS / SPW;               // waves per block
-  static_assert(G * SPW == NS, "slots per wave must divide the slot count");
-  constexpr int BS = 64 * G;
-  constexpr int QCAP = 64 + BS;             // ring buffer of queued particles
-  constexpr int NV = D + 1;                 // rho + j components
-  constexpr int REC = 8;                    // record: m (3), c, a, b (3)
+__device__ __forceinline__ double mfma64(double a, double b, double c) {
+  return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
+}
 
-  const int64_t k_item = xcd_element(blockIdx.x);
-  if (k_item >= (int64_t)*g.n_items) return;   // block-uniform
+template <int D, int NP, int W, bool A2>
+__global__ __launch_bounds__(64 * W) void k_gather_mf64(ListArgsM64 g) {
+  constexpr int NG = (NP + 3) / 4;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int slot0 = wave * SPW;
-  const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
-  const uint32_t item = g.item_order[k_item];
+  const int64_t k_item = (int64_t)xcd_element(blockIdx.x) * W + wave;
+  if (k_item >= (int64_t)*g.n_items) return;   // wave-uniform
+  const uint32_t item = __builtin_amdgcn_readfirstlane(g.item_order[k_item]);
   const uint2 it = g.items[item];
-  const uint32_t e = it.x;
-  const uint32_t base = g.cap_off[e] + it.y;
-  const uint32_t cnt = min(64u, g.n_live[e] - it.y);
+  const uint32_t e = __builtin_amdgcn_readfirstlane(it.x);
+  const uint32_t first = __builtin_amdgcn_readfirstlane(it.y);
+  const uint32_t base = __builtin_amdgcn_readfirstlane(g.cap_off[e] + first);
+  const uint32_t cnt = __builtin_amdgcn_readfirstlane(min(64u, g.n_live[e] - first));
+  const int kc = lane >> 4;   // row k of the lane's A and B operands of t
+  const double R = g.R, invR = g.invR, m2r = -2.0 * g.invR;
 
-  __shared__ T q_rec[QCAP][REC];
-  __shared__ int wave_pass[G];
-
-  const double cx = g.geom[(size_t)e * 8 + 0], cy = g.geom[(size_t)e * 8 + 1],
-               cz = g.geom[(size_t)e * 8 + 2], bound2 = g.geom[(size_t)e * 8 + 3];
-  double fnl[D + 1][D], frl[D + 1];
+  // the lane's node (column j = lane % 4) of every group: n - c_e (padding
+  // slots at infinity: C = -inf, so t = -inf and S = 0 exactly) and its row
+  // of B before the entry's shift (b[k] = 2 (n - c_e)[k] / R, 1 for k = 3)
+  double nx[NG], ny[NG], nz[NG], bk[NG];
 #pragma unroll
-  for (int f = 0; f <= D; ++f) {
-#pragma unroll
-    for (int k = 0; k < D; ++k) fnl[f][k] = g.face_normal[((size_t)e * (D + 1) + f) * D + k];
-    frl[f] = g.face_rhs[(size_t)e * (D + 1) + f] + g.R * (1.0 + 1e-12);
+  for (int ng = 0; ng < NG; ++ng) {
+    const double4 nt = g.node_tab[(size_t)e * (4 * NG) + 4 * ng + (lane & 3)];
+    const bool pad = nt.w > 0.0 && nt.x == 0.0 && nt.y == 0.0 && nt.z == 0.0;
+    nx[ng] = pad ? INFINITY : nt.x;
+    ny[ng] = nt.y;
+    nz[ng] = D == 3 ? nt.z : 0.0;
+    bk[ng] = kc == 0 ? 2.0 * invR * nt.x
+                     : (kc == 1 ? 2.0 * invR * nt.y : (kc == 2 ? 2.0 * invR * nz[ng] : 1.0));
   }
-  using CT = const T __attribute__((address_space(4))) *;
-  const CT ntab = (CT)g.node_tab + ((size_t)e * NS + slot0) * 4;
+  const auto s_geom = as_constant(g.geom);
+  const double cx = s_geom[(size_t)e * 8 + 0], cy = s_geom[(size_t)e * 8 + 1],
+               cz = D == 3 ? s_geom[(size_t)e * 8 + 2] : 0.0;
+  const auto s_ent = as_constant(reinterpret_cast<const uint32_t *>(g.ent + base));
+  const auto s_chk = as_constant(reinterpret_cast<const uint32_t *>(g.chunks));
 
-  T acc[SPW][NV];
+  double acc[NG];
 #pragma unroll
-  for (int i = 0; i < SPW; ++i)
-#pragma unroll
-    for (int v = 0; v < NV; ++v) acc[i][v] = T(0);
-  int q_head = 0, q_count = 0;   // identical in every thread of the block
+  for (int ng = 0; ng < NG; ++ng) acc[ng] = 0.0;
 
-  auto process_batch = [&](int count) {
-    if (lane < count) {
-      int slot = q_head + lane;
-      if (slot >= QCAP) slot -= QCAP;
-      T rec[REC];
+  struct Entry {
+    uint32_t mask;
+    double ox, oy, oz;
+    double p[4], w[4];
+  };
+  // entry i (clamped into the item: loads past the end are never evaluated)
+  auto fetch = [&](Entry &en, uint32_t i) {
+    const uint32_t l = min(i, cnt - 1);
+    const uint32_t ch = s_ent[2 * l], y = s_ent[2 * l + 1];
+    const uint32_t src = s_chk[2 * (size_t)ch];
+    en.mask = y & ENT_MASK;
+    const uint32_t code = y >> ENT_SHIFT_BIT;
+    double sh[3];
 #pragma unroll
-      for (int k = 0; k < REC; ++k) rec[k] = q_rec[slot][k];
+    for (int a = 0; a < 3; ++a) {
+      const uint32_t c2 = (code >> (2 * a)) & 3u;
+      sh[a] = a < D ? (c2 == 1 ? g.shift_len[a] : (c2 == 2 ? -g.shift_len[a] : 0.0)) : 0.0;
+    }
+    en.ox = (s_geom[(size_t)src * 8 + 0] + sh[0]) - cx;
+    en.oy = (s_geom[(size_t)src * 8 + 1] + sh[1]) - cy;
+    en.oz = D == 3 ? (s_geom[(size_t)src * 8 + 2] + sh[2]) - cz : 0.0;
+    const double *pp = g.pos_t + (size_t)ch * 256 + lane;
+    const double *ww = g.wgt_t + (size_t)ch * 256 + lane;
 #pragma unroll
-      for (int i = 0; i < SPW; ++i) {
-        const CT nd = ntab + i * 4;
-        T t = rec[3] - nd[3];
-#pragma unroll
-        for (int k = D - 1; k >= 0; --k) t = fma(nd[k], rec[k], t);
-        T s;
-        if constexpr (A2) {
-          s = sq_pos_unit(t);
-        } else {
-          const T w = t > T(0) ? t : T(0);
-          s = g.alpha == 2.0 ? w * w : (T)pow((double)w, g.alpha);
-        }
-        acc[i][0] = fma(rec[4], s, acc[i][0]);
-#pragma unroll
-        for (int k = 0; k < D; ++k) acc[i][1 + k] = fma(rec[5 + k], s, acc[i][1 + k]);
-      }
+    for (int sl = 0; sl < 4; ++sl) {
+      en.p[sl] = pp[64 * sl];
+      en.w[sl] = ww[64 * sl];
     }
   };
-
-  // the item's list entries, one per lane, resolved once: the sorted index
-  // of the chunk's first particle, its particle count, the shift code
-  uint32_t e_first = 0, e_count = 0, e_code = 0;
-  if ((uint32_t)lane < cnt) {
-    const uint2 en = g.ent[base + lane];
-    const uint2 ck = g.chunks[en.x];   // source element, first sorted index
-    e_first = ck.y;
-    e_count = min(64u, g.seg[ck.x + 1] - ck.y);
-    e_code = en.y >> ENT_SHIFT_BIT;
-  }
-
-  // the loads of list entry ei for this wave, one chunk particle per lane.
-  // Issued on every call with an in-bounds index (lanes past the chunk's
-  // count, and entries past the item's end, re-read a staged particle and are
-  // dropped by count = 0 or lane >= count), so the same number of loads is in
-  // flight on every path and the wait before an entry's use does not cover
-  // the entry fetched after it.
-  struct Pre {
-    double4 pp, ww;
-    uint32_t count, code;
-  };
-  auto fetch = [&](Pre &d, uint32_t ei) {
-    const int l = (int)min(ei, 63u);
-    const uint32_t s0 = (uint32_t)__builtin_amdgcn_readlane((int)e_first, l);
-    const uint32_t ce = (uint32_t)__builtin_amdgcn_readlane((int)e_count, l);
-    d.code = (uint32_t)__builtin_amdgcn_readlane((int)e_code, l);
-    d.count = ei < cnt ? ce : 0u;
-    const uint32_t idx = s0 + min((uint32_t)lane, ce > 0u ? ce - 1u : 0u);
-    d.pp = g.pos[idx];
-    d.ww = g.wgt[idx];
-  };
-  // prefilter and queue the fetched entry (all waves of the block take part)
-  auto step = [&](const Pre &d) {
-    bool pass = false;
-    T rec[REC];
-    if ((uint32_t)lane < d.count) {
-      double sh[3] = {0, 0, 0};
-#pragma unroll
-      for (int a = 0; a < D; ++a) {
-        const uint32_t c2 = (d.code >> (2 * a)) & 3u;
-        sh[a] = c2 == 1 ? g.shift_len[a] : (c2 == 2 ? -g.shift_len[a] : 0.0);
-      }
-      const double x[3] = {d.pp.x + sh[0], d.pp.y + sh[1], D == 3 ? d.pp.z + sh[2] : 0.0};
-      const double dl[3] = {x[0] - cx, x[1] - cy, x[2] - cz};
-      const double dd = fma(dl[0], dl[0], fma(dl[1], dl[1], D == 3 ? dl[2] * dl[2] : 0.0));
-      double viol = -INFINITY;
-#pragma unroll
-      for (int f = 0; f <= D; ++f) {
-        double ip = fnl[f][0] * x[0];
-#pragma unroll
-        for (int kk = 1; kk < D; ++kk) ip = fma(fnl[f][kk], x[kk], ip);
-        viol = fmax(viol, ip - frl[f]);
-      }
-      pass = (dd <= bound2) & (viol <= 0.0);
-      if (pass) {
-        const double two_inv_r = 2.0 * g.invR;
-        rec[0] = (T)(dl[0] * two_inv_r);
-        rec[1] = (T)(dl[1] * two_inv_r);
-        rec[2] = (T)(dl[2] * two_inv_r);
-        rec[3] = (T)fma(-dd, g.invR, g.R);
-        rec[4] = (T)d.ww.x;
-        rec[5] = (T)d.ww.y;
-        rec[6] = (T)d.ww.z;
-        rec[7] = (T)d.ww.w;
-      }
-    }
-    const uint64_t bal = __ballot(pass);
-    if (G > 1) {
-      if (lane == 0) wave_pass[wave] = __popcll(bal);
-      __syncthreads();
-    }
-    int off = q_count, added = 0;
-    if (G > 1) {
-#pragma unroll
-      for (int w = 0; w < G; ++w) {
-        if (w < wave) off += wave_pass[w];
-        added += wave_pass[w];
-      }
+  auto shape = [&](double t) {
+    if constexpr (A2) {
+      return sq_pos_unit(t);
     } else {
-      added = __popcll(bal);
+      const double w = t > 0.0 ? t : 0.0;
+      return g.alpha == 2.0 ? w * w : pow(w, g.alpha);
     }
-    if (pass) {
-      int slot = q_head + off + __popcll(bal & lt_mask);
-      if (slot >= QCAP) slot -= QCAP;
+  };
+  auto evaluate = [&](const Entry &en) {
+    const double ok = kc == 0 ? en.ox : (kc == 1 ? en.oy : (kc == 2 ? en.oz : 0.0));
 #pragma unroll
-      for (int kk = 0; kk < REC; ++kk) q_rec[slot][kk] = rec[kk];
-    }
-    q_count += added;
-    __syncthreads();
-    while (q_count >= 64) {
-      process_batch(64);
-      q_head += 64;
-      if (q_head >= QCAP) q_head -= QCAP;
-      q_count -= 64;
-      __syncthreads();   // the freed slots may be refilled next
+    for (int ng = 0; ng < NG; ++ng) {
+      if (!((en.mask >> ng) & 1u)) continue;
+      const double b = fma(m2r, ok, bk[ng]);
+      const double dx = nx[ng] - en.ox, dy = ny[ng] - en.oy, dz = nz[ng] - en.oz;
+      const double c = fma(-fma(dz, dz, fma(dy, dy, dx * dx)), invR, R);
+      double sv[4];
+#pragma unroll
+      for (int sl = 0; sl < 4; ++sl) sv[sl] = shape(mfma64(en.p[sl], b, c));
+#pragma unroll
+      for (int sl = 0; sl < 4; ++sl) acc[ng] = mfma64(sv[sl], en.w[sl], acc[ng]);
     }
   };
 
-  // wave w takes entries w, w + G, w + 2 G, ...; two named buffers, the loads
-  // of the next entry in flight while one is queued and evaluated
-  Pre pa, pb;
-  fetch(pa, (uint32_t)wave);
+  // two named buffers: the loads of the next entry in flight while one is
+  // evaluated
+  Entry ea, eb;
+  fetch(ea, 0);
 #pragma unroll 1
-  for (uint32_t i0 = 0; i0 < cnt; i0 += 2 * G) {
-    fetch(pb, i0 + G + (uint32_t)wave);
-    __builtin_amdgcn_sched_barrier(0);
-    step(pa);
-    __builtin_amdgcn_sched_barrier(0);
-    fetch(pa, i0 + 2 * G + (uint32_t)wave);
-    __builtin_amdgcn_sched_barrier(0);
-    step(pb);
-    __builtin_amdgcn_sched_barrier(0);
+  for (uint32_t i = 0; i < cnt; i += 2) {
+    fetch(eb, i + 1);
+    evaluate(ea);
+    fetch(ea, i + 2);
+    if (i + 1 < cnt) evaluate(eb);
   }
-  if (q_count > 0) process_batch(q_count);
 
-  // reduce across lanes in a fixed tree; lane 0 writes this wave's slots
-  double *dst = g.partial + (size_t)item * NS * 4;
+  // blocks (lane bits 2-3) added in a fixed butterfly; lanes 16 i + v then hold
+  // node slot 4 ng + i, component v
+  double *dst = g.partial + (size_t)item * (16 * NG);
 #pragma unroll
-  for (int i = 0; i < SPW; ++i) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      double val = v < NV ? acc[i][v < NV ? v : 0] : 0.0;
-      if (v < NV)
-        for (int o = 32; o > 0; o >>= 1) val += __shfl_xor(val, o, 64);
-      if (lane == 0) dst[(slot0 + i) * 4 + v] = val;
-    }
+  for (int ng = 0; ng < NG; ++ng) {
+    double v = acc[ng];
+    v += __shfl_xor(v, 4, 64);
+    v += __shfl_xor(v, 8, 64);
+    if ((lane & 12) == 0) dst[(4 * ng + (lane >> 4)) * 4 + (lane & 3)] = v;
   }
 }
 
@@ -1357,7 +1279,7 @@ __global__ __launch_bounds__(64) void k_shape_average(AverageArgs g) {
 void launch_gather_fp64(int D, int Np, const GatherArgs &ga, int K, bool a2, bool count);
 void launch_cull_chunks(int D, int Np, const CullArgs &ca);
 void launch_gather_cl(int D, int Np, const ListArgs &la, uint32_t n_items, bool a2);
-void launch_gather_cl64(int D, int Np, const ListArgs64 &la, uint32_t n_items, bool a2);
+void launch_gather_mf64(int D, int Np, const ListArgsM64 &la, uint32_t n_items, bool a2);
 void launch_deposit_adj(int D, int Np, const AdjArgs &ga, unsigned grid, int precision,
                         bool a2);
 void launch_shape_integral(int D, int Np, const IntegralArgs &ga, unsigned grid, bool a2);

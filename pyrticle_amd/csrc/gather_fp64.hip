@@ -1,7 +1,9 @@
-// gather_fp64.hip -- instantiations of the FP64 deposition gather (k_gather).
-// Nodes per wave: the whole element in 2D; 10 (P3) and 12 (P4) in 3D, the
-// fastest measured on the cylinder workloads (the result does not depend on
-// it beyond the fixed summation order across waves).
+// gather_fp64.hip -- instantiations of the FP64 deposition gathers: the
+// list gather on the FP64 matrix cores (k_gather_mf64) and the candidate
+// gather of the pair-counting mode (k_gather; nodes per wave: the whole
+// element in 2D, 10 (P3) and 12 (P4) in 3D, the fastest measured on the
+// cylinder workloads; the result does not depend on it beyond the fixed
+// summation order across waves).
 #include "deposit_kernels.h"
 
 namespace pyr {
@@ -40,32 +42,31 @@ void launch_gather_fp64(int D, int Np, const GatherArgs &ga, int K, bool a2, boo
   throw Error(PYR_ERR_INVALID_ARG, "deposition supports P1..P4 (2D) and P1..P4 (3D) elements");
 }
 
-// the list gather: node slots per wave (4 NG slots in all): the whole element
-// up to 16 slots, 10 for P3 and 12 for P4 tetrahedra
-template <int D, int NP, int SPW>
-static void launch_cl64(const ListArgs64 &la, uint32_t n_items, bool a2) {
-  constexpr int G = (4 * ((NP + 3) / 4)) / SPW;
-  dim3 grid(gather_grid((int)n_items)), block(64 * G);
+// the FP64 matrix-core list gather: MF64_W work items (waves) per workgroup
+constexpr int MF64_W = 4;
+template <int D, int NP>
+static void launch_mf64(const ListArgsM64 &la, uint32_t n_items, bool a2) {
+  dim3 grid(gather_grid((int)((n_items + MF64_W - 1) / MF64_W))), block(64 * MF64_W);
   hipStream_t s = stream();
-  if (a2) hipLaunchKernelGGL((k_gather_cl64<D, NP, SPW, true>), grid, block, 0, s, la);
-  else hipLaunchKernelGGL((k_gather_cl64<D, NP, SPW, false>), grid, block, 0, s, la);
+  if (a2) hipLaunchKernelGGL((k_gather_mf64<D, NP, MF64_W, true>), grid, block, 0, s, la);
+  else hipLaunchKernelGGL((k_gather_mf64<D, NP, MF64_W, false>), grid, block, 0, s, la);
   PYR_CHECK_LAUNCH();
 }
 
-void launch_gather_cl64(int D, int Np, const ListArgs64 &la, uint32_t n_items, bool a2) {
+void launch_gather_mf64(int D, int Np, const ListArgsM64 &la, uint32_t n_items, bool a2) {
   if (D == 2) {
     switch (Np) {
-      case 3: return launch_cl64<2, 3, 4>(la, n_items, a2);
-      case 6: return launch_cl64<2, 6, 8>(la, n_items, a2);
-      case 10: return launch_cl64<2, 10, 12>(la, n_items, a2);
-      case 15: return launch_cl64<2, 15, 16>(la, n_items, a2);
+      case 3: return launch_mf64<2, 3>(la, n_items, a2);
+      case 6: return launch_mf64<2, 6>(la, n_items, a2);
+      case 10: return launch_mf64<2, 10>(la, n_items, a2);
+      case 15: return launch_mf64<2, 15>(la, n_items, a2);
     }
   } else {
     switch (Np) {
-      case 4: return launch_cl64<3, 4, 4>(la, n_items, a2);
-      case 10: return launch_cl64<3, 10, 12>(la, n_items, a2);
-      case 20: return launch_cl64<3, 20, 10>(la, n_items, a2);
-      case 35: return launch_cl64<3, 35, 12>(la, n_items, a2);
+      case 4: return launch_mf64<3, 4>(la, n_items, a2);
+      case 10: return launch_mf64<3, 10>(la, n_items, a2);
+      case 20: return launch_mf64<3, 20>(la, n_items, a2);
+      case 35: return launch_mf64<3, 35>(la, n_items, a2);
     }
   }
   throw Error(PYR_ERR_INVALID_ARG, "deposition supports P1..P4 (2D) and P1..P4 (3D) elements");
