@@ -304,7 +304,7 @@ PROFILE_COUNTERS = ("FETCH_SIZE", "WRITE_SIZE", "TCC_HIT_sum TCC_MISS_sum", "VAL
 TRAFFIC_KERNELS = {
     "k_gather_cl": r"k_gather_cl<", "k_cull_chunks": r"k_cull_chunks<",
     "k_stage32c": r"k_stage32c\b", "k_combine_items": r"k_combine_items\b",
-    "k_gather_mf64": r"k_gather_mf64<", "k_stage64t": r"k_stage64t\b", "k_gather": r"k_gather<",
+    "k_gather_mf64": r"k_gather_mf64<", "k_stage64c": r"k_stage64c\b", "k_gather": r"k_gather<",
     "k_lsrk4_step": r"k_lsrk4_step_c<3, 3, double",
     "k_lsrk4_step_fp32c": r"k_lsrk4_step_c<3, 3, float, false", "k_stage": r"k_stage\(",
     "k_rs_scatter": r"k_rs_scatter\b", "k_shape_integral": r"k_shape_integral<",
@@ -961,7 +961,7 @@ def main(argv=None):
             "walk_node_visits_per_particle": walk_visits / max(1, n_last),
             "pairs_inside_R": n_in, "particles": n_last}
     roof64 = {"bound": "mfma", "kernel": "k_gather_mf64 (rho+j deposition over culled chunk "
-              "lists, FP64 matrix cores; with its operand staging k_stage64t)",
+              "lists, FP64 matrix cores)",
               "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
               "frac": achieved_tflops / FP64_PEAK_TFLOPS,
               "traffic": measured_traffic("k_gather_mf64"),

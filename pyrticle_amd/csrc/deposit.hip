@@ -287,47 +287,75 @@ __global__ __launch_bounds__(256) void k_stage32c(int D, int64_t nch,
   if (lane == 0) meta[c] = make_float4(ctr[0], ctr[1], ctr[2], sqrtf(r2) * (1.f + 1e-5f) + 1e-30f);
 }
 
-// Operand staging of the FP64 matrix-core gather, chunk c (one wave) = the
-// FP64-staged particles [first, first + count) of source element s, four
-// slices of 16 (slot 16 sl + r):
+// Chunk staging of the FP64 matrix-core gather, chunk c (one wave) = the
+// particles [first, first + count) of the element-sorted order, all in source
+// element s, four slices of 16 (slot 16 sl + r):
 //   pos_t[256 c + 64 sl + l] = row l / 16 of the slice's particle l % 16:
-//                              x - c_s (3), then -|x - c_s|^2 / R;
+//                              p = x - c_s (3), then -|p|^2 / R;
 //   wgt_t[256 c + 64 sl + l] = component l % 4 of particle
-//                              4 ((l / 4) % 4) + l / 16: (a, a v);
-// zeros for padding slots (slot >= count).
-__global__ __launch_bounds__(256) void k_stage64t(int64_t nch, const uint2 *__restrict__ chunks,
+//                              4 ((l / 4) % 4) + l / 16: (a, a v), a = q scale;
+// zeros for padding slots (slot >= count); and meta[c], the chunk's sphere of
+// k_stage32c (from the same FP32-rounded offsets, so both precisions cull the
+// same lists).  The rows pass through a wave-private LDS transpose.
+__global__ __launch_bounds__(256) void k_stage64c(int D, int64_t nch,
+                                                  const uint2 *__restrict__ chunks,
                                                   const uint32_t *__restrict__ seg,
-                                                  const double4 *__restrict__ pos,
-                                                  const double4 *__restrict__ wgt,
+                                                  const uint32_t *__restrict__ perm,
+                                                  const double *__restrict__ x,
+                                                  const double *__restrict__ q,
+                                                  const double *__restrict__ v, double scale,
                                                   const double *__restrict__ geom, double invR,
                                                   double *__restrict__ pos_t,
-                                                  double *__restrict__ wgt_t) {
-  const int lane = threadIdx.x & 63;
-  const int64_t c = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
-  if (c >= nch) return;   // uniform per wave
+                                                  double *__restrict__ wgt_t,
+                                                  float4 *__restrict__ meta) {
+  __shared__ double s_p[4][64][4], s_w[4][64][4];   // per wave: rows of its 64 slots
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t c = xcd_block(blockIdx.x, gridDim.x) * 4 + wv;
+  if (c >= nch) return;   // uniform per wave (the LDS rows are the wave's own)
   const uint2 ch = chunks[c];
   const uint32_t s = ch.x, first = ch.y;
   const uint32_t count = min(64u, seg[s + 1] - first);
-  const double sx = geom[(size_t)s * 8 + 0], sy = geom[(size_t)s * 8 + 1],
-               sz = geom[(size_t)s * 8 + 2];
-  const int kc = lane >> 4, vc = lane & 3;
-  for (int sl = 0; sl < 4; ++sl) {
-    const uint32_t pa = 16 * sl + (lane & 15);
-    double pv = 0.0;
-    if (pa < count) {
-      const double4 x = pos[first + pa];
-      const double dx = x.x - sx, dy = x.y - sy, dz = x.z - sz;
-      pv = kc == 0 ? dx : (kc == 1 ? dy : (kc == 2 ? dz : -fma(dz, dz, fma(dy, dy, dx * dx)) * invR));
+  const bool valid = (uint32_t)lane < count;
+  double p[3] = {0.0, 0.0, 0.0}, w[4] = {0.0, 0.0, 0.0, 0.0};
+  float xs[3] = {0.f, 0.f, 0.f};
+  if (valid) {
+    const uint32_t i = perm[first + lane];
+    const double a = q[i] * scale;
+    w[0] = a;
+    for (int k = 0; k < D; ++k) {
+      p[k] = x[(size_t)i * D + k] - geom[(size_t)s * 8 + k];
+      xs[k] = (float)p[k];
+      if (v) w[1 + k] = a * v[(size_t)i * D + k];
     }
-    pos_t[(size_t)c * 256 + 64 * sl + lane] = pv;
-    const uint32_t pb = 16 * sl + 4 * ((lane >> 2) & 3) + (lane >> 4);
-    double wv = 0.0;
-    if (pb < count) {
-      const double4 w = wgt[first + pb];
-      wv = vc == 0 ? w.x : (vc == 1 ? w.y : (vc == 2 ? w.z : w.w));
-    }
-    wgt_t[(size_t)c * 256 + 64 * sl + lane] = wv;
   }
+  s_p[wv][lane][0] = p[0];
+  s_p[wv][lane][1] = p[1];
+  s_p[wv][lane][2] = p[2];
+  s_p[wv][lane][3] = -fma(p[2], p[2], fma(p[1], p[1], p[0] * p[0])) * invR;
+  for (int k = 0; k < 4; ++k) s_w[wv][lane][k] = w[k];
+  wave_lds_fence();
+  for (int sl = 0; sl < 4; ++sl) {
+    pos_t[(size_t)c * 256 + 64 * sl + lane] = s_p[wv][16 * sl + (lane & 15)][lane >> 4];
+    wgt_t[(size_t)c * 256 + 64 * sl + lane] =
+        s_w[wv][16 * sl + 4 * ((lane >> 2) & 3) + (lane >> 4)][lane & 3];
+  }
+  float mn[3], mx[3];
+  for (int k = 0; k < 3; ++k) {
+    mn[k] = valid ? xs[k] : INFINITY;
+    mx[k] = valid ? xs[k] : -INFINITY;
+  }
+  for (int o = 32; o > 0; o >>= 1)
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = fminf(mn[k], __shfl_xor(mn[k], o, 64));
+      mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], o, 64));
+    }
+  float ctr[3];
+  for (int k = 0; k < 3; ++k) ctr[k] = 0.5f * (mn[k] + mx[k]);
+  float r2 = 0.f;
+  if (valid)
+    for (int k = 0; k < 3; ++k) r2 = fmaf(xs[k] - ctr[k], xs[k] - ctr[k], r2);
+  for (int o = 32; o > 0; o >>= 1) r2 = fmaxf(r2, __shfl_xor(r2, o, 64));
+  if (lane == 0) meta[c] = make_float4(ctr[0], ctr[1], ctr[2], sqrtf(r2) * (1.f + 1e-5f) + 1e-30f);
 }
 
 // rho and j of every target from its work items' partial sums, added in item
@@ -1138,8 +1166,11 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
     {
       ProfScope prof(PROF_SORT);
       if (n > 0) {
-        const bool fp64 = precision == PYR_FP64;
-        perm = sort_and_stage(mesh, c, n, x, q, j ? v : nullptr, elem, normalizer, fp64, sorted);
+        // the FP64 staging of the sorted order serves the pair-counting mode's
+        // candidate gather; the list gathers stage their own chunks
+        const bool stage64 = precision == PYR_FP64 && mesh->deposit_count != 0;
+        perm = sort_and_stage(mesh, c, n, x, q, j ? v : nullptr, elem, normalizer, stage64,
+                              sorted);
       } else {
         c->seg.reserve(sizeof(uint32_t) * ((size_t)K + 1));
         PYR_HIP(hipMemsetAsync(c->seg.ptr, 0, sizeof(uint32_t) * ((size_t)K + 1), s));
@@ -1212,21 +1243,35 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
       {
         ProfScope prof(PROF_SORT);
         nch = count_wait(c, 0);
-        // chunk nch: all zero, read by the list gather past the end of a list
-        c->pos32c.reserve(sizeof(float4) * 64 * (size_t)(nch + 1));
-        c->wgt32c.reserve(sizeof(float4) * 64 * (size_t)(nch + 1));
-        PYR_HIP(hipMemsetAsync(c->pos32c.as<float4>() + 64 * (size_t)nch, 0,
-                               sizeof(float4) * 64, s));
-        PYR_HIP(hipMemsetAsync(c->wgt32c.as<float4>() + 64 * (size_t)nch, 0,
-                               sizeof(float4) * 64, s));
         c->ch_meta.reserve(sizeof(float4) * (size_t)std::max<int64_t>(1, nch));
-        if (nch > 0) {
-          const double scale = normalizer * std::pow(p_scale(radius), alpha);
-          hipLaunchKernelGGL(k_stage32c, dim3((unsigned)((nch + 3) / 4)), dim3(256), 0, s, D,
-                             nch, cl.chunks, c->seg.as<uint32_t>(), perm, x, q, j ? v : nullptr,
-                             scale, c->geom.as<double>(), c->pos32c.as<float4>(),
-                             c->wgt32c.as<float>(), c->ch_meta.as<float4>());
-          PYR_CHECK_LAUNCH();
+        if (precision == PYR_FP32C) {
+          // chunk nch: all zero, read by the list gather past the end of a list
+          c->pos32c.reserve(sizeof(float4) * 64 * (size_t)(nch + 1));
+          c->wgt32c.reserve(sizeof(float4) * 64 * (size_t)(nch + 1));
+          PYR_HIP(hipMemsetAsync(c->pos32c.as<float4>() + 64 * (size_t)nch, 0,
+                                 sizeof(float4) * 64, s));
+          PYR_HIP(hipMemsetAsync(c->wgt32c.as<float4>() + 64 * (size_t)nch, 0,
+                                 sizeof(float4) * 64, s));
+          if (nch > 0) {
+            const double scale = normalizer * std::pow(p_scale(radius), alpha);
+            hipLaunchKernelGGL(k_stage32c, dim3((unsigned)((nch + 3) / 4)), dim3(256), 0, s, D,
+                               nch, cl.chunks, c->seg.as<uint32_t>(), perm, x, q,
+                               j ? v : nullptr, scale, c->geom.as<double>(),
+                               c->pos32c.as<float4>(), c->wgt32c.as<float>(),
+                               c->ch_meta.as<float4>());
+            PYR_CHECK_LAUNCH();
+          }
+        } else {
+          c->pos64t.reserve(sizeof(double) * 256 * (size_t)std::max<int64_t>(1, nch));
+          c->wgt64t.reserve(sizeof(double) * 256 * (size_t)std::max<int64_t>(1, nch));
+          if (nch > 0) {
+            hipLaunchKernelGGL(k_stage64c, dim3((unsigned)((nch + 3) / 4)), dim3(256), 0, s, D,
+                               nch, cl.chunks, c->seg.as<uint32_t>(), perm, x, q,
+                               j ? v : nullptr, normalizer, c->geom.as<double>(), 1.0 / radius,
+                               c->pos64t.as<double>(), c->wgt64t.as<double>(),
+                               c->ch_meta.as<float4>());
+            PYR_CHECK_LAUNCH();
+          }
         }
       }
       // per-target lists of the chunks that reach a node group: capacity =
@@ -1327,13 +1372,6 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
         if (n_items && precision == PYR_FP32C) {
           launch_gather_cl(D, Np, la, n_items, alpha == 2.0);
         } else if (n_items) {
-          c->pos64t.reserve(sizeof(double) * 256 * (size_t)std::max<int64_t>(1, nch));
-          c->wgt64t.reserve(sizeof(double) * 256 * (size_t)std::max<int64_t>(1, nch));
-          hipLaunchKernelGGL(k_stage64t, dim3((unsigned)((nch + 3) / 4)), dim3(256), 0, s, nch,
-                             cl.chunks, c->seg.as<uint32_t>(), c->pos.as<double4>(),
-                             c->wgt.as<double4>(), c->geom.as<double>(), 1.0 / radius,
-                             c->pos64t.as<double>(), c->wgt64t.as<double>());
-          PYR_CHECK_LAUNCH();
           ListArgsM64 lm{};
           lm.K = K;
           lm.R = radius;

@@ -441,12 +441,6 @@ __device__ __forceinline__ uint4 cache_chunk(const DevMesh &m, const T *__restri
   return *(const uint4 *)src;
 }
 
-__device__ __forceinline__ void wave_lds_fence() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Gives slots to the elements of the active lanes in `need` (lowest lane's
 // element first), then to the face neighbours of those elements, in slots no
 // lane holds (used), as many as fit; copies their data with the active lanes

@@ -536,6 +536,14 @@ void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *
 
 }  // namespace pyr
 
+// Orders a wave's LDS writes before its later LDS reads of other lanes' data
+// (the compiler sees no dependence between them).
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 namespace pyr {
 // Block b of a grid of nblocks -> the block index it handles, such that chunks
 // of 32 consecutive indices go to one XCD (the hardware deals consecutive
