@@ -254,7 +254,8 @@ int pyr_velocities_backward(int64_t n, int vdim, double vacuum_c, const double *
  * precision PYR_FP32C evaluates the shape and its products in FP32 (on the
  * matrix cores) and folds short FP32 partial sums into FP64 per-node sums;
  * positions relative to the element, q*normalizer and the outputs stay FP64.
- * Both precisions give bitwise reproducible results. */
+ * PYR_FP64 evaluates and sums in FP64 (on the FP64 matrix cores).  Both
+ * precisions give bitwise reproducible results. */
 int pyr_deposit(pyr_mesh *mesh, double radius, double alpha, double normalizer,
                 int precision, int64_t n, int vdim, const double *positions,
                 const double *charges, const double *velocities,
@@ -377,8 +378,8 @@ int pyr_deposit_list_stats(pyr_mesh *mesh, unsigned long long out[4]);
  * lists for radius R (built on first use, cached per radius). */
 int pyr_deposit_candidates(pyr_mesh *mesh, double radius, int64_t *n_pairs_out);
 
-/* Stable sort of particle indices by containing element (the counting/radix
- * sort the deposition uses): perm_out[j] is the particle at sorted position j
+/* Stable sort of particle indices by containing element (the radix sort the
+ * deposition uses): perm_out[j] is the particle at sorted position j
  * (numpy.argsort(kind="stable") of the element ids); seg_out (K+1, may be NULL)
  * the start of each element's run.  No reference counterpart. */
 int pyr_sort_by_element(pyr_mesh *mesh, int64_t n, const uint32_t *containing_elements,
