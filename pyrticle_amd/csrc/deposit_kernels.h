@@ -506,13 +506,22 @@ __global__ __launch_bounds__(256) void k_cull_chunks(CullArgs g) {
           if (near) ngm |= 1u << ng;
         }
       }
-      const uint64_t live = __ballot(ngm != 0);
-      if (ngm) {
-        const uint32_t at = base + n_out + (uint32_t)__popcll(live & lt_mask);
-        g.ent[at] = make_uint2(ch, ngm | (pcode << ENT_SHIFT_BIT));
-        g.ent_off[at] = make_float4(px, py, pz, 0.f);
+      // the batch's kept chunks written grouped by group mask (in lane order
+      // within a mask), so that the gather's pairs of consecutive entries
+      // mostly share their groups
+      uint64_t pend = __ballot(ngm != 0);
+      while (pend) {
+        const int l0 = __builtin_ctzll(pend);
+        const uint32_t m0 = (uint32_t)__builtin_amdgcn_readlane((int)ngm, l0);
+        const uint64_t grp = __ballot(ngm == m0) & pend;
+        if ((grp >> lane) & 1ull) {
+          const uint32_t at = base + n_out + (uint32_t)__popcll(grp & lt_mask);
+          g.ent[at] = make_uint2(ch, ngm | (pcode << ENT_SHIFT_BIT));
+          g.ent_off[at] = make_float4(px, py, pz, 0.f);
+        }
+        n_out += (uint32_t)__popcll(grp);
+        pend &= ~grp;
       }
-      n_out += (uint32_t)__popcll(live);
     }
   }
   if (lane == 0) g.n_live[e] = n_out;
