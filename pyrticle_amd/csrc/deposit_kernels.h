@@ -1050,6 +1050,50 @@ __global__ __launch_bounds__(64) void k_deposit_adj(AdjArgs g) {
 
 
 // ---------------------------------------------------------------------------
+// node skip of the per-particle gathers
+// ---------------------------------------------------------------------------
+// A sphere around the wave's valid particles (FP64: centre of their bounding
+// box, radius to the farthest rounded up); a wave holds at least one.
+__device__ __forceinline__ void wave_sphere(bool valid, const double4 &pp, double c[3],
+                                            double &r) {
+  double mn[3] = {valid ? pp.x : HUGE_VAL, valid ? pp.y : HUGE_VAL, valid ? pp.z : HUGE_VAL};
+  double mx[3] = {valid ? pp.x : -HUGE_VAL, valid ? pp.y : -HUGE_VAL,
+                  valid ? pp.z : -HUGE_VAL};
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = fmin(mn[k], __shfl_xor(mn[k], o, 64));
+      mx[k] = fmax(mx[k], __shfl_xor(mx[k], o, 64));
+    }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) c[k] = 0.5 * (mn[k] + mx[k]);
+  const double dx = pp.x - c[0], dy = pp.y - c[1], dz = pp.z - c[2];
+  double r2 = valid ? fma(dz, dz, fma(dy, dy, dx * dx)) : 0.0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o, 64));
+  r = sqrt(r2) * (1.0 + 1e-12) + 1e-300;
+}
+
+// The nodes of a target (rows of its FP64 node table: n - c_t, ...) that lie
+// within lim of the point cc (relative to c_t): bit i for node i, one lane per
+// node.  A node outside has every particle of the wave's sphere farther than
+// R from it when lim >= R + r (by the 1e-9 relative margin the callers add), so
+// its shape value is exactly 0 for all of them and the wave can skip it.
+template <int D, int NP>
+__device__ __forceinline__ uint64_t near_nodes(const double *nt, const double cc[3], double lim) {
+  static_assert(NP <= 64, "one lane per node");
+  const int lane = threadIdx.x & 63;
+  bool near = false;
+  if (lane < NP) {
+    const double dx = nt[lane * 4 + 0] - cc[0], dy = nt[lane * 4 + 1] - cc[1],
+                 dz = D == 3 ? nt[lane * 4 + 2] - cc[2] : 0.0;
+    near = fma(dz, dz, fma(dy, dy, dx * dx)) <= lim * lim;
+  }
+  return __ballot(near);
+}
+
+// ---------------------------------------------------------------------------
 // per-particle shape integral of the normalized deposition
 // ---------------------------------------------------------------------------
 // dep_normshape.hpp:124-165: I_p = sum_t |J_t| sum_i w_i S(n_ti - x_p) over the
@@ -1091,6 +1135,9 @@ __global__ __launch_bounds__(64) void k_shape_integral(IntegralArgs g) {
     const uint32_t jj = base + lane;
     const bool valid = jj < p1;
     const double4 pp = valid ? g.pos[jj] : make_double4(0, 0, 0, 0);
+    double wc[3], wr;
+    wave_sphere(valid, pp, wc, wr);
+    const double lim = g.R * (1.0 + 1e-9) + wr;
     double integral = 0;
     for (uint32_t ti = t_begin; ti < t_end; ++ti) {
       const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.tr[ti]);
@@ -1114,9 +1161,12 @@ __global__ __launch_bounds__(64) void k_shape_integral(IntegralArgs g) {
       // its square term 0
       const double c0 = A2 && !pass ? -HUGE_VAL : fma(-dd, g.invR, g.R);
       const CD nt = ntab_all + (size_t)t * NP * 4;
+      const double cc[3] = {wc[0] + sh[0] - gt[0], wc[1] + sh[1] - gt[1], wc[2] + sh[2] - gt[2]};
+      const uint64_t nmask = near_nodes<D, NP>(g.node_tab + (size_t)t * NP * 4, cc, lim);
       double el_integral = 0;
 #pragma unroll
       for (int i = 0; i < NP; ++i) {
+        if (!((nmask >> i) & 1ull)) continue;   // uniform: S = 0 for the whole wave
         double tt = c0 - nt[i * 4 + 3];
 #pragma unroll
         for (int k = D - 1; k >= 0; --k) tt = fma(nt[i * 4 + k], m[k], tt);
@@ -1175,6 +1225,9 @@ __global__ __launch_bounds__(64) void k_shape_average(AverageArgs g) {
     const uint32_t jj = base + lane;
     const bool valid = jj < p1;
     const double4 pp = valid ? g.b.pos[jj] : make_double4(0, 0, 0, 0);
+    double wc[3], wr;
+    wave_sphere(valid, pp, wc, wr);
+    const double lim = g.b.R * (1.0 + 1e-9) + wr;
     double aI = 0, aF[6] = {0, 0, 0, 0, 0, 0}, aS[2] = {0, 0};
     for (uint32_t ti = t_begin; ti < t_end; ++ti) {
       const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.b.tr[ti]);
@@ -1198,11 +1251,15 @@ __global__ __launch_bounds__(64) void k_shape_average(AverageArgs g) {
       // its square term 0
       const double c0 = A2 && !pass ? -HUGE_VAL : fma(-dd, g.b.invR, g.b.R);
       const CD nt = ntab_all + (size_t)t * NP * 4;
-      // c_i = S_i w_i for this element's nodes
+      const double cc[3] = {wc[0] + sh[0] - gt[0], wc[1] + sh[1] - gt[1], wc[2] + sh[2] - gt[2]};
+      const uint64_t nmask = near_nodes<D, NP>(g.b.node_tab + (size_t)t * NP * 4, cc, lim);
+      // c_i = S_i w_i for this element's nodes (0 for the skipped ones)
       double cw[NP];
       double eI = 0;
 #pragma unroll
       for (int i = 0; i < NP; ++i) {
+        cw[i] = 0.0;
+        if (!((nmask >> i) & 1ull)) continue;   // uniform: S = 0 for the whole wave
         double tt = c0 - nt[i * 4 + 3];
 #pragma unroll
         for (int k = D - 1; k >= 0; --k) tt = fma(nt[i * 4 + k], m[k], tt);
