@@ -911,6 +911,51 @@ __global__ __launch_bounds__(64 * W) void k_gather_mf64(ListArgsM64 g) {
 }
 
 // ---------------------------------------------------------------------------
+// node skip of the per-particle gathers
+// ---------------------------------------------------------------------------
+// A sphere around the wave's valid particles (FP64: centre of their bounding
+// box, radius to the farthest rounded up); a wave holds at least one.
+__device__ __forceinline__ void wave_sphere(bool valid, const double4 &pp, double c[3],
+                                            double &r) {
+  double mn[3] = {valid ? pp.x : HUGE_VAL, valid ? pp.y : HUGE_VAL, valid ? pp.z : HUGE_VAL};
+  double mx[3] = {valid ? pp.x : -HUGE_VAL, valid ? pp.y : -HUGE_VAL,
+                  valid ? pp.z : -HUGE_VAL};
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = fmin(mn[k], __shfl_xor(mn[k], o, 64));
+      mx[k] = fmax(mx[k], __shfl_xor(mx[k], o, 64));
+    }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) c[k] = 0.5 * (mn[k] + mx[k]);
+  const double dx = pp.x - c[0], dy = pp.y - c[1], dz = pp.z - c[2];
+  double r2 = valid ? fma(dz, dz, fma(dy, dy, dx * dx)) : 0.0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o, 64));
+  r = sqrt(r2) * (1.0 + 1e-12) + 1e-300;
+}
+
+// The nodes of a target (rows of its node table: n - c_t, ...) that lie within
+// lim of the point cc (relative to c_t): bit i for node i, one lane per node.
+// A node outside has every particle of the wave's sphere farther than R from
+// it when lim >= R + r (by the relative margin the callers add: 1e-9 over an
+// FP64 table, 1e-5 over an FP32 one), so its shape value is exactly 0 for all
+// of them and the wave can skip it.
+template <int D, int NP, class T = double>
+__device__ __forceinline__ uint64_t near_nodes(const T *nt, const double cc[3], double lim) {
+  static_assert(NP <= 64, "one lane per node");
+  const int lane = threadIdx.x & 63;
+  bool near = false;
+  if (lane < NP) {
+    const double dx = (double)nt[lane * 4 + 0] - cc[0], dy = (double)nt[lane * 4 + 1] - cc[1],
+                 dz = D == 3 ? (double)nt[lane * 4 + 2] - cc[2] : 0.0;
+    near = fma(dz, dz, fma(dy, dy, dx * dx)) <= lim * lim;
+  }
+  return __ballot(near);
+}
+
+// ---------------------------------------------------------------------------
 // adjoint of the deposition: per-particle gather of grad S . (rho_bar, j_bar)
 // ---------------------------------------------------------------------------
 // For particle p with a = q N and the nodes i within R:
@@ -962,6 +1007,9 @@ __global__ __launch_bounds__(64) void k_deposit_adj(AdjArgs g) {
     }
     const T wa = (T)ww.x;
     const T wv[3] = {(T)ww.y, (T)ww.z, (T)ww.w};
+    double wc[3], wr;
+    wave_sphere(valid, pp, wc, wr);
+    const double lim = g.R * (1.0 + (FP32 ? 1e-5 : 1e-9)) + wr;
     double gx[D], gv[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) {
@@ -988,6 +1036,9 @@ __global__ __launch_bounds__(64) void k_deposit_adj(AdjArgs g) {
       const T m[3] = {(T)(d[0] * two_inv_r), (T)(d[1] * two_inv_r), (T)(d[2] * two_inv_r)};
       const T c0 = (T)fma(-dd, g.invR, g.R);
       const CT nt = ntab_all + (size_t)t * NP * 4;
+      const double cc[3] = {wc[0] + sh[0] - gt[0], wc[1] + sh[1] - gt[1], wc[2] + sh[2] - gt[2]};
+      const uint64_t nmask = near_nodes<D, NP, T>(static_cast<const T *>(g.node_tab) +
+                                                      (size_t)t * NP * 4, cc, lim);
       const CD rb = (CD)g.rho_bar + (size_t)t * NP;
       const CD jb = (CD)g.j_bar + (size_t)t * NP * D;
       T sD = T(0), sDn[D], sS[D];
@@ -998,6 +1049,7 @@ __global__ __launch_bounds__(64) void k_deposit_adj(AdjArgs g) {
       }
 #pragma unroll
       for (int i = 0; i < NP; ++i) {
+        if (!((nmask >> i) & 1ull)) continue;   // uniform: w = 0 for the whole wave
         T tt = c0 - nt[i * 4 + 3];
 #pragma unroll
         for (int k = D - 1; k >= 0; --k) tt = fma(nt[i * 4 + k], m[k], tt);
@@ -1048,50 +1100,6 @@ __global__ __launch_bounds__(64) void k_deposit_adj(AdjArgs g) {
   }
 }
 
-
-// ---------------------------------------------------------------------------
-// node skip of the per-particle gathers
-// ---------------------------------------------------------------------------
-// A sphere around the wave's valid particles (FP64: centre of their bounding
-// box, radius to the farthest rounded up); a wave holds at least one.
-__device__ __forceinline__ void wave_sphere(bool valid, const double4 &pp, double c[3],
-                                            double &r) {
-  double mn[3] = {valid ? pp.x : HUGE_VAL, valid ? pp.y : HUGE_VAL, valid ? pp.z : HUGE_VAL};
-  double mx[3] = {valid ? pp.x : -HUGE_VAL, valid ? pp.y : -HUGE_VAL,
-                  valid ? pp.z : -HUGE_VAL};
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      mn[k] = fmin(mn[k], __shfl_xor(mn[k], o, 64));
-      mx[k] = fmax(mx[k], __shfl_xor(mx[k], o, 64));
-    }
-#pragma unroll
-  for (int k = 0; k < 3; ++k) c[k] = 0.5 * (mn[k] + mx[k]);
-  const double dx = pp.x - c[0], dy = pp.y - c[1], dz = pp.z - c[2];
-  double r2 = valid ? fma(dz, dz, fma(dy, dy, dx * dx)) : 0.0;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o, 64));
-  r = sqrt(r2) * (1.0 + 1e-12) + 1e-300;
-}
-
-// The nodes of a target (rows of its FP64 node table: n - c_t, ...) that lie
-// within lim of the point cc (relative to c_t): bit i for node i, one lane per
-// node.  A node outside has every particle of the wave's sphere farther than
-// R from it when lim >= R + r (by the 1e-9 relative margin the callers add), so
-// its shape value is exactly 0 for all of them and the wave can skip it.
-template <int D, int NP>
-__device__ __forceinline__ uint64_t near_nodes(const double *nt, const double cc[3], double lim) {
-  static_assert(NP <= 64, "one lane per node");
-  const int lane = threadIdx.x & 63;
-  bool near = false;
-  if (lane < NP) {
-    const double dx = nt[lane * 4 + 0] - cc[0], dy = nt[lane * 4 + 1] - cc[1],
-                 dz = D == 3 ? nt[lane * 4 + 2] - cc[2] : 0.0;
-    near = fma(dz, dz, fma(dy, dy, dx * dx)) <= lim * lim;
-  }
-  return __ballot(near);
-}
 
 // ---------------------------------------------------------------------------
 // per-particle shape integral of the normalized deposition
