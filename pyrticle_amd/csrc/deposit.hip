@@ -1161,7 +1161,6 @@ static void deposit_impl(pyr_mesh *mesh, double radius, double alpha, double nor
     hipStream_t s = stream();
     prepare_tables(mesh, c, radius);
     const uint32_t *perm = nullptr;
-    for (DevBuf *b : {&c->pos, &c->wgt}) b->reserve(sizeof(double4) * (size_t)std::max<int64_t>(1, n));
     if (n > 0) PYR_REQUIRE(x && q && elem, "null pointer");
     {
       ProfScope prof(PROF_SORT);
