@@ -266,8 +266,18 @@ __global__ __launch_bounds__(256) void k_stage32c(int D, int64_t nch,
   }
   const int64_t slot = 64 * c + lane;
   pos[slot] = make_float4(xs[0], xs[1], xs[2], 0.f);
-  const int64_t quad = slot & ~(int64_t)3;
-  for (int k = 0; k < 4; ++k) wgt_t[(quad + k) * 4 + (slot & 3)] = w[k];
+  {
+    // the quad's weights transposed through the wave's own LDS row: one
+    // 16-byte store per lane (component k of the quad's four particles)
+    __shared__ float4 s_w[4][64];
+    const int wv = threadIdx.x >> 6;
+    s_w[wv][lane] = make_float4(w[0], w[1], w[2], w[3]);
+    wave_lds_fence();
+    const int qb = lane & ~3, k = lane & 3;
+    const float *src = reinterpret_cast<const float *>(&s_w[wv][qb]);
+    reinterpret_cast<float4 *>(wgt_t)[slot] =
+        make_float4(src[0 * 4 + k], src[1 * 4 + k], src[2 * 4 + k], src[3 * 4 + k]);
+  }
   float mn[3], mx[3];
   for (int k = 0; k < 3; ++k) {
     mn[k] = valid ? xs[k] : INFINITY;
