@@ -171,13 +171,18 @@ def test_forces_3d(small_cylinder, cuda, dmesh3):
     assert rel_l2(F.cpu().numpy(), fo) < 1e-13
 
 
-def test_sort_by_element_is_stable_argsort(square, dmesh2, cuda):
+@pytest.mark.parametrize("n", [50_000, 1_500_000])
+def test_sort_by_element_is_stable_argsort(square, dmesh2, cuda, n):
+    """Both sort paths (LSD passes below 2^20 keys, rocPRIM's onesweep sort
+    above), with 1% of the particles outside the mesh (their ids clamp to K and
+    sort last, in index order)."""
     from pyrticle_amd import device as dv
     rng = np.random.default_rng(9)
-    el = rng.integers(0, square.ma.K, size=50_000).astype(np.uint32)
+    el = rng.integers(0, square.ma.K, size=n).astype(np.uint32)
+    el[rng.random(n) < 0.01] = dv.INVALID_ELEMENT
     perm, seg = dmesh2.sort_by_element(dv.element_tensor(el, cuda))
     assert np.array_equal(perm.cpu().numpy(), np.argsort(el, kind="stable"))
-    counts = np.bincount(el, minlength=square.ma.K)
+    counts = np.bincount(el[el != dv.INVALID_ELEMENT], minlength=square.ma.K)
     assert np.array_equal(seg.cpu().numpy(), np.concatenate([[0], np.cumsum(counts)]))
 
 
