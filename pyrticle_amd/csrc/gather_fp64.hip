@@ -47,7 +47,8 @@ void launch_gather_fp64(int D, int Np, const GatherArgs &ga, int K, bool a2, boo
 constexpr int MF64_W = 2;
 template <int D, int NP>
 static void launch_mf64(const ListArgsM64 &la, uint32_t n_items, bool a2) {
-  dim3 grid(gather_grid((int)((n_items + MF64_W - 1) / MF64_W))), block(64 * MF64_W);
+  dim3 grid(gather_grid<XCD_CHUNK_MF64>((int)((n_items + MF64_W - 1) / MF64_W))),
+      block(64 * MF64_W);
   hipStream_t s = stream();
   if (a2) hipLaunchKernelGGL((k_gather_mf64<D, NP, MF64_W, true>), grid, block, 0, s, la);
   else hipLaunchKernelGGL((k_gather_mf64<D, NP, MF64_W, false>), grid, block, 0, s, la);
