@@ -546,14 +546,15 @@ __device__ __forceinline__ void wave_lds_fence() {
 
 namespace pyr {
 // Block b of a grid of nblocks -> the block index it handles, such that chunks
-// of 32 consecutive indices go to one XCD (the hardware deals consecutive
+// of 128 consecutive indices go to one XCD (the hardware deals consecutive
 // blocks round-robin to the 8 XCDs).  Kernels that read particles in a
 // shuffled order (the staging after the element sort, the push in that order)
-// then fetch each element's particles through one L2 instead of all eight.  A
-// bijection on [0, nblocks): the blocks past the last whole group of 8 chunks
-// map to themselves.
+// then fetch each element's particles through one L2 instead of all eight
+// (runs of 128 against 32: steps 0.7% shorter on the 4M-particle cylinder,
+// runs of 8 no faster than 32).  A bijection on [0, nblocks): the blocks past
+// the last whole group of 8 chunks map to themselves.
 __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nblocks) {
-  constexpr int64_t C = 32;
+  constexpr int64_t C = 128;
   const int64_t full = nblocks / (8 * C) * (8 * C);
   if (b >= full) return b;
   const int64_t xcd = b & 7, slot = b >> 3;
